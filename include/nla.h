@@ -1,0 +1,121 @@
+/*
+ * nla.h — C ABI of libnlahip, the MI355X (gfx950) implementation of NextLA.jl's
+ * recursive triangular solve / multiply hot path.
+ *
+ * Conventions shared by every entry point
+ *   - Matrices are column-major device pointers with a leading dimension
+ *     (element (i, j) at p[i + j*ld], 0-based), as Julia's Matrix / ROCMatrix.
+ *   - n is the order of the triangular matrix A; m is the other dimension of B:
+ *     B is n x m for side 'L' and m x n for side 'R'.
+ *   - Only the `uplo` triangle of A is read; the diagonal is non-unit.
+ *   - trans: 'N', 'T' or 'C' ('C' equals 'T' for the real element types).
+ *   - dtype: NLA_F32 or NLA_F64; A and B share it.
+ *   - stream: a hipStream_t passed as void* (NULL = the null stream).  Work is
+ *     enqueued asynchronously; nothing synchronizes inside the plain path, and
+ *     results are visible after the caller synchronizes the stream.
+ *   - Return value: 0 on success; -k when argument k (1-based) is invalid,
+ *     checked before any device work, so B is untouched on error; a positive
+ *     NLA_ERR_* code when the HIP runtime reports a failure.
+ *   - The plain entry points allocate no device memory.
+ */
+#ifndef NLA_H
+#define NLA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nla_dtype {
+    NLA_F32 = 0,
+    NLA_F64 = 1,
+};
+
+enum nla_status {
+    NLA_OK = 0,
+    NLA_ERR_HIP = 1,          /* a HIP runtime call failed */
+    NLA_ERR_NO_DEVICE = 2,    /* no gfx950 device visible */
+    NLA_ERR_UNSUPPORTED = 3,  /* combination not supported by this build */
+    NLA_ERR_WORKSPACE = 4,    /* workspace too small / budget exceeded */
+    NLA_ERR_NOT_CONVERGED = 5 /* iterative refinement did not reach the target */
+};
+
+/* Library / device information.  Returns the ABI version (>= 1). */
+int32_t nla_abi_version(void);
+
+/* Name of the last error on the calling thread (static string, never NULL). */
+const char *nla_last_error(void);
+
+/*
+ * unified_rectrxm!(side, uplo, transpose, alpha, func, A, B)
+ * Replaces src/rectrxm.jl:43-76 (with the recursion src/rectrxm.jl:101-198).
+ *   func 'S' (TRSM): solve op(A)·X = alpha·B (side 'L') or X·op(A) = alpha·B
+ *                    (side 'R'); X overwrites B.
+ *   func 'M' (TRMM): B <- alpha·op(A)·B (side 'L') or alpha·B·op(A) (side 'R').
+ * Argument positions for error codes: 1 side, 2 uplo, 3 trans, 4 func,
+ * 5 dtype, 6 alpha, 7 n, 8 m, 9 A, 10 lda, 11 B, 12 ldb, 13 stream.
+ */
+int32_t nla_rectrxm(char side, char uplo, char trans, char func, int32_t dtype, double alpha, int64_t n,
+                    int64_t m, const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
+
+/*
+ * GEMM_ADD!(X, Y, C): C += X·Y, X is N x R, Y is R x M, C is N x M.
+ * Replaces src/matmul.jl:69-73 (kernel matmul!, src/matmul.jl:5-66).
+ */
+int32_t nla_gemm_add(int32_t dtype, int64_t N, int64_t R, int64_t M, const void *X, int64_t ldx, const void *Y,
+                     int64_t ldy, void *C, int64_t ldc, void *stream);
+
+/*
+ * GEMM_SUB!(C, X, Y): C -= X·Y, C is N x M, X is N x R, Y is R x M.
+ * Replaces src/matmul.jl:76-80.  C comes first, as in the reference.
+ */
+int32_t nla_gemm_sub(int32_t dtype, int64_t N, int64_t R, int64_t M, void *C, int64_t ldc, const void *X,
+                     int64_t ldx, const void *Y, int64_t ldy, void *stream);
+
+/*
+ * General GEMM update used by the recursion: C = beta·C + alpha·op(X)·op(Y),
+ * op(X) is M x K, op(Y) is K x N; transx/transy in {'N','T'}.
+ * No counterpart of its own in the reference: it is matmul! (src/matmul.jl:5-66)
+ * with the transA/transB arguments that kernel declares and the alpha scaling
+ * of src/rectrxm.jl:64,72 folded in.
+ */
+int32_t nla_gemm(int32_t dtype, char transx, char transy, int64_t M, int64_t N, int64_t K, double alpha,
+                 const void *X, int64_t ldx, const void *Y, int64_t ldy, double beta, void *C, int64_t ldc,
+                 void *stream);
+
+/*
+ * Diagonal-block leaves: LeftLowerTRSM!, LeftUpperTRSM!, RightLowerTRSM!,
+ * RightUpperTRSM! (src/trsm.jl:128-150) for func 'S', and LeftLowerTRMM!,
+ * LeftUpperTRMM!, RightLowerTRMM!, RightUpperTRMM! (src/trmm.jl:332-393) for
+ * func 'M'.  Solves A·X = B / X·A = B or forms A·B / B·A in place, with A
+ * n x n (uplo triangle, no transpose, alpha = 1).  The reference kernels are
+ * limited to n <= 1024 (TRSM) and n <= 16 (TRMM); here any n is accepted and
+ * blocks larger than one leaf go through the recursion.
+ * Argument positions: 1 func, 2 side, 3 uplo, 4 dtype, 5 n, 6 m, 7 A, 8 lda,
+ * 9 B, 10 ldb, 11 stream.
+ */
+int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int64_t m, const void *A, int64_t lda,
+                 void *B, int64_t ldb, void *stream);
+
+/*
+ * Kernel timing (instrumentation; no counterpart in the reference, which has
+ * no timers).  While enabled, every kernel the library launches is bracketed
+ * by HIP events on its stream and tagged with its algorithmic work.
+ *   nla_prof_enable(1) clears the records and starts recording; (0) stops.
+ *   nla_prof_summary(kind, ...) waits for the recorded events and returns the
+ *   launch count, summed kernel time (ms), algorithmic flops and bytes of one
+ *   kernel kind: NLA_KIND_GEMM (GEMM updates) or NLA_KIND_LEAF (diagonal-block
+ *   leaves).
+ */
+enum nla_kernel_kind { NLA_KIND_GEMM = 0, NLA_KIND_LEAF = 1 };
+
+int32_t nla_prof_enable(int32_t on);
+
+int32_t nla_prof_summary(int32_t kind, int64_t *launches, double *ms, double *flops, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NLA_H */

@@ -1,0 +1,381 @@
+// C ABI of libnlahip and the host-side recursion planner.
+//
+// nla_rectrxm restates unified_rectrxm! (src/rectrxm.jl:43-76) and its
+// recursion unified_rec (src/rectrxm.jl:101-198) for column-major device
+// matrices:
+//   - 'T'/'C' become a strided view of A with swapped strides and a flipped
+//     uplo (src/rectrxm.jl:56-59), so no transpose is ever materialised;
+//   - the 2x2 split recurses until a diagonal block fits one leaf launch;
+//     split points are multiples of the leaf size (64), which is also a
+//     multiple of every MFMA tile edge, instead of the reference's
+//     power-of-two rule (src/rectrxm.jl:129-134);
+//   - forward (A11 -> update -> A22) or backward order per the reference's
+//     table (src/rectrxm.jl:153-196), with the same GEMM_ADD!/GEMM_SUB!
+//     operands (src/rectrxm.jl:164-192);
+//   - alpha is folded into the kernels: for TRSM into the first kernel that
+//     touches each block of B (the reference scales B up front,
+//     src/rectrxm.jl:64), for TRMM into the last kernel that writes it (the
+//     reference scales afterwards, src/rectrxm.jl:72).
+// All launches go to the caller's stream in dependency order; nothing
+// synchronizes and nothing is allocated.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/nla.h"
+#include "nla_internal.h"
+
+namespace nla {
+
+static thread_local char g_last_error[256] = "no error";
+
+static int32_t hip_status(hipError_t e, const char *where) {
+    if (e == hipSuccess) return NLA_OK;
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s", where, hipGetErrorString(e));
+    return NLA_ERR_HIP;
+}
+
+static int32_t bad_arg(int k, const char *what) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "argument %d (%s) is invalid", k, what);
+    return -k;
+}
+
+// ---- kernel timing --------------------------------------------------------
+struct ProfRec {
+    int kind;
+    double flops, bytes;
+    hipEvent_t e0, e1;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+static std::vector<hipEvent_t> g_event_pool;
+
+static hipEvent_t take_event() {
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch through `fn`, bracketed by events when timing is enabled.
+template <typename F>
+static hipError_t timed(int kind, double flops, double bytes, hipStream_t s, F fn) {
+    if (!g_prof_on) return fn();
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    ProfRec r{kind, flops, bytes, take_event(), take_event()};
+    if (!r.e0 || !r.e1) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(r.e0, s);
+    if (e != hipSuccess) return e;
+    e = fn();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(r.e1, s);
+    if (e != hipSuccess) return e;
+    g_prof.push_back(r);
+    return hipSuccess;
+}
+
+static inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
+
+// A strided view: element (i, j) at p[i*rs + j*cs].
+template <typename T>
+struct View {
+    T *p;
+    int64_t rs, cs, nr, nc;
+    View sub(int64_t r0, int64_t nr_, int64_t c0, int64_t nc_) const {
+        return View{p + r0 * rs + c0 * cs, rs, cs, nr_, nc_};
+    }
+};
+
+static int leaf_nb_default() {
+    static int nb = [] {
+        const char *e = std::getenv("NLA_LEAF_NB");
+        int v = e ? std::atoi(e) : LEAF_MAX_NB;
+        if (v != 16 && v != 32 && v != 64) v = LEAF_MAX_NB;
+        return v;
+    }();
+    return nb;
+}
+
+template <typename T>
+struct Recursion {
+    char func, side, uplo;  // uplo of the effective (possibly transposed) A
+    int64_t nb;             // leaf size
+    hipStream_t stream;
+    hipError_t err = hipSuccess;
+
+    void leaf(View<T> A, View<T> B, int64_t n, T scale) {
+        if (err != hipSuccess) return;
+        LeafArgs<T> a;
+        a.trmm = (func == 'M');
+        a.nb = (int)n;
+        a.scale = scale;
+        const int64_t last = n - 1;
+        if (side == 'L') {
+            a.m = B.nc;
+            if (uplo == 'L') {
+                a.A = A.p; a.ars = A.rs; a.acs = A.cs;
+                a.B = B.p; a.brs = B.rs; a.bcs = B.cs;
+            } else {  // reversal J*U*J is lower
+                a.A = A.p + last * (A.rs + A.cs); a.ars = -A.rs; a.acs = -A.cs;
+                a.B = B.p + last * B.rs; a.brs = -B.rs; a.bcs = B.cs;
+            }
+        } else {  // X*op(A) = B solved as op(A)^T X^T = B^T
+            a.m = B.nr;
+            if (uplo == 'U') {  // op(A)^T is lower
+                a.A = A.p; a.ars = A.cs; a.acs = A.rs;
+                a.B = B.p; a.brs = B.cs; a.bcs = B.rs;
+            } else {
+                a.A = A.p + last * (A.rs + A.cs); a.ars = -A.cs; a.acs = -A.rs;
+                a.B = B.p + last * B.cs; a.brs = -B.cs; a.bcs = B.rs;
+            }
+        }
+        const double fl = (double)n * (double)n * (double)a.m;
+        const double by = ((double)n * (n + 1) / 2 + 2.0 * n * a.m) * sizeof(T);
+        err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_leaf<T>(a, stream); });
+    }
+
+    // C = beta*C + alpha*X*Y on views (GEMM_ADD!/GEMM_SUB!, src/matmul.jl:69-80).
+    void gemm(View<T> C, View<T> X, View<T> Y, T alpha, T beta) {
+        if (err != hipSuccess) return;
+        GemmArgs<T> g;
+        g.M = C.nr; g.N = C.nc; g.K = X.nc;
+        g.tx = (X.rs != 1); g.ldx = g.tx ? X.rs : X.cs;
+        g.ty = (Y.rs != 1); g.ldy = g.ty ? Y.rs : Y.cs;
+        g.X = X.p; g.Y = Y.p; g.C = C.p; g.ldc = C.cs;
+        g.alpha = alpha; g.beta = beta;
+        const double fl = 2.0 * g.M * g.N * g.K;
+        const double by = ((double)g.M * g.K + (double)g.K * g.N + 2.0 * g.M * g.N) * sizeof(T);
+        err = timed(NLA_KIND_GEMM, fl, by, stream, [&] { return launch_gemm<T>(g, stream); });
+    }
+
+    int64_t split(int64_t n) const {
+        int64_t half = (n + 1) / 2;
+        int64_t mid = ((half + nb - 1) / nb) * nb;
+        if (mid >= n) mid = n - 1;
+        if (mid < 1) mid = 1;
+        return mid;
+    }
+
+    // pre: TRSM scale owed by B's untouched blocks; post: TRMM scale owed by
+    // the last write of this B.
+    void rec(View<T> A, View<T> B, int64_t n, T pre, T post) {
+        if (err != hipSuccess) return;
+        if (n <= nb) {
+            leaf(A, B, n, func == 'S' ? pre : post);
+            return;
+        }
+        const int64_t mid = split(n), rem = n - mid;
+        View<T> A11 = A.sub(0, mid, 0, mid), A22 = A.sub(mid, rem, mid, rem);
+        View<T> A21 = A.sub(mid, rem, 0, mid), A12 = A.sub(0, mid, mid, rem);
+        View<T> B1, B2;
+        if (side == 'L') {
+            B1 = B.sub(0, mid, 0, B.nc);
+            B2 = B.sub(mid, rem, 0, B.nc);
+        } else {
+            B1 = B.sub(0, B.nr, 0, mid);
+            B2 = B.sub(0, B.nr, mid, rem);
+        }
+        const bool forward = (side == 'L' && uplo == 'L' && func == 'S') || (side == 'R' && uplo == 'U' && func == 'S') ||
+                             (side == 'L' && uplo == 'U' && func == 'M') || (side == 'R' && uplo == 'L' && func == 'M');
+        const T one = T(1);
+        if (func == 'S') {
+            if (forward) {
+                rec(A11, B1, mid, pre, one);
+                if (side == 'L') gemm(B2, A21, B1, -one, pre);  // B2 -= A21*B1
+                else gemm(B2, B1, A12, -one, pre);              // B2 -= B1*A12
+                rec(A22, B2, rem, one, one);
+            } else {
+                rec(A22, B2, rem, pre, one);
+                if (side == 'L') gemm(B1, A12, B2, -one, pre);  // B1 -= A12*B2
+                else gemm(B1, B2, A21, -one, pre);              // B1 -= B2*A21
+                rec(A11, B1, mid, one, one);
+            }
+        } else {
+            if (forward) {
+                rec(A11, B1, mid, one, one);
+                if (side == 'L') gemm(B1, A12, B2, post, post);  // B1 += A12*B2
+                else gemm(B1, B2, A21, post, post);              // B1 += B2*A21
+                rec(A22, B2, rem, one, post);
+            } else {
+                rec(A22, B2, rem, one, one);
+                if (side == 'L') gemm(B2, A21, B1, post, post);  // B2 += A21*B1
+                else gemm(B2, B1, A12, post, post);              // B2 += B1*A12
+                rec(A11, B1, mid, one, post);
+            }
+        }
+    }
+};
+
+template <typename T>
+static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double alpha, int64_t n, int64_t m,
+                             const void *Ap, int64_t lda, void *Bp, int64_t ldb, hipStream_t s) {
+    View<T> A{(T *)Ap, 1, lda, n, n};
+    View<T> B{(T *)Bp, 1, ldb, side == 'L' ? n : m, side == 'L' ? m : n};
+    if (alpha == 0.0) return hip_status(launch_zero<T>(B.nr, B.nc, B.p, ldb, s), "zero fill");
+    if (trans != 'N') {  // src/rectrxm.jl:56-59
+        A.rs = lda;
+        A.cs = 1;
+        uplo = (uplo == 'L') ? 'U' : 'L';
+    }
+    Recursion<T> r;
+    r.func = func;
+    r.side = side;
+    r.uplo = uplo;
+    r.nb = leaf_nb_default();
+    r.stream = s;
+    r.rec(A, B, n, (T)alpha, (T)alpha);
+    return hip_status(r.err, "kernel launch");
+}
+
+}  // namespace nla
+
+using namespace nla;
+
+extern "C" {
+
+int32_t nla_abi_version(void) { return 1; }
+
+const char *nla_last_error(void) { return g_last_error; }
+
+int32_t nla_rectrxm(char side, char uplo, char trans, char func, int32_t dtype, double alpha, int64_t n, int64_t m,
+                    const void *A, int64_t lda, void *B, int64_t ldb, void *stream) {
+    side = up(side); uplo = up(uplo); trans = up(trans); func = up(func);
+    if (side != 'L' && side != 'R') return bad_arg(1, "side");
+    if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
+    if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
+    if (func != 'S' && func != 'M') return bad_arg(4, "func");
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(5, "dtype");
+    if (n < 0) return bad_arg(7, "n");
+    if (m < 0) return bad_arg(8, "m");
+    if (n == 0 || m == 0) return NLA_OK;
+    if (A == nullptr) return bad_arg(9, "A");
+    if (lda < n) return bad_arg(10, "lda");
+    if (B == nullptr) return bad_arg(11, "B");
+    if (ldb < (side == 'L' ? n : m)) return bad_arg(12, "ldb");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == NLA_F64) return rectrxm_typed<double>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s);
+    return rectrxm_typed<float>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s);
+}
+
+int32_t nla_gemm(int32_t dtype, char transx, char transy, int64_t M, int64_t N, int64_t K, double alpha,
+                 const void *X, int64_t ldx, const void *Y, int64_t ldy, double beta, void *C, int64_t ldc,
+                 void *stream) {
+    transx = up(transx); transy = up(transy);
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(1, "dtype");
+    if (transx != 'N' && transx != 'T' && transx != 'C') return bad_arg(2, "transx");
+    if (transy != 'N' && transy != 'T' && transy != 'C') return bad_arg(3, "transy");
+    if (M < 0) return bad_arg(4, "M");
+    if (N < 0) return bad_arg(5, "N");
+    if (K < 0) return bad_arg(6, "K");
+    if (M == 0 || N == 0) return NLA_OK;
+    bool tx = transx != 'N', ty = transy != 'N';
+    if (K > 0 && X == nullptr) return bad_arg(8, "X");
+    if (K > 0 && ldx < (tx ? K : M)) return bad_arg(9, "ldx");
+    if (K > 0 && Y == nullptr) return bad_arg(10, "Y");
+    if (K > 0 && ldy < (ty ? N : K)) return bad_arg(11, "ldy");
+    if (C == nullptr) return bad_arg(13, "C");
+    if (ldc < M) return bad_arg(14, "ldc");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == NLA_F64) {
+        GemmArgs<double> g{M, N, K, tx, ty, alpha, beta, (const double *)X, ldx, (const double *)Y, ldy, (double *)C, ldc};
+        return hip_status(launch_gemm<double>(g, s), "gemm launch");
+    }
+    GemmArgs<float> g{M, N, K, tx, ty, (float)alpha, (float)beta, (const float *)X, ldx, (const float *)Y, ldy, (float *)C, ldc};
+    return hip_status(launch_gemm<float>(g, s), "gemm launch");
+}
+
+int32_t nla_gemm_add(int32_t dtype, int64_t N, int64_t R, int64_t M, const void *X, int64_t ldx, const void *Y,
+                     int64_t ldy, void *C, int64_t ldc, void *stream) {
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(1, "dtype");
+    if (N < 0) return bad_arg(2, "N");
+    if (R < 0) return bad_arg(3, "R");
+    if (M < 0) return bad_arg(4, "M");
+    if (N == 0 || M == 0 || R == 0) return NLA_OK;
+    if (X == nullptr) return bad_arg(5, "X");
+    if (ldx < N) return bad_arg(6, "ldx");
+    if (Y == nullptr) return bad_arg(7, "Y");
+    if (ldy < R) return bad_arg(8, "ldy");
+    if (C == nullptr) return bad_arg(9, "C");
+    if (ldc < N) return bad_arg(10, "ldc");
+    return nla_gemm(dtype, 'N', 'N', N, M, R, 1.0, X, ldx, Y, ldy, 1.0, C, ldc, stream);
+}
+
+int32_t nla_gemm_sub(int32_t dtype, int64_t N, int64_t R, int64_t M, void *C, int64_t ldc, const void *X,
+                     int64_t ldx, const void *Y, int64_t ldy, void *stream) {
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(1, "dtype");
+    if (N < 0) return bad_arg(2, "N");
+    if (R < 0) return bad_arg(3, "R");
+    if (M < 0) return bad_arg(4, "M");
+    if (N == 0 || M == 0 || R == 0) return NLA_OK;
+    if (C == nullptr) return bad_arg(5, "C");
+    if (ldc < N) return bad_arg(6, "ldc");
+    if (X == nullptr) return bad_arg(7, "X");
+    if (ldx < N) return bad_arg(8, "ldx");
+    if (Y == nullptr) return bad_arg(9, "Y");
+    if (ldy < R) return bad_arg(10, "ldy");
+    return nla_gemm(dtype, 'N', 'N', N, M, R, -1.0, X, ldx, Y, ldy, 1.0, C, ldc, stream);
+}
+
+int32_t nla_prof_enable(int32_t on) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto &r : g_prof) {
+        (void)hipEventSynchronize(r.e1);
+        g_event_pool.push_back(r.e0);
+        g_event_pool.push_back(r.e1);
+    }
+    g_prof.clear();
+    g_prof_on = (on != 0);
+    return NLA_OK;
+}
+
+int32_t nla_prof_summary(int32_t kind, int64_t *launches, double *ms, double *flops, double *bytes) {
+    if (kind != NLA_KIND_GEMM && kind != NLA_KIND_LEAF) return bad_arg(1, "kind");
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    int64_t n = 0;
+    double t = 0, f = 0, b = 0;
+    for (auto &r : g_prof) {
+        if (r.kind != kind) continue;
+        hipError_t e = hipEventSynchronize(r.e1);
+        if (e != hipSuccess) return hip_status(e, "event sync");
+        float dt = 0;
+        e = hipEventElapsedTime(&dt, r.e0, r.e1);
+        if (e != hipSuccess) return hip_status(e, "event time");
+        ++n;
+        t += dt;
+        f += r.flops;
+        b += r.bytes;
+    }
+    if (launches) *launches = n;
+    if (ms) *ms = t;
+    if (flops) *flops = f;
+    if (bytes) *bytes = b;
+    return NLA_OK;
+}
+
+int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int64_t m, const void *A, int64_t lda,
+                 void *B, int64_t ldb, void *stream) {
+    func = up(func); side = up(side); uplo = up(uplo);
+    if (func != 'S' && func != 'M') return bad_arg(1, "func");
+    if (side != 'L' && side != 'R') return bad_arg(2, "side");
+    if (uplo != 'L' && uplo != 'U') return bad_arg(3, "uplo");
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(4, "dtype");
+    if (n < 0) return bad_arg(5, "n");
+    if (m < 0) return bad_arg(6, "m");
+    if (n == 0 || m == 0) return NLA_OK;
+    if (A == nullptr) return bad_arg(7, "A");
+    if (lda < n) return bad_arg(8, "lda");
+    if (B == nullptr) return bad_arg(9, "B");
+    if (ldb < (side == 'L' ? n : m)) return bad_arg(10, "ldb");
+    return nla_rectrxm(side, uplo, 'N', func, dtype, 1.0, n, m, A, lda, B, ldb, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+// GEMM update of the recursive triangular solve / multiply on gfx950 MFMA.
+//
+//   C = beta*C + alpha*op(X)*op(Y)      op(X): M x K, op(Y): K x N, C: M x N
+//
+// Replaces matmul! (src/matmul.jl:5-66) as used by GEMM_ADD!/GEMM_SUB!
+// (src/matmul.jl:69-80) from unified_rec (src/rectrxm.jl:164-192).  The
+// reference kernel computes one output per work-item from 32x32 LDS tiles;
+// here every wave owns a (BM/WM) x (BN/WN) block of C held in MFMA
+// accumulators, operands are staged through double-buffered LDS, and the
+// alpha pre/post scaling of unified_rectrxm! (src/rectrxm.jl:64,72) is folded
+// into beta/alpha so B is never swept separately.
+//
+// Orientation: the MFMA computes D = C^T for each 16x16 (f64) / 32x32 (f32)
+// block, so that the lane index of D's column walks down a column of C and the
+// epilogue writes whole 128-byte segments of the column-major output.
+//
+// LDS layout per operand follows its contiguous dimension in HBM:
+//   outer-contiguous (X 'N', Y 'T'):   tile[k][outer]   (row stride BO + PO)
+//   k-contiguous     (X 'T', Y 'N'):   tile[outer][k]   (row stride BK + PK)
+// so HBM reads are 16-byte vectors along the contiguous dimension and LDS
+// writes are vectors too; the transpose happens in the per-lane MFMA operand
+// read instead.
+#include "nla_device.h"
+#include "nla_internal.h"
+
+namespace nla {
+
+template <typename T, int BO, int BK, int NT, bool KCONTIG, int VW>
+struct TileLoader {
+    // Each thread moves CH chunks of VW consecutive elements along the
+    // contiguous dimension of the operand.
+    static constexpr int CHUNKS = BO * BK / VW;
+    static constexpr int CH = CHUNKS / NT;
+    static_assert(CHUNKS % NT == 0, "tile does not divide among threads");
+    static constexpr int PO = (sizeof(T) == 8) ? 2 : 4;  // pads keep 16-byte alignment
+    static constexpr int PK = (sizeof(T) == 8) ? 2 : 4;
+    static constexpr int LDS_ELEMS = KCONTIG ? BO * (BK + PK) : BK * (BO + PO);
+
+    T r[CH][VW];
+
+    // Coordinates (outer, k) inside the tile of chunk c of thread t.
+    __device__ static inline void coords(int t, int c, int &o, int &k) {
+        int q = t + c * NT;
+        if (KCONTIG) {
+            k = (q % (BK / VW)) * VW;
+            o = q / (BK / VW);
+        } else {
+            o = (q % (BO / VW)) * VW;
+            k = q / (BO / VW);
+        }
+    }
+
+    // Element (outer, k) of the operand lives at P[outer*so + k*sk] with one
+    // of so/sk equal to 1.  o0/k0: tile origin; no/nk: extents for bounds.
+    __device__ inline void load(const T *__restrict__ P, int64_t so, int64_t sk, int64_t o0, int64_t k0,
+                                int64_t no, int64_t nk, int t) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            int o, k;
+            coords(t, c, o, k);
+            int64_t go = o0 + o, gk = k0 + k;
+            const T *p = P + go * so + gk * sk;
+            if (VW > 1) {
+                bool full = KCONTIG ? (go < no && gk + VW <= nk) : (gk < nk && go + VW <= no);
+                if (full) {
+                    typedef T vec_t __attribute__((ext_vector_type(VW)));
+                    vec_t v = *reinterpret_cast<const vec_t *>(p);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) r[c][e] = v[e];
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                int64_t eo = KCONTIG ? go : go + e;
+                int64_t ek = KCONTIG ? gk + e : gk;
+                r[c][e] = (eo < no && ek < nk) ? (KCONTIG ? p[e] : p[e * so]) : T(0);
+            }
+        }
+    }
+
+    __device__ inline void store(T *lds, int t) const {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            int o, k;
+            coords(t, c, o, k);
+            T *q = KCONTIG ? lds + o * (BK + PK) + k : lds + k * (BO + PO) + o;
+            if (VW > 1) {
+                typedef T vec_t __attribute__((ext_vector_type(VW)));
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) v[e] = r[c][e];
+                *reinterpret_cast<vec_t *>(q) = v;
+            } else {
+                q[0] = r[c][0];
+            }
+        }
+    }
+
+    // Operand element (outer, k) from the staged tile.
+    __device__ static inline T at(const T *lds, int o, int k) {
+        return KCONTIG ? lds[o * (BK + PK) + k] : lds[k * (BO + PO) + o];
+    }
+};
+
+// Hardware dispatches work-group g to XCD g % 8.  Renumber so each XCD owns
+// a contiguous range of logical tiles, then walk tiles in groups of GM row
+// tiles so the tiles resident on one XCD share X row panels and Y column
+// panels in that XCD's L2.
+__device__ inline void tile_coords(int G, int g, int tiles_m, int &tm, int &tn) {
+    const int NX = 8, GM = 8;
+    int xcd = g % NX, idx = g / NX;
+    int per = G / NX, rem = G % NX;
+    int gp = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    int tiles_n = G / tiles_m;
+    int group = GM * tiles_n;
+    int gid = gp / group;
+    int first = gid * GM;
+    int gm = tiles_m - first < GM ? tiles_m - first : GM;
+    int in = gp % group;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW>
+__global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N, int64_t K, T alpha,
+                                                           const T *__restrict__ X, int64_t ldx,
+                                                           const T *__restrict__ Y, int64_t ldy, T beta,
+                                                           T *__restrict__ C, int64_t ldc, int tiles_m) {
+    typedef Mfma<T> MF;
+    constexpr int NT = WM * WN * 64;
+    constexpr int TWM = BM / WM, TWN = BN / WN;
+    constexpr int RB = TWM / MF::BD, CB = TWN / MF::BD;
+    static_assert(RB >= 1 && CB >= 1 && TWM % MF::BD == 0 && TWN % MF::BD == 0, "bad wave tile");
+    static_assert(BK % MF::KD == 0, "bad BK");
+    // X tile: outer = rows of C, contiguous along k when TX.
+    typedef TileLoader<T, BM, BK, NT, TX, VW> LX;
+    // Y tile: outer = columns of C, contiguous along k when !TY.
+    typedef TileLoader<T, BN, BK, NT, !TY, VW> LY;
+
+    __shared__ __attribute__((aligned(16))) T lds_x[2][LX::LDS_ELEMS];
+    __shared__ __attribute__((aligned(16))) T lds_y[2][LY::LDS_ELEMS];
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    int tm, tn;
+    tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+
+    // op(X)(r, k): TX ? X[k + r*ldx] : X[r + k*ldx]
+    const int64_t xso = TX ? ldx : 1, xsk = TX ? 1 : ldx;
+    // op(Y)(k, c): TY ? Y[c + k*ldy] : Y[k + c*ldy]
+    const int64_t yso = TY ? 1 : ldy, ysk = TY ? ldy : 1;
+
+    typename MF::acc_t acc[CB][RB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = MF::zero();
+
+    LX lx;
+    LY ly;
+    const int nkt = (int)((K + BK - 1) / BK);
+    if (nkt > 0) {
+        lx.load(X, xso, xsk, row0, 0, M, K, t);
+        ly.load(Y, yso, ysk, col0, 0, N, K, t);
+        lx.store(lds_x[0], t);
+        ly.store(lds_y[0], t);
+    }
+    __syncthreads();
+
+    const int lo = lane % MF::BD, lk = lane / MF::BD;
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nkt;
+        if (more) {
+            lx.load(X, xso, xsk, row0, (int64_t)(kt + 1) * BK, M, K, t);
+            ly.load(Y, yso, ysk, col0, (int64_t)(kt + 1) * BK, N, K, t);
+        }
+        const T *sx = lds_x[cur];
+        const T *sy = lds_y[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += MF::KD) {
+            T a[CB], b[RB];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) a[cb] = LY::at(sy, wn * TWN + cb * MF::BD + lo, ks + lk);
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) b[rb] = LX::at(sx, wm * TWM + rb * MF::BD + lo, ks + lk);
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = MF::mma(a[cb], b[rb], acc[cb][rb]);
+        }
+        if (more) {
+            lx.store(lds_x[cur ^ 1], t);
+            ly.store(lds_y[cur ^ 1], t);
+        }
+        __syncthreads();
+    }
+
+    // Epilogue: D[i][j] of block (cb, rb) is C(row j, column i).
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int r = 0; r < MF::NR; ++r) {
+                int64_t row = row0 + wm * TWM + rb * MF::BD + MF::dj(lane, r);
+                int64_t col = col0 + wn * TWN + cb * MF::BD + MF::di(lane, r);
+                if (row < M && col < N) {
+                    T *p = C + row + col * ldc;
+                    T v = alpha * acc[cb][rb][r];
+                    *p = (beta == T(0)) ? v : v + beta * *p;
+                }
+            }
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW>
+static hipError_t launch_cfg(const GemmArgs<T> &a, hipStream_t s) {
+    int tiles_m = (int)((a.M + BM - 1) / BM);
+    int tiles_n = (int)((a.N + BN - 1) / BN);
+    int G = tiles_m * tiles_n;
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TX, TY, VW>), dim3(G), dim3(WM * WN * 64), 0, s, a.M,
+                       a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C, a.ldc, tiles_m);
+    return hipGetLastError();
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, int VW>
+static hipError_t launch_trans(const GemmArgs<T> &a, hipStream_t s) {
+    if (!a.tx && !a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW>(a, s);
+    if (a.tx && !a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, true, false, VW>(a, s);
+    if (!a.tx && a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, false, true, VW>(a, s);
+    return launch_cfg<T, BM, BN, BK, WM, WN, true, true, VW>(a, s);
+}
+
+static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+template <typename T>
+hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s) {
+    if (a.M <= 0 || a.N <= 0) return hipSuccess;
+    constexpr int VW = 16 / sizeof(T);
+    // 16-byte vector loads need every column start of X, Y (and the tile
+    // origins, which are multiples of the tile sizes) to be 16-byte aligned.
+    bool vec = aligned16(a.X) && aligned16(a.Y) && (a.ldx % VW == 0) && (a.ldy % VW == 0);
+    int64_t big_tiles = ((a.M + 127) / 128) * ((a.N + 127) / 128);
+    bool small = big_tiles < 512;
+    if (sizeof(T) == 8) {
+        if (small) {
+            if (vec) return launch_trans<T, 64, 64, 16, 2, 2, VW>(a, s);
+            return launch_trans<T, 64, 64, 16, 2, 2, 1>(a, s);
+        }
+        if (vec) return launch_trans<T, 128, 128, 16, 2, 2, VW>(a, s);
+        return launch_trans<T, 128, 128, 16, 2, 2, 1>(a, s);
+    } else {
+        if (small) {
+            if (vec) return launch_trans<T, 64, 64, 32, 2, 2, VW>(a, s);
+            return launch_trans<T, 64, 64, 32, 2, 2, 1>(a, s);
+        }
+        if (vec) return launch_trans<T, 128, 128, 32, 2, 2, VW>(a, s);
+        return launch_trans<T, 128, 128, 32, 2, 2, 1>(a, s);
+    }
+}
+
+template hipError_t launch_gemm<float>(const GemmArgs<float> &, hipStream_t);
+template hipError_t launch_gemm<double>(const GemmArgs<double> &, hipStream_t);
+
+}  // namespace nla

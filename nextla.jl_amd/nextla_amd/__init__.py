@@ -1,0 +1,188 @@
+"""Host-side mirror of NextLA.jl's triangular solve / multiply interface on
+MI355X (gfx950), backed by libnlahip.
+
+Names follow the reference with Julia's `!` spelled `_`:
+
+    unified_rectrxm_(side, uplo, transpose, alpha, func, A, B)  src/rectrxm.jl:43
+    GEMM_ADD_(A, B, C)   C += A*B                               src/matmul.jl:69
+    GEMM_SUB_(A, B, C)   A -= B*C                               src/matmul.jl:76
+    LeftLowerTRSM_(A, B) ... RightUpperTRSM_(A, B)              src/trsm.jl:128-150
+    LeftLowerTRMM_(A, B) ... RightUpperTRMM_(A, B)              src/trmm.jl:332-393
+
+A and B are 2-D torch tensors on a ROCm device, float32 or float64.  Julia
+matrices are column-major; a tensor with stride (1, ld) is used as is, and a
+row-major tensor (stride (ld, 1)) is handled through the transpose identities
+(X*op(A) = B  <=>  op(A)^T X^T = B^T), so no copy is made either way.  B is
+updated in place and returned.  Work is queued on torch's current stream of
+B's device; call torch.cuda.synchronize() (the reference's
+KernelAbstractions.synchronize) before reading results on the host.
+
+Deliberate differences from the reference: arguments are validated before any
+device work (invalid characters or shapes raise ValueError and leave B
+untouched), lower-case characters are accepted, and alpha == 0 sets B to zero
+without reading it (BLAS semantics).
+"""
+import torch
+
+from . import _lib
+from ._lib import NLA_F32, NLA_F64, NLAError, load
+
+__all__ = [
+    "unified_rectrxm_", "GEMM_ADD_", "GEMM_SUB_",
+    "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
+    "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_",
+    "trsm", "trmm", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
+]
+
+_DT = {torch.float32: NLA_F32, torch.float64: NLA_F64}
+
+
+def colmajor(t):
+    """A column-major (Julia-layout) copy of a 2-D tensor."""
+    return t.t().contiguous().t()
+
+
+def _layout(t, name):
+    """(transposed, ld): transposed=True when memory holds t^T column-major."""
+    if t.dim() != 2:
+        raise ValueError(f"{name} must be a matrix, got {t.dim()} dims")
+    r, c = t.shape
+    s0, s1 = t.stride()
+    if s0 == 1 and (c <= 1 or s1 >= max(1, r)):
+        return False, max(s1 if c > 1 else r, 1)
+    if s1 == 1 and (r <= 1 or s0 >= max(1, c)):
+        return True, max(s0 if r > 1 else c, 1)
+    raise ValueError(f"{name} must have unit stride in one dimension, got strides {t.stride()}")
+
+
+def _stream(t):
+    return ctypes_stream(torch.cuda.current_stream(t.device))
+
+
+def ctypes_stream(s):
+    return s.cuda_stream
+
+
+def _check_pair(A, B):
+    if A.dtype not in _DT:
+        raise ValueError(f"unsupported element type {A.dtype} (Float32/Float64 only)")
+    if B.dtype != A.dtype:
+        raise ValueError(f"A is {A.dtype} but B is {B.dtype}")
+    if not (A.is_cuda and B.is_cuda) or A.device != B.device:
+        raise ValueError("A and B must live on the same ROCm device")
+
+
+def _flip(c, a, b):
+    return b if c == a else a
+
+
+def unified_rectrxm_(side, uplo, transpose, alpha, func, A, B):
+    """unified_rectrxm!(side, uplo, transpose, alpha, func, A, B) of
+    src/rectrxm.jl:43-76.
+
+    func 'S': solve op(A)*X = alpha*B ('L') or X*op(A) = alpha*B ('R').
+    func 'M': B <- alpha*op(A)*B ('L') or alpha*B*op(A) ('R').
+    Returns B (overwritten)."""
+    for k, (v, ok) in enumerate(((side, "LRlr"), (uplo, "LUlu"), (transpose, "NTCntc"), (func, "SMsm")), 1):
+        if not isinstance(v, str) or len(v) != 1 or v not in ok:
+            raise ValueError(f"argument {k} has invalid value {v!r}")
+    side, uplo, transpose, func = side.upper(), uplo.upper(), transpose.upper(), func.upper()
+    _check_pair(A, B)
+    n = A.shape[0]
+    if A.dim() != 2 or A.shape[1] != n:
+        raise ValueError(f"A must be square, got {tuple(A.shape)}")
+    if B.dim() != 2:
+        raise ValueError("B must be a matrix")
+    if (side == "L" and B.shape[0] != n) or (side == "R" and B.shape[1] != n):
+        raise ValueError(f"B {tuple(B.shape)} does not match A {tuple(A.shape)} for side {side!r}")
+    a_t, lda = _layout(A, "A")
+    b_t, ldb = _layout(B, "B")
+    tr = transpose != "N"
+    if a_t:  # memory holds A^T: same op with the transpose toggled, uplo flipped
+        uplo = _flip(uplo, "L", "U")
+        tr = not tr
+    if b_t:  # memory holds B^T: solve/multiply on the transposed equation
+        side = _flip(side, "L", "R")
+        tr = not tr
+    # the stored column-major matrix is n x m for side 'L', m x n for side 'R'
+    rc, cc = (B.shape[1], B.shape[0]) if b_t else (B.shape[0], B.shape[1])
+    m = cc if side == "L" else rc
+    st =_lib.load().nla_rectrxm(_lib.ch(side), _lib.ch(uplo), _lib.ch("T" if tr else "N"), _lib.ch(func),
+                                 _DT[A.dtype], float(alpha), n, m, A.data_ptr(), lda, B.data_ptr(), ldb,
+                                 _stream(B))
+    _lib.check(st, "unified_rectrxm!")
+    return B
+
+
+def _gemm(alpha, X, Y, beta, C, what):
+    """C = beta*C + alpha*X*Y on semantic matrices of any supported layout."""
+    for t, nm in ((X, "X"), (Y, "Y"), (C, "C")):
+        if t.dtype not in _DT or t.dtype != C.dtype:
+            raise ValueError(f"{what}: {nm} has element type {t.dtype}, expected {C.dtype} (Float32/Float64)")
+        if not t.is_cuda or t.device != C.device:
+            raise ValueError(f"{what}: all operands must live on one ROCm device")
+    N, R = X.shape
+    R2, M = Y.shape
+    if R2 != R or tuple(C.shape) != (N, M):
+        raise ValueError(f"{what}: shapes {tuple(C.shape)} != {tuple(X.shape)} * {tuple(Y.shape)}")
+    xt, ldx = _layout(X, "X")
+    yt, ldy = _layout(Y, "Y")
+    ct, ldc = _layout(C, "C")
+    if not ct:  # C = b*C + a*op(X)*op(Y)
+        args = ("T" if xt else "N", "T" if yt else "N", N, M, R, X, ldx, Y, ldy)
+    else:  # C^T = b*C^T + a*Y^T*X^T
+        args = ("N" if yt else "T", "N" if xt else "T", M, N, R, Y, ldy, X, ldx)
+    tx, ty, mm, nn, kk, P, ldp, Q, ldq = args
+    st = _lib.load().nla_gemm(_DT[C.dtype], _lib.ch(tx), _lib.ch(ty), mm, nn, kk, float(alpha), P.data_ptr(), ldp,
+                              Q.data_ptr(), ldq, float(beta), C.data_ptr(), ldc, _stream(C))
+    _lib.check(st, what)
+    return C
+
+
+def GEMM_ADD_(A, B, C):
+    """GEMM_ADD!(A, B, C) of src/matmul.jl:69-73: C += A*B; returns C."""
+    return _gemm(1.0, A, B, 1.0, C, "GEMM_ADD!")
+
+
+def GEMM_SUB_(A, B, C):
+    """GEMM_SUB!(A, B, C) of src/matmul.jl:76-80: A -= B*C; returns A."""
+    return _gemm(-1.0, B, C, 1.0, A, "GEMM_SUB!")
+
+
+def _leaf(func, side, uplo):
+    def f(A, B):
+        return unified_rectrxm_(side, uplo, "N", 1.0, func, A, B)
+    return f
+
+
+LeftLowerTRSM_ = _leaf("S", "L", "L")
+LeftUpperTRSM_ = _leaf("S", "L", "U")
+RightLowerTRSM_ = _leaf("S", "R", "L")
+RightUpperTRSM_ = _leaf("S", "R", "U")
+LeftLowerTRMM_ = _leaf("M", "L", "L")
+LeftUpperTRMM_ = _leaf("M", "L", "U")
+RightLowerTRMM_ = _leaf("M", "R", "L")
+RightUpperTRMM_ = _leaf("M", "R", "U")
+
+for _f, _doc in ((LeftLowerTRSM_, "A\\B, A lower (src/trsm.jl:128)"), (LeftUpperTRSM_, "A\\B, A upper (src/trsm.jl:134)"),
+                 (RightLowerTRSM_, "B/A, A lower (src/trsm.jl:140)"), (RightUpperTRSM_, "B/A, A upper (src/trsm.jl:146)"),
+                 (LeftLowerTRMM_, "A*B, A lower (src/trmm.jl:332)"), (LeftUpperTRMM_, "A*B, A upper (src/trmm.jl:349)"),
+                 (RightLowerTRMM_, "B*A, A lower (src/trmm.jl:366)"), (RightUpperTRMM_, "B*A, A upper (src/trmm.jl:383)")):
+    _f.__doc__ = f"In-place {_doc}; any n (the reference leaf is limited to one block)."
+
+
+def trsm(side, uplo, transa, diag, A, B, alpha=1.0):
+    """trsm(side, uplo, transa, diag, A, B, alpha) of src/trsm.jl:186-204.
+    Unlike the reference wrapper, transa is honoured; only diag 'N' is
+    supported (the reference ignores diag)."""
+    if diag not in ("N", "n"):
+        raise ValueError("diag 'U' is not supported; only non-unit triangles")
+    return unified_rectrxm_(side, uplo, transa, alpha, "S", A, B)
+
+
+def trmm(side, uplo, transa, diag, A, B, alpha=1.0):
+    """trmm(side, uplo, transa, diag, A, B, alpha) of src/trmm.jl:430-447
+    (transa honoured, diag 'N' only)."""
+    if diag not in ("N", "n"):
+        raise ValueError("diag 'U' is not supported; only non-unit triangles")
+    return unified_rectrxm_(side, uplo, transa, alpha, "M", A, B)

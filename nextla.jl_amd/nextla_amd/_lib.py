@@ -1,0 +1,94 @@
+"""ctypes binding of libnlahip (declared in include/nla.h).
+
+The library is the only compute path: if it is missing, every entry point
+raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libnlahip.so"))
+
+NLA_F32 = 0
+NLA_F64 = 1
+
+_c_char = ctypes.c_char
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); must list every function of include/nla.h
+SIGNATURES = {
+    "nla_abi_version": (_i32, []),
+    "nla_last_error": (ctypes.c_char_p, []),
+    "nla_rectrxm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_gemm_add": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_gemm_sub": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_gemm": (_i32, [_i32, _c_char, _c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
+    "nla_leaf": (_i32, [_c_char, _c_char, _c_char, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_prof_enable": (_i32, [_i32]),
+    "nla_prof_summary": (_i32, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
+                                ctypes.POINTER(_dbl)]),
+}
+
+NLA_KIND_GEMM = 0
+NLA_KIND_LEAF = 1
+
+
+def prof_enable(on=True):
+    check(load().nla_prof_enable(1 if on else 0), "nla_prof_enable")
+
+
+def prof_summary(kind):
+    """(launches, ms, flops, bytes) of the kernels recorded since prof_enable."""
+    n, ms, fl, by = _i64(), _dbl(), _dbl(), _dbl()
+    check(load().nla_prof_summary(kind, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)),
+          "nla_prof_summary")
+    return n.value, ms.value, fl.value, by.value
+
+_lock = threading.Lock()
+_lib = None
+
+
+class NLAError(RuntimeError):
+    """A HIP/runtime failure reported by libnlahip (positive status)."""
+
+
+def load(path=None):
+    """Load libnlahip once and attach the C signatures."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise RuntimeError(
+                f"libnlahip not built: {p} is missing (run `make -C nextla.jl_amd` or __graft_entry__.build())")
+        lib = ctypes.CDLL(p)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def check(status, what):
+    """Turn a C status into the reference's error behaviour (ArgumentError /
+    ErrorException in Julia; ValueError / NLAError here)."""
+    if status == 0:
+        return
+    msg = load().nla_last_error().decode(errors="replace")
+    if status < 0:
+        raise ValueError(f"{what}: {msg}")
+    raise NLAError(f"{what}: status {status}: {msg}")
+
+
+def ch(c):
+    """One-character argument as the C ABI's `char`."""
+    if not isinstance(c, str) or len(c) != 1:
+        raise ValueError(f"expected a single character, got {c!r}")
+    return c.encode("ascii")

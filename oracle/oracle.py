@@ -1,0 +1,174 @@
+"""CPU oracle for NextLA.jl's recursive triangular solve / multiply.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the bench's
+CPU-baseline leg, and never by the product path (nextla.jl_amd/).
+
+Two oracles, as in the reference's own tests (test/unified_rectrxm.jl:36-40):
+  * `rectrxm` etc. — the C restatement of unified_rectrxm! (liboracle.so,
+    rectrxm_oracle.c), following src/rectrxm.jl:43-198, src/matmul.jl:5-80,
+    src/trsm.jl:5-150 and src/trmm.jl:43-393 operation by operation;
+  * `blas` — LinearAlgebra.BLAS.trsm!/trmm! of the reference tests, here the
+    same OpenBLAS 0.3.29 routines (dtrsm/strsm/dtrmm/strmm) through scipy.
+
+Error measures: `backward_error` (TRSM, needs no reference solution) and
+`product_error` (TRMM), both normwise in Frobenius norm and computed in FP64.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        c, i64, dbl, vp, i32 = ctypes.c_char, ctypes.c_long, ctypes.c_double, ctypes.c_void_p, ctypes.c_int
+        for sfx in ("d", "s"):
+            f = getattr(lib, f"oracle_rectrxm_{sfx}")
+            f.argtypes = [c, c, c, c, dbl, i64, i64, vp, i64, vp, i64, i32]
+            f.restype = i32
+            f = getattr(lib, f"oracle_rectrxm_thr_{sfx}")
+            f.argtypes = [c, c, c, c, dbl, i64, i64, vp, i64, vp, i64, i64, i32]
+            f.restype = i32
+            f = getattr(lib, f"oracle_gemm_add_{sfx}")
+            f.argtypes = [i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]
+            f.restype = None
+            f = getattr(lib, f"oracle_gemm_sub_{sfx}")
+            f.argtypes = [i64, i64, i64, vp, i64, vp, i64, vp, i64, i32]
+            f.restype = None
+            f = getattr(lib, f"oracle_leaf_{sfx}")
+            f.argtypes = [c, c, c, i64, i64, vp, i64, vp, i64, i32]
+            f.restype = None
+        _lib = lib
+    return _lib
+
+
+def _sfx(a):
+    if a.dtype == np.float64:
+        return "d"
+    if a.dtype == np.float32:
+        return "s"
+    raise TypeError(f"oracle supports float32/float64, got {a.dtype}")
+
+
+def _f(a):
+    """Fortran-ordered copy (column-major, like Julia)."""
+    return np.array(a, order="F", copy=True)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def rectrxm(side, uplo, trans, alpha, func, A, B, nthreads=1, threshold=None):
+    """unified_rectrxm!(side, uplo, trans, alpha, func, A, B) restated on the
+    CPU (src/rectrxm.jl:43-76); returns the new B (inputs are not modified).
+    `threshold` overrides the reference's leaf thresholds (256 for 'S', 16 for
+    'M', src/rectrxm.jl:52,63) to exercise the recursion on small inputs."""
+    A = _f(A)
+    X = _f(B)
+    assert A.dtype == X.dtype
+    n = A.shape[0]
+    m = X.shape[1] if side == "L" else X.shape[0]
+    lib = load()
+    if threshold is None:
+        fn = getattr(lib, f"oracle_rectrxm_{_sfx(A)}")
+        fn(side.encode(), uplo.encode(), trans.encode(), func.encode(), float(alpha), n, m, _ptr(A), max(1, n),
+           _ptr(X), max(1, X.shape[0]), int(nthreads))
+    else:
+        fn = getattr(lib, f"oracle_rectrxm_thr_{_sfx(A)}")
+        fn(side.encode(), uplo.encode(), trans.encode(), func.encode(), float(alpha), n, m, _ptr(A), max(1, n),
+           _ptr(X), max(1, X.shape[0]), int(threshold), int(nthreads))
+    return X
+
+
+def gemm_add(X, Y, C, nthreads=1):
+    """GEMM_ADD!(X, Y, C): returns C + X*Y (src/matmul.jl:69-73)."""
+    X, Y, C = _f(X), _f(Y), _f(C)
+    N, R = X.shape
+    M = Y.shape[1]
+    getattr(load(), f"oracle_gemm_add_{_sfx(C)}")(N, R, M, _ptr(X), max(1, N), _ptr(Y), max(1, R), _ptr(C),
+                                                  max(1, N), int(nthreads))
+    return C
+
+
+def gemm_sub(C, X, Y, nthreads=1):
+    """GEMM_SUB!(C, X, Y): returns C - X*Y (src/matmul.jl:76-80)."""
+    X, Y, C = _f(X), _f(Y), _f(C)
+    N, R = X.shape
+    M = Y.shape[1]
+    getattr(load(), f"oracle_gemm_sub_{_sfx(C)}")(N, R, M, _ptr(C), max(1, N), _ptr(X), max(1, N), _ptr(Y),
+                                                  max(1, R), int(nthreads))
+    return C
+
+
+def leaf(func, side, uplo, A, B, nthreads=1):
+    """One reference leaf kernel (LeftLowerTRSM! ... RightUpperTRMM!) applied
+    to the whole of A (src/trsm.jl:128-150, src/trmm.jl:332-393)."""
+    A, X = _f(A), _f(B)
+    n = A.shape[0]
+    m = X.shape[1] if side == "L" else X.shape[0]
+    getattr(load(), f"oracle_leaf_{_sfx(A)}")(func.encode(), side.encode(), uplo.encode(), n, m, _ptr(A),
+                                              max(1, n), _ptr(X), max(1, X.shape[0]), int(nthreads))
+    return X
+
+
+def blas(side, uplo, trans, alpha, func, A, B):
+    """LinearAlgebra.BLAS.trsm!/trmm!(side, uplo, trans, 'N', alpha, A, B) as
+    called by test/unified_rectrxm.jl:36-40, through scipy's OpenBLAS."""
+    from scipy.linalg import blas as sb
+    A = _f(A)
+    B = _f(B)
+    pre = "d" if A.dtype == np.float64 else "s"
+    fn = getattr(sb, pre + ("trsm" if func == "S" else "trmm"))
+    t = {"N": 0, "T": 1, "C": 2}[trans]
+    return fn(alpha, A, B, side=0 if side == "L" else 1, lower=1 if uplo == "L" else 0, trans_a=t, diag=0)
+
+
+def op_matrix(uplo, trans, A):
+    """op(A) restricted to its `uplo` triangle, in FP64."""
+    A = np.asarray(A, dtype=np.float64)
+    T = np.tril(A) if uplo == "L" else np.triu(A)
+    return T.T if trans in ("T", "C") else T
+
+
+def backward_error(side, uplo, trans, alpha, A, B, X):
+    """Normwise backward error of a TRSM solution in FP64:
+    ||alpha*B - op(A)X|| / (||op(A)|| ||X|| + |alpha| ||B||)  (Frobenius)."""
+    T = op_matrix(uplo, trans, A)
+    B = np.asarray(B, dtype=np.float64)
+    X = np.asarray(X, dtype=np.float64)
+    R = alpha * B - (T @ X if side == "L" else X @ T)
+    den = np.linalg.norm(T) * np.linalg.norm(X) + abs(alpha) * np.linalg.norm(B)
+    return float(np.linalg.norm(R) / den) if den > 0 else float(np.linalg.norm(R))
+
+
+def product_error(side, uplo, trans, alpha, A, B, Y):
+    """Normwise error of a TRMM result against the FP64 product:
+    ||Y - alpha*op(A)B|| / (|alpha| ||op(A)|| ||B||)  (Frobenius)."""
+    T = op_matrix(uplo, trans, A)
+    B = np.asarray(B, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    Z = alpha * (T @ B if side == "L" else B @ T)
+    den = abs(alpha) * np.linalg.norm(T) * np.linalg.norm(B)
+    return float(np.linalg.norm(Y - Z) / den) if den > 0 else float(np.linalg.norm(Y - Z))
+
+
+def rel_error(X, Y):
+    """norm(X - Y)/norm(Y) of test/unified_rectrxm.jl:42."""
+    X = np.asarray(X, dtype=np.float64)
+    Y = np.asarray(Y, dtype=np.float64)
+    d = np.linalg.norm(Y)
+    return float(np.linalg.norm(X - Y) / d) if d > 0 else float(np.linalg.norm(X - Y))

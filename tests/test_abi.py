@@ -1,0 +1,94 @@
+"""The C-ABI library loads, exports every symbol include/nla.h declares, and
+validates arguments before any device work (no GPU needed: these calls return
+before the first HIP call)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "nla.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(nla_\w+)\s*\(", txt)))
+
+
+def test_header_lists_entry_points():
+    fns = header_functions()
+    for f in ("nla_rectrxm", "nla_gemm_add", "nla_gemm_sub", "nla_gemm", "nla_leaf"):
+        assert f in fns
+
+
+def test_library_exports_every_declared_symbol(nla):
+    from nextla_amd import _lib
+    lib = _lib.load()
+    for name in header_functions():
+        assert hasattr(lib, name), name
+    assert set(header_functions()) == set(_lib.SIGNATURES)
+    assert lib.nla_abi_version() >= 1
+
+
+def _rect(lib, *args):
+    side, uplo, trans, func, dtype, alpha, n, m, A, lda, B, ldb = args
+    return lib.nla_rectrxm(side, uplo, trans, func, dtype, alpha, n, m, A, lda, B, ldb, None)
+
+
+GOOD = [b"L", b"L", b"N", b"S", 1, 1.0, 8, 4, 16, 8, 32, 8]
+
+
+@pytest.mark.parametrize("pos,val", [(0, b"X"), (1, b"Q"), (2, b"Z"), (3, b"K"), (4, 7), (6, -1), (7, -2),
+                                     (8, None), (9, 3), (10, None), (11, 2)])
+def test_rectrxm_rejects_invalid_arguments(nla, pos, val):
+    from nextla_amd import _lib
+    lib = _lib.load()
+    args = list(GOOD)
+    args[pos] = val
+    st = _rect(lib, *args)
+    assert st == -(pos + 1), (pos, st)
+    assert b"invalid" in lib.nla_last_error()
+
+
+def test_rectrxm_empty_is_noop(nla):
+    from nextla_amd import _lib
+    lib = _lib.load()
+    for n, m in ((0, 4), (8, 0), (0, 0)):
+        assert _rect(lib, b"R", b"U", b"T", b"M", 0, 2.0, n, m, None, 1, None, 1) == 0
+
+
+def test_right_side_ldb_rule(nla):
+    from nextla_amd import _lib
+    lib = _lib.load()
+    # side 'R': B is m x n, so ldb must be >= m (here m=10 > n=8)
+    st = _rect(lib, b"R", b"L", b"N", b"S", 1, 1.0, 8, 10, 16, 8, 32, 9)
+    assert st == -12
+
+
+def test_gemm_and_leaf_validation(nla):
+    from nextla_amd import _lib
+    lib = _lib.load()
+    assert lib.nla_gemm(3, b"N", b"N", 4, 4, 4, 1.0, 8, 4, 8, 4, 1.0, 8, 4, None) == -1
+    assert lib.nla_gemm(1, b"X", b"N", 4, 4, 4, 1.0, 8, 4, 8, 4, 1.0, 8, 4, None) == -2
+    assert lib.nla_gemm(1, b"N", b"N", 4, 4, 4, 1.0, 8, 3, 8, 4, 1.0, 8, 4, None) == -9
+    assert lib.nla_gemm(1, b"N", b"T", 4, 5, 4, 1.0, 8, 4, 8, 4, 1.0, 8, 4, None) == -11
+    assert lib.nla_gemm(1, b"N", b"N", 0, 4, 4, 1.0, None, 1, None, 1, 1.0, None, 1, None) == 0
+    assert lib.nla_gemm_add(1, 4, 4, 4, 8, 3, 8, 4, 8, 4, None) == -6
+    assert lib.nla_gemm_sub(1, 4, 4, 4, 8, 2, 8, 4, 8, 4, None) == -6
+    assert lib.nla_leaf(b"Q", b"L", b"L", 1, 4, 4, 8, 4, 8, 4, None) == -1
+    assert lib.nla_leaf(b"S", b"L", b"L", 1, 4, 4, 8, 4, 8, 3, None) == -10
+
+
+def test_python_mirror_validates_before_device_work(nla):
+    import torch
+    A = torch.zeros(4, 4, dtype=torch.float64)
+    B = torch.zeros(4, 2, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        nla.unified_rectrxm_("X", "L", "N", 1.0, "S", A, B)
+    with pytest.raises(ValueError):
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, B.to(torch.float32))
+    with pytest.raises(ValueError):  # host tensors are rejected, never computed on the CPU
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, B)
+    with pytest.raises(ValueError):
+        nla.trsm("L", "L", "N", "U", A, B)

@@ -1,0 +1,220 @@
+"""Pins the CPU oracle (oracle/rectrxm_oracle.c) before it is trusted as the
+checker: against the BLAS fixtures the reference tests compare with, against
+live OpenBLAS at recursion sizes, against closed forms and invariants, and
+against exact rational arithmetic on tiny dyadic inputs."""
+import itertools
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+COMBOS = list(itertools.product("LR", "LU", "NTC", "SM"))
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    data = np.load(os.path.join(HERE, "golden", "blas_fixtures.npz"))
+    with open(os.path.join(HERE, "golden", "blas_fixtures.json")) as f:
+        manifest = json.load(f)
+    return data, manifest["cases"]
+
+
+def test_fixture_inventory(fixtures):
+    data, cases = fixtures
+    assert len(cases) == 146
+    combos = {(c["side"], c["uplo"], c["trans"], c["func"]) for c in cases}
+    assert combos == set(COMBOS)
+
+
+def test_oracle_matches_blas_fixtures(fixtures):
+    """Every fixture case: relative forward error below the reference's
+    tolerances (1e-14 FP64, test/unified_rectrxm.jl:8; 1e-5 FP32, test/trsm.jl:8)."""
+    data, cases = fixtures
+    for c in cases:
+        A, B, X = data[c["A"]], data[c["B"]], data[c["key"]]
+        got = O.rectrxm(c["side"], c["uplo"], c["trans"], c["alpha"], c["func"], A, B)
+        tol = 1e-14 if c["dtype"] == "float64" else 1e-5
+        assert O.rel_error(got, X) < tol, c
+
+
+@pytest.mark.parametrize("n,m", [(257, 3), (300, 5), (513, 2)])
+def test_oracle_trsm_recursion_vs_blas(n, m):
+    """TRSM recursion with the reference's threshold 256: n=300 and 513 take
+    the non-power-of-two split branch (src/rectrxm.jl:132)."""
+    rng = np.random.default_rng(n)
+    for side, uplo, trans in itertools.product("LR", "LU", "NT"):
+        A = O._f(np.tril(rng.random((n, n)) + 1) + 10 * np.eye(n))
+        if uplo == "U":
+            A = O._f(A.T)
+        B = rng.standard_normal((n, m) if side == "L" else (m, n))
+        X = O.rectrxm(side, uplo, trans, 1.5, "S", A, B, nthreads=4)
+        Y = O.blas(side, uplo, trans, 1.5, "S", A, B)
+        assert O.rel_error(X, Y) < 1e-14
+        assert O.backward_error(side, uplo, trans, 1.5, A, B, X) < 10 * n * np.finfo(np.float64).eps
+
+
+@pytest.mark.parametrize("n", [17, 40, 100])
+def test_oracle_trmm_recursion_vs_blas(n):
+    """TRMM recursion (threshold 16, src/rectrxm.jl:52) incl. odd splits."""
+    rng = np.random.default_rng(n)
+    for side, uplo, trans in itertools.product("LR", "LU", "NTC"):
+        A = O._f(rng.standard_normal((n, n)))
+        B = rng.standard_normal((n, 6) if side == "L" else (6, n))
+        X = O.rectrxm(side, uplo, trans, -0.5, "M", A, B)
+        Y = O.blas(side, uplo, trans, -0.5, "M", A, B)
+        assert O.rel_error(X, Y) < 1e-14
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_oracle_small_threshold_all_combos(dt):
+    """Force deep recursion (threshold 3) so every GEMM_ADD!/GEMM_SUB! branch
+    of src/rectrxm.jl:153-196 runs, in both precisions."""
+    rng = np.random.default_rng(7)
+    n, m = 29, 4
+    tol = 1e-13 if dt == np.float64 else 2e-5
+    for side, uplo, trans, func in COMBOS:
+        A = (np.tril(rng.random((n, n)) + 1) + 10 * np.eye(n)).astype(dt)
+        if uplo == "U":
+            A = np.ascontiguousarray(A.T)
+        B = rng.standard_normal((n, m) if side == "L" else (m, n)).astype(dt)
+        X = O.rectrxm(side, uplo, trans, 1.25, func, A, B, threshold=3)
+        Y = O.blas(side, uplo, trans, 1.25, func, A.astype(np.float64), B.astype(np.float64))
+        assert O.rel_error(X, Y) < tol, (side, uplo, trans, func)
+
+
+def test_closed_forms():
+    rng = np.random.default_rng(3)
+    n, m = 300, 5
+    B = rng.standard_normal((n, m))
+    # identity: TRSM gives alpha*B, TRMM gives alpha*B
+    for func in "SM":
+        X = O.rectrxm("L", "L", "N", 2.5, func, np.eye(n), B)
+        np.testing.assert_allclose(X, 2.5 * B, rtol=0, atol=1e-15)
+    # diagonal: X = alpha*B/d, and alpha*d*B
+    d = 1 + rng.random(n)
+    D = np.diag(d)
+    X = O.rectrxm("L", "U", "T", 1.5, "S", D, B)
+    np.testing.assert_allclose(X, 1.5 * B / d[:, None], rtol=1e-15)
+    X = O.rectrxm("R", "L", "N", 1.5, "M", D, B.T)
+    np.testing.assert_allclose(X, 1.5 * B.T * d[None, :], rtol=1e-15)
+    # unit lower all-ones L: L^{-1} is bidiagonal (1, -1), so X_i = B_i - B_{i-1}
+    L = np.tril(np.ones((n, n)))
+    X = O.rectrxm("L", "L", "N", 1.0, "S", L, B)
+    expect = B.copy()
+    expect[1:] -= B[:-1]
+    np.testing.assert_allclose(X, expect, rtol=0, atol=1e-12)
+    # and L*B is the running sum
+    Y = O.rectrxm("L", "L", "N", 1.0, "M", L, B)
+    np.testing.assert_allclose(Y, np.cumsum(B, axis=0), rtol=1e-13, atol=1e-12)
+    # nilpotent N (strictly lower): (I+N)^{-1} = sum_k (-N)^k, k < n
+    k = 12
+    Nm = np.tril(rng.random((k, k)) * 0.5, -1)
+    inv = np.zeros((k, k))
+    P = np.eye(k)
+    for _ in range(k):
+        inv += P
+        P = P @ (-Nm)
+    Bk = rng.standard_normal((k, 3))
+    X = O.rectrxm("L", "L", "N", 1.0, "S", np.eye(k) + Nm, Bk, threshold=2)
+    np.testing.assert_allclose(X, inv @ Bk, rtol=1e-12, atol=1e-13)
+
+
+def test_dualities_and_linearity():
+    rng = np.random.default_rng(5)
+    n, m = 270, 9
+    A = np.tril(rng.standard_normal((n, n)) / np.sqrt(n)) + np.diag(2 + rng.random(n))
+    B = rng.standard_normal((n, m))
+    XL = O.rectrxm("L", "L", "N", 1.0, "S", A, B)
+    # side duality: X*A^T = B^T  <=>  A X^T = B
+    XR = O.rectrxm("R", "L", "T", 1.0, "S", A, B.T)
+    np.testing.assert_allclose(XR.T, XL, rtol=1e-12, atol=1e-14)
+    # uplo duality through the reversal permutation J A J
+    J = np.eye(n)[::-1]
+    XU = O.rectrxm("L", "U", "N", 1.0, "S", J @ A @ J, J @ B)
+    np.testing.assert_allclose(J @ XU, XL, rtol=1e-12, atol=1e-14)
+    # linearity in alpha
+    X3 = O.rectrxm("L", "L", "N", -3.0, "S", A, B)
+    np.testing.assert_allclose(X3, -3.0 * XL, rtol=1e-14, atol=1e-15)
+    # TRMM -> TRSM round trip recovers B
+    Y = O.rectrxm("L", "L", "N", 1.0, "M", A, B)
+    Z = O.rectrxm("L", "L", "N", 1.0, "S", A, Y)
+    assert O.rel_error(Z, B) < 1e-13
+
+
+def _exact_solve(side, uplo, trans, func, alpha, A, B):
+    n = len(A)
+    T = [[Fraction(int(A[i][j])) if (j <= i if uplo == "L" else j >= i) else Fraction(0) for j in range(n)]
+         for i in range(n)]
+    if trans != "N":
+        T = [[T[j][i] for j in range(n)] for i in range(n)]
+    Bf = [[Fraction(int(x)) for x in row] for row in B]
+    if side == "R":  # X T = B  <=>  T^T X^T = B^T
+        T = [[T[j][i] for j in range(n)] for i in range(n)]
+        Bf = [[Bf[j][i] for j in range(len(Bf))] for i in range(len(Bf[0]))]
+    m = len(Bf[0])
+    a = Fraction(alpha)
+    if func == "M":
+        Y = [[a * sum(T[i][k] * Bf[k][c] for k in range(n)) for c in range(m)] for i in range(n)]
+    else:
+        lower = all(T[i][j] == 0 for i in range(n) for j in range(i + 1, n))
+        Y = [[None] * m for _ in range(n)]
+        order = range(n) if lower else range(n - 1, -1, -1)
+        for c in range(m):
+            for i in order:
+                s = a * Bf[i][c] - sum(T[i][k] * Y[k][c] for k in range(n) if Y[k][c] is not None and k != i)
+                Y[i][c] = s / T[i][i]
+    if side == "R":
+        Y = [[Y[j][i] for j in range(n)] for i in range(m)]
+    return np.array([[float(v) for v in row] for row in Y])
+
+
+def test_exact_dyadic_all_combos():
+    """Integer off-diagonals, power-of-two diagonals, integer B: every
+    intermediate of the restated recursion is a short dyadic rational, so the
+    oracle must agree exactly with rational arithmetic (a wrong split, index,
+    sign or update order changes the exact answer)."""
+    rng = np.random.default_rng(11)
+    for n, thr in ((5, 2), (7, 3), (9, 2), (12, 4)):
+        for side, uplo, trans, func in COMBOS:
+            A = rng.integers(-3, 4, size=(n, n)).astype(np.float64)
+            A[np.diag_indices(n)] = rng.choice([1.0, 2.0, -4.0], size=n)
+            A = np.tril(A) if uplo == "L" else np.triu(A)
+            m = 3
+            B = rng.integers(-5, 6, size=(n, m) if side == "L" else (m, n)).astype(np.float64)
+            X = O.rectrxm(side, uplo, trans, 2.0, func, A, B, threshold=thr)
+            E = _exact_solve(side, uplo, trans, func, 2, A, B)
+            np.testing.assert_array_equal(X, E, err_msg=str((n, side, uplo, trans, func)))
+
+
+def test_leaf_and_gemm_entry_points():
+    rng = np.random.default_rng(2)
+    n, m = 24, 5
+    for func, side, uplo in itertools.product("SM", "LR", "LU"):
+        A = np.tril(rng.random((n, n)) + 1) + 10 * np.eye(n)
+        A = A if uplo == "L" else np.ascontiguousarray(A.T)
+        B = rng.random((n, m) if side == "L" else (m, n)) + 1
+        X = O.leaf(func, side, uplo, A, B)
+        Y = O.blas(side, uplo, "N", 1.0, func, A, B)
+        assert O.rel_error(X, Y) < 1e-14
+    X = rng.standard_normal((33, 70))
+    Y = rng.standard_normal((70, 9))
+    C = rng.standard_normal((33, 9))
+    np.testing.assert_allclose(O.gemm_add(X, Y, C), C + X @ Y, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(O.gemm_sub(C, X, Y), C - X @ Y, rtol=1e-13, atol=1e-13)
+
+
+def test_planted_mistake_is_caught(fixtures):
+    """The fixtures are sharp enough to reject a plausible restatement error:
+    solving with the wrong triangle (a transposed operand) or dropping alpha."""
+    data, cases = fixtures
+    c = next(x for x in cases if x["key"] == "alpha/X/d/LLNS")
+    A, B, X = data[c["A"]], data[c["B"]], data[c["key"]]
+    wrong_trans = O.rectrxm("L", "L", "T", c["alpha"], "S", A, B)
+    dropped_alpha = O.rectrxm("L", "L", "N", 1.0, "S", A, B)
+    assert O.rel_error(wrong_trans, X) > 1e-3
+    assert O.rel_error(dropped_alpha, X) > 1e-3
