@@ -95,6 +95,16 @@ struct View {
     }
 };
 
+// Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),
+// 0 = register/LDS substitution kernel.
+static int leaf_mode_default() {
+    static int mode = [] {
+        const char *e = std::getenv("NLA_LEAF_MODE");
+        return (e && std::strcmp(e, "subst") == 0) ? 0 : 1;
+    }();
+    return mode;
+}
+
 static int leaf_nb_default() {
     static int nb = [] {
         const char *e = std::getenv("NLA_LEAF_NB");
@@ -110,10 +120,55 @@ struct Recursion {
     char func, side, uplo;  // uplo of the effective (possibly transposed) A
     int64_t nb;             // leaf size
     hipStream_t stream;
+    const T *W = nullptr;   // inverted diagonal blocks (TRSM, leaf mode 1)
+    int mode = 1;
     hipError_t err = hipSuccess;
 
-    void leaf(View<T> A, View<T> B, int64_t n, T scale) {
+    // Leaf as an MFMA product: TRSM applies inv(A_kk) from W, TRMM applies
+    // A_kk itself with the other triangle masked to zero.
+    void leaf_mfma(View<T> A, View<T> B, int64_t n, int64_t off, T scale) {
+        GemmArgs<T> g;
+        g.alpha = scale;
+        g.beta = T(0);
+        g.K = n;
+        g.C = B.p;
+        g.ldc = B.cs;
+        const bool lower = (uplo == 'L');
+        // the A_kk operand: inverse block (column-major, ld nb) or the view
+        const T *ap;
+        bool at;
+        int64_t lda;
+        if (func == 'S') {
+            ap = W + (off / nb) * nb * nb;
+            at = false;
+            lda = nb;
+        } else {
+            ap = A.p;
+            at = (A.rs != 1);
+            lda = at ? A.rs : A.cs;
+        }
+        if (side == 'L') {  // B_k (n x m) <- op * B_k
+            g.M = n; g.N = B.nc; g.leaf = 1;
+            g.X = ap; g.tx = at; g.ldx = lda;
+            g.Y = B.p; g.ty = false; g.ldy = B.cs;
+            g.tri = (func == 'M') ? (lower ? 1 : 2) : 0;
+        } else {  // B_k (m x n) <- B_k * op
+            g.M = B.nr; g.N = n; g.leaf = 2;
+            g.X = B.p; g.tx = false; g.ldx = B.cs;
+            g.Y = ap; g.ty = at; g.ldy = lda;
+            g.tri = (func == 'M') ? (lower ? 3 : 4) : 0;
+        }
+        const double fl = (double)n * (double)n * (double)(side == 'L' ? B.nc : B.nr);
+        const double by = ((double)n * (n + 1) / 2 + 2.0 * n * (side == 'L' ? B.nc : B.nr)) * sizeof(T);
+        err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_gemm<T>(g, stream); });
+    }
+
+    void leaf(View<T> A, View<T> B, int64_t n, int64_t off, T scale) {
         if (err != hipSuccess) return;
+        if (mode == 1) {
+            leaf_mfma(A, B, n, off, scale);
+            return;
+        }
         LeafArgs<T> a;
         a.trmm = (func == 'M');
         a.nb = (int)n;
@@ -167,10 +222,11 @@ struct Recursion {
 
     // pre: TRSM scale owed by B's untouched blocks; post: TRMM scale owed by
     // the last write of this B.
-    void rec(View<T> A, View<T> B, int64_t n, T pre, T post) {
+    // off: position of this diagonal block in the whole matrix.
+    void rec(View<T> A, View<T> B, int64_t n, int64_t off, T pre, T post) {
         if (err != hipSuccess) return;
         if (n <= nb) {
-            leaf(A, B, n, func == 'S' ? pre : post);
+            leaf(A, B, n, off, func == 'S' ? pre : post);
             return;
         }
         const int64_t mid = split(n), rem = n - mid;
@@ -189,27 +245,27 @@ struct Recursion {
         const T one = T(1);
         if (func == 'S') {
             if (forward) {
-                rec(A11, B1, mid, pre, one);
+                rec(A11, B1, mid, off, pre, one);
                 if (side == 'L') gemm(B2, A21, B1, -one, pre);  // B2 -= A21*B1
                 else gemm(B2, B1, A12, -one, pre);              // B2 -= B1*A12
-                rec(A22, B2, rem, one, one);
+                rec(A22, B2, rem, off + mid, one, one);
             } else {
-                rec(A22, B2, rem, pre, one);
+                rec(A22, B2, rem, off + mid, pre, one);
                 if (side == 'L') gemm(B1, A12, B2, -one, pre);  // B1 -= A12*B2
                 else gemm(B1, B2, A21, -one, pre);              // B1 -= B2*A21
-                rec(A11, B1, mid, one, one);
+                rec(A11, B1, mid, off, one, one);
             }
         } else {
             if (forward) {
-                rec(A11, B1, mid, one, one);
+                rec(A11, B1, mid, off, one, one);
                 if (side == 'L') gemm(B1, A12, B2, post, post);  // B1 += A12*B2
                 else gemm(B1, B2, A21, post, post);              // B1 += B2*A21
-                rec(A22, B2, rem, one, post);
+                rec(A22, B2, rem, off + mid, one, post);
             } else {
-                rec(A22, B2, rem, one, one);
+                rec(A22, B2, rem, off + mid, one, one);
                 if (side == 'L') gemm(B2, A21, B1, post, post);  // B2 += A21*B1
                 else gemm(B2, B1, A12, post, post);              // B2 += B1*A12
-                rec(A11, B1, mid, one, post);
+                rec(A11, B1, mid, off, one, post);
             }
         }
     }
@@ -232,7 +288,29 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     r.uplo = uplo;
     r.nb = leaf_nb_default();
     r.stream = s;
-    r.rec(A, B, n, (T)alpha, (T)alpha);
+    r.mode = leaf_mode_default();
+    T *W = nullptr;
+    if (func == 'S' && r.mode == 1) {
+        // inverses of all diagonal leaf blocks, from one batched launch; the
+        // buffer comes from the stream-ordered pool and returns to it below
+        const int64_t blocks = (n + r.nb - 1) / r.nb;
+        hipError_t e = hipMallocAsync((void **)&W, (size_t)(blocks * r.nb * r.nb) * sizeof(T), s);
+        if (e != hipSuccess) return hip_status(e, "workspace");
+        const double fl = (double)blocks * r.nb * r.nb * r.nb / 3.0;
+        const double by = (double)blocks * r.nb * r.nb * 1.5 * sizeof(T);
+        e = timed(NLA_KIND_LEAF, fl, by, s,
+                  [&] { return launch_trtri_blocks<T>(n, (int)r.nb, A.p, A.rs, A.cs, uplo == 'L', W, s); });
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(W, s);
+            return hip_status(e, "diagonal inverses");
+        }
+        r.W = W;
+    }
+    r.rec(A, B, n, 0, (T)alpha, (T)alpha);
+    if (W) {
+        hipError_t e = hipFreeAsync(W, s);
+        if (r.err == hipSuccess) r.err = e;
+    }
     return hip_status(r.err, "kernel launch");
 }
 

@@ -79,6 +79,24 @@ struct TileLoader {
         }
     }
 
+    // Zero the elements outside a triangle: keep (outer, k) when
+    // (KEEP_LE ? k <= outer : k >= outer), in global coordinates.
+    template <bool KEEP_LE>
+    __device__ inline void mask(int64_t o0, int64_t k0, int t) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            int o, k;
+            coords(t, c, o, k);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                int64_t eo = o0 + (KCONTIG ? o : o + e);
+                int64_t ek = k0 + (KCONTIG ? k + e : k);
+                bool keep = KEEP_LE ? (ek <= eo) : (ek >= eo);
+                if (!keep) r[c][e] = T(0);
+            }
+        }
+    }
+
     __device__ inline void store(T *lds, int t) const {
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -122,11 +140,13 @@ __device__ inline void tile_coords(int G, int g, int tiles_m, int &tm, int &tn) 
     tn = in / gm;
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW>
+// TRI: triangle mask on one operand (see GemmArgs::tri).  C is not declared
+// __restrict__: a leaf update writes the strip it reads.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
 __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N, int64_t K, T alpha,
                                                            const T *__restrict__ X, int64_t ldx,
                                                            const T *__restrict__ Y, int64_t ldy, T beta,
-                                                           T *__restrict__ C, int64_t ldc, int tiles_m) {
+                                                           T *C, int64_t ldc, int tiles_m) {
     typedef Mfma<T> MF;
     constexpr int NT = WM * WN * 64;
     constexpr int TWM = BM / WM, TWN = BN / WN;
@@ -161,10 +181,19 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
 
     LX lx;
     LY ly;
+    // op(X)(r,k): outer r, inner k.  op(Y)(k,c): outer c, inner k; the Y
+    // triangle conditions are c <= k (lower) / c >= k (upper).
+    auto masks = [&](int64_t k0) {
+        if (TRI == 1) lx.template mask<true>(row0, k0, t);
+        if (TRI == 2) lx.template mask<false>(row0, k0, t);
+        if (TRI == 3) ly.template mask<false>(col0, k0, t);
+        if (TRI == 4) ly.template mask<true>(col0, k0, t);
+    };
     const int nkt = (int)((K + BK - 1) / BK);
     if (nkt > 0) {
         lx.load(X, xso, xsk, row0, 0, M, K, t);
         ly.load(Y, yso, ysk, col0, 0, N, K, t);
+        masks(0);
         lx.store(lds_x[0], t);
         ly.store(lds_y[0], t);
     }
@@ -193,6 +222,7 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
                 for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = MF::mma(a[cb], b[rb], acc[cb][rb]);
         }
         if (more) {
+            masks((int64_t)(kt + 1) * BK);
             lx.store(lds_x[cur ^ 1], t);
             ly.store(lds_y[cur ^ 1], t);
         }
@@ -216,22 +246,33 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
             }
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW>
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
 static hipError_t launch_cfg(const GemmArgs<T> &a, hipStream_t s) {
     int tiles_m = (int)((a.M + BM - 1) / BM);
     int tiles_n = (int)((a.N + BN - 1) / BN);
     int G = tiles_m * tiles_n;
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TX, TY, VW>), dim3(G), dim3(WM * WN * 64), 0, s, a.M,
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI>), dim3(G), dim3(WM * WN * 64), 0, s, a.M,
                        a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C, a.ldc, tiles_m);
     return hipGetLastError();
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, int VW>
+template <typename T, int BM, int BN, int BK, int WM, int WN, int VW, int TRI>
 static hipError_t launch_trans(const GemmArgs<T> &a, hipStream_t s) {
-    if (!a.tx && !a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW>(a, s);
-    if (a.tx && !a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, true, false, VW>(a, s);
-    if (!a.tx && a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, false, true, VW>(a, s);
-    return launch_cfg<T, BM, BN, BK, WM, WN, true, true, VW>(a, s);
+    if (!a.tx && !a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW, TRI>(a, s);
+    if (a.tx && !a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, true, false, VW, TRI>(a, s);
+    if (!a.tx && a.ty) return launch_cfg<T, BM, BN, BK, WM, WN, false, true, VW, TRI>(a, s);
+    return launch_cfg<T, BM, BN, BK, WM, WN, true, true, VW, TRI>(a, s);
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN, int VW>
+static hipError_t launch_tri(const GemmArgs<T> &a, hipStream_t s) {
+    switch (a.tri) {
+        case 1: return launch_trans<T, BM, BN, BK, WM, WN, VW, 1>(a, s);
+        case 2: return launch_trans<T, BM, BN, BK, WM, WN, VW, 2>(a, s);
+        case 3: return launch_trans<T, BM, BN, BK, WM, WN, VW, 3>(a, s);
+        case 4: return launch_trans<T, BM, BN, BK, WM, WN, VW, 4>(a, s);
+        default: return launch_trans<T, BM, BN, BK, WM, WN, VW, 0>(a, s);
+    }
 }
 
 static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
@@ -243,22 +284,43 @@ hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s) {
     // 16-byte vector loads need every column start of X, Y (and the tile
     // origins, which are multiples of the tile sizes) to be 16-byte aligned.
     bool vec = aligned16(a.X) && aligned16(a.Y) && (a.ldx % VW == 0) && (a.ldy % VW == 0);
+    if (a.leaf) {
+        // one leaf (<= 64) along M (left) or N (right) and along K: a single
+        // tile covers the leaf dimension, K is consumed in two steps.
+        if (a.leaf == 1) {  // left: C (nb x m) = W (nb x nb) * B (nb x m)
+            if (a.M > 64 || a.K > 64) return hipErrorInvalidValue;
+            if constexpr (sizeof(T) == 8) {
+                if (vec) return launch_tri<T, 64, 32, 32, 2, 2, VW>(a, s);
+                return launch_tri<T, 64, 32, 32, 2, 2, 1>(a, s);
+            }
+            if (vec) return launch_tri<T, 64, 64, 32, 2, 2, VW>(a, s);
+            return launch_tri<T, 64, 64, 32, 2, 2, 1>(a, s);
+        }
+        // right: C (m x nb) = B (m x nb) * W (nb x nb)
+        if (a.N > 64 || a.K > 64) return hipErrorInvalidValue;
+        if constexpr (sizeof(T) == 8) {
+            if (vec) return launch_tri<T, 32, 64, 32, 2, 2, VW>(a, s);
+            return launch_tri<T, 32, 64, 32, 2, 2, 1>(a, s);
+        }
+        if (vec) return launch_tri<T, 64, 64, 32, 2, 2, VW>(a, s);
+        return launch_tri<T, 64, 64, 32, 2, 2, 1>(a, s);
+    }
     int64_t big_tiles = ((a.M + 127) / 128) * ((a.N + 127) / 128);
     bool small = big_tiles < 512;
-    if (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 8) {
         if (small) {
-            if (vec) return launch_trans<T, 64, 64, 16, 2, 2, VW>(a, s);
-            return launch_trans<T, 64, 64, 16, 2, 2, 1>(a, s);
+            if (vec) return launch_trans<T, 64, 64, 16, 2, 2, VW, 0>(a, s);
+            return launch_trans<T, 64, 64, 16, 2, 2, 1, 0>(a, s);
         }
-        if (vec) return launch_trans<T, 128, 128, 16, 2, 2, VW>(a, s);
-        return launch_trans<T, 128, 128, 16, 2, 2, 1>(a, s);
+        if (vec) return launch_trans<T, 128, 128, 16, 2, 2, VW, 0>(a, s);
+        return launch_trans<T, 128, 128, 16, 2, 2, 1, 0>(a, s);
     } else {
         if (small) {
-            if (vec) return launch_trans<T, 64, 64, 32, 2, 2, VW>(a, s);
-            return launch_trans<T, 64, 64, 32, 2, 2, 1>(a, s);
+            if (vec) return launch_trans<T, 64, 64, 32, 2, 2, VW, 0>(a, s);
+            return launch_trans<T, 64, 64, 32, 2, 2, 1, 0>(a, s);
         }
-        if (vec) return launch_trans<T, 128, 128, 32, 2, 2, VW>(a, s);
-        return launch_trans<T, 128, 128, 32, 2, 2, 1>(a, s);
+        if (vec) return launch_trans<T, 128, 128, 32, 2, 2, VW, 0>(a, s);
+        return launch_trans<T, 128, 128, 32, 2, 2, 1, 0>(a, s);
     }
 }
 

@@ -10,6 +10,13 @@ namespace nla {
 constexpr int LEAF_MAX_NB = 64;
 
 // C = beta*C + alpha*op(X)*op(Y); op(X) is M x K, op(Y) is K x N.
+// tri: 0 = dense operands; 1/2 = op(X) restricted to its lower/upper
+// triangle (k <= r / k >= r); 3/4 = op(Y) restricted to its lower/upper
+// triangle (c <= k / c >= k).  leaf != 0 selects the tile shape used for
+// diagonal-block leaves: one leaf (<= 64) along M (leaf = 1, left side) or N
+// (leaf = 2, right side) and along K.  C may then alias the operand it
+// overwrites, because one work-group covers the whole leaf dimension of the
+// strip it reads and writes.
 template <typename T>
 struct GemmArgs {
     int64_t M, N, K;
@@ -21,6 +28,8 @@ struct GemmArgs {
     int64_t ldy;
     T *C;
     int64_t ldc;
+    int tri = 0;
+    int leaf = 0;  // 0: general; 1: left leaf (M <= 64); 2: right leaf (N <= 64)
 };
 
 // Canonical leaf: L(i,j) = A[i*ars + j*acs] (lower, nb x nb),
@@ -43,6 +52,15 @@ hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s);
 
 template <typename T>
 hipError_t launch_leaf(const LeafArgs<T> &a, hipStream_t s);
+
+// Inverses of the diagonal blocks of an n x n triangular view
+// M(i,j) = A[i*rs + j*cs] (lower if lower, else upper): block b covers rows
+// and columns [b*nb, min(n, (b+1)*nb)); W receives inv(M_bb) as an nb x nb
+// column-major block at W + b*nb*nb (zero outside the triangle and padded
+// with the identity past n).
+template <typename T>
+hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,
+                               hipStream_t s);
 
 template <typename T>
 hipError_t launch_zero(int64_t rows, int64_t cols, T *B, int64_t ldb, hipStream_t s);

@@ -18,8 +18,9 @@
 // registers; the block of L sits in LDS and is read with wave-uniform
 // addresses (one broadcast LDS read serves all 64 lanes), so the sweep has no
 // barriers at all (the reference takes one barrier per row, src/trsm.jl:21-27).
-// For the left side the RHS tile is contiguous along i in HBM, so it is staged
-// through LDS to turn 64 strided column reads into coalesced row reads.
+// The RHS tile is staged through LDS so that HBM is always walked along its
+// contiguous direction (down columns for the left side, along rows for the
+// right side) whatever the lane-to-column mapping.
 #include "nla_device.h"
 #include "nla_internal.h"
 
@@ -27,7 +28,10 @@ namespace nla {
 
 constexpr int LEAF_COLS = 64;  // RHS columns per work-group (one per lane)
 
-template <typename T, int NB, bool STAGE, bool TRMM>
+// ICONTIG: Bc is contiguous along i (left side) rather than along c (right
+// side); the RHS tile is staged through LDS either way, walking the
+// contiguous direction so every wave-wide load and store is coalesced.
+template <typename T, int NB, bool ICONTIG, bool TRMM>
 __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, const T *__restrict__ A, int64_t ars,
                                                          int64_t acs, T *__restrict__ B, int64_t brs, int64_t bcs,
                                                          T scale) {
@@ -35,30 +39,41 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
     // (row i is a dot product over j <= i); store L in the matching order.
     __shared__ __attribute__((aligned(16))) T sl[NB * NB];
     __shared__ T srinv[NB];
-    __shared__ T sb[STAGE ? LEAF_COLS * (NB + 1) : 1];
+    __shared__ T sb[LEAF_COLS * (NB + 1)];
 
     const int t = threadIdx.x;
     const int64_t c0 = (int64_t)blockIdx.x * LEAF_COLS;
 
-    // Stage L, padded to NB with an identity block.
-    for (int e = t; e < NB * NB; e += LEAF_COLS) {
-        int i, j;
-        if (ars == 1 || ars == -1) {
-            i = e % NB;
-            j = e / NB;
-        } else {
-            j = e % NB;
-            i = e / NB;
+    // Stage L, padded to NB with an identity block.  Loads are issued in
+    // batches of SB before any LDS write so HBM/L2 latency overlaps.
+    constexpr int SB = 16;
+    const bool arow = (ars == 1 || ars == -1);  // walk the contiguous direction
+#pragma unroll 1
+    for (int e0 = 0; e0 < NB * NB; e0 += SB * LEAF_COLS) {
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            int e = e0 + u * LEAF_COLS + t;
+            int i = arow ? e % NB : e / NB;
+            int j = arow ? e / NB : e % NB;
+            bool in = e < NB * NB;
+            if (in && i < nb && j < nb)
+                v[u] = (j <= i) ? A[i * ars + j * acs] : T(0);
+            else
+                v[u] = (i == j) ? T(1) : T(0);
         }
-        T v;
-        if (i < nb && j < nb)
-            v = (j <= i) ? A[i * ars + j * acs] : T(0);
-        else
-            v = (i == j) ? T(1) : T(0);
-        if (TRMM)
-            sl[i * NB + j] = v;
-        else
-            sl[j * NB + i] = v;
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            int e = e0 + u * LEAF_COLS + t;
+            int i = arow ? e % NB : e / NB;
+            int j = arow ? e / NB : e % NB;
+            if (e < NB * NB) {
+                if (TRMM)
+                    sl[i * NB + j] = v[u];
+                else
+                    sl[j * NB + i] = v[u];
+            }
+        }
     }
     if (!TRMM) {
         for (int i = t; i < NB; i += LEAF_COLS) srinv[i] = (i < nb) ? T(1) / A[i * ars + i * acs] : T(1);
@@ -66,21 +81,28 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
 
     // Load this lane's column of Bc.
     T x[NB];
-    const int64_t col = c0 + t;
-    if (STAGE) {
-        for (int e = t; e < NB * LEAF_COLS; e += LEAF_COLS) {
-            int i = e % NB, c = e / NB;
+#pragma unroll 1
+    for (int e0 = 0; e0 < NB * LEAF_COLS; e0 += SB * LEAF_COLS) {
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            int e = e0 + u * LEAF_COLS + t;
+            int i = ICONTIG ? e % NB : e / LEAF_COLS;
+            int c = ICONTIG ? e / NB : e % LEAF_COLS;
             int64_t gc = c0 + c;
-            sb[c * (NB + 1) + i] = (i < nb && gc < m) ? B[i * brs + gc * bcs] : T(0);
+            v[u] = (i < nb && gc < m) ? B[i * brs + gc * bcs] : T(0);
         }
-        __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NB; ++i) x[i] = sb[t * (NB + 1) + i];
-    } else {
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NB; ++i) x[i] = (i < nb && col < m) ? B[i * brs + col * bcs] : T(0);
+        for (int u = 0; u < SB; ++u) {
+            int e = e0 + u * LEAF_COLS + t;
+            int i = ICONTIG ? e % NB : e / LEAF_COLS;
+            int c = ICONTIG ? e / NB : e % LEAF_COLS;
+            sb[c * (NB + 1) + i] = v[u];
+        }
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = sb[t * (NB + 1) + i];
 
     if (!TRMM) {
         // alpha of unified_rectrxm! (src/rectrxm.jl:64) applied on first touch.
@@ -108,48 +130,130 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
         }
     }
 
-    if (STAGE) {
 #pragma unroll
-        for (int i = 0; i < NB; ++i) sb[t * (NB + 1) + i] = x[i];
-        __syncthreads();
-        for (int e = t; e < NB * LEAF_COLS; e += LEAF_COLS) {
-            int i = e % NB, c = e / NB;
-            int64_t gc = c0 + c;
-            if (i < nb && gc < m) B[i * brs + gc * bcs] = sb[c * (NB + 1) + i];
+    for (int i = 0; i < NB; ++i) sb[t * (NB + 1) + i] = x[i];
+    __syncthreads();
+#pragma unroll 16
+    for (int e = t; e < NB * LEAF_COLS; e += LEAF_COLS) {
+        int i = ICONTIG ? e % NB : e / LEAF_COLS;
+        int c = ICONTIG ? e / NB : e % LEAF_COLS;
+        int64_t gc = c0 + c;
+        if (i < nb && gc < m) B[i * brs + gc * bcs] = sb[c * (NB + 1) + i];
+    }
+}
+
+// Inverse of one NB x NB diagonal block per work-group, by forward
+// substitution on the columns of the identity (one column per lane, the
+// block in LDS).  Each column of the inverse is a backward-stable triangular
+// solve; the inverses turn the diagonal-block leaves of the recursion into
+// MFMA products.
+template <typename T, int NB>
+__global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__restrict__ A, int64_t rs,
+                                                          int64_t cs, int lower, T *__restrict__ W) {
+    __shared__ __attribute__((aligned(16))) T sl[NB * NB];
+    __shared__ T srinv[NB];
+    const int t = threadIdx.x;
+    const int64_t off = (int64_t)blockIdx.x * NB;
+    const int nb = (int)(n - off < NB ? n - off : NB);
+    // canonical lower L: the block itself, or J*U*J for an upper block
+    const T *base = lower ? A + off * (rs + cs) : A + (off + nb - 1) * (rs + cs);
+    const int64_t ars = lower ? rs : -rs, acs = lower ? cs : -cs;
+    const bool arow = (ars == 1 || ars == -1);
+    constexpr int SB = 16;
+#pragma unroll 1
+    for (int e0 = 0; e0 < NB * NB; e0 += SB * LEAF_COLS) {
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            int e = e0 + u * LEAF_COLS + t;
+            int i = arow ? e % NB : e / NB;
+            int j = arow ? e / NB : e % NB;
+            if (e < NB * NB && i < nb && j < nb)
+                v[u] = (j <= i) ? base[i * ars + j * acs] : T(0);
+            else
+                v[u] = (i == j) ? T(1) : T(0);
         }
-    } else {
-        if (col < m) {
 #pragma unroll
-            for (int i = 0; i < NB; ++i)
-                if (i < nb) B[i * brs + col * bcs] = x[i];
+        for (int u = 0; u < SB; ++u) {
+            int e = e0 + u * LEAF_COLS + t;
+            int i = arow ? e % NB : e / NB;
+            int j = arow ? e / NB : e % NB;
+            if (e < NB * NB) sl[j * NB + i] = v[u];
+        }
+    }
+    for (int i = t; i < NB; i += LEAF_COLS) srinv[i] = (i < nb) ? T(1) / base[i * ars + i * acs] : T(1);
+    __syncthreads();
+    if (t >= NB) return;
+    T x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = (i == t) ? T(1) : T(0);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        x[j] = x[j] * srinv[j];
+#pragma unroll
+        for (int r = j + 1; r < NB; ++r) x[r] = x[r] - sl[j * NB + r] * x[j];
+    }
+    // column t of inv(L); for an upper block inv(U) = J inv(L) J
+    T *w = W + (int64_t)blockIdx.x * NB * NB;
+    if (lower || t >= nb) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) w[i + t * NB] = x[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            if (i < nb) w[(nb - 1 - i) + (nb - 1 - t) * NB] = x[i];
+            else w[i + (nb - 1 - t) * NB] = T(0);
         }
     }
 }
 
-template <typename T, int NB, bool STAGE>
+template <typename T>
+hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,
+                               hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + nb - 1) / nb;
+    if (nb == 64)
+        hipLaunchKernelGGL((trtri_kernel<T, 64>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
+                           lower ? 1 : 0, W);
+    else if (nb == 32)
+        hipLaunchKernelGGL((trtri_kernel<T, 32>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
+                           lower ? 1 : 0, W);
+    else if (nb == 16)
+        hipLaunchKernelGGL((trtri_kernel<T, 16>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
+                           lower ? 1 : 0, W);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template hipError_t launch_trtri_blocks<float>(int64_t, int, const float *, int64_t, int64_t, bool, float *,
+                                               hipStream_t);
+template hipError_t launch_trtri_blocks<double>(int64_t, int, const double *, int64_t, int64_t, bool, double *,
+                                                hipStream_t);
+
+template <typename T, int NB, bool ICONTIG>
 static hipError_t launch_leaf_nb(const LeafArgs<T> &a, hipStream_t s) {
     int64_t groups = (a.m + LEAF_COLS - 1) / LEAF_COLS;
     if (a.trmm)
-        hipLaunchKernelGGL((leaf_kernel<T, NB, STAGE, true>), dim3((unsigned)groups), dim3(LEAF_COLS), 0, s, a.nb,
+        hipLaunchKernelGGL((leaf_kernel<T, NB, ICONTIG, true>), dim3((unsigned)groups), dim3(LEAF_COLS), 0, s, a.nb,
                            a.m, a.A, a.ars, a.acs, a.B, a.brs, a.bcs, a.scale);
     else
-        hipLaunchKernelGGL((leaf_kernel<T, NB, STAGE, false>), dim3((unsigned)groups), dim3(LEAF_COLS), 0, s, a.nb,
+        hipLaunchKernelGGL((leaf_kernel<T, NB, ICONTIG, false>), dim3((unsigned)groups), dim3(LEAF_COLS), 0, s, a.nb,
                            a.m, a.A, a.ars, a.acs, a.B, a.brs, a.bcs, a.scale);
     return hipGetLastError();
 }
 
-template <typename T, bool STAGE>
+template <typename T, bool ICONTIG>
 static hipError_t launch_leaf_stage(const LeafArgs<T> &a, hipStream_t s) {
-    if (a.nb <= 16) return launch_leaf_nb<T, 16, STAGE>(a, s);
-    if (a.nb <= 32) return launch_leaf_nb<T, 32, STAGE>(a, s);
-    return launch_leaf_nb<T, 64, STAGE>(a, s);
+    if (a.nb <= 16) return launch_leaf_nb<T, 16, ICONTIG>(a, s);
+    if (a.nb <= 32) return launch_leaf_nb<T, 32, ICONTIG>(a, s);
+    return launch_leaf_nb<T, 64, ICONTIG>(a, s);
 }
 
 template <typename T>
 hipError_t launch_leaf(const LeafArgs<T> &a, hipStream_t s) {
     if (a.m <= 0 || a.nb <= 0) return hipSuccess;
     if (a.nb > LEAF_MAX_NB) return hipErrorInvalidValue;
-    // Stage through LDS when Bc is contiguous along i (left side).
     if (a.brs == 1 || a.brs == -1) return launch_leaf_stage<T, true>(a, s);
     return launch_leaf_stage<T, false>(a, s);
 }

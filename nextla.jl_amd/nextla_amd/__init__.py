@@ -47,6 +47,8 @@ def _layout(t, name):
     if t.dim() != 2:
         raise ValueError(f"{name} must be a matrix, got {t.dim()} dims")
     r, c = t.shape
+    if r == 0 or c == 0:  # empty: strides are meaningless, nothing is read
+        return False, max(r, 1)
     s0, s1 = t.stride()
     if s0 == 1 and (c <= 1 or s1 >= max(1, r)):
         return False, max(s1 if c > 1 else r, 1)
@@ -56,11 +58,16 @@ def _layout(t, name):
 
 
 def _stream(t):
-    return ctypes_stream(torch.cuda.current_stream(t.device))
+    """hipStream_t handle of torch's current stream on t's device."""
+    return torch.cuda.current_stream(t.device).cuda_stream
 
 
-def ctypes_stream(s):
-    return s.cuda_stream
+def _require_device(*ts):
+    """All operands on one ROCm device: host tensors are rejected, never
+    computed on the CPU."""
+    d = ts[0].device
+    if not all(t.is_cuda and t.device == d for t in ts):
+        raise ValueError("all operands must live on the same ROCm device")
 
 
 def _check_pair(A, B):
@@ -68,8 +75,7 @@ def _check_pair(A, B):
         raise ValueError(f"unsupported element type {A.dtype} (Float32/Float64 only)")
     if B.dtype != A.dtype:
         raise ValueError(f"A is {A.dtype} but B is {B.dtype}")
-    if not (A.is_cuda and B.is_cuda) or A.device != B.device:
-        raise ValueError("A and B must live on the same ROCm device")
+    _require_device(A, B)
 
 
 def _flip(c, a, b):
@@ -119,8 +125,7 @@ def _gemm(alpha, X, Y, beta, C, what):
     for t, nm in ((X, "X"), (Y, "Y"), (C, "C")):
         if t.dtype not in _DT or t.dtype != C.dtype:
             raise ValueError(f"{what}: {nm} has element type {t.dtype}, expected {C.dtype} (Float32/Float64)")
-        if not t.is_cuda or t.device != C.device:
-            raise ValueError(f"{what}: all operands must live on one ROCm device")
+    _require_device(X, Y, C)
     N, R = X.shape
     R2, M = Y.shape
     if R2 != R or tuple(C.shape) != (N, M):
