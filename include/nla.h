@@ -99,6 +99,22 @@ int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int6
                  void *B, int64_t ldb, void *stream);
 
 /*
+ * Process-wide choice of the methods used below the recursion (no
+ * counterpart in the reference, whose thresholds are constants,
+ * src/rectrxm.jl:52,63).  Defaults come from the environment
+ * (NLA_LEAF_MODE=subst, NLA_PANEL=0) and can be changed between calls:
+ *   NLA_CFG_LEAF_MODE: 1 = inverted 64-blocks applied on MFMA (default),
+ *                      0 = register/LDS substitution leaf;
+ *   NLA_CFG_PANEL:     1 = diagonal blocks up to 256 solved by one fused
+ *                      panel launch (default, needs leaf mode 1), 0 = recurse
+ *                      to 64-leaves.
+ * Returns the previous value, or -1 for an unknown key.
+ */
+enum nla_config_key { NLA_CFG_LEAF_MODE = 0, NLA_CFG_PANEL = 1 };
+
+int32_t nla_set_config(int32_t key, int32_t value);
+
+/*
  * Kernel timing (instrumentation; no counterpart in the reference, which has
  * no timers).  While enabled, every kernel the library launches is bracketed
  * by HIP events on its stream and tagged with its algorithmic work.

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -95,14 +96,36 @@ struct View {
     }
 };
 
+// Methods below the recursion (nla_set_config); -1 = not yet read from the
+// environment.
+static std::atomic<int> g_leaf_mode{-1};
+static std::atomic<int> g_panel{-1};
+
 // Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),
 // 0 = register/LDS substitution kernel.
 static int leaf_mode_default() {
-    static int mode = [] {
+    int v = g_leaf_mode.load();
+    if (v < 0) {
         const char *e = std::getenv("NLA_LEAF_MODE");
-        return (e && std::strcmp(e, "subst") == 0) ? 0 : 1;
-    }();
-    return mode;
+        v = (e && std::strcmp(e, "subst") == 0) ? 0 : 1;
+        int expect = -1;
+        g_leaf_mode.compare_exchange_strong(expect, v);
+        v = g_leaf_mode.load();
+    }
+    return v;
+}
+
+// Fused diagonal panels on (1) or off (0, recursion down to 64-leaves).
+static bool panel_default() {
+    int v = g_panel.load();
+    if (v < 0) {
+        const char *e = std::getenv("NLA_PANEL");
+        v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+        int expect = -1;
+        g_panel.compare_exchange_strong(expect, v);
+        v = g_panel.load();
+    }
+    return v != 0;
 }
 
 static int leaf_nb_default() {
@@ -122,6 +145,7 @@ struct Recursion {
     hipStream_t stream;
     const T *W = nullptr;   // inverted diagonal blocks (TRSM, leaf mode 1)
     int mode = 1;
+    bool use_panel = true;  // fuse diagonal blocks <= PANEL_MAX into one launch
     hipError_t err = hipSuccess;
 
     // Leaf as an MFMA product: TRSM applies inv(A_kk) from W, TRMM applies
@@ -161,6 +185,31 @@ struct Recursion {
         const double fl = (double)n * (double)n * (double)(side == 'L' ? B.nc : B.nr);
         const double by = ((double)n * (n + 1) / 2 + 2.0 * n * (side == 'L' ? B.nc : B.nr)) * sizeof(T);
         err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_gemm<T>(g, stream); });
+    }
+
+    // A whole diagonal block (n <= PANEL_MAX) in one fused launch.
+    void panel(View<T> A, View<T> B, int64_t n, int64_t off, T scale) {
+        PanelArgs<T> a;
+        a.trmm = (func == 'M');
+        a.nbig = (int)n;
+        a.scale = scale;
+        a.W = (func == 'S') ? W + (off / nb) * nb * nb : nullptr;
+        if (side == 'L') {
+            a.lower = (uplo == 'L');
+            a.m = B.nc;
+            a.M = A.p; a.mrs = A.rs; a.mcs = A.cs;
+            a.B = B.p; a.brs = B.rs; a.bcs = B.cs;
+            a.wrs = 1; a.wcs = nb;
+        } else {  // op(A)^T X^T = B^T
+            a.lower = (uplo == 'U');
+            a.m = B.nr;
+            a.M = A.p; a.mrs = A.cs; a.mcs = A.rs;
+            a.B = B.p; a.brs = B.cs; a.bcs = B.rs;
+            a.wrs = nb; a.wcs = 1;
+        }
+        const double fl = (double)n * (double)n * (double)a.m;
+        const double by = ((double)n * (n + 1) / 2 + 2.0 * n * a.m) * sizeof(T);
+        err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_panel<T>(a, stream); });
     }
 
     void leaf(View<T> A, View<T> B, int64_t n, int64_t off, T scale) {
@@ -225,6 +274,10 @@ struct Recursion {
     // off: position of this diagonal block in the whole matrix.
     void rec(View<T> A, View<T> B, int64_t n, int64_t off, T pre, T post) {
         if (err != hipSuccess) return;
+        if (mode == 1 && nb == 64 && n <= PANEL_MAX && use_panel) {
+            panel(A, B, n, off, func == 'S' ? pre : post);
+            return;
+        }
         if (n <= nb) {
             leaf(A, B, n, off, func == 'S' ? pre : post);
             return;
@@ -289,6 +342,7 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     r.nb = leaf_nb_default();
     r.stream = s;
     r.mode = leaf_mode_default();
+    r.use_panel = panel_default();
     T *W = nullptr;
     if (func == 'S' && r.mode == 1) {
         // inverses of all diagonal leaf blocks, from one batched launch; the
@@ -401,6 +455,20 @@ int32_t nla_gemm_sub(int32_t dtype, int64_t N, int64_t R, int64_t M, void *C, in
     if (Y == nullptr) return bad_arg(9, "Y");
     if (ldy < R) return bad_arg(10, "ldy");
     return nla_gemm(dtype, 'N', 'N', N, M, R, -1.0, X, ldx, Y, ldy, 1.0, C, ldc, stream);
+}
+
+int32_t nla_set_config(int32_t key, int32_t value) {
+    if (key == NLA_CFG_LEAF_MODE) {
+        int prev = leaf_mode_default();
+        g_leaf_mode.store(value ? 1 : 0);
+        return prev;
+    }
+    if (key == NLA_CFG_PANEL) {
+        int prev = panel_default() ? 1 : 0;
+        g_panel.store(value ? 1 : 0);
+        return prev;
+    }
+    return -1;
 }
 
 int32_t nla_prof_enable(int32_t on) {

@@ -25,6 +25,11 @@
 
 namespace nla {
 
+// Pointers into LDS keep their address space so that every access compiles
+// to ds_read / ds_write (a generic pointer becomes a flat access, which also
+// waits on outstanding global loads).
+#define NLA_LDS __attribute__((address_space(3)))
+
 template <typename T, int BO, int BK, int NT, bool KCONTIG, int VW>
 struct TileLoader {
     // Each thread moves CH chunks of VW consecutive elements along the
@@ -37,6 +42,7 @@ struct TileLoader {
     static constexpr int LDS_ELEMS = KCONTIG ? BO * (BK + PK) : BK * (BO + PO);
 
     T r[CH][VW];
+    uint32_t okbits;  // CHECK loads: bit (c*VW+e) set when element is in bounds
 
     // Coordinates (outer, k) inside the tile of chunk c of thread t.
     __device__ static inline void coords(int t, int c, int &o, int &k) {
@@ -52,6 +58,12 @@ struct TileLoader {
 
     // Element (outer, k) of the operand lives at P[outer*so + k*sk] with one
     // of so/sk equal to 1.  o0/k0: tile origin; no/nk: extents for bounds.
+    // CHECK = false: the whole tile is in bounds, plain vector loads.
+    // CHECK = true: element-wise loads without branches: an out-of-bounds
+    // element is read from P itself (always valid) and replaced by zero, so
+    // the loads of a k-step stay in flight together (a branch per element
+    // would make the compiler wait for every earlier load at each join).
+    template <bool CHECK>
     __device__ inline void load(const T *__restrict__ P, int64_t so, int64_t sk, int64_t o0, int64_t k0,
                                 int64_t no, int64_t nk, int t) {
 #pragma unroll
@@ -59,24 +71,38 @@ struct TileLoader {
             int o, k;
             coords(t, c, o, k);
             int64_t go = o0 + o, gk = k0 + k;
-            const T *p = P + go * so + gk * sk;
-            if (VW > 1) {
-                bool full = KCONTIG ? (go < no && gk + VW <= nk) : (gk < nk && go + VW <= no);
-                if (full) {
+            if (!CHECK) {
+                const T *p = P + go * so + gk * sk;
+                if (VW > 1) {
                     typedef T vec_t __attribute__((ext_vector_type(VW)));
                     vec_t v = *reinterpret_cast<const vec_t *>(p);
 #pragma unroll
                     for (int e = 0; e < VW; ++e) r[c][e] = v[e];
-                    continue;
+                } else {
+                    r[c][0] = *p;
+                }
+            } else {
+                if (c == 0) okbits = 0;
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    int64_t eo = KCONTIG ? go : go + e;
+                    int64_t ek = KCONTIG ? gk + e : gk;
+                    bool ok = (eo < no) && (ek < nk);
+                    r[c][e] = *(ok ? P + eo * so + ek * sk : P);
+                    okbits |= (ok ? 1u : 0u) << (c * VW + e);
                 }
             }
-#pragma unroll
-            for (int e = 0; e < VW; ++e) {
-                int64_t eo = KCONTIG ? go : go + e;
-                int64_t ek = KCONTIG ? gk + e : gk;
-                r[c][e] = (eo < no && ek < nk) ? (KCONTIG ? p[e] : p[e * so]) : T(0);
-            }
         }
+    }
+
+    // Zero the out-of-bounds elements of a CHECK load.  Done just before the
+    // LDS store, so no instruction needs the loaded values any earlier.
+    __device__ inline void apply_bounds() {
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+                if (!((okbits >> (c * VW + e)) & 1u)) r[c][e] = T(0);
     }
 
     // Zero the elements outside a triangle: keep (outer, k) when
@@ -97,18 +123,18 @@ struct TileLoader {
         }
     }
 
-    __device__ inline void store(T *lds, int t) const {
+    __device__ inline void store(NLA_LDS T *lds, int t) const {
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             int o, k;
             coords(t, c, o, k);
-            T *q = KCONTIG ? lds + o * (BK + PK) + k : lds + k * (BO + PO) + o;
+            NLA_LDS T *q = KCONTIG ? lds + o * (BK + PK) + k : lds + k * (BO + PO) + o;
             if (VW > 1) {
                 typedef T vec_t __attribute__((ext_vector_type(VW)));
                 vec_t v;
 #pragma unroll
                 for (int e = 0; e < VW; ++e) v[e] = r[c][e];
-                *reinterpret_cast<vec_t *>(q) = v;
+                *reinterpret_cast<NLA_LDS vec_t *>(q) = v;
             } else {
                 q[0] = r[c][0];
             }
@@ -116,7 +142,7 @@ struct TileLoader {
     }
 
     // Operand element (outer, k) from the staged tile.
-    __device__ static inline T at(const T *lds, int o, int k) {
+    __device__ static inline T at(const NLA_LDS T *lds, int o, int k) {
         return KCONTIG ? lds[o * (BK + PK) + k] : lds[k * (BO + PO) + o];
     }
 };
@@ -140,13 +166,13 @@ __device__ inline void tile_coords(int G, int g, int tiles_m, int &tm, int &tn) 
     tn = in / gm;
 }
 
-// TRI: triangle mask on one operand (see GemmArgs::tri).  C is not declared
-// __restrict__: a leaf update writes the strip it reads.
-template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
-__global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N, int64_t K, T alpha,
-                                                           const T *__restrict__ X, int64_t ldx,
-                                                           const T *__restrict__ Y, int64_t ldy, T beta,
-                                                           T *C, int64_t ldc, int tiles_m) {
+// One BM x BN tile of C.  CHECK selects bounds-checked loads and stores
+// (edge tiles, or K not a multiple of BK) or the unchecked vector path.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI, bool CHECK>
+__device__ __forceinline__ void gemm_tile(int64_t M, int64_t N, int64_t K, T alpha, const T *__restrict__ X,
+                                          int64_t ldx, const T *__restrict__ Y, int64_t ldy, T beta, T *C,
+                                          int64_t ldc, int64_t row0, int64_t col0, NLA_LDS T *lds_xb,
+                                          NLA_LDS T *lds_yb) {
     typedef Mfma<T> MF;
     constexpr int NT = WM * WN * 64;
     constexpr int TWM = BM / WM, TWN = BN / WN;
@@ -154,19 +180,15 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
     static_assert(RB >= 1 && CB >= 1 && TWM % MF::BD == 0 && TWN % MF::BD == 0, "bad wave tile");
     static_assert(BK % MF::KD == 0, "bad BK");
     // X tile: outer = rows of C, contiguous along k when TX.
-    typedef TileLoader<T, BM, BK, NT, TX, VW> LX;
+    typedef TileLoader<T, BM, BK, NT, TX, CHECK ? 1 : VW> LX;
     // Y tile: outer = columns of C, contiguous along k when !TY.
-    typedef TileLoader<T, BN, BK, NT, !TY, VW> LY;
-
-    __shared__ __attribute__((aligned(16))) T lds_x[2][LX::LDS_ELEMS];
-    __shared__ __attribute__((aligned(16))) T lds_y[2][LY::LDS_ELEMS];
+    typedef TileLoader<T, BN, BK, NT, !TY, CHECK ? 1 : VW> LY;
+    NLA_LDS T *lds_x[2] = {lds_xb, lds_xb + LX::LDS_ELEMS};
+    NLA_LDS T *lds_y[2] = {lds_yb, lds_yb + LY::LDS_ELEMS};
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave % WM, wn = wave / WM;
-    int tm, tn;
-    tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
-    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
 
     // op(X)(r, k): TX ? X[k + r*ldx] : X[r + k*ldx]
     const int64_t xso = TX ? ldx : 1, xsk = TX ? 1 : ldx;
@@ -184,6 +206,10 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
     // op(X)(r,k): outer r, inner k.  op(Y)(k,c): outer c, inner k; the Y
     // triangle conditions are c <= k (lower) / c >= k (upper).
     auto masks = [&](int64_t k0) {
+        if (CHECK) {
+            lx.apply_bounds();
+            ly.apply_bounds();
+        }
         if (TRI == 1) lx.template mask<true>(row0, k0, t);
         if (TRI == 2) lx.template mask<false>(row0, k0, t);
         if (TRI == 3) ly.template mask<false>(col0, k0, t);
@@ -191,8 +217,8 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
     };
     const int nkt = (int)((K + BK - 1) / BK);
     if (nkt > 0) {
-        lx.load(X, xso, xsk, row0, 0, M, K, t);
-        ly.load(Y, yso, ysk, col0, 0, N, K, t);
+        lx.template load<CHECK>(X, xso, xsk, row0, 0, M, K, t);
+        ly.template load<CHECK>(Y, yso, ysk, col0, 0, N, K, t);
         masks(0);
         lx.store(lds_x[0], t);
         ly.store(lds_y[0], t);
@@ -204,11 +230,11 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
         const int cur = kt & 1;
         const bool more = kt + 1 < nkt;
         if (more) {
-            lx.load(X, xso, xsk, row0, (int64_t)(kt + 1) * BK, M, K, t);
-            ly.load(Y, yso, ysk, col0, (int64_t)(kt + 1) * BK, N, K, t);
+            lx.template load<CHECK>(X, xso, xsk, row0, (int64_t)(kt + 1) * BK, M, K, t);
+            ly.template load<CHECK>(Y, yso, ysk, col0, (int64_t)(kt + 1) * BK, N, K, t);
         }
-        const T *sx = lds_x[cur];
-        const T *sy = lds_y[cur];
+        const NLA_LDS T *sx = lds_x[cur];
+        const NLA_LDS T *sy = lds_y[cur];
 #pragma unroll
         for (int ks = 0; ks < BK; ks += MF::KD) {
             T a[CB], b[RB];
@@ -229,21 +255,71 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
         __syncthreads();
     }
 
-    // Epilogue: D[i][j] of block (cb, rb) is C(row j, column i).
+    // Epilogue: D[i][j] of block (cb, rb) is C(row j, column i).  beta is
+    // uniform, so the choice is made once; with beta != 0 the C values of one
+    // column block are all requested before any is used.
+    if (beta == T(0)) {
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
+        for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-            for (int r = 0; r < MF::NR; ++r) {
-                int64_t row = row0 + wm * TWM + rb * MF::BD + MF::dj(lane, r);
-                int64_t col = col0 + wn * TWN + cb * MF::BD + MF::di(lane, r);
-                if (row < M && col < N) {
-                    T *p = C + row + col * ldc;
-                    T v = alpha * acc[cb][rb][r];
-                    *p = (beta == T(0)) ? v : v + beta * *p;
+                for (int r = 0; r < MF::NR; ++r) {
+                    int64_t row = row0 + wm * TWM + rb * MF::BD + MF::dj(lane, r);
+                    int64_t col = col0 + wn * TWN + cb * MF::BD + MF::di(lane, r);
+                    if (!CHECK || (row < M && col < N)) C[row + col * ldc] = alpha * acc[cb][rb][r];
                 }
-            }
+    } else {
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+            T cv[RB][MF::NR];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) {
+                    int64_t row = row0 + wm * TWM + rb * MF::BD + MF::dj(lane, r);
+                    int64_t col = col0 + wn * TWN + cb * MF::BD + MF::di(lane, r);
+                    bool ok = !CHECK || (row < M && col < N);
+                    cv[rb][r] = *(ok ? C + row + col * ldc : C);
+                }
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) {
+                    int64_t row = row0 + wm * TWM + rb * MF::BD + MF::dj(lane, r);
+                    int64_t col = col0 + wn * TWN + cb * MF::BD + MF::di(lane, r);
+                    if (!CHECK || (row < M && col < N))
+                        C[row + col * ldc] = alpha * acc[cb][rb][r] + beta * cv[rb][r];
+                }
+        }
+    }
+}
+
+// TRI: triangle mask on one operand (see GemmArgs::tri).  C is not declared
+// __restrict__: a leaf update writes the strip it reads.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
+__global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N, int64_t K, T alpha,
+                                                           const T *__restrict__ X, int64_t ldx,
+                                                           const T *__restrict__ Y, int64_t ldy, T beta,
+                                                           T *C, int64_t ldc, int tiles_m) {
+    constexpr int NT = WM * WN * 64;
+    typedef TileLoader<T, BM, BK, NT, TX, VW> LX;
+    typedef TileLoader<T, BN, BK, NT, !TY, VW> LY;
+    __shared__ __attribute__((aligned(16))) T lds_x[2 * LX::LDS_ELEMS];
+    __shared__ __attribute__((aligned(16))) T lds_y[2 * LY::LDS_ELEMS];
+    int tm, tn;
+    tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+    // interior tiles with whole k-steps take the unchecked vector path
+    const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0) && TRI == 0;
+    if (interior)
+        gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI, false>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc,
+                                                                 row0, col0, (NLA_LDS T *)lds_x,
+                                                                 (NLA_LDS T *)lds_y);
+    else
+        gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI, true>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc,
+                                                                row0, col0, (NLA_LDS T *)lds_x,
+                                                                (NLA_LDS T *)lds_y);
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
@@ -305,22 +381,36 @@ hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s) {
         if (vec) return launch_tri<T, 64, 64, 32, 2, 2, VW>(a, s);
         return launch_tri<T, 64, 64, 32, 2, 2, 1>(a, s);
     }
-    int64_t big_tiles = ((a.M + 127) / 128) * ((a.N + 127) / 128);
-    bool small = big_tiles < 512;
+    // Tile shape by how many tiles the problem yields: the largest shape that
+    // still gives at least one tile per CU (256) wins; 8- and 16-wave
+    // work-groups keep two or more waves on every SIMD.
+    auto tiles = [&](int bm, int bn) { return ((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn); };
     if constexpr (sizeof(T) == 8) {
-        if (small) {
-            if (vec) return launch_trans<T, 64, 64, 16, 2, 2, VW, 0>(a, s);
-            return launch_trans<T, 64, 64, 16, 2, 2, 1, 0>(a, s);
+        if (tiles(256, 128) >= 256) {
+            if (vec) return launch_trans<T, 256, 128, 16, 4, 2, VW, 0>(a, s);
+            return launch_trans<T, 256, 128, 16, 4, 2, 1, 0>(a, s);
         }
-        if (vec) return launch_trans<T, 128, 128, 16, 2, 2, VW, 0>(a, s);
-        return launch_trans<T, 128, 128, 16, 2, 2, 1, 0>(a, s);
+        if (tiles(128, 128) >= 256) {
+            if (vec) return launch_trans<T, 128, 128, 16, 4, 2, VW, 0>(a, s);
+            return launch_trans<T, 128, 128, 16, 4, 2, 1, 0>(a, s);
+        }
+        if (vec) return launch_trans<T, 64, 128, 16, 2, 2, VW, 0>(a, s);
+        return launch_trans<T, 64, 128, 16, 2, 2, 1, 0>(a, s);
     } else {
-        if (small) {
-            if (vec) return launch_trans<T, 64, 64, 32, 2, 2, VW, 0>(a, s);
-            return launch_trans<T, 64, 64, 32, 2, 2, 1, 0>(a, s);
+        if (tiles(256, 256) >= 256) {
+            if (vec) return launch_trans<T, 256, 256, 32, 4, 4, VW, 0>(a, s);
+            return launch_trans<T, 256, 256, 32, 4, 4, 1, 0>(a, s);
         }
-        if (vec) return launch_trans<T, 128, 128, 32, 2, 2, VW, 0>(a, s);
-        return launch_trans<T, 128, 128, 32, 2, 2, 1, 0>(a, s);
+        if (tiles(256, 128) >= 256) {
+            if (vec) return launch_trans<T, 256, 128, 16, 4, 2, VW, 0>(a, s);
+            return launch_trans<T, 256, 128, 16, 4, 2, 1, 0>(a, s);
+        }
+        if (tiles(128, 128) >= 256) {
+            if (vec) return launch_trans<T, 128, 128, 32, 2, 2, VW, 0>(a, s);
+            return launch_trans<T, 128, 128, 32, 2, 2, 1, 0>(a, s);
+        }
+        if (vec) return launch_trans<T, 64, 64, 32, 2, 2, VW, 0>(a, s);
+        return launch_trans<T, 64, 64, 32, 2, 2, 1, 0>(a, s);
     }
 }
 

@@ -47,6 +47,29 @@ struct LeafArgs {
     T scale;
 };
 
+// Largest diagonal block the fused panel kernel solves in one launch.
+constexpr int PANEL_MAX = 256;
+
+// Fused diagonal panel: M(i,j) = M[i*mrs + j*mcs] (nbig x nbig, lower or
+// upper), Bc(i,c) = B[i*brs + c*bcs] (nbig x m), inverse 64-blocks of M's
+// diagonal at W + k*64*64 with element (i,j) at [i*wrs + j*wcs] (TRSM only).
+template <typename T>
+struct PanelArgs {
+    bool trmm, lower;
+    int nbig;
+    int64_t m;
+    const T *M;
+    int64_t mrs, mcs;
+    T *B;
+    int64_t brs, bcs;
+    const T *W;
+    int64_t wrs, wcs;
+    T scale;
+};
+
+template <typename T>
+hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s);
+
 template <typename T>
 hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s);
 

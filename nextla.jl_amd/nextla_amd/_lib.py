@@ -28,6 +28,7 @@ SIGNATURES = {
     "nla_gemm_sub": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_gemm": (_i32, [_i32, _c_char, _c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
     "nla_leaf": (_i32, [_c_char, _c_char, _c_char, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_set_config": (_i32, [_i32, _i32]),
     "nla_prof_enable": (_i32, [_i32]),
     "nla_prof_summary": (_i32, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
                                 ctypes.POINTER(_dbl)]),
@@ -35,6 +36,13 @@ SIGNATURES = {
 
 NLA_KIND_GEMM = 0
 NLA_KIND_LEAF = 1
+NLA_CFG_LEAF_MODE = 0
+NLA_CFG_PANEL = 1
+
+
+def set_config(key, value):
+    """Process-wide method switch below the recursion; returns the old value."""
+    return load().nla_set_config(key, value)
 
 
 def prof_enable(on=True):

@@ -116,13 +116,29 @@ def test_trmm_leaf_functions(nla):
 SIZES = [(1, 1), (2, 3), (63, 5), (64, 64), (65, 1), (100, 65), (129, 7), (300, 130), (513, 31), (1000, 64),
          (1031, 2)]
 
+# methods below the recursion: fused 256-panels (default), 64-leaves as MFMA
+# products of inverted blocks, 64-leaves by register/LDS substitution
+METHODS = {"panel": (1, 1), "mfma-leaf": (1, 0), "subst-leaf": (0, 0)}
+
+
+@pytest.fixture(params=list(METHODS))
+def method(request, nla):
+    from nextla_amd import _lib
+    leaf_mode, panel = METHODS[request.param]
+    old_l = _lib.set_config(_lib.NLA_CFG_LEAF_MODE, leaf_mode)
+    old_p = _lib.set_config(_lib.NLA_CFG_PANEL, panel)
+    yield request.param
+    _lib.set_config(_lib.NLA_CFG_LEAF_MODE, old_l)
+    _lib.set_config(_lib.NLA_CFG_PANEL, old_p)
+
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("n,m", SIZES)
-def test_recursion_all_combos(nla, dt, n, m):
+def test_recursion_all_combos(nla, method, dt, n, m):
     """All side/uplo/trans/func at sizes that hit single leaves, ragged leaves,
     odd splits and multi-level recursion; alpha != 1; backward error bound and
-    agreement with the C restatement of the reference algorithm."""
+    agreement with the C restatement of the reference algorithm; for each
+    method below the recursion."""
     rng = np.random.default_rng(n * 1000 + m)
     for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
         A = O._f(np.tril(rng.uniform(-1, 1, (n, n)) / np.sqrt(n)) + np.diag(2 + rng.random(n))).astype(dt)
