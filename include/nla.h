@@ -99,6 +99,51 @@ int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int6
                  void *B, int64_t ldb, void *stream);
 
 /*
+ * Batched solves/multiplies of many independent systems (no counterpart in
+ * the reference, which makes one unified_rectrxm! call per system).  All
+ * systems of a batch share side, uplo, trans, func and dtype; each has its
+ * own n (<= 1024), m, A, B and alpha.  Every system is solved as one fused
+ * diagonal panel by a persistent kernel that pulls (system, 16-column strip)
+ * tasks, longest first, from a device queue.
+ *
+ *   nla_batch_workspace_bytes  device workspace the plan needs (descriptors,
+ *                              task queue and, for TRSM, the inverted 64x64
+ *                              diagonal blocks of every system).
+ *   nla_batch_plan_create      validates the items and uploads the plan into
+ *                              the caller's workspace (blocks the host until
+ *                              the upload is done); the plan keeps pointers,
+ *                              so A_i and B_i must stay allocated.
+ *   nla_batch_plan_execute     enqueues the batch on `stream`: kernels only,
+ *                              no allocation, no memset, no host copy, so it
+ *                              can be captured into a hipGraph; with
+ *                              use_graph != 0 the library captures it itself
+ *                              on the first call and replays the graph after.
+ *   nla_batch_plan_destroy     releases the host handle (and its graph).
+ * Argument positions for nla_batch_plan_create: 1 side, 2 uplo, 3 trans,
+ * 4 func, 5 dtype, 6 items, 7 count, 8 workspace, 9 workspace_bytes, 10 plan.
+ */
+typedef struct nla_batch_item {
+    int64_t n;       /* order of A_i (<= 1024) */
+    int64_t m;       /* other dimension of B_i */
+    const void *A;   /* column-major, leading dimension lda */
+    int64_t lda;
+    void *B;         /* column-major, leading dimension ldb */
+    int64_t ldb;
+    double alpha;
+} nla_batch_item;
+
+typedef struct nla_batch_plan_s *nla_batch_plan;
+
+int64_t nla_batch_workspace_bytes(char func, int32_t dtype, const nla_batch_item *items, int64_t count);
+
+int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32_t dtype, const nla_batch_item *items,
+                              int64_t count, void *workspace, int64_t workspace_bytes, nla_batch_plan *plan);
+
+int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_graph);
+
+int32_t nla_batch_plan_destroy(nla_batch_plan plan);
+
+/*
  * Process-wide choice of the methods used below the recursion (no
  * counterpart in the reference, whose thresholds are constants,
  * src/rectrxm.jl:52,63).  Defaults come from the environment

@@ -84,6 +84,9 @@ static hipError_t timed(int kind, double flops, double bytes, hipStream_t s, F f
     return hipSuccess;
 }
 
+int32_t report_hip(hipError_t e, const char *where) { return hip_status(e, where); }
+int32_t report_bad_arg(int k, const char *what) { return bad_arg(k, what); }
+
 static inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
 
 // A strided view: element (i, j) at p[i*rs + j*cs].

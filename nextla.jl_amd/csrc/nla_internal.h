@@ -47,8 +47,10 @@ struct LeafArgs {
     T scale;
 };
 
-// Largest diagonal block the fused panel kernel solves in one launch.
-constexpr int PANEL_MAX = 256;
+// Largest diagonal block the fused panel kernel solves in one launch, and the
+// RHS columns per work-group.
+constexpr int PANEL_MAX = 512;
+constexpr int PANEL_STRIP = 16;
 
 // Fused diagonal panel: M(i,j) = M[i*mrs + j*mcs] (nbig x nbig, lower or
 // upper), Bc(i,c) = B[i*brs + c*bcs] (nbig x m), inverse 64-blocks of M's
@@ -69,6 +71,46 @@ struct PanelArgs {
 
 template <typename T>
 hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s);
+
+// Largest system the persistent batched solver takes (one panel per system).
+constexpr int BATCH_MAX = 1024;
+
+// One system of a batch in the panel's canonical form (see PanelArgs).
+template <typename T>
+struct BatchSys {
+    int n, lower;
+    int64_t m;
+    const T *M;
+    int64_t mrs, mcs;
+    T *B;
+    int64_t brs, bcs;
+    const T *W;  // this system's inverse 64-blocks (TRSM)
+    int64_t wrs, wcs;
+    T scale;
+};
+
+struct BatchTask {
+    int sys, strip;
+};
+
+// RHS columns per batched task.
+template <typename T>
+struct BatchStrip {
+    static constexpr int W = 16;
+};
+
+// Kernel entry of the persistent batched solver (for occupancy queries).
+template <typename T>
+const void *batch_kernel_ptr(bool trmm);
+
+template <typename T>
+hipError_t launch_batch(const BatchSys<T> *sys, const BatchTask *tasks, int ntasks, int *counter, bool trmm,
+                        int groups, hipStream_t s);
+
+// Inverse 64-blocks of every system of a batch (TRSM): one work-group per
+// (system, block) pair listed in blk (system index, block index).
+template <typename T>
+hipError_t launch_batch_trtri(const BatchSys<T> *sys, const int2 *blk, int nblk, hipStream_t s);
 
 template <typename T>
 hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s);

@@ -148,12 +148,10 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
 // solve; the inverses turn the diagonal-block leaves of the recursion into
 // MFMA products.
 template <typename T, int NB>
-__global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__restrict__ A, int64_t rs,
-                                                          int64_t cs, int lower, T *__restrict__ W) {
-    __shared__ __attribute__((aligned(16))) T sl[NB * NB];
-    __shared__ T srinv[NB];
+__device__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
+                            T *__restrict__ W, T *sl, T *srinv) {
     const int t = threadIdx.x;
-    const int64_t off = (int64_t)blockIdx.x * NB;
+    const int64_t off = blk * NB;
     const int nb = (int)(n - off < NB ? n - off : NB);
     // canonical lower L: the block itself, or J*U*J for an upper block
     const T *base = lower ? A + off * (rs + cs) : A + (off + nb - 1) * (rs + cs);
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__
         for (int r = j + 1; r < NB; ++r) x[r] = x[r] - sl[j * NB + r] * x[j];
     }
     // column t of inv(L); for an upper block inv(U) = J inv(L) J
-    T *w = W + (int64_t)blockIdx.x * NB * NB;
+    T *w = W + blk * NB * NB;
     if (lower || t >= nb) {
 #pragma unroll
         for (int i = 0; i < NB; ++i) w[i + t * NB] = x[i];
@@ -206,6 +204,46 @@ __global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__
         }
     }
 }
+
+// Inverse of one NB x NB diagonal block per work-group, by forward
+// substitution on the columns of the identity (one column per lane, the
+// block in LDS).  Each column of the inverse is a backward-stable triangular
+// solve; the inverses turn the diagonal-block leaves of the recursion into
+// MFMA products.
+template <typename T, int NB>
+__global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__restrict__ A, int64_t rs,
+                                                          int64_t cs, int lower, T *__restrict__ W) {
+    __shared__ __attribute__((aligned(16))) T sl[NB * NB];
+    __shared__ T srinv[NB];
+    trtri_block<T, NB>(n, blockIdx.x, A, rs, cs, lower != 0, W, sl, srinv);
+}
+
+// Batched form: one work-group per (system, block) pair.
+template <typename T>
+__global__ __launch_bounds__(LEAF_COLS) void batch_trtri_kernel(const BatchSys<T> *__restrict__ sys,
+                                                                const int2 *__restrict__ blk) {
+    __shared__ __attribute__((aligned(16))) T sl[64 * 64];
+    __shared__ T srinv[64];
+    const int2 b = blk[blockIdx.x];
+    const BatchSys<T> &S = sys[b.x];
+    // the diagonal blocks of the effective matrix in its natural orientation:
+    // for a right-side system M = op(A)^T, whose blocks are the transposed
+    // blocks of op(A); the inverses are stored for op(A) (read transposed).
+    const bool right = (S.wrs != 1);
+    const int64_t rs = right ? S.mcs : S.mrs, cs = right ? S.mrs : S.mcs;
+    const bool lower = right ? (S.lower == 0) : (S.lower != 0);
+    trtri_block<T, 64>(S.n, b.y, S.M, rs, cs, lower, (T *)S.W, sl, srinv);
+}
+
+template <typename T>
+hipError_t launch_batch_trtri(const BatchSys<T> *sys, const int2 *blk, int nblk, hipStream_t s) {
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_trtri_kernel<T>, dim3(nblk), dim3(LEAF_COLS), 0, s, sys, blk);
+    return hipGetLastError();
+}
+
+template hipError_t launch_batch_trtri<float>(const BatchSys<float> *, const int2 *, int, hipStream_t);
+template hipError_t launch_batch_trtri<double>(const BatchSys<double> *, const int2 *, int, hipStream_t);
 
 template <typename T>
 hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,

@@ -1,21 +1,21 @@
-// Fused diagonal-panel kernel: the bottom levels of the recursion in one
-// launch.
+// Fused diagonal-panel solve: the bottom levels of the recursion in one
+// launch, and the persistent batched solver built on the same routine.
 //
-// For a diagonal block of up to NBIG rows (a multiple of 64) of the effective
-// triangular matrix M (lower or upper), every work-group owns a strip of W
-// right-hand-side columns, keeps that strip of B in LDS for the whole block
-// and runs the blocked algorithm the recursion would run below this size:
+// For a diagonal block of up to NBIG rows of the effective triangular matrix
+// M (lower or upper), a work-group owns a strip of W right-hand-side columns,
+// keeps that strip of B in LDS for the whole block and runs the blocked
+// algorithm the recursion would run below this size:
 //   TRSM lower: for k: X_k = inv(M_kk) B_k ; B_i -= M_ik X_k (i > k)
 //   TRSM upper: the same from the last 64-block backwards (i < k)
 //   TRMM lower: from the last block up: B_i = tri(M_ii) B_i + sum_{k<i} M_ik B_k
 //   TRMM upper: from the first block down, sum over k > i
 // with inv(M_kk) from the batched diagonal inverses.  Each phase is a
-// 64 x W x 64 MFMA product; no work-group depends on another (each owns its
-// columns), so the panel needs one launch instead of the 2*(NBIG/64)-1
-// launches of leaves and small-K updates it replaces.
+// 64 x W x 64 MFMA product whose M-operand fragments are fetched one phase
+// ahead; no work-group depends on another (each owns its columns).
 //
 // Replaces the bottom of unified_rec (src/rectrxm.jl:101-198) together with the
-// leaves it calls (src/trsm.jl:128-150, src/trmm.jl:332-393).
+// leaves it calls (src/trsm.jl:128-150, src/trmm.jl:332-393); the batched form
+// has no counterpart in the reference (one call per system there).
 #include "nla_device.h"
 #include "nla_internal.h"
 
@@ -23,65 +23,146 @@ namespace nla {
 
 #define NLA_LDS __attribute__((address_space(3)))
 
-// MFMA output tile 64 x W per phase, split over 4 waves.
+// 16x16x4 f32 MFMA (2048 flops, 32 cycles): D[i][j] in lane j+16*(i/4),
+// register i%4 (verified on gfx950 by a lane-layout probe).
+struct Mfma16f {
+    static constexpr int BD = 16, KD = 4, NR = 4;
+    typedef f4_t acc_t;
+    __device__ static inline acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+    __device__ static inline acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline int di(int l, int r) { return 4 * (l >> 4) + r; }
+    __device__ static inline int dj(int l, int) { return l & 15; }
+};
+
+// Output tile of one phase: 64 rows x W columns over 4 waves.  With 16x16
+// MFMA blocks and W = 16, wave w owns rows 16w..16w+15 of the 64-row block;
+// narrow strips give many work-groups (m/16) for the column parallelism.
 template <typename T>
-struct PanelShape;
+struct Mfma16;
 template <>
-struct PanelShape<double> {  // 16x16 blocks: wave w owns row block w, both column blocks
-    static constexpr int W = 32;
-    static constexpr int RBW = 1, CBW = 2;  // blocks per wave (rows, cols)
+struct Mfma16<double> : Mfma<double> {};
+template <>
+struct Mfma16<float> : Mfma16f {};
+
+template <typename T, int W_>
+struct PanelShape {
+    typedef Mfma16<T> MF;
+    static constexpr int W = W_, CBW = W_ / 16;
+    static_assert(W_ % 16 == 0, "strip width");
     __device__ static inline int rb0(int wave) { return wave; }
     __device__ static inline int cb0(int) { return 0; }
 };
-template <>
-struct PanelShape<float> {  // 32x32 blocks: wave w owns block (w%2, w/2)
-    static constexpr int W = 64;
-    static constexpr int RBW = 1, CBW = 1;
-    __device__ static inline int rb0(int wave) { return wave & 1; }
-    __device__ static inline int cb0(int wave) { return wave >> 1; }
+
+// Phase p of the blocked algorithm for nsb 64-blocks.  kind: 0 = diagonal
+// (TRSM: inv(M_kk) block; TRMM: masked M_ii), 1 = off-diagonal M_ik.
+struct Phase {
+    int i, k, kind;
+    bool first, last;  // first / last phase of an output block group
 };
 
-template <typename T, int NBIG, bool TRMM>
-__global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T *__restrict__ Mp, int64_t mrs,
-                                                    int64_t mcs, int lower, T *__restrict__ Bp, int64_t brs,
-                                                    int64_t bcs, const T *__restrict__ Wp, int64_t wrs,
-                                                    int64_t wcs, T scale) {
-    typedef Mfma<T> MF;
-    typedef PanelShape<T> PS;
+__device__ inline Phase phase_of(int p, int nsb, bool lower, bool trmm) {
+    Phase ph;
+    if (!trmm) {
+        // groups per step: 1 diagonal + (nsb-1-step) updates
+        int step = 0, base = 0;
+        while (p >= base + (nsb - step)) {
+            base += nsb - step;
+            ++step;
+        }
+        int q = p - base;
+        int k = lower ? step : nsb - 1 - step;
+        ph.k = k;
+        if (q == 0) {
+            ph.i = k;
+            ph.kind = 0;
+        } else {
+            ph.i = lower ? k + q : q - 1;
+            ph.kind = 1;
+        }
+        ph.first = true;
+        ph.last = (q == nsb - 1 - step);  // last phase of this step
+    } else {
+        // output block i (bottom-up for lower): diagonal, then its sources
+        int step = 0, base = 0;
+        auto cnt = [&](int s) { return nsb - s; };  // diagonal + sources (k < i lower, k > i upper)
+        while (p >= base + cnt(step)) {
+            base += cnt(step);
+            ++step;
+        }
+        int q = p - base;
+        int i = lower ? nsb - 1 - step : step;
+        ph.i = i;
+        if (q == 0) {
+            ph.k = i;
+            ph.kind = 0;
+        } else {
+            ph.k = lower ? q - 1 : i + q;
+            ph.kind = 1;
+        }
+        ph.first = (q == 0);
+        ph.last = (q == cnt(step) - 1);
+    }
+    return ph;
+}
+
+__device__ inline int phase_count(int nsb, bool trmm) {
+    (void)trmm;
+    return nsb * (nsb + 1) / 2;
+}
+
+// Solve / multiply one strip of W columns (starting at c0) of one diagonal
+// panel; bs is the LDS strip (W x LDB).
+template <typename T, int NBIG, int WS, bool TRMM>
+__device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict__ Mp, int64_t mrs, int64_t mcs,
+                            bool lower, T *__restrict__ Bp, int64_t brs, int64_t bcs, const T *__restrict__ Wp,
+                            int64_t wrs, int64_t wcs, T scale, NLA_LDS T *bs) {
+    typedef PanelShape<T, WS> PS;
+    typedef typename PS::MF MF;
     constexpr int W = PS::W;
-    constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);  // LDS row stride of one strip column
-    constexpr int NKS = 64 / MF::KD;                      // k-steps per 64-deep phase
-    __shared__ __attribute__((aligned(16))) T bs_raw[W * LDB];
-    NLA_LDS T *bs = (NLA_LDS T *)bs_raw;
+    constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
+    constexpr int NKS = 64 / MF::KD;  // k-steps per 64-deep phase
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * W;
     const int nsb = (nbig + 63) / 64;
 
     // ---- load the strip (coalesced along B's contiguous direction) ----
-    {
-        const bool rowc = (brs == 1 || brs == -1);
+    const bool rowc = (brs == 1 || brs == -1);
+    const int nrow = nsb * 64;  // rows past nbig up to the 64-boundary are zeroed
 #pragma unroll 8
-        for (int e = t; e < W * NBIG; e += 256) {
-            int r = rowc ? e % NBIG : e / W;
-            int c = rowc ? e / NBIG : e % W;
-            int64_t gc = c0 + c;
-            bool ok = r < nbig && gc < m;
-            T v = *(ok ? Bp + r * brs + gc * bcs : Bp);
-            bs[c * LDB + r] = ok ? (TRMM ? v : v * scale) : T(0);
-        }
+    for (int e = t; e < W * nrow; e += 256) {
+        int r = rowc ? e % nrow : e / W;
+        int c = rowc ? e / nrow : e % W;
+        int64_t gc = c0 + c;
+        bool ok = r < nbig && gc < m;
+        T v = *(ok ? Bp + r * brs + gc * bcs : Bp);
+        bs[c * LDB + r] = ok ? (TRMM ? v : v * scale) : T(0);
     }
     __syncthreads();
 
     const int lo = lane % MF::BD, lk = lane / MF::BD;
     const int rb = PS::rb0(wave), cbase = PS::cb0(wave);
     typename MF::acc_t acc[PS::CBW];
+    auto bsize = [&](int b) { return (b == nsb - 1) ? nbig - 64 * b : 64; };
 
-    // acc += op * Bs[rows of block kb]; op(i, k) = A[i*ars + k*acs] for
-    // i < ni, k < nk (zero outside, and outside the triangle when tri != 0:
-    // 1 keep k <= i, 2 keep k >= i).
-    auto phase = [&](const T *A, int64_t ars, int64_t acs, int ni, int nk, int kb, int tri) {
-        T af[NKS];
+    // M-operand fragment of phase ph: op(i, k) for this lane's row, zero
+    // outside the block and (TRMM diagonal) outside the triangle.
+    auto fetch = [&](const Phase &ph, T *af) {
+        const T *A;
+        int64_t ars, acs;
+        int tri = 0;
+        if (ph.kind == 0 && !TRMM) {
+            A = Wp + (int64_t)ph.k * 64 * 64;
+            ars = wrs;
+            acs = wcs;
+        } else {
+            A = Mp + (int64_t)(64 * ph.i) * mrs + (int64_t)(64 * ph.k) * mcs;
+            ars = mrs;
+            acs = mcs;
+            if (ph.kind == 0) tri = lower ? 1 : 2;
+        }
+        const int ni = bsize(ph.i), nk = bsize(ph.k);
         const int i = rb * MF::BD + lo;
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
@@ -92,7 +173,9 @@ __global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T
             T v = *(ok ? A + i * ars + k * acs : A);
             af[s] = ok ? v : T(0);
         }
-        const NLA_LDS T *src = bs + kb * 64;
+    };
+    auto compute = [&](const Phase &ph, const T *af) {
+        const NLA_LDS T *src = bs + ph.k * 64;
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
 #pragma unroll
@@ -103,11 +186,6 @@ __global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T
             }
         }
     };
-    auto zero_acc = [&]() {
-#pragma unroll
-        for (int cb = 0; cb < PS::CBW; ++cb) acc[cb] = MF::zero();
-    };
-    // Bs[block ib rows] = alpha * acc + beta * Bs[...] for this wave's tile
     auto write_block = [&](int ib, T alpha, T beta) {
         NLA_LDS T *dst = bs + ib * 64;
 #pragma unroll
@@ -121,70 +199,143 @@ __global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T
                 *p = (beta == T(0)) ? v : v + beta * *p;
             }
     };
-    auto bsize = [&](int b) { return (b == nsb - 1) ? nbig - 64 * b : 64; };
-    const T *Wb = Wp;
-
-    if (!TRMM) {
-        for (int step = 0; step < nsb; ++step) {
-            const int k = lower ? step : nsb - 1 - step;
-            // X_k = inv(M_kk) B_k
-            zero_acc();
-            phase(Wb + (int64_t)k * 64 * 64, wrs, wcs, bsize(k), bsize(k), k, 0);
-            __syncthreads();
-            write_block(k, T(1), T(0));
-            __syncthreads();
-            // B_i -= M_ik X_k for the blocks still to be solved
-            const int i0 = lower ? k + 1 : 0, i1 = lower ? nsb : k;
-            for (int i = i0; i < i1; ++i) {
-                zero_acc();
-                phase(Mp + (int64_t)(64 * i) * mrs + (int64_t)(64 * k) * mcs, mrs, mcs, bsize(i), bsize(k), k, 0);
-                write_block(i, T(-1), T(1));
+    auto run = [&](const Phase &ph, const T *af) {
+        if (ph.first) {
+#pragma unroll
+            for (int cb = 0; cb < PS::CBW; ++cb) acc[cb] = MF::zero();
+        }
+        compute(ph, af);
+        if (!TRMM) {
+            if (ph.kind == 0) {  // X_k = inv(M_kk) B_k, in place
+                __syncthreads();
+                write_block(ph.k, T(1), T(0));
+                __syncthreads();
+            } else {
+                write_block(ph.i, T(-1), T(1));
             }
-            __syncthreads();
-        }
-    } else {
-        for (int step = 0; step < nsb; ++step) {
-            const int i = lower ? nsb - 1 - step : step;
-            zero_acc();
-            phase(Mp + (int64_t)(64 * i) * (mrs + mcs), mrs, mcs, bsize(i), bsize(i), i, lower ? 1 : 2);
-            const int k0 = lower ? 0 : i + 1, k1 = lower ? i : nsb;
-            for (int k = k0; k < k1; ++k)
-                phase(Mp + (int64_t)(64 * i) * mrs + (int64_t)(64 * k) * mcs, mrs, mcs, bsize(i), bsize(k), k, 0);
+            if (ph.last) __syncthreads();
+        } else if (ph.last) {
             __syncthreads();  // every wave has read B_i
-            write_block(i, T(1), T(0));
+            write_block(ph.i, T(1), T(0));
             __syncthreads();
         }
+    };
+
+    // phases with the M-operand of the next phase in flight (two register
+    // buffers, the loop body written twice so both stay statically indexed)
+    const int np = phase_count(nsb, TRMM);
+    T fa[NKS], fb[NKS];
+    Phase pa = phase_of(0, nsb, lower, TRMM), pb;
+    fetch(pa, fa);
+    for (int p = 0; p < np; p += 2) {
+        const bool hb = p + 1 < np;
+        if (hb) {
+            pb = phase_of(p + 1, nsb, lower, TRMM);
+            fetch(pb, fb);
+        }
+        run(pa, fa);
+        if (!hb) break;
+        const bool ha = p + 2 < np;
+        if (ha) {
+            pa = phase_of(p + 2, nsb, lower, TRMM);
+            fetch(pa, fa);
+        }
+        run(pb, fb);
     }
 
     // ---- write the strip back ----
-    {
-        const bool rowc = (brs == 1 || brs == -1);
 #pragma unroll 8
-        for (int e = t; e < W * NBIG; e += 256) {
-            int r = rowc ? e % NBIG : e / W;
-            int c = rowc ? e / NBIG : e % W;
-            int64_t gc = c0 + c;
-            if (r < nbig && gc < m) Bp[r * brs + gc * bcs] = TRMM ? bs[c * LDB + r] * scale : bs[c * LDB + r];
-        }
+    for (int e = t; e < W * nrow; e += 256) {
+        int r = rowc ? e % nrow : e / W;
+        int c = rowc ? e / nrow : e % W;
+        int64_t gc = c0 + c;
+        if (r < nbig && gc < m) Bp[r * brs + gc * bcs] = TRMM ? bs[c * LDB + r] * scale : bs[c * LDB + r];
     }
+}
+
+template <typename T, int NBIG, int WS, bool TRMM>
+__global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T *__restrict__ Mp, int64_t mrs,
+                                                    int64_t mcs, int lower, T *__restrict__ Bp, int64_t brs,
+                                                    int64_t bcs, const T *__restrict__ Wp, int64_t wrs,
+                                                    int64_t wcs, T scale) {
+    constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
+    __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
+    panel_strip<T, NBIG, WS, TRMM>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, Bp, brs, bcs, Wp, wrs,
+                                   wcs, scale, (NLA_LDS T *)bs_raw);
 }
 
 template <typename T>
 hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s) {
     if (a.m <= 0 || a.nbig <= 0) return hipSuccess;
     if (a.nbig > PANEL_MAX) return hipErrorInvalidValue;
-    constexpr int W = PanelShape<T>::W;
+    constexpr int W = PANEL_STRIP;
     const int64_t groups = (a.m + W - 1) / W;
     if (a.trmm)
-        hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig, a.m,
-                           a.M, a.mrs, a.mcs, a.lower ? 1 : 0, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+        hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
+                           a.m, a.M, a.mrs, a.mcs, a.lower ? 1 : 0, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     else
-        hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
+        hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
                            a.m, a.M, a.mrs, a.mcs, a.lower ? 1 : 0, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     return hipGetLastError();
 }
 
 template hipError_t launch_panel<float>(const PanelArgs<float> &, hipStream_t);
 template hipError_t launch_panel<double>(const PanelArgs<double> &, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Persistent batched solver: work-groups pull (system, strip) tasks from a
+// queue ordered by decreasing cost until it is empty.  Every system is one
+// diagonal panel of up to BATCH_MAX rows.
+
+template <typename T, bool TRMM>
+__global__ __launch_bounds__(256) void batch_kernel(const BatchSys<T> *__restrict__ sys,
+                                                    const BatchTask *__restrict__ tasks, int ntasks,
+                                                    int *__restrict__ counter) {
+    constexpr int WS = BatchStrip<T>::W;
+    constexpr int LDB = BATCH_MAX + (sizeof(T) == 8 ? 2 : 4);
+    __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
+    __shared__ int s_idx;
+    for (;;) {
+        if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1);
+        __syncthreads();
+        const int idx = s_idx;
+        __syncthreads();
+        if (idx >= ntasks) break;  // every wave of the group sees the same idx
+        const BatchTask tk = tasks[idx];
+        const BatchSys<T> &S = sys[tk.sys];
+        panel_strip<T, BATCH_MAX, WS, TRMM>(S.n, S.m, (int64_t)tk.strip * WS, S.M, S.mrs, S.mcs, S.lower != 0, S.B,
+                                            S.brs, S.bcs, S.W, S.wrs, S.wcs, S.scale, (NLA_LDS T *)bs_raw);
+        __syncthreads();
+    }
+}
+
+__global__ void zero_int_kernel(int *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+template <typename T>
+hipError_t launch_batch(const BatchSys<T> *sys, const BatchTask *tasks, int ntasks, int *counter, bool trmm,
+                        int groups, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, counter);
+    if (trmm)
+        hipLaunchKernelGGL((batch_kernel<T, true>), dim3(groups), dim3(256), 0, s, sys, tasks, ntasks, counter);
+    else
+        hipLaunchKernelGGL((batch_kernel<T, false>), dim3(groups), dim3(256), 0, s, sys, tasks, ntasks, counter);
+    return hipGetLastError();
+}
+
+template <typename T>
+const void *batch_kernel_ptr(bool trmm) {
+    return trmm ? (const void *)batch_kernel<T, true> : (const void *)batch_kernel<T, false>;
+}
+
+template const void *batch_kernel_ptr<float>(bool);
+template const void *batch_kernel_ptr<double>(bool);
+
+template hipError_t launch_batch<float>(const BatchSys<float> *, const BatchTask *, int, int *, bool, int,
+                                        hipStream_t);
+template hipError_t launch_batch<double>(const BatchSys<double> *, const BatchTask *, int, int *, bool, int,
+                                         hipStream_t);
 
 }  // namespace nla

@@ -19,6 +19,12 @@ _i64 = ctypes.c_int64
 _dbl = ctypes.c_double
 _vp = ctypes.c_void_p
 
+class BatchItem(ctypes.Structure):
+    """nla_batch_item of include/nla.h."""
+    _fields_ = [("n", ctypes.c_int64), ("m", ctypes.c_int64), ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("alpha", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); must list every function of include/nla.h
 SIGNATURES = {
     "nla_abi_version": (_i32, []),
@@ -28,6 +34,11 @@ SIGNATURES = {
     "nla_gemm_sub": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_gemm": (_i32, [_i32, _c_char, _c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
     "nla_leaf": (_i32, [_c_char, _c_char, _c_char, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_batch_workspace_bytes": (_i64, [_c_char, _i32, ctypes.POINTER(BatchItem), _i64]),
+    "nla_batch_plan_create": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, ctypes.POINTER(BatchItem), _i64, _vp,
+                                     _i64, ctypes.POINTER(_vp)]),
+    "nla_batch_plan_execute": (_i32, [_vp, _vp, _i32]),
+    "nla_batch_plan_destroy": (_i32, [_vp]),
     "nla_set_config": (_i32, [_i32, _i32]),
     "nla_prof_enable": (_i32, [_i32]),
     "nla_prof_summary": (_i32, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),

@@ -92,3 +92,20 @@ def test_python_mirror_validates_before_device_work(nla):
         nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, B)
     with pytest.raises(ValueError):
         nla.trsm("L", "L", "N", "U", A, B)
+
+
+def test_batch_validation(nla):
+    import ctypes
+    from nextla_amd import _lib
+    lib = _lib.load()
+    items = (_lib.BatchItem * 2)()
+    items[0].n, items[0].m, items[0].A, items[0].lda, items[0].B, items[0].ldb, items[0].alpha = 8, 4, 64, 8, 64, 8, 1.0
+    items[1].n, items[1].m, items[1].A, items[1].lda, items[1].B, items[1].ldb, items[1].alpha = 2000, 4, 64, 2000, 64, 2000, 1.0
+    need = lib.nla_batch_workspace_bytes(b"S", 1, items, 1)
+    assert need > 0
+    plan = ctypes.c_void_p()
+    assert lib.nla_batch_plan_create(b"X", b"L", b"N", b"S", 1, items, 1, 256, need, ctypes.byref(plan)) == -1
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 2, 256, need, ctypes.byref(plan)) == -6
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 256, need - 1, ctypes.byref(plan)) == -9
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 264, need, ctypes.byref(plan)) == -8
+    assert lib.nla_batch_plan_destroy(None) == 0

@@ -1,0 +1,107 @@
+"""Batched solves / multiplies (nla_batch_*): parity with one plain call per
+system, the oracle's error bounds, bitwise equality of a system alone and
+inside a mixed batch, and hipGraph replays against the eager launch."""
+import itertools
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(DEV).t()
+
+
+def make_system(rng, n, m, side, uplo, dt):
+    A = np.tril(rng.uniform(-1, 1, (n, n)) / max(n, 1)) + np.diag(1 + rng.random(n))
+    if uplo == "U":
+        A = A.T.copy()
+    B = rng.standard_normal((n, m) if side == "L" else (m, n))
+    return A.astype(dt), B.astype(dt)
+
+
+SHAPES = [(1, 1), (16, 5), (64, 16), (65, 33), (100, 1), (200, 100), (512, 17), (700, 40), (1024, 3)]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_batch_all_combos(nla, dt):
+    from nextla_amd import batch
+    rng = np.random.default_rng(5)
+    eps = np.finfo(dt).eps
+    for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
+        hostA, hostB, As, Bs, refs = [], [], [], [], []
+        for n, m in SHAPES:
+            A, B = make_system(rng, n, m, side, uplo, dt)
+            hostA.append(A)
+            hostB.append(B)
+            As.append(dev(A))
+            Bs.append(dev(B))
+            # one plain call per system (the reference's usage pattern)
+            Bp = dev(B)
+            nla.unified_rectrxm_(side, uplo, trans, 1.25, func, dev(A), Bp)
+            refs.append(Bp)
+        batch.batched_rectrxm_(side, uplo, trans, func, As, Bs, alphas=1.25)
+        torch.cuda.synchronize()
+        for A, B, X, R in zip(hostA, hostB, Bs, refs):
+            n = A.shape[0]
+            Xh = X.cpu().numpy()
+            e = (O.backward_error if func == "S" else O.product_error)(side, uplo, trans, 1.25, A, B, Xh)
+            assert e <= 10 * n * eps, (side, uplo, trans, func, n, e)
+            assert O.rel_error(Xh, R.cpu().numpy()) <= 100 * n * eps, (side, uplo, trans, func, n)
+
+
+def test_alone_equals_in_mix(nla):
+    """A system's result does not depend on what else is in the batch."""
+    from nextla_amd import batch
+    rng = np.random.default_rng(6)
+    A, B = make_system(rng, 300, 45, "L", "L", np.float64)
+    alone = [dev(B)]
+    batch.batched_rectrxm_("L", "L", "N", "S", [dev(A)], alone)
+    As, Bs = [], []
+    for n, m in [(1000, 7), (300, 45), (33, 200), (512, 512), (5, 5)]:
+        if (n, m) == (300, 45):
+            As.append(dev(A))
+            Bs.append(dev(B))
+            mine = len(Bs) - 1
+        else:
+            a, b = make_system(rng, n, m, "L", "L", np.float64)
+            As.append(dev(a))
+            Bs.append(dev(b))
+    batch.batched_rectrxm_("L", "L", "N", "S", As, Bs)
+    torch.cuda.synchronize()
+    assert torch.equal(alone[0], Bs[mine])
+
+
+def test_graph_replays_match_eager(nla):
+    from nextla_amd import batch
+    rng = np.random.default_rng(7)
+    systems = [make_system(rng, n, m, "L", "L", np.float64) for n, m in SHAPES]
+    As = [dev(a) for a, _ in systems]
+    B0 = [dev(b) for _, b in systems]
+    Bs = [b.clone() for b in B0]
+    plan = batch.BatchPlan("L", "L", "N", "S", As, Bs, 0.5)
+    plan.execute(use_graph=False)
+    torch.cuda.synchronize()
+    eager = [b.clone() for b in Bs]
+    for rep in range(4):
+        for b, b0 in zip(Bs, B0):
+            b.copy_(b0)  # the program's own eager work between replays
+        torch.cuda.synchronize()
+        plan.execute(use_graph=True)
+        torch.cuda.synchronize()
+        for b, e in zip(Bs, eager):
+            assert torch.equal(b, e), rep
+    plan.close()
+
+
+def test_batch_rejects_oversized_system(nla):
+    from nextla_amd import batch
+    A = torch.eye(1025, dtype=torch.float64, device=DEV)
+    B = torch.zeros(1025, 2, dtype=torch.float64, device=DEV).t().contiguous().t()
+    with pytest.raises(ValueError):
+        batch.BatchPlan("L", "L", "N", "S", [A], [B])
