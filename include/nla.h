@@ -110,9 +110,9 @@ int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int6
  *                              task queue and, for TRSM, the inverted 64x64
  *                              diagonal blocks of every system).
  *   nla_batch_plan_create      validates the items and uploads the plan into
- *                              the caller's workspace (blocks the host until
- *                              the upload is done); the plan keeps pointers,
- *                              so A_i and B_i must stay allocated.
+ *                              the caller's workspace, ordered on `stream`;
+ *                              the plan keeps pointers, so A_i and B_i must
+ *                              stay allocated.
  *   nla_batch_plan_execute     enqueues the batch on `stream`: kernels only,
  *                              no allocation, no memset, no host copy, so it
  *                              can be captured into a hipGraph; with
@@ -120,7 +120,8 @@ int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int6
  *                              on the first call and replays the graph after.
  *   nla_batch_plan_destroy     releases the host handle (and its graph).
  * Argument positions for nla_batch_plan_create: 1 side, 2 uplo, 3 trans,
- * 4 func, 5 dtype, 6 items, 7 count, 8 workspace, 9 workspace_bytes, 10 plan.
+ * 4 func, 5 dtype, 6 items, 7 count, 8 workspace, 9 workspace_bytes,
+ * 10 stream, 11 plan.
  */
 typedef struct nla_batch_item {
     int64_t n;       /* order of A_i (<= 1024) */
@@ -137,7 +138,8 @@ typedef struct nla_batch_plan_s *nla_batch_plan;
 int64_t nla_batch_workspace_bytes(char func, int32_t dtype, const nla_batch_item *items, int64_t count);
 
 int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32_t dtype, const nla_batch_item *items,
-                              int64_t count, void *workspace, int64_t workspace_bytes, nla_batch_plan *plan);
+                              int64_t count, void *workspace, int64_t workspace_bytes, void *stream,
+                              nla_batch_plan *plan);
 
 int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_graph);
 

@@ -55,6 +55,7 @@ static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, i
 }  // namespace nla
 
 struct nla_batch_plan_s {
+    std::vector<char> host;  // staged descriptors, alive until the upload has run
     int32_t dtype;
     bool trmm;
     char *ws;
@@ -69,7 +70,7 @@ namespace nla {
 
 template <typename T>
 static int32_t build(char side, char uplo, char trans, char func, const nla_batch_item *items, int64_t count,
-                     nla_batch_plan_s *P) {
+                     nla_batch_plan_s *P, hipStream_t stream) {
     const Layout &L = P->L;
     std::vector<BatchSys<T>> sys((size_t)count);
     std::vector<BatchTask> tasks;
@@ -122,11 +123,14 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
         int na = (sys[(size_t)a.sys].n + 63) / 64, nb2 = (sys[(size_t)b.sys].n + 63) / 64;
         return na * (na + 1) > nb2 * (nb2 + 1);
     });
-    hipError_t e = hipMemcpy(P->ws + L.sys, sys.data(), sys.size() * sizeof(BatchSys<T>), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !tasks.empty())
-        e = hipMemcpy(P->ws + L.tasks, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !blk.empty())
-        e = hipMemcpy(P->ws + L.blk, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice);
+    // one staged image of the descriptor region, copied in stream order
+    P->host.assign((size_t)(L.inv - L.sys), 0);
+    std::memcpy(P->host.data(), sys.data(), sys.size() * sizeof(BatchSys<T>));
+    if (!tasks.empty())
+        std::memcpy(P->host.data() + (L.tasks - L.sys), tasks.data(), tasks.size() * sizeof(BatchTask));
+    if (!blk.empty()) std::memcpy(P->host.data() + (L.blk - L.sys), blk.data(), blk.size() * sizeof(int2));
+    // stream-ordered: no wait on other streams; the image lives in the plan
+    hipError_t e = hipMemcpyAsync(P->ws + L.sys, P->host.data(), P->host.size(), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return report_hip(e, "batch plan upload");
     P->ntasks = (int)tasks.size();
     P->nblk = (int)blk.size();
@@ -167,7 +171,8 @@ int64_t nla_batch_workspace_bytes(char func, int32_t dtype, const nla_batch_item
 }
 
 int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32_t dtype, const nla_batch_item *items,
-                              int64_t count, void *workspace, int64_t workspace_bytes, nla_batch_plan *plan) {
+                              int64_t count, void *workspace, int64_t workspace_bytes, void *stream,
+                              nla_batch_plan *plan) {
     side = upc(side); uplo = upc(uplo); trans = upc(trans); func = upc(func);
     if (side != 'L' && side != 'R') return report_bad_arg(1, "side");
     if (uplo != 'L' && uplo != 'U') return report_bad_arg(2, "uplo");
@@ -183,7 +188,7 @@ int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32
         if (!it.A || !it.B || it.lda < it.n || it.ldb < (side == 'L' ? it.n : it.m))
             return report_bad_arg(6, "items (pointer or leading dimension)");
     }
-    if (plan == nullptr) return report_bad_arg(10, "plan");
+    if (plan == nullptr) return report_bad_arg(11, "plan");
     Layout L = layout_of(func, dtype, items, count);
     if (workspace == nullptr || workspace_bytes < L.total) return report_bad_arg(9, "workspace_bytes");
     if (((uintptr_t)workspace & 255) != 0) return report_bad_arg(8, "workspace (needs 256-byte alignment)");
@@ -193,8 +198,9 @@ int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32
     P->trmm = (func == 'M');
     P->ws = (char *)workspace;
     P->L = L;
-    int32_t st = dtype == NLA_F64 ? build<double>(side, uplo, trans, func, items, count, P)
-                                  : build<float>(side, uplo, trans, func, items, count, P);
+    hipStream_t s = (hipStream_t)stream;
+    int32_t st = dtype == NLA_F64 ? build<double>(side, uplo, trans, func, items, count, P, s)
+                                  : build<float>(side, uplo, trans, func, items, count, P, s);
     if (st != NLA_OK) {
         delete P;
         return st;

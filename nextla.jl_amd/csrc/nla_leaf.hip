@@ -142,6 +142,96 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
     }
 }
 
+// Inverse of one 64 x 64 triangular block with 256 threads, in three stages
+// (the blocked scheme of LAPACK's trtri):
+//   1. the four 16 x 16 diagonal sub-blocks by substitution on the columns of
+//      the identity (64 lanes, one column each, 16-step chains);
+//   2. 32 x 32 inverses: inv([A 0; C B]) = [inv A, 0; -inv B * C * inv A, inv B];
+//   3. the same step from 32 to 64.
+// sl: the canonical lower block (column-major, 64 x 64); si: the inverse
+// being built; st: 32 x 32 scratch.  Upper blocks are inverted as J U J.
+template <typename T>
+__device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
+                              T *__restrict__ W, T *sl, T *si, T *st) {
+    constexpr int NB = 64, NT = 256;
+    const int t = threadIdx.x;
+    const int64_t off = blk * NB;
+    const int nb = (int)(n - off < NB ? n - off : NB);
+    const T *base = lower ? A + off * (rs + cs) : A + (off + nb - 1) * (rs + cs);
+    const int64_t ars = lower ? rs : -rs, acs = lower ? cs : -cs;
+    const bool arow = (ars == 1 || ars == -1);
+    // stage L (identity padding past nb); zero the inverse
+#pragma unroll 4
+    for (int e = t; e < NB * NB; e += NT) {
+        int i = arow ? e % NB : e / NB;
+        int j = arow ? e / NB : e % NB;
+        bool ok = i < nb && j < nb && j <= i;
+        T v = *(ok ? base + i * ars + j * acs : base);
+        sl[j * NB + i] = ok ? v : (i == j ? T(1) : T(0));
+        si[e] = T(0);
+    }
+    __syncthreads();
+    // 1. sub-block b, column c: y = inv(D_b) e_c
+    if (t < 64) {
+        const int b = t >> 4, c = t & 15, o = 16 * b;
+        T x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = (i == c) ? T(1) : T(0);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            x[j] = x[j] / sl[(o + j) * NB + o + j];
+#pragma unroll
+            for (int r = j + 1; r < 16; ++r) x[r] -= sl[(o + j) * NB + o + r] * x[j];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) si[(o + c) * NB + o + i] = x[i];
+    }
+    __syncthreads();
+    // 2. two 32-blocks p: st_p = C_p * inv(D_2p) (16 x 16), then
+    //    si[32p+16.., 32p..] = -inv(D_2p+1) * st_p
+    for (int e = t; e < 2 * 256; e += NT) {
+        int p = e >> 8, q = e & 255, i = q & 15, j = q >> 4, o = 32 * p;
+        T acc = T(0);
+        for (int k = j; k < 16; ++k) acc += sl[(o + k) * NB + o + 16 + i] * si[(o + j) * NB + o + k];
+        st[p * 256 + j * 16 + i] = acc;
+    }
+    __syncthreads();
+    for (int e = t; e < 2 * 256; e += NT) {
+        int p = e >> 8, q = e & 255, i = q & 15, j = q >> 4, o = 32 * p;
+        T acc = T(0);
+        for (int k = 0; k <= i; ++k) acc += si[(o + 16 + k) * NB + o + 16 + i] * st[p * 256 + j * 16 + k];
+        si[(o + j) * NB + o + 16 + i] = -acc;
+    }
+    __syncthreads();
+    // 3. st = Q * inv(P) (32 x 32, inv(P) lower), si[32.., 0..32] = -inv(R) * st
+    for (int e = t; e < 1024; e += NT) {
+        int i = e & 31, j = e >> 5;
+        T acc = T(0);
+        for (int k = j; k < 32; ++k) acc += sl[k * NB + 32 + i] * si[j * NB + k];
+        st[j * 32 + i] = acc;
+    }
+    __syncthreads();
+    for (int e = t; e < 1024; e += NT) {
+        int i = e & 31, j = e >> 5;
+        T acc = T(0);
+        for (int k = 0; k <= i; ++k) acc += si[(32 + k) * NB + 32 + i] * st[j * 32 + k];
+        si[j * NB + 32 + i] = -acc;
+    }
+    __syncthreads();
+    // write: column-major 64 x 64; an upper block's inverse is J inv(L) J
+    T *w = W + blk * NB * NB;
+#pragma unroll 4
+    for (int e = t; e < NB * NB; e += NT) {
+        int i = e % NB, j = e / NB;  // element (i, j) of the stored inverse
+        T v;
+        if (lower || i >= nb || j >= nb)
+            v = (i < nb && j < nb) || lower ? si[j * NB + i] : (i == j ? T(1) : T(0));
+        else
+            v = si[(nb - 1 - j) * NB + (nb - 1 - i)];
+        w[e] = v;
+    }
+}
+
 // Inverse of one NB x NB diagonal block per work-group, by forward
 // substitution on the columns of the identity (one column per lane, the
 // block in LDS).  Each column of the inverse is a backward-stable triangular
@@ -218,12 +308,23 @@ __global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__
     trtri_block<T, NB>(n, blockIdx.x, A, rs, cs, lower != 0, W, sl, srinv);
 }
 
+// The 64-block inverse used by the recursion and the batched solver.
+template <typename T>
+__global__ __launch_bounds__(256) void trtri64_kernel(int64_t n, const T *__restrict__ A, int64_t rs, int64_t cs,
+                                                      int lower, T *__restrict__ W) {
+    __shared__ __attribute__((aligned(16))) T sl[64 * 64];
+    __shared__ __attribute__((aligned(16))) T si[64 * 64];
+    __shared__ __attribute__((aligned(16))) T st[32 * 32];
+    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, W, sl, si, st);
+}
+
 // Batched form: one work-group per (system, block) pair.
 template <typename T>
-__global__ __launch_bounds__(LEAF_COLS) void batch_trtri_kernel(const BatchSys<T> *__restrict__ sys,
-                                                                const int2 *__restrict__ blk) {
+__global__ __launch_bounds__(256) void batch_trtri_kernel(const BatchSys<T> *__restrict__ sys,
+                                                         const int2 *__restrict__ blk) {
     __shared__ __attribute__((aligned(16))) T sl[64 * 64];
-    __shared__ T srinv[64];
+    __shared__ __attribute__((aligned(16))) T si[64 * 64];
+    __shared__ __attribute__((aligned(16))) T st[32 * 32];
     const int2 b = blk[blockIdx.x];
     const BatchSys<T> &S = sys[b.x];
     // the diagonal blocks of the effective matrix in its natural orientation:
@@ -232,13 +333,13 @@ __global__ __launch_bounds__(LEAF_COLS) void batch_trtri_kernel(const BatchSys<T
     const bool right = (S.wrs != 1);
     const int64_t rs = right ? S.mcs : S.mrs, cs = right ? S.mrs : S.mcs;
     const bool lower = right ? (S.lower == 0) : (S.lower != 0);
-    trtri_block<T, 64>(S.n, b.y, S.M, rs, cs, lower, (T *)S.W, sl, srinv);
+    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, (T *)S.W, sl, si, st);
 }
 
 template <typename T>
 hipError_t launch_batch_trtri(const BatchSys<T> *sys, const int2 *blk, int nblk, hipStream_t s) {
     if (nblk <= 0) return hipSuccess;
-    hipLaunchKernelGGL(batch_trtri_kernel<T>, dim3(nblk), dim3(LEAF_COLS), 0, s, sys, blk);
+    hipLaunchKernelGGL(batch_trtri_kernel<T>, dim3(nblk), dim3(256), 0, s, sys, blk);
     return hipGetLastError();
 }
 
@@ -251,8 +352,8 @@ hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + nb - 1) / nb;
     if (nb == 64)
-        hipLaunchKernelGGL((trtri_kernel<T, 64>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
-                           lower ? 1 : 0, W);
+        hipLaunchKernelGGL((trtri64_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, n, A, rs, cs, lower ? 1 : 0,
+                           W);
     else if (nb == 32)
         hipLaunchKernelGGL((trtri_kernel<T, 32>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
                            lower ? 1 : 0, W);

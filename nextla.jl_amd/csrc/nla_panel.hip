@@ -146,9 +146,12 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
     typename MF::acc_t acc[PS::CBW];
     auto bsize = [&](int b) { return (b == nsb - 1) ? nbig - 64 * b : 64; };
 
-    // M-operand fragment of phase ph: op(i, k) for this lane's row, zero
-    // outside the block and (TRMM diagonal) outside the triangle.
-    auto fetch = [&](const Phase &ph, T *af) {
+    // M-operand fragment of phase ph: op(i, k) for this lane's row.  Elements
+    // outside the block (or, for the TRMM diagonal, outside the triangle) are
+    // read from a valid address and flagged in *okb; they are zeroed when the
+    // fragment is used, one phase later, so no instruction waits for the
+    // loads before then.
+    auto fetch = [&](const Phase &ph, T *af, uint32_t *okb) {
         const T *A;
         int64_t ars, acs;
         int tri = 0;
@@ -164,25 +167,28 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
         }
         const int ni = bsize(ph.i), nk = bsize(ph.k);
         const int i = rb * MF::BD + lo;
+        uint32_t bits = 0;
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
             const int k = s * MF::KD + lk;
             bool ok = i < ni && k < nk;
             if (tri == 1) ok = ok && k <= i;
             if (tri == 2) ok = ok && k >= i;
-            T v = *(ok ? A + i * ars + k * acs : A);
-            af[s] = ok ? v : T(0);
+            af[s] = *(ok ? A + i * ars + k * acs : A);
+            bits |= (ok ? 1u : 0u) << s;
         }
+        *okb = bits;
     };
-    auto compute = [&](const Phase &ph, const T *af) {
+    auto compute = [&](const Phase &ph, const T *af, uint32_t okb) {
         const NLA_LDS T *src = bs + ph.k * 64;
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
+            const T a = ((okb >> s) & 1u) ? af[s] : T(0);
 #pragma unroll
             for (int cb = 0; cb < PS::CBW; ++cb) {
                 const int col = (cbase + cb) * MF::BD + lo;
                 T bv = src[col * LDB + s * MF::KD + lk];
-                acc[cb] = MF::mma(af[s], bv, acc[cb]);
+                acc[cb] = MF::mma(a, bv, acc[cb]);
             }
         }
     };
@@ -199,12 +205,12 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
                 *p = (beta == T(0)) ? v : v + beta * *p;
             }
     };
-    auto run = [&](const Phase &ph, const T *af) {
+    auto run = [&](const Phase &ph, const T *af, uint32_t okb) {
         if (ph.first) {
 #pragma unroll
             for (int cb = 0; cb < PS::CBW; ++cb) acc[cb] = MF::zero();
         }
-        compute(ph, af);
+        compute(ph, af, okb);
         if (!TRMM) {
             if (ph.kind == 0) {  // X_k = inv(M_kk) B_k, in place
                 __syncthreads();
@@ -225,22 +231,23 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
     // buffers, the loop body written twice so both stay statically indexed)
     const int np = phase_count(nsb, TRMM);
     T fa[NKS], fb[NKS];
+    uint32_t oka = 0, okb = 0;
     Phase pa = phase_of(0, nsb, lower, TRMM), pb;
-    fetch(pa, fa);
+    fetch(pa, fa, &oka);
     for (int p = 0; p < np; p += 2) {
         const bool hb = p + 1 < np;
         if (hb) {
             pb = phase_of(p + 1, nsb, lower, TRMM);
-            fetch(pb, fb);
+            fetch(pb, fb, &okb);
         }
-        run(pa, fa);
+        run(pa, fa, oka);
         if (!hb) break;
         const bool ha = p + 2 < np;
         if (ha) {
             pa = phase_of(p + 2, nsb, lower, TRMM);
-            fetch(pa, fa);
+            fetch(pa, fa, &oka);
         }
-        run(pb, fb);
+        run(pb, fb, okb);
     }
 
     // ---- write the strip back ----

@@ -36,7 +36,7 @@ SIGNATURES = {
     "nla_leaf": (_i32, [_c_char, _c_char, _c_char, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_batch_workspace_bytes": (_i64, [_c_char, _i32, ctypes.POINTER(BatchItem), _i64]),
     "nla_batch_plan_create": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, ctypes.POINTER(BatchItem), _i64, _vp,
-                                     _i64, ctypes.POINTER(_vp)]),
+                                     _i64, _vp, ctypes.POINTER(_vp)]),
     "nla_batch_plan_execute": (_i32, [_vp, _vp, _i32]),
     "nla_batch_plan_destroy": (_i32, [_vp]),
     "nla_set_config": (_i32, [_i32, _i32]),

@@ -23,7 +23,7 @@ class BatchPlan:
     (the plan keeps their addresses); execute() may be called repeatedly,
     e.g. after refilling the B_i in place."""
 
-    def __init__(self, side, uplo, trans, func, As, Bs, alphas=1.0):
+    def __init__(self, side, uplo, trans, func, As, Bs, alphas=1.0, stream=None):
         side, uplo, trans, func = side.upper(), uplo.upper(), trans.upper(), func.upper()
         if len(As) != len(Bs):
             raise ValueError("As and Bs differ in length")
@@ -68,8 +68,9 @@ class BatchPlan:
         self.workspace = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
         self._plan = ctypes.c_void_p()
         self._keep = (list(As), list(Bs))
+        s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         st = lib.nla_batch_plan_create(_lib.ch(side), _lib.ch(uplo), _lib.ch(trans), _lib.ch(func), code, items,
-                                       count, self.workspace.data_ptr(), nbytes, ctypes.byref(self._plan))
+                                       count, self.workspace.data_ptr(), nbytes, s, ctypes.byref(self._plan))
         _lib.check(st, "nla_batch_plan_create")
         self.count = count
 

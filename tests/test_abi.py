@@ -104,8 +104,10 @@ def test_batch_validation(nla):
     need = lib.nla_batch_workspace_bytes(b"S", 1, items, 1)
     assert need > 0
     plan = ctypes.c_void_p()
-    assert lib.nla_batch_plan_create(b"X", b"L", b"N", b"S", 1, items, 1, 256, need, ctypes.byref(plan)) == -1
-    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 2, 256, need, ctypes.byref(plan)) == -6
-    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 256, need - 1, ctypes.byref(plan)) == -9
-    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 264, need, ctypes.byref(plan)) == -8
+    assert lib.nla_batch_plan_create(b"X", b"L", b"N", b"S", 1, items, 1, 256, need, None, ctypes.byref(plan)) == -1
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 2, 256, need, None, ctypes.byref(plan)) == -6
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 256, need - 1, None,
+                                     ctypes.byref(plan)) == -9
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 264, need, None, ctypes.byref(plan)) == -8
+    assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 256, need, None, None) == -11
     assert lib.nla_batch_plan_destroy(None) == 0
