@@ -16,7 +16,11 @@
  *   - Return value: 0 on success; -k when argument k (1-based) is invalid,
  *     checked before any device work, so B is untouched on error; a positive
  *     NLA_ERR_* code when the HIP runtime reports a failure.
- *   - The plain entry points allocate no device memory.
+ *   - Device memory: the plain TRSM path takes one buffer for the inverses
+ *     of its diagonal 64-blocks (n*64 elements) from the stream-ordered pool
+ *     (hipMallocAsync) and returns it on the same stream; TRMM, the GEMM
+ *     entry points and the leaves allocate nothing.  The batched and
+ *     mixed-precision forms work in a caller-provided workspace.
  */
 #ifndef NLA_H
 #define NLA_H
@@ -146,13 +150,56 @@ int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_gr
 int32_t nla_batch_plan_destroy(nla_batch_plan plan);
 
 /*
+ * Mixed-precision TRSM with FP64 iterative refinement (no counterpart in the
+ * reference; the FP64 solution it converges to is that of unified_rectrxm!'s
+ * TRSM, src/rectrxm.jl:43-76).  Solves
+ * op(A)·X = alpha·B (side 'L') or X·op(A) = alpha·B (side 'R') for FP64 A, B:
+ *   - correction solves run the recursion on an fp32 copy of A's triangle,
+ *     with fp32 diagonal blocks and GEMM updates in `precision`:
+ *     NLA_LP_BF16 / NLA_LP_FP16 round both operands after a power-of-two
+ *     scaling per operand block and accumulate in fp32 on MFMA;
+ *     NLA_LP_FP32 is the plain fp32 update;
+ *   - every step forms R = alpha·B - op(A)·X (or alpha·B - X·op(A)) in FP64
+ *     (MFMA GEMM restricted to the triangle), and its normwise backward
+ *     error ||R||_F / (||A||_F ||X||_F + ||alpha·B||_F); the loop stops once
+ *     that is <= tol, otherwise X += the correction solve of R;
+ *   - a step that less than halves the backward error raises the update
+ *     precision for the following steps (bf16 -> fp16 -> fp32).
+ * X overwrites B.  The call synchronizes `stream` once per step (the
+ * stopping test reads two norms on the host).  NLA_ERR_NOT_CONVERGED: after
+ * max_iter (<= NLA_MIXED_MAX_ITER) correction steps the error is still above
+ * tol; B then holds the last iterate and info its error.  The workspace
+ * (nla_mixed_workspace_bytes, 256-byte aligned) holds about
+ * 4*n^2 + 20*n*m bytes.
+ * Argument positions: 1 side, 2 uplo, 3 trans, 4 precision, 5 alpha, 6 n,
+ * 7 m, 8 A, 9 lda, 10 B, 11 ldb, 12 tol, 13 max_iter, 14 workspace,
+ * 15 workspace_bytes, 16 stream, 17 info.
+ */
+enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2 };
+
+#define NLA_MIXED_MAX_ITER 30
+
+typedef struct nla_mixed_info {
+    int32_t iterations;     /* correction steps applied */
+    int32_t precision;      /* update precision of the last step (nla_precision) */
+    double backward_error;  /* of the X returned in B */
+    double history[NLA_MIXED_MAX_ITER + 1]; /* backward error after 0, 1, ... steps */
+} nla_mixed_info;
+
+int64_t nla_mixed_workspace_bytes(char side, int64_t n, int64_t m);
+
+int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, double alpha, int64_t n, int64_t m,
+                          const double *A, int64_t lda, double *B, int64_t ldb, double tol, int32_t max_iter,
+                          void *workspace, int64_t workspace_bytes, void *stream, nla_mixed_info *info);
+
+/*
  * Process-wide choice of the methods used below the recursion (no
  * counterpart in the reference, whose thresholds are constants,
  * src/rectrxm.jl:52,63).  Defaults come from the environment
  * (NLA_LEAF_MODE=subst, NLA_PANEL=0) and can be changed between calls:
  *   NLA_CFG_LEAF_MODE: 1 = inverted 64-blocks applied on MFMA (default),
  *                      0 = register/LDS substitution leaf;
- *   NLA_CFG_PANEL:     1 = diagonal blocks up to 256 solved by one fused
+ *   NLA_CFG_PANEL:     1 = diagonal blocks up to 512 solved by one fused
  *                      panel launch (default, needs leaf mode 1), 0 = recurse
  *                      to 64-leaves.
  * Returns the previous value, or -1 for an unknown key.
@@ -168,10 +215,11 @@ int32_t nla_set_config(int32_t key, int32_t value);
  *   nla_prof_enable(1) clears the records and starts recording; (0) stops.
  *   nla_prof_summary(kind, ...) waits for the recorded events and returns the
  *   launch count, summed kernel time (ms), algorithmic flops and bytes of one
- *   kernel kind: NLA_KIND_GEMM (GEMM updates) or NLA_KIND_LEAF (diagonal-block
- *   leaves).
+ *   kernel kind: NLA_KIND_GEMM (fp32/fp64 GEMM updates), NLA_KIND_LEAF
+ *   (diagonal-block leaves and panels), NLA_KIND_LP_GEMM (bf16/fp16 updates of
+ *   the mixed-precision solve) or NLA_KIND_RESIDUAL (its FP64 residuals).
  */
-enum nla_kernel_kind { NLA_KIND_GEMM = 0, NLA_KIND_LEAF = 1 };
+enum nla_kernel_kind { NLA_KIND_GEMM = 0, NLA_KIND_LEAF = 1, NLA_KIND_LP_GEMM = 2, NLA_KIND_RESIDUAL = 3 };
 
 int32_t nla_prof_enable(int32_t on);
 

@@ -24,7 +24,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
+#include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/nla.h"
@@ -150,6 +152,12 @@ struct Recursion {
     int mode = 1;
     bool use_panel = true;  // fuse diagonal blocks <= PANEL_MAX into one launch
     hipError_t err = hipSuccess;
+    // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
+    // -1 plain fp32.  Each update takes two absolute-maximum slots from amax
+    // (zeroed before the solve).
+    int lp = -1;
+    unsigned *amax = nullptr;
+    int64_t amax_used = 0, amax_cap = 0;
 
     // Leaf as an MFMA product: TRSM applies inv(A_kk) from W, TRMM applies
     // A_kk itself with the other triangle masked to zero.
@@ -250,9 +258,40 @@ struct Recursion {
         err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_leaf<T>(a, stream); });
     }
 
+    // The update with operands rounded to bf16/fp16 after power-of-two
+    // scaling per operand block (scales from the blocks' absolute maxima).
+    void lp_gemm(View<float> C, View<float> X, View<float> Y, float alpha, float beta) {
+        if (amax_used + 2 > amax_cap) {
+            err = hipErrorInvalidValue;
+            return;
+        }
+        unsigned *slot = amax + amax_used;
+        amax_used += 2;
+        LpGemmArgs g;
+        g.M = C.nr; g.N = C.nc; g.K = X.nc;
+        g.tx = (X.rs != 1); g.ldx = g.tx ? X.rs : X.cs;
+        g.ty = (Y.rs != 1); g.ldy = g.ty ? Y.rs : Y.cs;
+        g.X = X.p; g.Y = Y.p; g.C = C.p; g.ldc = C.cs;
+        g.alpha = alpha; g.beta = beta;
+        g.lp = lp;
+        g.amax = slot;
+        err = launch_absmax(X.nr, X.nc, X.p, X.rs, X.cs, slot, stream);
+        if (err == hipSuccess) err = launch_absmax(Y.nr, Y.nc, Y.p, Y.rs, Y.cs, slot + 1, stream);
+        if (err != hipSuccess) return;
+        const double fl = 2.0 * g.M * g.N * g.K;
+        const double by = ((double)g.M * g.K + (double)g.K * g.N + 2.0 * g.M * g.N) * sizeof(float);
+        err = timed(NLA_KIND_LP_GEMM, fl, by, stream, [&] { return launch_lp_gemm(g, stream); });
+    }
+
     // C = beta*C + alpha*X*Y on views (GEMM_ADD!/GEMM_SUB!, src/matmul.jl:69-80).
     void gemm(View<T> C, View<T> X, View<T> Y, T alpha, T beta) {
         if (err != hipSuccess) return;
+        if constexpr (std::is_same<T, float>::value) {
+            if (lp >= 0) {
+                lp_gemm(C, X, Y, alpha, beta);
+                return;
+            }
+        }
         GemmArgs<T> g;
         g.M = C.nr; g.N = C.nc; g.K = X.nc;
         g.tx = (X.rs != 1); g.ldx = g.tx ? X.rs : X.cs;
@@ -371,6 +410,176 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     return hip_status(r.err, "kernel launch");
 }
 
+// ---- mixed precision with FP64 refinement ---------------------------------
+
+// Workspace of nla_rectrxm_mixed: an fp32 copy of A's stored triangle and
+// the inverses of its diagonal 64-blocks (the correction solves), alpha*B and
+// the residual in FP64, the fp32 correction, the absolute-maximum slots of
+// the low-precision updates and the norm reductions.  256-byte aligned parts.
+struct MixedLayout {
+    int64_t rows, cols;  // shape of B
+    int64_t ld32, ldr;   // leading dimension of the fp32 A copy / of S, R, D
+    int64_t amax_slots;
+    int64_t a32, w, s, r, d, amax, part, norms, total;
+};
+
+static inline int64_t al256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
+    MixedLayout L{};
+    L.rows = side == 'L' ? n : m;
+    L.cols = side == 'L' ? m : n;
+    L.ld32 = (n + 63) / 64 * 64;
+    L.ldr = (L.rows + 63) / 64 * 64;
+    const int64_t nblk = (n + 63) / 64;
+    L.amax_slots = 2 * (2 * nblk + 2);  // an update per internal node of the recursion
+    int64_t o = 0;
+    L.a32 = o;   o = al256(o + L.ld32 * n * 4);
+    L.w = o;     o = al256(o + nblk * 64 * 64 * 4);
+    L.s = o;     o = al256(o + L.ldr * L.cols * 8);
+    L.r = o;     o = al256(o + L.ldr * L.cols * 8);
+    L.d = o;     o = al256(o + L.ldr * L.cols * 4);
+    L.amax = o;  o = al256(o + L.amax_slots * 4);
+    L.part = o;  o = al256(o + NORM_PARTS_MAX * 2 * 8);
+    L.norms = o; o = al256(o + 8 * 8);
+    L.total = o;
+    return L;
+}
+
+// A power of two at or just above |x| (1 for zero or non-finite x).
+static double pow2_above(double x) {
+    if (!(x > 0.0) || !std::isfinite(x)) return 1.0;
+    int e;
+    (void)std::frexp(x, &e);
+    return std::ldexp(1.0, e);
+}
+
+static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precision, double alpha, int64_t n,
+                                  int64_t m, const double *A, int64_t lda, double *B, int64_t ldb, double tol,
+                                  int max_iter, char *ws, const MixedLayout &L, hipStream_t s,
+                                  nla_mixed_info *info) {
+    float *A32 = (float *)(ws + L.a32);
+    float *W = (float *)(ws + L.w);
+    double *S = (double *)(ws + L.s), *R = (double *)(ws + L.r);
+    float *D = (float *)(ws + L.d);
+    unsigned *amax = (unsigned *)(ws + L.amax);
+    double *part = (double *)(ws + L.part), *nrm = (double *)(ws + L.norms);
+    const bool tr = (trans != 'N');
+    const char ue = tr ? (uplo == 'L' ? 'U' : 'L') : uplo;  // uplo of op(A)
+    hipError_t e = hipSuccess;
+#define NLA_CK(x)                                          \
+    do {                                                   \
+        e = (x);                                           \
+        if (e != hipSuccess) return hip_status(e, "mixed"); \
+    } while (0)
+
+    // fp32 copy of the stored triangle, viewed as op(A); inverses of its
+    // diagonal 64-blocks for the correction solves
+    NLA_CK(launch_tri_to_f32(n, A, lda, uplo == 'L', A32, L.ld32, s));
+    View<float> A32v{A32, 1, L.ld32, n, n};
+    if (tr) {
+        A32v.rs = L.ld32;
+        A32v.cs = 1;
+    }
+    NLA_CK(launch_trtri_blocks<float>(n, 64, A32v.p, A32v.rs, A32v.cs, ue == 'L', W, s));
+    // S = alpha*B; the norms of A and S enter every stopping test
+    NLA_CK(launch_scale_copy(L.rows, L.cols, alpha, B, ldb, S, L.ldr, s));
+    NLA_CK(launch_norms(n, n, A, lda, uplo == 'L' ? 1 : 2, part, nrm + 0, s));
+    NLA_CK(launch_norms(L.rows, L.cols, S, L.ldr, 0, part, nrm + 2, s));
+    double h[4];
+    NLA_CK(hipMemcpyAsync(h, nrm, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    NLA_CK(hipStreamSynchronize(s));
+    const double nA = std::sqrt(h[0]), nS = std::sqrt(h[2]);
+
+    // D <- op(A)^-1 D (or D op(A)^-1) on the fp32 copy, updates in `prec`
+    auto lp_solve = [&](int prec) -> hipError_t {
+        Recursion<float> r;
+        r.func = 'S';
+        r.side = side;
+        r.uplo = ue;
+        r.nb = 64;
+        r.stream = s;
+        r.mode = 1;
+        r.use_panel = true;
+        r.W = W;
+        if (prec != NLA_LP_FP32) {
+            r.lp = prec;
+            r.amax = amax;
+            r.amax_cap = L.amax_slots;
+            hipError_t e2 = launch_fill_u32(amax, L.amax_slots, 0u, s);
+            if (e2 != hipSuccess) return e2;
+        }
+        View<float> Dv{D, 1, L.ldr, L.rows, L.cols};
+        r.rec(A32v, Dv, n, 0, 1.0f, 1.0f);
+        return r.err;
+    };
+
+    // X0: the low-precision solve of S, scaled by a power of two so the fp32
+    // buffer holds O(1) values
+    int prec = precision;
+    double sig = pow2_above(h[3]);
+    NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, S, L.ldr, D, L.ldr, s));
+    NLA_CK(lp_solve(prec));
+    NLA_CK(launch_up(L.rows, L.cols, sig, D, L.ldr, 0.0, B, ldb, s));
+
+    // R = S - op(A) X (side L) or S - X op(A) (side R): the FP64 MFMA GEMM
+    // with the triangle of op(A) masked and the k-steps beyond it skipped
+    GemmArgs<double> g;
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.C = R;
+    g.ldc = L.ldr;
+    g.K = n;
+    if (side == 'L') {
+        g.M = n; g.N = m;
+        g.X = A; g.tx = tr; g.ldx = lda;
+        g.Y = B; g.ty = false; g.ldy = ldb;
+        g.tri = (ue == 'L') ? 1 : 2;
+    } else {
+        g.M = m; g.N = n;
+        g.X = B; g.tx = false; g.ldx = ldb;
+        g.Y = A; g.ty = tr; g.ldy = lda;
+        g.tri = (ue == 'L') ? 3 : 4;
+    }
+    const double res_fl = (double)n * n * m;
+    const double res_by = ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8;
+
+    double prev = INFINITY, bwd = INFINITY;
+    int it = 0;
+    for (;; ++it) {
+        NLA_CK(launch_scale_copy(L.rows, L.cols, 1.0, S, L.ldr, R, L.ldr, s));
+        NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, [&] { return launch_gemm<double>(g, s); }));
+        NLA_CK(launch_norms(L.rows, L.cols, R, L.ldr, 0, part, nrm + 4, s));
+        NLA_CK(launch_norms(L.rows, L.cols, B, ldb, 0, part, nrm + 6, s));
+        NLA_CK(hipMemcpyAsync(h, nrm + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        NLA_CK(hipStreamSynchronize(s));
+        const double nR = std::sqrt(h[0]), nX = std::sqrt(h[2]);
+        const double den = nA * nX + nS;
+        bwd = den > 0.0 ? nR / den : (nR > 0.0 ? INFINITY : 0.0);
+        if (info && it <= NLA_MIXED_MAX_ITER) info->history[it] = bwd;
+        if (bwd <= tol || it >= max_iter || !std::isfinite(bwd)) break;
+        // too little progress: raise the update precision for the next steps
+        if (it > 0 && bwd > 0.5 * prev && prec != NLA_LP_FP32) prec = (prec == NLA_LP_BF16) ? NLA_LP_FP16 : NLA_LP_FP32;
+        prev = bwd;
+        sig = pow2_above(h[1]);
+        NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, R, L.ldr, D, L.ldr, s));
+        NLA_CK(lp_solve(prec));
+        NLA_CK(launch_up(L.rows, L.cols, sig, D, L.ldr, 1.0, B, ldb, s));
+    }
+#undef NLA_CK
+    if (info) {
+        info->iterations = it;
+        info->precision = prec;
+        info->backward_error = bwd;
+    }
+    if (!(bwd <= tol)) {
+        std::snprintf(g_last_error, sizeof(g_last_error),
+                      "refinement stopped at backward error %.3e after %d steps (target %.3e)", bwd, it, tol);
+        return NLA_ERR_NOT_CONVERGED;
+    }
+    return NLA_OK;
+}
+
 }  // namespace nla
 
 using namespace nla;
@@ -460,6 +669,43 @@ int32_t nla_gemm_sub(int32_t dtype, int64_t N, int64_t R, int64_t M, void *C, in
     return nla_gemm(dtype, 'N', 'N', N, M, R, -1.0, X, ldx, Y, ldy, 1.0, C, ldc, stream);
 }
 
+int64_t nla_mixed_workspace_bytes(char side, int64_t n, int64_t m) {
+    side = up(side);
+    if ((side != 'L' && side != 'R') || n < 0 || m < 0) return -1;
+    return mixed_layout(side, n, m).total;
+}
+
+int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, double alpha, int64_t n, int64_t m,
+                          const double *A, int64_t lda, double *B, int64_t ldb, double tol, int32_t max_iter,
+                          void *workspace, int64_t workspace_bytes, void *stream, nla_mixed_info *info) {
+    side = up(side); uplo = up(uplo); trans = up(trans);
+    if (side != 'L' && side != 'R') return bad_arg(1, "side");
+    if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
+    if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
+    if (precision != NLA_LP_BF16 && precision != NLA_LP_FP16 && precision != NLA_LP_FP32)
+        return bad_arg(4, "precision");
+    if (!std::isfinite(alpha)) return bad_arg(5, "alpha");
+    if (n < 0) return bad_arg(6, "n");
+    if (m < 0) return bad_arg(7, "m");
+    if (!(tol > 0.0)) return bad_arg(12, "tol");
+    if (max_iter < 0 || max_iter > NLA_MIXED_MAX_ITER) return bad_arg(13, "max_iter");
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->precision = precision;
+    }
+    if (n == 0 || m == 0) return NLA_OK;
+    if (A == nullptr) return bad_arg(8, "A");
+    if (lda < n) return bad_arg(9, "lda");
+    if (B == nullptr) return bad_arg(10, "B");
+    if (ldb < (side == 'L' ? n : m)) return bad_arg(11, "ldb");
+    MixedLayout L = mixed_layout(side, n, m);
+    if (workspace == nullptr || ((uintptr_t)workspace & 255) != 0)
+        return bad_arg(14, "workspace (needs 256-byte alignment)");
+    if (workspace_bytes < L.total) return bad_arg(15, "workspace_bytes");
+    return rectrxm_mixed_impl(side, uplo, trans, precision, alpha, n, m, A, lda, B, ldb, tol, max_iter,
+                              (char *)workspace, L, (hipStream_t)stream, info);
+}
+
 int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_LEAF_MODE) {
         int prev = leaf_mode_default();
@@ -487,7 +733,7 @@ int32_t nla_prof_enable(int32_t on) {
 }
 
 int32_t nla_prof_summary(int32_t kind, int64_t *launches, double *ms, double *flops, double *bytes) {
-    if (kind != NLA_KIND_GEMM && kind != NLA_KIND_LEAF) return bad_arg(1, "kind");
+    if (kind < NLA_KIND_GEMM || kind > NLA_KIND_RESIDUAL) return bad_arg(1, "kind");
     std::lock_guard<std::mutex> g(g_prof_mu);
     int64_t n = 0;
     double t = 0, f = 0, b = 0;

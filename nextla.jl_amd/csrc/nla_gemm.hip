@@ -215,11 +215,19 @@ __device__ __forceinline__ void gemm_tile(int64_t M, int64_t N, int64_t K, T alp
         if (TRI == 3) ly.template mask<false>(col0, k0, t);
         if (TRI == 4) ly.template mask<true>(col0, k0, t);
     };
-    const int nkt = (int)((K + BK - 1) / BK);
+    // k-steps that the triangle zeroes entirely are skipped: the operand's
+    // rows (columns) of this tile only reach k up to (from) the tile edge.
+    // Tile origins are multiples of BM/BN, which are multiples of BK.
+    int64_t kbeg = 0, kend = K;
+    if (TRI == 1) kend = row0 + BM < K ? row0 + BM : K;
+    if (TRI == 2) kbeg = row0 < K ? row0 : K;
+    if (TRI == 3) kbeg = col0 < K ? col0 : K;
+    if (TRI == 4) kend = col0 + BN < K ? col0 + BN : K;
+    const int nkt = (int)((kend - kbeg + BK - 1) / BK);
     if (nkt > 0) {
-        lx.template load<CHECK>(X, xso, xsk, row0, 0, M, K, t);
-        ly.template load<CHECK>(Y, yso, ysk, col0, 0, N, K, t);
-        masks(0);
+        lx.template load<CHECK>(X, xso, xsk, row0, kbeg, M, K, t);
+        ly.template load<CHECK>(Y, yso, ysk, col0, kbeg, N, K, t);
+        masks(kbeg);
         lx.store(lds_x[0], t);
         ly.store(lds_y[0], t);
     }
@@ -229,9 +237,10 @@ __device__ __forceinline__ void gemm_tile(int64_t M, int64_t N, int64_t K, T alp
     for (int kt = 0; kt < nkt; ++kt) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nkt;
+        const int64_t knext = kbeg + (int64_t)(kt + 1) * BK;
         if (more) {
-            lx.template load<CHECK>(X, xso, xsk, row0, (int64_t)(kt + 1) * BK, M, K, t);
-            ly.template load<CHECK>(Y, yso, ysk, col0, (int64_t)(kt + 1) * BK, N, K, t);
+            lx.template load<CHECK>(X, xso, xsk, row0, knext, M, K, t);
+            ly.template load<CHECK>(Y, yso, ysk, col0, knext, N, K, t);
         }
         const NLA_LDS T *sx = lds_x[cur];
         const NLA_LDS T *sy = lds_y[cur];
@@ -248,7 +257,7 @@ __device__ __forceinline__ void gemm_tile(int64_t M, int64_t N, int64_t K, T alp
                 for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = MF::mma(a[cb], b[rb], acc[cb][rb]);
         }
         if (more) {
-            masks((int64_t)(kt + 1) * BK);
+            masks(knext);
             lx.store(lds_x[cur ^ 1], t);
             ly.store(lds_y[cur ^ 1], t);
         }
@@ -309,9 +318,13 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
     __shared__ __attribute__((aligned(16))) T lds_y[2 * LY::LDS_ELEMS];
     int tm, tn;
     tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
+    // with a triangular operand the longest k-ranges belong to the last row
+    // (TRI 1) or column (TRI 4) tiles: start those first
+    if (TRI == 1) tm = tiles_m - 1 - tm;
+    if (TRI == 4) tn = (int)(gridDim.x / tiles_m) - 1 - tn;
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
     // interior tiles with whole k-steps take the unchecked vector path
-    const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0) && TRI == 0;
+    const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0);
     if (interior)
         gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI, false>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc,
                                                                  row0, col0, (NLA_LDS T *)lds_x,
@@ -351,6 +364,33 @@ static hipError_t launch_tri(const GemmArgs<T> &a, hipStream_t s) {
     }
 }
 
+// A dense product with one triangular operand: op(X) (TRI 1/2, op(Y) dense,
+// untransposed) or op(Y) (TRI 3/4, op(X) dense, untransposed) -- the shapes
+// of the residual alpha*B - op(A)*X and alpha*B - X*op(A).
+template <typename T, int BM, int BN, int BK, int WM, int WN, int VW>
+static hipError_t launch_tri_dense(const GemmArgs<T> &a, hipStream_t s) {
+    switch (a.tri) {
+        case 1:
+            if (a.ty) return hipErrorInvalidValue;
+            return a.tx ? launch_cfg<T, BM, BN, BK, WM, WN, true, false, VW, 1>(a, s)
+                        : launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW, 1>(a, s);
+        case 2:
+            if (a.ty) return hipErrorInvalidValue;
+            return a.tx ? launch_cfg<T, BM, BN, BK, WM, WN, true, false, VW, 2>(a, s)
+                        : launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW, 2>(a, s);
+        case 3:
+            if (a.tx) return hipErrorInvalidValue;
+            return a.ty ? launch_cfg<T, BM, BN, BK, WM, WN, false, true, VW, 3>(a, s)
+                        : launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW, 3>(a, s);
+        case 4:
+            if (a.tx) return hipErrorInvalidValue;
+            return a.ty ? launch_cfg<T, BM, BN, BK, WM, WN, false, true, VW, 4>(a, s)
+                        : launch_cfg<T, BM, BN, BK, WM, WN, false, false, VW, 4>(a, s);
+        default:
+            return launch_trans<T, BM, BN, BK, WM, WN, VW, 0>(a, s);
+    }
+}
+
 static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 template <typename T>
@@ -387,16 +427,17 @@ hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s) {
     auto tiles = [&](int bm, int bn) { return ((a.M + bm - 1) / bm) * ((a.N + bn - 1) / bn); };
     if constexpr (sizeof(T) == 8) {
         if (tiles(256, 128) >= 256) {
-            if (vec) return launch_trans<T, 256, 128, 16, 4, 2, VW, 0>(a, s);
-            return launch_trans<T, 256, 128, 16, 4, 2, 1, 0>(a, s);
+            if (vec) return launch_tri_dense<T, 256, 128, 16, 4, 2, VW>(a, s);
+            return launch_tri_dense<T, 256, 128, 16, 4, 2, 1>(a, s);
         }
         if (tiles(128, 128) >= 256) {
-            if (vec) return launch_trans<T, 128, 128, 16, 4, 2, VW, 0>(a, s);
-            return launch_trans<T, 128, 128, 16, 4, 2, 1, 0>(a, s);
+            if (vec) return launch_tri_dense<T, 128, 128, 16, 4, 2, VW>(a, s);
+            return launch_tri_dense<T, 128, 128, 16, 4, 2, 1>(a, s);
         }
-        if (vec) return launch_trans<T, 64, 128, 16, 2, 2, VW, 0>(a, s);
-        return launch_trans<T, 64, 128, 16, 2, 2, 1, 0>(a, s);
+        if (vec) return launch_tri_dense<T, 64, 128, 16, 2, 2, VW>(a, s);
+        return launch_tri_dense<T, 64, 128, 16, 2, 2, 1>(a, s);
     } else {
+        if (a.tri) return hipErrorInvalidValue;  // triangular products are FP64 only
         if (tiles(256, 256) >= 256) {
             if (vec) return launch_trans<T, 256, 256, 32, 4, 4, VW, 0>(a, s);
             return launch_trans<T, 256, 256, 32, 4, 4, 1, 0>(a, s);

@@ -12,11 +12,12 @@ constexpr int LEAF_MAX_NB = 64;
 // C = beta*C + alpha*op(X)*op(Y); op(X) is M x K, op(Y) is K x N.
 // tri: 0 = dense operands; 1/2 = op(X) restricted to its lower/upper
 // triangle (k <= r / k >= r); 3/4 = op(Y) restricted to its lower/upper
-// triangle (c <= k / c >= k).  leaf != 0 selects the tile shape used for
-// diagonal-block leaves: one leaf (<= 64) along M (leaf = 1, left side) or N
-// (leaf = 2, right side) and along K.  C may then alias the operand it
-// overwrites, because one work-group covers the whole leaf dimension of the
-// strip it reads and writes.
+// triangle (c <= k / c >= k); k-steps outside the triangle are skipped
+// (general tile shapes take tri for FP64 only).  leaf != 0 selects the tile
+// shape used for diagonal-block leaves: one leaf (<= 64) along M (leaf = 1,
+// left side) or N (leaf = 2, right side) and along K.  C may then alias the
+// operand it overwrites, because one work-group covers the whole leaf
+// dimension of the strip it reads and writes.
 template <typename T>
 struct GemmArgs {
     int64_t M, N, K;
@@ -129,5 +130,48 @@ hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_
 
 template <typename T>
 hipError_t launch_zero(int64_t rows, int64_t cols, T *B, int64_t ldb, hipStream_t s);
+
+// ---- mixed precision ------------------------------------------------------
+
+// Low-precision GEMM update C = beta*C + alpha*op(X)*op(Y) on fp32 storage:
+// op(X) and op(Y) are scaled by powers of two chosen from amax[0] / amax[1]
+// (absolute maxima of the two operand blocks as float bit patterns; nullptr
+// for no scaling), rounded to bf16 (lp = 0) or fp16 (lp = 1), multiplied on
+// MFMA with fp32 accumulation and unscaled on the way out.  X, Y need
+// 16-byte aligned columns (rows when transposed) and ldx, ldy % 4 == 0.
+struct LpGemmArgs {
+    int64_t M, N, K;
+    bool tx, ty;
+    float alpha, beta;
+    const float *X;
+    int64_t ldx;
+    const float *Y;
+    int64_t ldy;
+    float *C;
+    int64_t ldc;
+    int lp;
+    const unsigned *amax;
+};
+hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s);
+// amax[0] = max(amax[0], max |P(r,c)|) over P(r,c) = P[r*rs + c*cs]; the slot
+// must hold 0 (or an earlier maximum) before the launch.
+hipError_t launch_absmax(int64_t rows, int64_t cols, const float *P, int64_t rs, int64_t cs, unsigned *amax,
+                         hipStream_t s);
+hipError_t launch_fill_u32(unsigned *p, int64_t n, unsigned v, hipStream_t s);
+// Column-major elementwise steps of the refinement loop.
+hipError_t launch_tri_to_f32(int64_t n, const double *A, int64_t lda, bool lower, float *A32, int64_t ld32,
+                             hipStream_t s);
+hipError_t launch_scale_copy(int64_t rows, int64_t cols, double alpha, const double *B, int64_t ldb, double *S,
+                             int64_t lds, hipStream_t s);
+hipError_t launch_down(int64_t rows, int64_t cols, double scale, const double *R, int64_t ldr, float *D,
+                       int64_t ldd, hipStream_t s);
+hipError_t launch_up(int64_t rows, int64_t cols, double sigma, const float *D, int64_t ldd, double beta, double *X,
+                     int64_t ldx, hipStream_t s);
+// out[0] = sum of squares, out[1] = max |.| of a column-major matrix (tri:
+// 0 whole, 1 lower triangle, 2 upper triangle); part holds NORM_PARTS_MAX
+// pairs of partial results.
+constexpr int NORM_PARTS_MAX = 1024;
+hipError_t launch_norms(int64_t rows, int64_t cols, const double *P, int64_t ld, int tri, double *part,
+                        double *out, hipStream_t s);
 
 }  // namespace nla

@@ -1,0 +1,539 @@
+// Kernels of the mixed-precision triangular solve with FP64 iterative
+// refinement (nla_rectrxm_mixed in include/nla.h).
+//
+// The refinement loop needs, besides the plain recursion:
+//   - a low-precision GEMM update for the correction solves: op(X), op(Y)
+//     stored as fp32, scaled by a power of two per operand block (from the
+//     block's absolute maximum, so fp16 neither overflows nor flushes the
+//     block's large entries), rounded to bf16 or fp16 on their way into LDS,
+//     multiplied on the 16x16x32 MFMA with fp32 accumulation, and unscaled in
+//     the epilogue;
+//   - conversions between the FP64 iterate / residual and the fp32 solve
+//     buffers, and the norms of the stopping test, as HBM-bound elementwise
+//     kernels with deterministic (fixed-order) reductions.
+// The FP64 residual itself is the MFMA GEMM of nla_gemm.hip restricted to
+// the triangle of op(A).
+#include <math.h>
+
+#include "nla_device.h"
+#include "nla_internal.h"
+
+namespace nla {
+
+#define NLA_LDS __attribute__((address_space(3)))
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4_t __attribute__((ext_vector_type(4)));
+
+template <int LP>
+struct LpType;
+template <>
+struct LpType<0> {  // bf16: 8 significant bits, fp32's exponent range
+    typedef __bf16 elem_t;
+    typedef bf16x8_t op_t;
+    typedef bf16x4_t quad_t;
+    __device__ static inline f4_t mma(op_t a, op_t b, f4_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct LpType<1> {  // fp16: 11 significant bits, largest finite 65504
+    typedef _Float16 elem_t;
+    typedef f16x8_t op_t;
+    typedef f16x4_t quad_t;
+    __device__ static inline f4_t mma(op_t a, op_t b, f4_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Scale that puts a block with absolute maximum `amax` (bit pattern of a
+// non-negative float) just below 2^15: exact (a power of two), and the
+// largest scaled entry stays below fp16's largest finite value.
+__device__ inline float block_scale(unsigned bits) {
+    float mx = __uint_as_float(bits);
+    if (!(mx > 0.0f) || !(mx < INFINITY)) return 1.0f;
+    int e;
+    (void)frexpf(mx, &e);  // mx < 2^e
+    int k = 15 - e;        // kept inside fp32's normal range
+    k = k > 126 ? 126 : (k < -126 ? -126 : k);
+    return ldexpf(1.0f, k);
+}
+
+// Stages a BO x BK block of an fp32 operand into LDS as rows of k (one row
+// per outer index, row stride BK + 8 elements) in the low-precision type.
+// Each thread moves chunks of 4 outer x 4 k elements: four 16-byte loads
+// along the operand's contiguous dimension, four 8-byte LDS stores along k.
+template <int LP, int BO, int BK, int NT, bool KCONTIG>
+struct LpLoader {
+    typedef typename LpType<LP>::elem_t E;
+    typedef typename LpType<LP>::quad_t Q;
+    static constexpr int CHUNKS = (BO / 4) * (BK / 4);
+    static constexpr int CH = CHUNKS / NT;
+    static_assert(CHUNKS % NT == 0, "tile does not divide among threads");
+    static_assert(CH * 16 <= 32, "bounds bits do not fit");
+    static constexpr int LDK = BK + 8;
+    static constexpr int LDS_ELEMS = BO * LDK;
+
+    float r[CH][4][4];  // [chunk][outer offset][k offset]
+    uint32_t okbits;
+
+    __device__ static inline void coords(int t, int c, int &o, int &k) {
+        int q = t + c * NT;
+        if (KCONTIG) {  // neighbouring threads take neighbouring k
+            k = (q % (BK / 4)) * 4;
+            o = (q / (BK / 4)) * 4;
+        } else {  // neighbouring threads take neighbouring outer indices
+            o = (q % (BO / 4)) * 4;
+            k = (q / (BO / 4)) * 4;
+        }
+    }
+
+    // Element (outer, k) at P[outer*so + k*sk]; (KCONTIG ? sk : so) == 1.
+    template <bool CHECK>
+    __device__ inline void load(const float *__restrict__ P, int64_t so, int64_t sk, int64_t o0, int64_t k0,
+                                int64_t no, int64_t nk, int t) {
+        if (CHECK) okbits = 0;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            int o, k;
+            coords(t, c, o, k);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                // v walks the non-contiguous dimension; 4 contiguous elements per load
+                int64_t go = o0 + o + (KCONTIG ? v : 0), gk = k0 + k + (KCONTIG ? 0 : v);
+                if (!CHECK) {
+                    f4_t x = *reinterpret_cast<const f4_t *>(P + go * so + gk * sk);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (KCONTIG) r[c][v][e] = x[e];
+                        else r[c][e][v] = x[e];
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        int64_t eo = KCONTIG ? go : go + e, ek = KCONTIG ? gk + e : gk;
+                        bool ok = (eo < no) && (ek < nk);
+                        float x = *(ok ? P + eo * so + ek * sk : P);
+                        if (KCONTIG) r[c][v][e] = x;
+                        else r[c][e][v] = x;
+                        okbits |= (ok ? 1u : 0u) << (c * 16 + v * 4 + e);
+                    }
+                }
+            }
+        }
+    }
+
+    // Zero the out-of-bounds elements of a CHECK load (bit order of load()).
+    __device__ inline void apply_bounds() {
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (!((okbits >> (c * 16 + v * 4 + e)) & 1u)) {
+                        if (KCONTIG) r[c][v][e] = 0.0f;
+                        else r[c][e][v] = 0.0f;
+                    }
+    }
+
+    __device__ inline void store(NLA_LDS E *lds, int t, float sc) const {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            int o, k;
+            coords(t, c, o, k);
+#pragma unroll
+            for (int oo = 0; oo < 4; ++oo) {
+                Q q;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) q[kk] = (E)(r[c][oo][kk] * sc);
+                *reinterpret_cast<NLA_LDS Q *>(lds + (o + oo) * LDK + k) = q;
+            }
+        }
+    }
+};
+
+// One BM x BN tile of C = beta*C + alpha*op(X)*op(Y) (fp32 C).  The MFMA
+// computes C^T per 16x16 block (A operand from op(Y), B operand from op(X)),
+// so the 16 lanes of a result register run down one column of C.
+template <int LP, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, bool CHECK>
+__device__ __forceinline__ void lp_tile(int64_t M, int64_t N, int64_t K, float alpha, const float *__restrict__ X,
+                                        int64_t ldx, const float *__restrict__ Y, int64_t ldy, float beta, float *C,
+                                        int64_t ldc, int64_t row0, int64_t col0, float sx, float sy,
+                                        NLA_LDS typename LpType<LP>::elem_t *lds_xb,
+                                        NLA_LDS typename LpType<LP>::elem_t *lds_yb) {
+    typedef LpType<LP> LT;
+    typedef typename LT::elem_t E;
+    typedef typename LT::op_t OP;
+    constexpr int NT = WM * WN * 64;
+    constexpr int TWM = BM / WM, TWN = BN / WN;
+    constexpr int RB = TWM / 16, CB = TWN / 16;
+    static_assert(BK % 32 == 0, "BK must be a multiple of the MFMA depth");
+    typedef LpLoader<LP, BM, BK, NT, TX> LX;   // rows of C; k-contiguous when TX
+    typedef LpLoader<LP, BN, BK, NT, !TY> LY;  // columns of C; k-contiguous unless TY
+    NLA_LDS E *lds_x[2] = {lds_xb, lds_xb + LX::LDS_ELEMS};
+    NLA_LDS E *lds_y[2] = {lds_yb, lds_yb + LY::LDS_ELEMS};
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int64_t xso = TX ? ldx : 1, xsk = TX ? 1 : ldx;
+    const int64_t yso = TY ? 1 : ldy, ysk = TY ? ldy : 1;
+
+    f4_t acc[CB][RB];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = f4_t{0.0f, 0.0f, 0.0f, 0.0f};
+
+    LX lx;
+    LY ly;
+    const int nkt = (int)((K + BK - 1) / BK);
+    if (nkt > 0) {
+        lx.template load<CHECK>(X, xso, xsk, row0, 0, M, K, t);
+        ly.template load<CHECK>(Y, yso, ysk, col0, 0, N, K, t);
+        if (CHECK) {
+            lx.apply_bounds();
+            ly.apply_bounds();
+        }
+        lx.store(lds_x[0], t, sx);
+        ly.store(lds_y[0], t, sy);
+    }
+    __syncthreads();
+
+    const int lo = lane & 15, lk = (lane >> 4) * 8;
+    for (int kt = 0; kt < nkt; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nkt;
+        if (more) {
+            lx.template load<CHECK>(X, xso, xsk, row0, (int64_t)(kt + 1) * BK, M, K, t);
+            ly.template load<CHECK>(Y, yso, ysk, col0, (int64_t)(kt + 1) * BK, N, K, t);
+        }
+        const NLA_LDS E *sx_ = lds_x[cur];
+        const NLA_LDS E *sy_ = lds_y[cur];
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 32) {
+            OP a[CB], b[RB];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+                a[cb] = *reinterpret_cast<const NLA_LDS OP *>(sy_ + (wn * TWN + cb * 16 + lo) * LY::LDK + ks + lk);
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+                b[rb] = *reinterpret_cast<const NLA_LDS OP *>(sx_ + (wm * TWM + rb * 16 + lo) * LX::LDK + ks + lk);
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = LT::mma(a[cb], b[rb], acc[cb][rb]);
+        }
+        if (more) {
+            if (CHECK) {
+                lx.apply_bounds();
+                ly.apply_bounds();
+            }
+            lx.store(lds_x[cur ^ 1], t, sx);
+            ly.store(lds_y[cur ^ 1], t, sy);
+        }
+        __syncthreads();
+    }
+
+    // D[i][j] (lane j + 16*(i/4), register i%4) is C(row j, column i).
+    // unscale in double: sx*sy may lie outside fp32's range
+    const double f = (double)alpha / ((double)sx * (double)sy);
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+        float cv[RB][4];
+        if (beta != 0.0f) {
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int64_t row = row0 + wm * TWM + rb * 16 + lo;
+                    int64_t col = col0 + wn * TWN + cb * 16 + (lane >> 4) * 4 + r;
+                    bool ok = !CHECK || (row < M && col < N);
+                    cv[rb][r] = *(ok ? C + row + col * ldc : C);
+                }
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int64_t row = row0 + wm * TWM + rb * 16 + lo;
+                int64_t col = col0 + wn * TWN + cb * 16 + (lane >> 4) * 4 + r;
+                if (!CHECK || (row < M && col < N)) {
+                    float v = (float)(f * (double)acc[cb][rb][r]);
+                    C[row + col * ldc] = beta != 0.0f ? v + beta * cv[rb][r] : v;
+                }
+            }
+    }
+}
+
+__device__ inline void lp_tile_coords(int G, int g, int tiles_m, int &tm, int &tn) {
+    // XCD-aware renumbering as in the FP32/FP64 GEMM: work-group g runs on
+    // XCD g % 8; each XCD walks a contiguous range of tiles in groups of 8
+    // row tiles.
+    const int NX = 8, GM = 8;
+    int xcd = g % NX, idx = g / NX;
+    int per = G / NX, rem = G % NX;
+    int gp = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    int tiles_n = G / tiles_m;
+    int group = GM * tiles_n;
+    int gid = gp / group;
+    int first = gid * GM;
+    int gm = tiles_m - first < GM ? tiles_m - first : GM;
+    int in = gp % group;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
+template <int LP, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY>
+__global__ __launch_bounds__(WM *WN * 64) void lp_gemm_kernel(int64_t M, int64_t N, int64_t K, float alpha,
+                                                              const float *__restrict__ X, int64_t ldx,
+                                                              const float *__restrict__ Y, int64_t ldy, float beta,
+                                                              float *C, int64_t ldc, int tiles_m,
+                                                              const unsigned *__restrict__ amax) {
+    typedef typename LpType<LP>::elem_t E;
+    constexpr int NT = WM * WN * 64;
+    typedef LpLoader<LP, BM, BK, NT, TX> LX;
+    typedef LpLoader<LP, BN, BK, NT, !TY> LY;
+    __shared__ __attribute__((aligned(16))) E lds_x[2 * LX::LDS_ELEMS];
+    __shared__ __attribute__((aligned(16))) E lds_y[2 * LY::LDS_ELEMS];
+    int tm, tn;
+    lp_tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
+    const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+    const float sx = amax ? block_scale(amax[0]) : 1.0f;
+    const float sy = amax ? block_scale(amax[1]) : 1.0f;
+    const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0);
+    if (interior)
+        lp_tile<LP, BM, BN, BK, WM, WN, TX, TY, false>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc, row0, col0,
+                                                        sx, sy, (NLA_LDS E *)lds_x, (NLA_LDS E *)lds_y);
+    else
+        lp_tile<LP, BM, BN, BK, WM, WN, TX, TY, true>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc, row0, col0,
+                                                       sx, sy, (NLA_LDS E *)lds_x, (NLA_LDS E *)lds_y);
+}
+
+template <int LP, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY>
+static hipError_t lp_launch_cfg(const LpGemmArgs &a, hipStream_t s) {
+    int tiles_m = (int)((a.M + BM - 1) / BM);
+    int tiles_n = (int)((a.N + BN - 1) / BN);
+    hipLaunchKernelGGL((lp_gemm_kernel<LP, BM, BN, BK, WM, WN, TX, TY>), dim3(tiles_m * tiles_n),
+                       dim3(WM * WN * 64), 0, s, a.M, a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C,
+                       a.ldc, tiles_m, (const unsigned *)a.amax);
+    return hipGetLastError();
+}
+
+template <int LP, int BM, int BN, int BK, int WM, int WN>
+static hipError_t lp_launch_trans(const LpGemmArgs &a, hipStream_t s) {
+    if (!a.tx && !a.ty) return lp_launch_cfg<LP, BM, BN, BK, WM, WN, false, false>(a, s);
+    if (a.tx && !a.ty) return lp_launch_cfg<LP, BM, BN, BK, WM, WN, true, false>(a, s);
+    if (!a.tx && a.ty) return lp_launch_cfg<LP, BM, BN, BK, WM, WN, false, true>(a, s);
+    return lp_launch_cfg<LP, BM, BN, BK, WM, WN, true, true>(a, s);
+}
+
+hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s) {
+    if (a.M <= 0 || a.N <= 0) return hipSuccess;
+    // Unchecked interior tiles issue 16-byte loads: every column (row, for a
+    // transposed operand) must start 16-byte aligned.
+    const bool vec = ((uintptr_t)a.X & 15) == 0 && ((uintptr_t)a.Y & 15) == 0 && a.ldx % 4 == 0 && a.ldy % 4 == 0;
+    if (!vec) return hipErrorInvalidValue;
+    if (a.lp == 0) return lp_launch_trans<0, 128, 128, 32, 2, 2>(a, s);
+    if (a.lp == 1) return lp_launch_trans<1, 128, 128, 32, 2, 2>(a, s);
+    return hipErrorInvalidValue;
+}
+
+// ---- elementwise and reduction kernels -----------------------------------
+
+// Column-major rows x cols walks: threads along the rows (the unit-stride
+// dimension), work-groups of the grid's y dimension along the columns.
+#define NLA_EW_BEGIN                                                                \
+    for (int64_t c = blockIdx.y; c < cols; c += gridDim.y)                          \
+        for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; \
+             r += (int64_t)gridDim.x * blockDim.x) {
+#define NLA_EW_END }
+
+static inline dim3 ew_grid(int64_t rows, int64_t cols, int64_t max_blocks = 1 << 20) {
+    int64_t bx = (rows + 255) / 256;
+    if (bx > 64) bx = 64;
+    int64_t by = cols < 65535 ? cols : 65535;
+    if (bx * by > max_blocks) by = max_blocks / bx > 0 ? max_blocks / bx : 1;
+    return dim3((unsigned)bx, (unsigned)by);
+}
+
+// amax[0] = max(amax[0], max |P(r,c)|), P(r,c) = P[r*rs + c*cs]; the walk
+// runs along whichever of the two dimensions has unit stride.  Non-negative
+// floats order like their bit patterns, so the maximum is an integer max.
+__global__ void absmax_kernel(int64_t rows, int64_t cols, const float *__restrict__ P, int64_t rs, int64_t cs,
+                              unsigned *amax) {
+    __shared__ unsigned red[256];
+    unsigned m = 0;
+    NLA_EW_BEGIN
+    unsigned b = __float_as_uint(fabsf(P[r * rs + c * cs]));
+    m = b > m ? b : m;
+    NLA_EW_END
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] != 0) atomicMax(amax, red[0]);
+}
+
+hipError_t launch_absmax(int64_t rows, int64_t cols, const float *P, int64_t rs, int64_t cs, unsigned *amax,
+                         hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    if (rs != 1) {  // walk the transposed view so threads stay on the unit stride
+        int64_t t = rows; rows = cols; cols = t;
+        t = rs; rs = cs; cs = t;
+    }
+    hipLaunchKernelGGL(absmax_kernel, ew_grid(rows, cols, 2048), dim3(256), 0, s, rows, cols, P, rs, cs, amax);
+    return hipGetLastError();
+}
+
+__global__ void fill_u32_kernel(unsigned *p, int64_t n, unsigned v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill_u32(unsigned *p, int64_t n, unsigned v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// A32 = fp32(A) on the stored triangle (lower: r >= c, upper: r <= c).
+__global__ void tri_to_f32_kernel(int64_t rows, int64_t cols, const double *__restrict__ A, int64_t lda, int lower,
+                                  float *__restrict__ A32, int64_t ld32) {
+    NLA_EW_BEGIN
+    if (lower ? r >= c : r <= c) A32[r + c * ld32] = (float)A[r + c * lda];
+    NLA_EW_END
+}
+
+// S = alpha * B (FP64).
+__global__ void scale_copy_kernel(int64_t rows, int64_t cols, double alpha, const double *__restrict__ B, int64_t ldb,
+                                  double *__restrict__ S, int64_t lds) {
+    NLA_EW_BEGIN
+    S[r + c * lds] = alpha * B[r + c * ldb];
+    NLA_EW_END
+}
+
+// D = fp32(scale * R).
+__global__ void down_kernel(int64_t rows, int64_t cols, double scale, const double *__restrict__ R, int64_t ldr,
+                            float *__restrict__ D, int64_t ldd) {
+    NLA_EW_BEGIN
+    D[r + c * ldd] = (float)(scale * R[r + c * ldr]);
+    NLA_EW_END
+}
+
+// X = beta * X + sigma * fp64(D)   (beta = 0: X = sigma * D)
+__global__ void up_kernel(int64_t rows, int64_t cols, double sigma, const float *__restrict__ D, int64_t ldd, double beta,
+                          double *__restrict__ X, int64_t ldx) {
+    NLA_EW_BEGIN
+    double d = sigma * (double)D[r + c * ldd];
+    X[r + c * ldx] = beta != 0.0 ? beta * X[r + c * ldx] + d : d;
+    NLA_EW_END
+}
+
+hipError_t launch_tri_to_f32(int64_t n, const double *A, int64_t lda, bool lower, float *A32, int64_t ld32,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tri_to_f32_kernel, ew_grid(n, n), dim3(256), 0, s, n, n, A, lda, lower ? 1 : 0, A32, ld32);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_copy(int64_t rows, int64_t cols, double alpha, const double *B, int64_t ldb, double *S,
+                             int64_t lds, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_copy_kernel, ew_grid(rows, cols), dim3(256), 0, s, rows, cols, alpha, B, ldb, S, lds);
+    return hipGetLastError();
+}
+
+hipError_t launch_down(int64_t rows, int64_t cols, double scale, const double *R, int64_t ldr, float *D,
+                       int64_t ldd, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(down_kernel, ew_grid(rows, cols), dim3(256), 0, s, rows, cols, scale, R, ldr, D, ldd);
+    return hipGetLastError();
+}
+
+hipError_t launch_up(int64_t rows, int64_t cols, double sigma, const float *D, int64_t ldd, double beta, double *X,
+                     int64_t ldx, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(up_kernel, ew_grid(rows, cols), dim3(256), 0, s, rows, cols, sigma, D, ldd, beta, X, ldx);
+    return hipGetLastError();
+}
+
+// Sum of squares and absolute maximum of a column-major rows x cols matrix
+// (restricted to a triangle when tri = 1 lower / 2 upper), in two passes
+// with a fixed reduction order: NORM_PARTS partial pairs, then one
+// work-group folds them into out[0] = sum of squares, out[1] = max |.|.
+constexpr int NORM_PARTS = NORM_PARTS_MAX;
+
+__global__ void norm_partial_kernel(int64_t rows, int64_t cols, const double *__restrict__ P, int64_t ld, int tri,
+                                    double *__restrict__ part) {
+    __shared__ double ss[256], mx[256];
+    double sum = 0.0, m = 0.0;
+    NLA_EW_BEGIN
+    if ((tri == 1 && r < c) || (tri == 2 && r > c)) continue;
+    double v = P[r + c * ld];
+    sum += v * v;
+    double a = fabs(v);
+    m = a > m ? a : m;
+    NLA_EW_END
+    ss[threadIdx.x] = sum;
+    mx[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            mx[threadIdx.x] = mx[threadIdx.x] > mx[threadIdx.x + w] ? mx[threadIdx.x] : mx[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        part[2 * b] = ss[0];
+        part[2 * b + 1] = mx[0];
+    }
+}
+
+__global__ void norm_final_kernel(int nparts, const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double ss[256], mx[256];
+    double s = 0.0, m = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        s += part[2 * i];
+        m = part[2 * i + 1] > m ? part[2 * i + 1] : m;
+    }
+    ss[threadIdx.x] = s;
+    mx[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ss[threadIdx.x] += ss[threadIdx.x + w];
+            mx[threadIdx.x] = mx[threadIdx.x] > mx[threadIdx.x + w] ? mx[threadIdx.x] : mx[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = ss[0];
+        out[1] = mx[0];
+    }
+}
+
+hipError_t launch_norms(int64_t rows, int64_t cols, const double *P, int64_t ld, int tri, double *part,
+                        double *out, hipStream_t s) {
+    int nparts = 0;
+    if (rows > 0 && cols > 0) {
+        dim3 g = ew_grid(rows, cols, NORM_PARTS);
+        nparts = (int)(g.x * g.y);
+        hipLaunchKernelGGL(norm_partial_kernel, g, dim3(256), 0, s, rows, cols, P, ld, tri, part);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, nparts, part, out);
+    return hipGetLastError();
+}
+
+}  // namespace nla

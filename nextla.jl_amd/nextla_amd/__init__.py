@@ -94,6 +94,18 @@ def unified_rectrxm_(side, uplo, transpose, alpha, func, A, B):
             raise ValueError(f"argument {k} has invalid value {v!r}")
     side, uplo, transpose, func = side.upper(), uplo.upper(), transpose.upper(), func.upper()
     _check_pair(A, B)
+    side, uplo, tr, n, m, lda, ldb = _canonical(side, uplo, transpose, A, B)
+    st = _lib.load().nla_rectrxm(_lib.ch(side), _lib.ch(uplo), _lib.ch("T" if tr else "N"), _lib.ch(func),
+                                 _DT[A.dtype], float(alpha), n, m, A.data_ptr(), lda, B.data_ptr(), ldb,
+                                 _stream(B))
+    _lib.check(st, "unified_rectrxm!")
+    return B
+
+
+def _canonical(side, uplo, transpose, A, B):
+    """The column-major call equivalent to (side, uplo, transpose) on the
+    semantic matrices A, B whatever their memory layout:
+    (side, uplo, transposed, n, m, lda, ldb) of the stored matrices."""
     n = A.shape[0]
     if A.dim() != 2 or A.shape[1] != n:
         raise ValueError(f"A must be square, got {tuple(A.shape)}")
@@ -113,11 +125,7 @@ def unified_rectrxm_(side, uplo, transpose, alpha, func, A, B):
     # the stored column-major matrix is n x m for side 'L', m x n for side 'R'
     rc, cc = (B.shape[1], B.shape[0]) if b_t else (B.shape[0], B.shape[1])
     m = cc if side == "L" else rc
-    st =_lib.load().nla_rectrxm(_lib.ch(side), _lib.ch(uplo), _lib.ch("T" if tr else "N"), _lib.ch(func),
-                                 _DT[A.dtype], float(alpha), n, m, A.data_ptr(), lda, B.data_ptr(), ldb,
-                                 _stream(B))
-    _lib.check(st, "unified_rectrxm!")
-    return B
+    return side, uplo, tr, n, m, lda, ldb
 
 
 def _gemm(alpha, X, Y, beta, C, what):

@@ -25,6 +25,15 @@ class BatchItem(ctypes.Structure):
                 ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("alpha", ctypes.c_double)]
 
 
+NLA_MIXED_MAX_ITER = 30
+
+
+class MixedInfo(ctypes.Structure):
+    """nla_mixed_info of include/nla.h."""
+    _fields_ = [("iterations", ctypes.c_int32), ("precision", ctypes.c_int32), ("backward_error", ctypes.c_double),
+                ("history", ctypes.c_double * (NLA_MIXED_MAX_ITER + 1))]
+
+
 # name -> (restype, argtypes); must list every function of include/nla.h
 SIGNATURES = {
     "nla_abi_version": (_i32, []),
@@ -39,6 +48,9 @@ SIGNATURES = {
                                      _i64, _vp, ctypes.POINTER(_vp)]),
     "nla_batch_plan_execute": (_i32, [_vp, _vp, _i32]),
     "nla_batch_plan_destroy": (_i32, [_vp]),
+    "nla_mixed_workspace_bytes": (_i64, [_c_char, _i64, _i64]),
+    "nla_rectrxm_mixed": (_i32, [_c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _dbl, _i32,
+                                 _vp, _i64, _vp, ctypes.POINTER(MixedInfo)]),
     "nla_set_config": (_i32, [_i32, _i32]),
     "nla_prof_enable": (_i32, [_i32]),
     "nla_prof_summary": (_i32, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
@@ -47,6 +59,12 @@ SIGNATURES = {
 
 NLA_KIND_GEMM = 0
 NLA_KIND_LEAF = 1
+NLA_KIND_LP_GEMM = 2
+NLA_KIND_RESIDUAL = 3
+NLA_LP_BF16 = 0
+NLA_LP_FP16 = 1
+NLA_LP_FP32 = 2
+NLA_ERR_NOT_CONVERGED = 5
 NLA_CFG_LEAF_MODE = 0
 NLA_CFG_PANEL = 1
 

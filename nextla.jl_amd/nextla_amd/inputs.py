@@ -84,3 +84,98 @@ def device_workload1(n, m, uplo, side, dtype, seed, device="cuda"):
     b = device_colmajor(rows, cols, dtype, device)
     b.normal_(0.0, 1.0, generator=g)
     return a, b
+
+
+def log_diagonal(n, c, dtype=np.float64):
+    """Diagonal 2*c^(i/(n-1)), i = 0..n-1: log-spaced from 2 to 2c."""
+    t = np.arange(n, dtype=np.float64) / max(n - 1, 1)
+    return (2.0 * np.power(float(c), t)).astype(dtype)
+
+
+def precision_tri(n, c, rng, dtype=np.float64):
+    """Lower A for the refinement workload: off-diagonals U(-1,1)/sqrt(n),
+    log-spaced diagonal 2 .. 2c (log_diagonal); cond(A) grows about linearly
+    with c, and calibrate_precision_tri picks c for a measured target."""
+    a = tri_mask((rng.random((n, n)) * 2.0 - 1.0) / math.sqrt(n), "L")
+    a[np.diag_indices(n)] = log_diagonal(n, c)
+    return np.asfortranarray(a.astype(dtype))
+
+
+def cond1_estimate(solve, solve_t, anorm1, x0, iters=5):
+    """kappa_1(A) = ||A||_1 * ||A^-1||_1 with ||A^-1||_1 from Hager's
+    estimator (Hager 1984; Higham 1988, the algorithm of LAPACK's xLACON).
+    solve(x) = A^-1 x and solve_t(x) = A^-T x act on float64 column vectors
+    (numpy arrays or torch tensors of shape (n, 1)); x0 is the starting
+    vector, all entries 1/n.  The estimate is a lower bound, exact for most
+    matrices and within a small factor otherwise."""
+    x = x0
+    est = 0.0
+    for k in range(iters):
+        y = solve(x)
+        est_new = float(abs(y).sum())
+        xi = y * 0.0 + 1.0
+        xi[y < 0] = -1.0
+        z = solve_t(xi)
+        zabs = abs(z)
+        if float(zabs.max()) <= float((z * x).sum()) or (k > 0 and est_new <= est):
+            est = max(est, est_new)
+            break
+        est = est_new
+        j = int(zabs.argmax())
+        x = x * 0.0
+        x[j, 0] = 1.0
+    return anorm1 * est
+
+
+def device_cond1(A, uplo="L"):
+    """kappa_1 of a triangular device matrix (zero outside its triangle),
+    with the solves done by the library's own TRSM."""
+    import torch
+    from . import unified_rectrxm_
+    n = A.shape[0]
+    anorm = float(A.abs().sum(0).max())
+
+    def solve(x):
+        y = x.clone()
+        unified_rectrxm_("L", uplo, "N", 1.0, "S", A, y)
+        return y
+
+    def solve_t(x):
+        y = x.clone()
+        unified_rectrxm_("L", uplo, "T", 1.0, "S", A, y)
+        return y
+
+    x0 = torch.full((n, 1), 1.0 / n, dtype=torch.float64, device=A.device)
+    return cond1_estimate(solve, solve_t, anorm, x0)
+
+
+def device_precision_tri(n, m, c, seed, device="cuda"):
+    """The refinement workload in HBM: lower A as precision_tri (FP64) with
+    diagonal scale c, and B (n x m) N(0,1), column-major."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    a = device_colmajor(n, n, torch.float64, device)
+    a.uniform_(-1.0, 1.0, generator=g)
+    a.mul_(1.0 / math.sqrt(n))
+    a.copy_(torch.tril(a))
+    a.diagonal().copy_(torch.from_numpy(log_diagonal(n, c)).to(device))
+    b = device_colmajor(n, m, torch.float64, device)
+    b.normal_(0.0, 1.0, generator=g)
+    return a, b
+
+
+def calibrate_precision_tri(n, m, target, seed, device="cuda", rounds=3, tol=0.15):
+    """Choose the diagonal scale c so that the measured kappa_1(A) of
+    device_precision_tri is within `tol` (relative, in log10 terms: about
+    +-40 % for tol 0.15) of `target`.  Returns (A, B, c, measured kappa_1)."""
+    c = float(target)
+    A = B = None
+    kappa = None
+    for _ in range(rounds):
+        A, B = device_precision_tri(n, m, c, seed, device)
+        kappa = device_cond1(A, "L")
+        if abs(math.log10(kappa) - math.log10(target)) <= tol:
+            break
+        c *= target / kappa
+    return A, B, c, kappa

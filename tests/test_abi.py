@@ -111,3 +111,31 @@ def test_batch_validation(nla):
     assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 264, need, None, ctypes.byref(plan)) == -8
     assert lib.nla_batch_plan_create(b"L", b"L", b"N", b"S", 1, items, 1, 256, need, None, None) == -11
     assert lib.nla_batch_plan_destroy(None) == 0
+
+
+def test_mixed_validation(nla):
+    import ctypes
+    from nextla_amd import _lib
+    lib = _lib.load()
+    need = lib.nla_mixed_workspace_bytes(b"L", 100, 8)
+    assert need >= 4 * 100 * 100 + 20 * 100 * 8
+    assert lib.nla_mixed_workspace_bytes(b"Q", 100, 8) == -1
+    info = _lib.MixedInfo()
+    good = [b"L", b"L", b"N", 1, 1.0, 100, 8, 256, 100, 512, 100, 1e-14, 10, 1024, need, None, ctypes.byref(info)]
+
+    def call(pos, val):
+        args = list(good)
+        args[pos] = val
+        return lib.nla_rectrxm_mixed(*args)
+
+    assert call(0, b"X") == -1
+    assert call(3, 7) == -4
+    assert call(4, float("nan")) == -5
+    assert call(5, -1) == -6
+    assert call(11, 0.0) == -12
+    assert call(12, 31) == -13
+    assert call(8, 99) == -9
+    assert call(10, 50) == -11
+    assert call(13, 1032) == -14
+    assert call(14, need - 1) == -15
+    assert call(6, 0) == 0 and info.iterations == 0  # empty: nothing to do

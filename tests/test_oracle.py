@@ -218,3 +218,20 @@ def test_planted_mistake_is_caught(fixtures):
     dropped_alpha = O.rectrxm("L", "L", "N", 1.0, "S", A, B)
     assert O.rel_error(wrong_trans, X) > 1e-3
     assert O.rel_error(dropped_alpha, X) > 1e-3
+
+
+def test_cond1_estimator_against_exact():
+    """Hager's 1-norm estimator (used to calibrate the refinement workload)
+    against kappa_1 from an explicit inverse on small matrices."""
+    import scipy.linalg as sl
+    from nextla_amd import inputs as I
+    rng = I._rng(1)
+    for c in (1e2, 1e4, 1e6):
+        A = I.precision_tri(400, c, rng)
+        exact = np.linalg.norm(A, 1) * np.linalg.norm(np.linalg.inv(A), 1)
+        est = I.cond1_estimate(lambda x: sl.solve_triangular(A, x, lower=True),
+                               lambda x: sl.solve_triangular(A, x, lower=True, trans="T"),
+                               np.abs(A).sum(0).max(), np.full((400, 1), 1 / 400))
+        assert 0.5 * exact <= est <= exact * (1 + 1e-12), (c, est, exact)
+        # the diagonal scale sets the 2-norm condition number to within a few percent
+        assert 0.9 * c <= np.linalg.cond(A, 2) <= 1.2 * c
