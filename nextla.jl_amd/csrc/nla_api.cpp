@@ -420,7 +420,7 @@ struct MixedLayout {
     int64_t rows, cols;  // shape of B
     int64_t ld32, ldr;   // leading dimension of the fp32 A copy / of S, R, D
     int64_t amax_slots;
-    int64_t a32, w, s, r, d, amax, part, norms, total;
+    int64_t a32, w, s, r, d, amax, part, norms, queue, total;
 };
 
 static inline int64_t al256(int64_t x) { return (x + 255) & ~(int64_t)255; }
@@ -442,6 +442,7 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     L.amax = o;  o = al256(o + L.amax_slots * 4);
     L.part = o;  o = al256(o + NORM_PARTS_MAX * 2 * 8);
     L.norms = o; o = al256(o + 8 * 8);
+    L.queue = o; o = al256(o + 4);
     L.total = o;
     return L;
 }
@@ -541,6 +542,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         g.Y = A; g.ty = tr; g.ldy = lda;
         g.tri = (ue == 'L') ? 3 : 4;
     }
+    g.queue = (int *)(ws + L.queue);
     const double res_fl = (double)n * n * m;
     const double res_by = ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8;
 
@@ -548,6 +550,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     int it = 0;
     for (;; ++it) {
         NLA_CK(launch_scale_copy(L.rows, L.cols, 1.0, S, L.ldr, R, L.ldr, s));
+        NLA_CK(launch_fill_u32((unsigned *)g.queue, 1, 0u, s));
         NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, [&] { return launch_gemm<double>(g, s); }));
         NLA_CK(launch_norms(L.rows, L.cols, R, L.ldr, 0, part, nrm + 4, s));
         NLA_CK(launch_norms(L.rows, L.cols, B, ldb, 0, part, nrm + 6, s));

@@ -318,10 +318,6 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
     __shared__ __attribute__((aligned(16))) T lds_y[2 * LY::LDS_ELEMS];
     int tm, tn;
     tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
-    // with a triangular operand the longest k-ranges belong to the last row
-    // (TRI 1) or column (TRI 4) tiles: start those first
-    if (TRI == 1) tm = tiles_m - 1 - tm;
-    if (TRI == 4) tn = (int)(gridDim.x / tiles_m) - 1 - tn;
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
     // interior tiles with whole k-steps take the unchecked vector path
     const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0);
@@ -335,11 +331,89 @@ __global__ __launch_bounds__(WM *WN * 64) void gemm_kernel(int64_t M, int64_t N,
                                                                 (NLA_LDS T *)lds_y);
 }
 
+// A triangular operand gives the tiles k-ranges from one k-step to all of K.
+// With one work-group per tile, the work-groups that draw the long tiles set
+// the runtime (the hardware hands a CU its next work-group in a fixed
+// rotation, so a CU keeps meeting tiles of the same rows).  Here a grid of
+// resident work-groups takes tiles from a counter instead, longest first;
+// all columns (rows) of a row (column) of tiles are taken together, so the
+// work-groups running at one time share that operand panel in L2.  *queue
+// must be zero at launch; every work-group leaves once the counter passes
+// the tile count.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
+__global__ __launch_bounds__(WM *WN * 64) void gemm_tri_queue_kernel(int64_t M, int64_t N, int64_t K, T alpha,
+                                                                     const T *__restrict__ X, int64_t ldx,
+                                                                     const T *__restrict__ Y, int64_t ldy, T beta,
+                                                                     T *C, int64_t ldc, int tiles_m, int tiles_n,
+                                                                     int *queue) {
+    static_assert(TRI >= 1 && TRI <= 4, "triangular operand expected");
+    constexpr int NT = WM * WN * 64;
+    typedef TileLoader<T, BM, BK, NT, TX, VW> LX;
+    typedef TileLoader<T, BN, BK, NT, !TY, VW> LY;
+    __shared__ __attribute__((aligned(16))) T lds_x[2 * LX::LDS_ELEMS];
+    __shared__ __attribute__((aligned(16))) T lds_y[2 * LY::LDS_ELEMS];
+    __shared__ int s_tile;
+    const int ntiles = tiles_m * tiles_n;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(queue, 1);
+        __syncthreads();
+        const int t = s_tile;
+        __syncthreads();  // read by all before thread 0 takes the next one
+        if (t >= ntiles) break;
+        int tm, tn;
+        if (TRI == 1) {  // k-range grows with the row: last rows first
+            tm = tiles_m - 1 - t / tiles_n;
+            tn = t % tiles_n;
+        } else if (TRI == 2) {
+            tm = t / tiles_n;
+            tn = t % tiles_n;
+        } else if (TRI == 3) {
+            tn = t / tiles_m;
+            tm = t % tiles_m;
+        } else {
+            tn = tiles_n - 1 - t / tiles_m;
+            tm = t % tiles_m;
+        }
+        const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
+        const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0);
+        if (interior)
+            gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI, false>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc,
+                                                                     row0, col0, (NLA_LDS T *)lds_x,
+                                                                     (NLA_LDS T *)lds_y);
+        else
+            gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI, true>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc,
+                                                                    row0, col0, (NLA_LDS T *)lds_x,
+                                                                    (NLA_LDS T *)lds_y);
+    }
+}
+
 template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW, int TRI>
 static hipError_t launch_cfg(const GemmArgs<T> &a, hipStream_t s) {
     int tiles_m = (int)((a.M + BM - 1) / BM);
     int tiles_n = (int)((a.N + BN - 1) / BN);
     int G = tiles_m * tiles_n;
+    if constexpr (TRI != 0) {
+        if (a.queue != nullptr && a.leaf == 0) {
+            constexpr int NT = WM * WN * 64;
+            static const int occ = [] {
+                int o = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &o, (const void *)gemm_tri_queue_kernel<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI>, NT, 0) !=
+                    hipSuccess)
+                    o = 1;
+                return o > 0 ? o : 1;
+            }();
+            int dev = 0, cus = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e != hipSuccess) return e;
+            int grid = cus * occ < G ? cus * occ : G;
+            hipLaunchKernelGGL((gemm_tri_queue_kernel<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI>), dim3(grid), dim3(NT),
+                               0, s, a.M, a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C, a.ldc, tiles_m,
+                               tiles_n, a.queue);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, BK, WM, WN, TX, TY, VW, TRI>), dim3(G), dim3(WM * WN * 64), 0, s, a.M,
                        a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C, a.ldc, tiles_m);
     return hipGetLastError();

@@ -31,6 +31,10 @@ struct GemmArgs {
     int64_t ldc;
     int tri = 0;
     int leaf = 0;  // 0: general; 1: left leaf (M <= 64); 2: right leaf (N <= 64)
+    // General shapes with tri != 0: a zeroed device counter through which a
+    // persistent grid takes the tiles longest first (nullptr: one
+    // work-group per tile).
+    int *queue = nullptr;
 };
 
 // Canonical leaf: L(i,j) = A[i*ars + j*acs] (lower, nb x nb),
