@@ -9,11 +9,21 @@ n^2*m flops over the mean solve time (HIP events on the solve's stream).
 Also reported on the same line:
   roofline      the dominant kernel (the MFMA GEMM update): algorithmic flops
                 over its own time, from per-launch HIP events, vs FP64 peak;
+                `traffic` is its HBM bytes per launch from the committed
+                rocprofv3 FETCH_SIZE / WRITE_SIZE profile of this workload;
   cpu_baseline  the oracle (C restatement of the reference algorithm, OpenMP
                 over the box's host cores) on a bounded sample: the same A with
                 a subset of the right-hand sides;
-  extra         FP32 and TRMM rates, the whole-operation roofline fraction and
-                the backward error of the timed solve.
+  extra         FP32 and TRMM rates, the whole-operation roofline fraction, the
+                backward error and p99 of the timed solve, and two more legs:
+                refined_mixed  FP64-accurate left-lower TRSM, n = 16384 with
+                               4096 RHS, cond_1(A) ~ 1e4: fp16 / bf16 updates
+                               with FP64 refinement to backward error <= 1e-14,
+                               against the plain FP64 solve of the same system;
+                batched        10,000 lower TRSMs (n log-uniform on [16, 1024],
+                               RHS log-uniform on [1, 256]) in one
+                               graph-captured persistent launch, against a
+                               loop of plain calls.
 """
 import argparse
 import json
@@ -70,6 +80,142 @@ def backward_error(torch, side, uplo, trans, alpha, A, B0, X):
     return (torch.linalg.norm(R) / den).item()
 
 
+def traffic_from_profile(n, m):
+    """HBM bytes per launch of the GEMM update measured by rocprofv3 for this
+    workload (profiles/, n = m = 8192 only), or None."""
+    import re
+    if (n, m) != (8192, 8192):
+        return None
+    path = os.path.join(ROOT, "profiles", "gemm_kernel_f64_trsm_n8192_pmc_traffic.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    hit = re.search(r"traffic_bytes_per_launch:\s*(\d+)", txt)
+    return int(hit.group(1)) if hit else None
+
+
+def event_ms(torch, fn, reps):
+    """Mean over reps of fn's time on the current stream (HIP events)."""
+    s = torch.cuda.current_stream()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return sum(ts) / len(ts), ts
+
+
+def mixed_leg(nla, torch):
+    """FP64-accurate TRSM by fp16 / bf16 correction solves and FP64
+    refinement vs the plain FP64 solve on the same system."""
+    from nextla_amd import inputs
+    from nextla_amd.mixed import rectrxm_mixed_, workspace_bytes
+    n, m = 16384, 4096
+    A, B0, c, kappa = inputs.calibrate_precision_tri(n, m, 1e4, seed=5)
+    P = B0.clone()
+    ws = torch.empty(workspace_bytes("L", n, m) + 256, dtype=torch.uint8, device=A.device)
+
+    def plain():
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, P)
+
+    for _ in range(2):
+        P.copy_(B0)
+        plain()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        P.copy_(B0)
+        t, _ = event_ms(torch, plain, 1)
+        ts.append(t)
+    tp = sum(ts) / len(ts)
+    out = {"workload": "left-lower TRSM n=16384, 4096 RHS N(0,1); A off-diagonal U(-1,1)/sqrt(n), "
+                       "log-spaced diagonal calibrated to cond_1 ~ 1e4; target backward error 1e-14",
+           "cond1_measured": kappa, "plain_fp64_ms": round(tp, 3),
+           "plain_fp64_tflops": round(float(n) * n * m / tp / 1e9, 3)}
+    X = B0.clone()
+    for prec in ("fp16", "bf16"):
+        X.copy_(B0)
+        rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws, raise_on_stall=False)
+        torch.cuda.synchronize()
+        ts, info = [], None
+        for _ in range(2):
+            X.copy_(B0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, info = rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws, raise_on_stall=False)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        tm = sum(ts) / len(ts)
+        fwd = (torch.linalg.norm(X - P) / torch.linalg.norm(P)).item()
+        out[prec] = {"ms": round(tm, 3), "speedup_vs_plain_fp64": round(tp / tm, 4),
+                     "effective_tflops": round(float(n) * n * m / tm / 1e9, 3), "iterations": info["iterations"],
+                     "final_update_precision": info["precision"], "backward_error": info["backward_error"],
+                     "converged": info["converged"], "rel_diff_vs_plain_fp64": fwd}
+        log(f"bench: refined {prec}: {tm:.2f} ms vs plain {tp:.2f} ms ({tp / tm:.3f}x), "
+            f"{info['iterations']} steps, bwd {info['backward_error']:.2e}")
+    del A, B0, P, X, ws
+    torch.cuda.empty_cache()
+    return out
+
+
+def batch_leg(nla, torch, dtype, count=10000, seed=0):
+    """10,000 independent lower TRSMs in one graph-captured persistent launch
+    vs a loop of plain calls; roofline summed per system."""
+    import numpy as np
+    from nextla_amd.batch import BatchPlan
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    ns = np.exp(rng.uniform(np.log(16), np.log(1024), count)).astype(int)
+    ms = np.exp(rng.uniform(np.log(1), np.log(256), count)).astype(int)
+    As, B0s = [], []
+    for n, m in zip(ns, ms):
+        A = torch.empty((n, n), dtype=dtype, device="cuda").t()
+        A.uniform_(-1.0, 1.0, generator=g).mul_(1.0 / n)
+        A.diagonal().uniform_(1.0, 2.0, generator=g)
+        As.append(A)
+        B0s.append(torch.empty((m, n), dtype=dtype, device="cuda").t().normal_(generator=g))
+    Bs = [b.clone() for b in B0s]
+    s = 8 if dtype == torch.float64 else 4
+    flops = float(np.sum(ns.astype(float) ** 2 * ms))
+    peak = PEAK["f64" if dtype == torch.float64 else "f32"] * 1e12
+    t_roof = float(np.sum(np.maximum(ns.astype(float) ** 2 * ms / peak,
+                                     (ns * (ns + 1) / 2.0 + 2.0 * ns * ms) * s / (HBM_TBS * 1e12))))
+    plan = BatchPlan("L", "L", "N", "S", As, Bs, 1.0)
+
+    def restore():
+        for b, b0 in zip(Bs, B0s):
+            b.copy_(b0)
+
+    for _ in range(2):
+        restore()
+        plan.execute(use_graph=True)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        restore()
+        t, _ = event_ms(torch, lambda: plan.execute(use_graph=True), 1)
+        ts.append(t)
+    tb = sum(ts) / len(ts)
+    restore()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for A, B in zip(As, Bs):
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, B)
+    torch.cuda.synchronize()
+    tl = (time.perf_counter() - t0) * 1e3
+    plan.close()
+    del As, Bs, B0s
+    torch.cuda.empty_cache()
+    return {"systems": count, "ms": round(tb, 3), "tflops": round(flops / tb / 1e9, 3),
+            "roofline_frac": round(t_roof * 1e3 / tb, 4), "loop_of_plain_calls_ms": round(tl, 1),
+            "speedup_vs_loop": round(tl / tb, 1)}
+
+
 def cpu_baseline(A_host, n, seconds_target=12.0):
     """The oracle on host cores over a bounded number of RHS columns: a short
     calibration run at 128 columns, then a sample sized for ~seconds_target."""
@@ -113,7 +259,7 @@ def main():
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM extra rates")
+    ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM rates and the mixed and batched legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-GPU by design "
@@ -148,8 +294,9 @@ def main():
     _lib.prof_enable(False)
     g_tf = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     roofline = {"bound": "mfma", "achieved": round(g_tf, 3), "peak": PEAK["f64"], "unit": "TFLOP/s",
-                "frac": round(g_tf / PEAK["f64"], 4), "traffic": None,
+                "frac": round(g_tf / PEAK["f64"], 4), "traffic": traffic_from_profile(n, m),
                 "kernel": "gemm_kernel<double> (MFMA f64 16x16x4 GEMM update)",
+                "traffic_kernel": "gemm_kernel<double,256,128,...> (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
                 "launches_per_solve": g_n // 3, "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
                 "gemm_time_share": round(g_ms / max(g_ms + l_ms, 1e-9), 4)}
     leaf = {"launches_per_solve": l_n // 3, "avg_launch_ms": round(l_ms / max(l_n, 1), 4),
@@ -167,6 +314,14 @@ def main():
             mm = sum(t2) / len(t2)
             extra[label + "_tflops"] = round(flops / (mm * 1e-3) / 1e12, 3)
             log(f"bench: {label} {extra[label + '_tflops']} TFLOP/s")
+        torch.cuda.empty_cache()
+        extra["refined_mixed"] = mixed_leg(nla, torch)
+        extra["batched"] = {}
+        for label, dt in (("f64", torch.float64), ("f32", torch.float32)):
+            extra["batched"][label] = batch_leg(nla, torch, dt)
+            b = extra["batched"][label]
+            log(f"bench: batched {label}: {b['ms']} ms ({b['tflops']} TFLOP/s, roofline {b['roofline_frac']}), "
+                f"loop of plain calls {b['loop_of_plain_calls_ms']} ms")
     torch.cuda.synchronize()
 
     cpu = None
