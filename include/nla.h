@@ -16,11 +16,13 @@
  *   - Return value: 0 on success; -k when argument k (1-based) is invalid,
  *     checked before any device work, so B is untouched on error; a positive
  *     NLA_ERR_* code when the HIP runtime reports a failure.
- *   - Device memory: the plain TRSM path takes one buffer for the inverses
- *     of its diagonal 64-blocks (n*64 elements) from the stream-ordered pool
- *     (hipMallocAsync) and returns it on the same stream; TRMM, the GEMM
- *     entry points and the leaves allocate nothing.  The batched and
- *     mixed-precision forms work in a caller-provided workspace.
+ *   - Device memory: the plain TRSM path keeps one buffer per (device,
+ *     stream) for the inverses of its diagonal 64-blocks (n*64 elements),
+ *     allocated in stream order the first time a stream needs it and grown
+ *     when a larger n arrives; it is reused by later calls on that stream and
+ *     held until the process ends.  TRMM, the GEMM entry points and the
+ *     leaves allocate nothing.  The batched and mixed-precision forms work in
+ *     a caller-provided workspace.
  */
 #ifndef NLA_H
 #define NLA_H

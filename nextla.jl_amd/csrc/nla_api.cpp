@@ -25,6 +25,7 @@
 #include <cstring>
 #include <atomic>
 #include <cmath>
+#include <memory>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -366,6 +367,57 @@ struct Recursion {
     }
 };
 
+// Workspace of the plain TRSM (the inverted diagonal blocks), cached per
+// (device, stream) and grown on demand.  A stream-ordered malloc / free per
+// call made the host wait once a stream had a backlog (the pool cannot hand
+// back a block whose free has not executed yet), so a pipeline of solves ran
+// no faster than one at a time.  Calls on one stream are ordered, so its
+// buffer is reused without any wait; the entry's lock keeps two host threads
+// that enqueue on the same stream from interleaving their launches.  A
+// larger request frees the old buffer in stream order and takes a new one.
+struct StreamWorkspace {
+    int dev;
+    hipStream_t stream;
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    std::mutex mu;
+};
+
+static StreamWorkspace *stream_workspace(hipStream_t s, hipError_t *err) {
+    static std::mutex mu;
+    static std::vector<std::unique_ptr<StreamWorkspace>> cache;
+    int dev = 0;
+    *err = hipGetDevice(&dev);
+    if (*err != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &w : cache)
+        if (w->dev == dev && w->stream == s) return w.get();
+    cache.emplace_back(new StreamWorkspace());
+    cache.back()->dev = dev;
+    cache.back()->stream = s;
+    return cache.back().get();
+}
+
+// Called with w->mu held.
+static hipError_t reserve(StreamWorkspace *w, size_t bytes) {
+    if (bytes <= w->bytes) return hipSuccess;
+    hipError_t e = hipSuccess;
+    if (w->ptr) {
+        e = hipFreeAsync(w->ptr, w->stream);
+        w->ptr = nullptr;
+        w->bytes = 0;
+        if (e != hipSuccess) return e;
+    }
+    size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+    e = hipMallocAsync(&w->ptr, want, w->stream);
+    if (e != hipSuccess) {
+        w->ptr = nullptr;
+        return e;
+    }
+    w->bytes = want;
+    return hipSuccess;
+}
+
 template <typename T>
 static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double alpha, int64_t n, int64_t m,
                              const void *Ap, int64_t lda, void *Bp, int64_t ldb, hipStream_t s) {
@@ -386,27 +438,26 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     r.mode = leaf_mode_default();
     r.use_panel = panel_default();
     T *W = nullptr;
+    std::unique_lock<std::mutex> hold;
     if (func == 'S' && r.mode == 1) {
-        // inverses of all diagonal leaf blocks, from one batched launch; the
-        // buffer comes from the stream-ordered pool and returns to it below
+        // inverses of all diagonal leaf blocks, from one batched launch, into
+        // this stream's cached workspace
         const int64_t blocks = (n + r.nb - 1) / r.nb;
-        hipError_t e = hipMallocAsync((void **)&W, (size_t)(blocks * r.nb * r.nb) * sizeof(T), s);
+        hipError_t e = hipSuccess;
+        StreamWorkspace *w = stream_workspace(s, &e);
+        if (!w) return hip_status(e, "workspace");
+        hold = std::unique_lock<std::mutex>(w->mu);
+        e = reserve(w, (size_t)(blocks * r.nb * r.nb) * sizeof(T));
         if (e != hipSuccess) return hip_status(e, "workspace");
+        W = (T *)w->ptr;
         const double fl = (double)blocks * r.nb * r.nb * r.nb / 3.0;
         const double by = (double)blocks * r.nb * r.nb * 1.5 * sizeof(T);
         e = timed(NLA_KIND_LEAF, fl, by, s,
                   [&] { return launch_trtri_blocks<T>(n, (int)r.nb, A.p, A.rs, A.cs, uplo == 'L', W, s); });
-        if (e != hipSuccess) {
-            (void)hipFreeAsync(W, s);
-            return hip_status(e, "diagonal inverses");
-        }
+        if (e != hipSuccess) return hip_status(e, "diagonal inverses");
         r.W = W;
     }
     r.rec(A, B, n, 0, (T)alpha, (T)alpha);
-    if (W) {
-        hipError_t e = hipFreeAsync(W, s);
-        if (r.err == hipSuccess) r.err = e;
-    }
     return hip_status(r.err, "kernel launch");
 }
 

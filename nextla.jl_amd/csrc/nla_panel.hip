@@ -143,7 +143,9 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
 
     const int lo = lane % MF::BD, lk = lane / MF::BD;
     const int rb = PS::rb0(wave), cbase = PS::cb0(wave);
-    typename MF::acc_t acc[PS::CBW];
+    // two accumulator sets (even / odd k-steps) keep two independent MFMA
+    // chains in flight per wave; they are summed when a block is written
+    typename MF::acc_t acc[PS::CBW], acc2[PS::CBW];
     auto bsize = [&](int b) { return (b == nsb - 1) ? nbig - 64 * b : 64; };
 
     // M-operand fragment of phase ph: op(i, k) for this lane's row.  Elements
@@ -188,7 +190,8 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
             for (int cb = 0; cb < PS::CBW; ++cb) {
                 const int col = (cbase + cb) * MF::BD + lo;
                 T bv = src[col * LDB + s * MF::KD + lk];
-                acc[cb] = MF::mma(a, bv, acc[cb]);
+                if (s & 1) acc2[cb] = MF::mma(a, bv, acc2[cb]);
+                else acc[cb] = MF::mma(a, bv, acc[cb]);
             }
         }
     };
@@ -201,14 +204,17 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
                 const int row = rb * MF::BD + MF::di(lane, r);
                 const int col = (cbase + cb) * MF::BD + MF::dj(lane, r);
                 NLA_LDS T *p = dst + col * LDB + row;
-                T v = alpha * acc[cb][r];
+                T v = alpha * (acc[cb][r] + acc2[cb][r]);
                 *p = (beta == T(0)) ? v : v + beta * *p;
             }
     };
     auto run = [&](const Phase &ph, const T *af, uint32_t okb) {
         if (ph.first) {
 #pragma unroll
-            for (int cb = 0; cb < PS::CBW; ++cb) acc[cb] = MF::zero();
+            for (int cb = 0; cb < PS::CBW; ++cb) {
+                acc[cb] = MF::zero();
+                acc2[cb] = MF::zero();
+            }
         }
         compute(ph, af, okb);
         if (!TRMM) {
