@@ -201,12 +201,23 @@ int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, d
  * (NLA_LEAF_MODE=subst, NLA_PANEL=0) and can be changed between calls:
  *   NLA_CFG_LEAF_MODE: 1 = inverted 64-blocks applied on MFMA (default),
  *                      0 = register/LDS substitution leaf;
- *   NLA_CFG_PANEL:     1 = diagonal blocks up to 512 solved by one fused
- *                      panel launch (default, needs leaf mode 1), 0 = recurse
- *                      to 64-leaves.
- * Returns the previous value, or -1 for an unknown key.
+ *   NLA_CFG_PANEL:     1 = diagonal blocks up to the panel size solved by
+ *                      one fused panel launch (default, needs leaf mode 1),
+ *                      0 = recurse to 64-leaves;
+ *   NLA_CFG_PANEL_ROWS_F32 / _F64: panel size for that dtype, a multiple of
+ *                      64 in [64, 512] (defaults 256 for FP32, 512 for FP64);
+ *   NLA_CFG_PANEL_STRIP_F32 / _F64: right-hand-side columns per panel
+ *                      work-group, 16 (default) or 32.
+ * Returns the previous value, or -1 for an unknown key or invalid value.
  */
-enum nla_config_key { NLA_CFG_LEAF_MODE = 0, NLA_CFG_PANEL = 1 };
+enum nla_config_key {
+    NLA_CFG_LEAF_MODE = 0,
+    NLA_CFG_PANEL = 1,
+    NLA_CFG_PANEL_ROWS_F32 = 2,
+    NLA_CFG_PANEL_ROWS_F64 = 3,
+    NLA_CFG_PANEL_STRIP_F32 = 4,
+    NLA_CFG_PANEL_STRIP_F64 = 5
+};
 
 int32_t nla_set_config(int32_t key, int32_t value);
 

@@ -106,6 +106,13 @@ struct View {
 // environment.
 static std::atomic<int> g_leaf_mode{-1};
 static std::atomic<int> g_panel{-1};
+// Panel shape per dtype (index NLA_F32 / NLA_F64): the largest diagonal
+// block handed to one fused panel launch (a multiple of 64, <= PANEL_MAX)
+// and its RHS columns per work-group.  Defaults from a sweep on MI355X at
+// n = 8192 (rows 256/384/512 x strips 16/32): FP64 512 rows, FP32 256 rows
+// (3-5 % faster than 512 there), 16 columns for both.
+static std::atomic<int> g_panel_rows[2] = {{256}, {PANEL_MAX}};
+static std::atomic<int> g_panel_strip[2] = {{PANEL_STRIP}, {PANEL_STRIP}};
 
 // Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),
 // 0 = register/LDS substitution kernel.
@@ -151,7 +158,8 @@ struct Recursion {
     hipStream_t stream;
     const T *W = nullptr;   // inverted diagonal blocks (TRSM, leaf mode 1)
     int mode = 1;
-    bool use_panel = true;  // fuse diagonal blocks <= PANEL_MAX into one launch
+    bool use_panel = true;  // fuse diagonal blocks <= panel_rows into one launch
+    int panel_rows = PANEL_MAX, panel_strip = PANEL_STRIP;
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
     // -1 plain fp32.  Each update takes two absolute-maximum slots from amax
@@ -219,6 +227,7 @@ struct Recursion {
             a.B = B.p; a.brs = B.cs; a.bcs = B.rs;
             a.wrs = nb; a.wcs = 1;
         }
+        a.strip = panel_strip;
         const double fl = (double)n * (double)n * (double)a.m;
         const double by = ((double)n * (n + 1) / 2 + 2.0 * n * a.m) * sizeof(T);
         err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_panel<T>(a, stream); });
@@ -317,7 +326,7 @@ struct Recursion {
     // off: position of this diagonal block in the whole matrix.
     void rec(View<T> A, View<T> B, int64_t n, int64_t off, T pre, T post) {
         if (err != hipSuccess) return;
-        if (mode == 1 && nb == 64 && n <= PANEL_MAX && use_panel) {
+        if (mode == 1 && nb == 64 && n <= panel_rows && use_panel) {
             panel(A, B, n, off, func == 'S' ? pre : post);
             return;
         }
@@ -437,6 +446,11 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     r.stream = s;
     r.mode = leaf_mode_default();
     r.use_panel = panel_default();
+    {
+        const int di = std::is_same<T, double>::value ? NLA_F64 : NLA_F32;
+        r.panel_rows = g_panel_rows[di].load();
+        r.panel_strip = g_panel_strip[di].load();
+    }
     T *W = nullptr;
     std::unique_lock<std::mutex> hold;
     if (func == 'S' && r.mode == 1) {
@@ -770,6 +784,14 @@ int32_t nla_set_config(int32_t key, int32_t value) {
         int prev = panel_default() ? 1 : 0;
         g_panel.store(value ? 1 : 0);
         return prev;
+    }
+    if (key == NLA_CFG_PANEL_ROWS_F32 || key == NLA_CFG_PANEL_ROWS_F64) {
+        if (value < 64 || value > PANEL_MAX || value % 64 != 0) return -1;
+        return g_panel_rows[key == NLA_CFG_PANEL_ROWS_F64 ? NLA_F64 : NLA_F32].exchange(value);
+    }
+    if (key == NLA_CFG_PANEL_STRIP_F32 || key == NLA_CFG_PANEL_STRIP_F64) {
+        if (value != 16 && value != 32) return -1;
+        return g_panel_strip[key == NLA_CFG_PANEL_STRIP_F64 ? NLA_F64 : NLA_F32].exchange(value);
     }
     return -1;
 }

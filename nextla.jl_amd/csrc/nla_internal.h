@@ -53,7 +53,7 @@ struct LeafArgs {
 };
 
 // Largest diagonal block the fused panel kernel solves in one launch, and the
-// RHS columns per work-group.
+// default RHS columns per work-group (16 or 32 per launch).
 constexpr int PANEL_MAX = 512;
 constexpr int PANEL_STRIP = 16;
 
@@ -72,6 +72,7 @@ struct PanelArgs {
     const T *W;
     int64_t wrs, wcs;
     T scale;
+    int strip = PANEL_STRIP;  // RHS columns per work-group: 16 or 32
 };
 
 template <typename T>

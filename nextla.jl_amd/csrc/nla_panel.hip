@@ -128,16 +128,46 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
     const int nsb = (nbig + 63) / 64;
 
     // ---- load the strip (coalesced along B's contiguous direction) ----
+    // Each thread issues a batch of independent loads before it stores any
+    // of them, so their latencies overlap; no division per element.
+    //   rowc (unit row stride): thread t takes rows t, t+256, ... of every
+    //   column; otherwise (unit column stride) it takes column t % W of rows
+    //   t / W, t / W + 256 / W, ...
     const bool rowc = (brs == 1 || brs == -1);
     const int nrow = nsb * 64;  // rows past nbig up to the 64-boundary are zeroed
-#pragma unroll 8
-    for (int e = t; e < W * nrow; e += 256) {
-        int r = rowc ? e % nrow : e / W;
-        int c = rowc ? e / nrow : e % W;
-        int64_t gc = c0 + c;
-        bool ok = r < nbig && gc < m;
-        T v = *(ok ? Bp + r * brs + gc * bcs : Bp);
-        bs[c * LDB + r] = ok ? (TRMM ? v : v * scale) : T(0);
+    static_assert(256 % W == 0, "strip width must divide the work-group");
+    constexpr int RSTEP = 256 / W;
+    if (rowc) {
+        for (int r = t; r < nrow; r += 256) {
+            T v[W];
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const bool ok = r < nbig && c0 + c < m;
+                v[c] = *(ok ? Bp + r * brs + (c0 + c) * bcs : Bp);
+                v[c] = ok ? (TRMM ? v[c] : v[c] * scale) : T(0);
+            }
+#pragma unroll
+            for (int c = 0; c < W; ++c) bs[c * LDB + r] = v[c];
+        }
+    } else {
+        const int c = t % W;
+        const bool cok = c0 + c < m;
+        constexpr int SL = 8;
+        for (int r0 = t / W; r0 < nrow; r0 += RSTEP * SL) {
+            T v[SL];
+#pragma unroll
+            for (int u = 0; u < SL; ++u) {
+                const int r = r0 + u * RSTEP;
+                const bool ok = cok && r < nbig;
+                v[u] = *(ok ? Bp + r * brs + (c0 + c) * bcs : Bp);
+                v[u] = ok ? (TRMM ? v[u] : v[u] * scale) : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < SL; ++u) {
+                const int r = r0 + u * RSTEP;
+                if (r < nrow) bs[c * LDB + r] = v[u];
+            }
+        }
     }
     __syncthreads();
 
@@ -183,15 +213,22 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
     };
     auto compute = [&](const Phase &ph, const T *af, uint32_t okb) {
         const NLA_LDS T *src = bs + ph.k * 64;
+        // all LDS operands of the phase are requested before the first MFMA
+        T bv[NKS][PS::CBW];
+#pragma unroll
+        for (int s = 0; s < NKS; ++s)
+#pragma unroll
+            for (int cb = 0; cb < PS::CBW; ++cb)
+                bv[s][cb] = src[((cbase + cb) * MF::BD + lo) * LDB + s * MF::KD + lk];
+        // keep the scheduler from sinking the reads back next to their MFMAs
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int s = 0; s < NKS; ++s) {
             const T a = ((okb >> s) & 1u) ? af[s] : T(0);
 #pragma unroll
             for (int cb = 0; cb < PS::CBW; ++cb) {
-                const int col = (cbase + cb) * MF::BD + lo;
-                T bv = src[col * LDB + s * MF::KD + lk];
-                if (s & 1) acc2[cb] = MF::mma(a, bv, acc2[cb]);
-                else acc[cb] = MF::mma(a, bv, acc[cb]);
+                if (s & 1) acc2[cb] = MF::mma(a, bv[s][cb], acc2[cb]);
+                else acc[cb] = MF::mma(a, bv[s][cb], acc[cb]);
             }
         }
     };
@@ -233,36 +270,54 @@ __device__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict
         }
     };
 
-    // phases with the M-operand of the next phase in flight (two register
-    // buffers, the loop body written twice so both stay statically indexed)
+    // phases with the M-operand fragments of the next PD - 1 phases in
+    // flight: a ring of PD register buffers, the loop body unrolled PD
+    // times so every buffer index is a compile-time constant.  Fetches are
+    // unconditional (past the last phase they repeat it): a fetch under a
+    // branch made the compiler wait for it at the join, before the phase
+    // that did not need it.
+    constexpr int PD = 2;
     const int np = phase_count(nsb, TRMM);
-    T fa[NKS], fb[NKS];
-    uint32_t oka = 0, okb = 0;
-    Phase pa = phase_of(0, nsb, lower, TRMM), pb;
-    fetch(pa, fa, &oka);
-    for (int p = 0; p < np; p += 2) {
-        const bool hb = p + 1 < np;
-        if (hb) {
-            pb = phase_of(p + 1, nsb, lower, TRMM);
-            fetch(pb, fb, &okb);
-        }
-        run(pa, fa, oka);
-        if (!hb) break;
-        const bool ha = p + 2 < np;
-        if (ha) {
-            pa = phase_of(p + 2, nsb, lower, TRMM);
-            fetch(pa, fa, &oka);
-        }
-        run(pb, fb, okb);
+    T fr[PD][NKS];
+    uint32_t okr[PD];
+    Phase phr[PD];
+#pragma unroll
+    for (int j = 0; j < PD - 1; ++j) {
+        phr[j] = phase_of(j < np ? j : np - 1, nsb, lower, TRMM);
+        fetch(phr[j], fr[j], &okr[j]);
     }
+    const int np_full = np - np % PD;
+    for (int p = 0; p < np_full; p += PD) {
+#pragma unroll
+        for (int u = 0; u < PD; ++u) {
+            const int nxt = p + u + PD - 1;
+            const int slot = (u + PD - 1) % PD;
+            phr[slot] = phase_of(nxt < np ? nxt : np - 1, nsb, lower, TRMM);
+            fetch(phr[slot], fr[slot], &okr[slot]);
+            run(phr[u], fr[u], okr[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PD - 1; ++u)
+        if (np_full + u < np) run(phr[u], fr[u], okr[u]);
 
     // ---- write the strip back ----
-#pragma unroll 8
-    for (int e = t; e < W * nrow; e += 256) {
-        int r = rowc ? e % nrow : e / W;
-        int c = rowc ? e / nrow : e % W;
-        int64_t gc = c0 + c;
-        if (r < nbig && gc < m) Bp[r * brs + gc * bcs] = TRMM ? bs[c * LDB + r] * scale : bs[c * LDB + r];
+    if (rowc) {
+        for (int r = t; r < nbig; r += 256) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const T v = bs[c * LDB + r];
+                if (c0 + c < m) Bp[r * brs + (c0 + c) * bcs] = TRMM ? v * scale : v;
+            }
+        }
+    } else {
+        const int c = t % W;
+        if (c0 + c < m) {
+            for (int r = t / W; r < nbig; r += RSTEP) {
+                const T v = bs[c * LDB + r];
+                Bp[r * brs + (c0 + c) * bcs] = TRMM ? v * scale : v;
+            }
+        }
     }
 }
 
@@ -277,11 +332,8 @@ __global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T
                                    wcs, scale, (NLA_LDS T *)bs_raw);
 }
 
-template <typename T>
-hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s) {
-    if (a.m <= 0 || a.nbig <= 0) return hipSuccess;
-    if (a.nbig > PANEL_MAX) return hipErrorInvalidValue;
-    constexpr int W = PANEL_STRIP;
+template <typename T, int W>
+static hipError_t launch_panel_w(const PanelArgs<T> &a, hipStream_t s) {
     const int64_t groups = (a.m + W - 1) / W;
     if (a.trmm)
         hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
@@ -290,6 +342,15 @@ hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s) {
         hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
                            a.m, a.M, a.mrs, a.mcs, a.lower ? 1 : 0, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s) {
+    if (a.m <= 0 || a.nbig <= 0) return hipSuccess;
+    if (a.nbig > PANEL_MAX) return hipErrorInvalidValue;
+    if (a.strip == 32) return launch_panel_w<T, 32>(a, s);
+    if (a.strip != 16) return hipErrorInvalidValue;
+    return launch_panel_w<T, 16>(a, s);
 }
 
 template hipError_t launch_panel<float>(const PanelArgs<float> &, hipStream_t);

@@ -67,6 +67,10 @@ NLA_LP_FP32 = 2
 NLA_ERR_NOT_CONVERGED = 5
 NLA_CFG_LEAF_MODE = 0
 NLA_CFG_PANEL = 1
+NLA_CFG_PANEL_ROWS_F32 = 2
+NLA_CFG_PANEL_ROWS_F64 = 3
+NLA_CFG_PANEL_STRIP_F32 = 4
+NLA_CFG_PANEL_STRIP_F64 = 5
 
 
 def set_config(key, value):

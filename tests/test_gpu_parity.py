@@ -153,6 +153,44 @@ def test_recursion_all_combos(nla, method, dt, n, m):
         assert O.rel_error(X, ref) < tol, (side, uplo, trans, func, O.rel_error(X, ref))
 
 
+PANEL_SHAPES = [(256, 16), (512, 32), (192, 32)]
+
+
+@pytest.mark.parametrize("rows,strip", PANEL_SHAPES)
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_panel_shapes(nla, rows, strip, dt):
+    """Every panel size / strip width the configuration (and a tuner) may
+    pick gives the same results as the oracle: all combos at sizes around
+    the panel boundaries."""
+    from nextla_amd import _lib
+    keys = ((_lib.NLA_CFG_PANEL_ROWS_F64, _lib.NLA_CFG_PANEL_STRIP_F64) if dt == np.float64
+            else (_lib.NLA_CFG_PANEL_ROWS_F32, _lib.NLA_CFG_PANEL_STRIP_F32))
+    old = (_lib.set_config(keys[0], rows), _lib.set_config(keys[1], strip))
+    assert old[0] > 0 and old[1] > 0
+    try:
+        rng = np.random.default_rng(rows + strip)
+        for n, m in ((rows, 33), (rows + 70, 50), (3 * rows + 5, 40)):
+            for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
+                A = O._f(np.tril(rng.uniform(-1, 1, (n, n)) / np.sqrt(n)) + np.diag(2 + rng.random(n))).astype(dt)
+                if uplo == "U":
+                    A = np.asfortranarray(A.T)
+                B = rng.standard_normal((n, m) if side == "L" else (m, n)).astype(dt)
+                alpha = 1.5 if func == "S" else -0.5
+                X = run(nla, side, uplo, trans, alpha, func, A, B)
+                check_bound(side, uplo, trans, alpha, func, A, B, X)
+    finally:
+        _lib.set_config(keys[0], old[0])
+        _lib.set_config(keys[1], old[1])
+
+
+def test_config_rejects_invalid_panel_shapes(nla):
+    from nextla_amd import _lib
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 100) == -1
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 1024) == -1
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_STRIP_F32, 24) == -1
+    assert _lib.set_config(99, 1) == -1
+
+
 def test_odd_leading_dimensions(nla):
     """Sub-matrix views with odd lda/ldb take the unaligned (scalar) load path."""
     rng = np.random.default_rng(9)
