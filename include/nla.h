@@ -207,7 +207,12 @@ int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, d
  *   NLA_CFG_PANEL_ROWS_F32 / _F64: panel size for that dtype, a multiple of
  *                      64 in [64, 512] (defaults 256 for FP32, 512 for FP64);
  *   NLA_CFG_PANEL_STRIP_F32 / _F64: right-hand-side columns per panel
- *                      work-group, 16 (default) or 32.
+ *                      work-group, 16 (default) or 32;
+ *   NLA_CFG_BATCH_CLASSES: size classes new batch plans may use, bit c for
+ *                      systems up to 128 << c rows (c = 0..3; the 1024 class
+ *                      is always on; default all).  A system goes to the
+ *                      smallest enabled class that holds it; smaller classes
+ *                      use less LDS per work-group, so more of them run per CU.
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {
@@ -216,7 +221,8 @@ enum nla_config_key {
     NLA_CFG_PANEL_ROWS_F32 = 2,
     NLA_CFG_PANEL_ROWS_F64 = 3,
     NLA_CFG_PANEL_STRIP_F32 = 4,
-    NLA_CFG_PANEL_STRIP_F64 = 5
+    NLA_CFG_PANEL_STRIP_F64 = 5,
+    NLA_CFG_BATCH_CLASSES = 6
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);

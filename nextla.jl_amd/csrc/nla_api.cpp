@@ -113,6 +113,9 @@ static std::atomic<int> g_panel{-1};
 // (3-5 % faster than 512 there), 16 columns for both.
 static std::atomic<int> g_panel_rows[2] = {{256}, {PANEL_MAX}};
 static std::atomic<int> g_panel_strip[2] = {{PANEL_STRIP}, {PANEL_STRIP}};
+static std::atomic<int> g_batch_classes{0xF};
+
+int batch_class_mask() { return g_batch_classes.load(); }
 
 // Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),
 // 0 = register/LDS substitution kernel.
@@ -788,6 +791,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_PANEL_ROWS_F32 || key == NLA_CFG_PANEL_ROWS_F64) {
         if (value < 64 || value > PANEL_MAX || value % 64 != 0) return -1;
         return g_panel_rows[key == NLA_CFG_PANEL_ROWS_F64 ? NLA_F64 : NLA_F32].exchange(value);
+    }
+    if (key == NLA_CFG_BATCH_CLASSES) {
+        if (value < 0 || value > 0xF) return -1;
+        return g_batch_classes.exchange(value | 0x8);
     }
     if (key == NLA_CFG_PANEL_STRIP_F32 || key == NLA_CFG_PANEL_STRIP_F64) {
         if (value != 16 && value != 32) return -1;

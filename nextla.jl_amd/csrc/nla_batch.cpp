@@ -4,11 +4,14 @@
 // A plan puts every system into the panel kernel's canonical form (the same
 // transformation nla_rectrxm applies to one system: op(A) as a strided view
 // with flipped uplo for 'T'/'C'; right-side systems solved on the transpose),
-// lists one task per (system, 16-column strip), longest first (the cost of a
-// strip grows with the square of the number of 64-blocks), and uploads the
-// descriptors into the caller's workspace.  Executing a plan launches three
-// kernels on the caller's stream and nothing else, so it can be captured in a
-// hipGraph and replayed.
+// lists one task per (system, 16-column strip), grouped by size class (the
+// LDS strip of the kernel that solves a class fits its largest system, so
+// small systems run at higher occupancy) and longest first within a class
+// (the cost of a strip grows with the square of the number of 64-blocks), and
+// uploads the descriptors into the caller's workspace.  Executing a plan
+// launches the inversion kernel (TRSM) and, per non-empty class, a counter
+// reset and the persistent solver, on the caller's stream and nothing else,
+// so it can be captured in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -59,7 +62,9 @@ struct nla_batch_plan_s {
     int32_t dtype;
     bool trmm;
     char *ws;
-    int ntasks, nblk, groups;
+    int ntasks, nblk;
+    // per size class (nla::BATCH_CLASSES): first task, task count, grid
+    int cls_nbig[4], cls_start[4], cls_count[4], cls_groups[4];
     nla::Layout L;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -118,11 +123,33 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
         }
         for (int64_t s = 0; s < (it.m + 15) / 16; ++s) tasks.push_back(BatchTask{(int)i, (int)s});
     }
-    // longest tasks first: cost of a strip ~ number of 64-block phases
+    // Size class of a system: the smallest enabled class that holds it (the
+    // largest class is always enabled).  Tasks are grouped by class, largest
+    // class first, and longest first within a class (the cost of a strip
+    // grows with the number of 64-block phases).
+    const int mask = batch_class_mask();
+    auto cls_of = [&](int n) {
+        for (int c = 0; c < 4; ++c)
+            if (((mask >> c) & 1 || c == 3) && n <= BATCH_CLASSES[c]) return c;
+        return 3;
+    };
     std::stable_sort(tasks.begin(), tasks.end(), [&](const BatchTask &a, const BatchTask &b) {
-        int na = (sys[(size_t)a.sys].n + 63) / 64, nb2 = (sys[(size_t)b.sys].n + 63) / 64;
-        return na * (na + 1) > nb2 * (nb2 + 1);
+        const int na = sys[(size_t)a.sys].n, nb2 = sys[(size_t)b.sys].n;
+        const int ca = cls_of(na), cb = cls_of(nb2);
+        if (ca != cb) return ca > cb;
+        const int pa = (na + 63) / 64, pb = (nb2 + 63) / 64;
+        return pa * (pa + 1) > pb * (pb + 1);
     });
+    for (int c = 0; c < 4; ++c) {
+        P->cls_nbig[c] = BATCH_CLASSES[c];
+        P->cls_start[c] = 0;
+        P->cls_count[c] = 0;
+        P->cls_groups[c] = 0;
+    }
+    for (size_t k = 0; k < tasks.size(); ++k) {
+        const int c = cls_of(sys[(size_t)tasks[k].sys].n);
+        if (P->cls_count[c]++ == 0) P->cls_start[c] = (int)k;
+    }
     // one staged image of the descriptor region, copied in stream order
     P->host.assign((size_t)(L.inv - L.sys), 0);
     std::memcpy(P->host.data(), sys.data(), sys.size() * sizeof(BatchSys<T>));
@@ -134,15 +161,20 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
     if (e != hipSuccess) return report_hip(e, "batch plan upload");
     P->ntasks = (int)tasks.size();
     P->nblk = (int)blk.size();
-    // persistent grid: every CU filled to the kernel's occupancy
-    int dev = 0, cus = 0, per_cu = 0;
+    // persistent grid per class: every CU filled to that kernel's occupancy
+    int dev = 0, cus = 0;
     e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_ptr<T>(func == 'M'), 256, 0);
     if (e != hipSuccess) return report_hip(e, "batch occupancy");
-    P->groups = std::max(1, cus * std::max(1, per_cu));
-    if (P->groups > P->ntasks) P->groups = std::max(1, P->ntasks);
+    for (int c = 0; c < 4; ++c) {
+        if (P->cls_count[c] == 0) continue;
+        int per_cu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_ptr<T>(func == 'M', BATCH_CLASSES[c]),
+                                                         256, 0);
+        if (e != hipSuccess) return report_hip(e, "batch occupancy");
+        const int g = std::max(1, cus * std::max(1, per_cu));
+        P->cls_groups[c] = std::min(g, P->cls_count[c]);
+    }
     return NLA_OK;
 }
 
@@ -153,8 +185,13 @@ static hipError_t enqueue(nla_batch_plan_s *P, hipStream_t s) {
     hipError_t e = hipSuccess;
     if (!P->trmm) e = launch_batch_trtri<T>(sys, (const int2 *)(P->ws + L.blk), P->nblk, s);
     if (e != hipSuccess) return e;
-    return launch_batch<T>(sys, (const BatchTask *)(P->ws + L.tasks), P->ntasks, (int *)(P->ws + L.counter),
-                           P->trmm, P->groups, s);
+    const BatchTask *tasks = (const BatchTask *)(P->ws + L.tasks);
+    int *counters = (int *)(P->ws + L.counter);
+    for (int c = 3; c >= 0 && e == hipSuccess; --c)
+        if (P->cls_count[c] > 0)
+            e = launch_batch<T>(sys, tasks + P->cls_start[c], P->cls_count[c], counters + c, P->trmm, P->cls_nbig[c],
+                                P->cls_groups[c], s);
+    return e;
 }
 
 }  // namespace nla

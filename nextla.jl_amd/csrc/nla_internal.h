@@ -105,13 +105,23 @@ struct BatchStrip {
     static constexpr int W = 16;
 };
 
-// Kernel entry of the persistent batched solver (for occupancy queries).
-template <typename T>
-const void *batch_kernel_ptr(bool trmm);
+// Size classes of the persistent batched solver: the kernel for class NBIG
+// takes systems of up to NBIG rows and sizes its LDS strip for NBIG.
+constexpr int BATCH_CLASSES[4] = {128, 256, 512, 1024};
 
+// Classes a new batch plan may use: bit c enables BATCH_CLASSES[c] (the
+// largest is always used); set with nla_set_config(NLA_CFG_BATCH_CLASSES).
+int batch_class_mask();
+
+// Kernel entry of the persistent batched solver for one size class (for
+// occupancy queries); nullptr for an unknown class.
+template <typename T>
+const void *batch_kernel_ptr(bool trmm, int nbig);
+
+// Tasks [0, ntasks) of one size class; *counter is cleared by a kernel first.
 template <typename T>
 hipError_t launch_batch(const BatchSys<T> *sys, const BatchTask *tasks, int ntasks, int *counter, bool trmm,
-                        int groups, hipStream_t s);
+                        int nbig, int groups, hipStream_t s);
 
 // Inverse 64-blocks of every system of a batch (TRSM): one work-group per
 // (system, block) pair listed in blk (system index, block index).

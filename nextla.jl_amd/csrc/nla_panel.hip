@@ -359,14 +359,16 @@ template hipError_t launch_panel<double>(const PanelArgs<double> &, hipStream_t)
 // ---------------------------------------------------------------------------
 // Persistent batched solver: work-groups pull (system, strip) tasks from a
 // queue ordered by decreasing cost until it is empty.  Every system is one
-// diagonal panel of up to BATCH_MAX rows.
+// diagonal panel of up to NBIG rows; the kernel is instantiated per size
+// class (NBIG = 128 ... 1024) so that the LDS strip, and with it the number
+// of work-groups a CU holds, fits the largest system of the class.
 
-template <typename T, bool TRMM>
+template <typename T, bool TRMM, int NBIG>
 __global__ __launch_bounds__(256) void batch_kernel(const BatchSys<T> *__restrict__ sys,
                                                     const BatchTask *__restrict__ tasks, int ntasks,
                                                     int *__restrict__ counter) {
     constexpr int WS = BatchStrip<T>::W;
-    constexpr int LDB = BATCH_MAX + (sizeof(T) == 8 ? 2 : 4);
+    constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
     __shared__ int s_idx;
     for (;;) {
@@ -377,8 +379,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchSys<T> *__restric
         if (idx >= ntasks) break;  // every wave of the group sees the same idx
         const BatchTask tk = tasks[idx];
         const BatchSys<T> &S = sys[tk.sys];
-        panel_strip<T, BATCH_MAX, WS, TRMM>(S.n, S.m, (int64_t)tk.strip * WS, S.M, S.mrs, S.mcs, S.lower != 0, S.B,
-                                            S.brs, S.bcs, S.W, S.wrs, S.wcs, S.scale, (NLA_LDS T *)bs_raw);
+        panel_strip<T, NBIG, WS, TRMM>(S.n, S.m, (int64_t)tk.strip * WS, S.M, S.mrs, S.mcs, S.lower != 0, S.B, S.brs,
+                                       S.bcs, S.W, S.wrs, S.wcs, S.scale, (NLA_LDS T *)bs_raw);
         __syncthreads();
     }
 }
@@ -387,29 +389,41 @@ __global__ void zero_int_kernel(int *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
+template <typename T, bool TRMM>
+static const void *batch_ptr_t(int nbig) {
+    switch (nbig) {
+        case 128: return (const void *)batch_kernel<T, TRMM, 128>;
+        case 256: return (const void *)batch_kernel<T, TRMM, 256>;
+        case 512: return (const void *)batch_kernel<T, TRMM, 512>;
+        case 1024: return (const void *)batch_kernel<T, TRMM, 1024>;
+        default: return nullptr;
+    }
+}
+
+template <typename T>
+const void *batch_kernel_ptr(bool trmm, int nbig) {
+    return trmm ? batch_ptr_t<T, true>(nbig) : batch_ptr_t<T, false>(nbig);
+}
+
 template <typename T>
 hipError_t launch_batch(const BatchSys<T> *sys, const BatchTask *tasks, int ntasks, int *counter, bool trmm,
-                        int groups, hipStream_t s) {
+                        int nbig, int groups, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
+    const void *k = batch_kernel_ptr<T>(trmm, nbig);
+    if (!k) return hipErrorInvalidValue;
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, counter);
-    if (trmm)
-        hipLaunchKernelGGL((batch_kernel<T, true>), dim3(groups), dim3(256), 0, s, sys, tasks, ntasks, counter);
-    else
-        hipLaunchKernelGGL((batch_kernel<T, false>), dim3(groups), dim3(256), 0, s, sys, tasks, ntasks, counter);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    void *args[] = {(void *)&sys, (void *)&tasks, (void *)&ntasks, (void *)&counter};
+    return hipLaunchKernel(k, dim3(groups), dim3(256), args, 0, s);
 }
 
-template <typename T>
-const void *batch_kernel_ptr(bool trmm) {
-    return trmm ? (const void *)batch_kernel<T, true> : (const void *)batch_kernel<T, false>;
-}
+template const void *batch_kernel_ptr<float>(bool, int);
+template const void *batch_kernel_ptr<double>(bool, int);
 
-template const void *batch_kernel_ptr<float>(bool);
-template const void *batch_kernel_ptr<double>(bool);
-
-template hipError_t launch_batch<float>(const BatchSys<float> *, const BatchTask *, int, int *, bool, int,
+template hipError_t launch_batch<float>(const BatchSys<float> *, const BatchTask *, int, int *, bool, int, int,
                                         hipStream_t);
-template hipError_t launch_batch<double>(const BatchSys<double> *, const BatchTask *, int, int *, bool, int,
+template hipError_t launch_batch<double>(const BatchSys<double> *, const BatchTask *, int, int *, bool, int, int,
                                          hipStream_t);
 
 }  // namespace nla

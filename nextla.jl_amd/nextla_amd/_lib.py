@@ -71,6 +71,7 @@ NLA_CFG_PANEL_ROWS_F32 = 2
 NLA_CFG_PANEL_ROWS_F64 = 3
 NLA_CFG_PANEL_STRIP_F32 = 4
 NLA_CFG_PANEL_STRIP_F64 = 5
+NLA_CFG_BATCH_CLASSES = 6
 
 
 def set_config(key, value):

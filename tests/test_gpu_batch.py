@@ -105,3 +105,30 @@ def test_batch_rejects_oversized_system(nla):
     B = torch.zeros(1025, 2, dtype=torch.float64, device=DEV).t().contiguous().t()
     with pytest.raises(ValueError):
         batch.BatchPlan("L", "L", "N", "S", [A], [B])
+
+
+@pytest.mark.parametrize("mask", [0x8, 0xA, 0xF])
+def test_size_classes_bitwise_identical(nla, mask):
+    """Whatever size classes a plan may use (NLA_CFG_BATCH_CLASSES), every
+    system gets the same bits: the classes differ only in LDS stride."""
+    from nextla_amd import _lib, batch
+    shapes = [(16, 3), (128, 20), (129, 7), (256, 33), (300, 16), (512, 5), (513, 40), (1024, 17)]
+    results = {}
+    for cur in (0xF, mask):
+        old = _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, cur)
+        try:
+            for side, uplo, func in (("L", "L", "S"), ("R", "U", "M")):
+                rng2 = np.random.default_rng(3)
+                As, Bs = [], []
+                for n, m in shapes:
+                    A, B = make_system(rng2, n, m, side, uplo, np.float64)
+                    As.append(dev(A))
+                    Bs.append(dev(B))
+                batch.batched_rectrxm_(side, uplo, "N", func, As, Bs, alphas=0.75)
+                torch.cuda.synchronize()
+                results.setdefault((side, uplo, func), []).append([b.cpu().numpy() for b in Bs])
+        finally:
+            _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, old)
+    for key, (ref, got) in results.items():
+        for r, g in zip(ref, got):
+            assert np.array_equal(r, g), key

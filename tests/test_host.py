@@ -112,3 +112,26 @@ def test_shape_errors(mirror):
     with pytest.raises(ValueError):
         mirror.GEMM_ADD_(torch.zeros(3, 4, dtype=torch.float64), torch.zeros(5, 2, dtype=torch.float64),
                          torch.zeros(3, 2, dtype=torch.float64))
+
+
+def test_tuner_picks_fastest_and_restores(nla):
+    """The tuner's selection and configuration handling, with stand-in timings
+    (no GPU): it returns the fastest candidate and leaves the process-wide
+    configuration as it found it until apply() is called."""
+    from nextla_amd import _lib, tune
+    before = _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 512)
+    _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, before)
+    fake = {(256, 16): 3.0, (256, 32): 4.0, (384, 16): 2.5, (384, 32): 2.9, (512, 16): 2.6, (512, 32): 5.0}
+    res = tune.tune_panels("f64", measure=lambda r, s: fake[(r, s)])
+    assert (res["rows"], res["strip"]) == (384, 16)
+    assert res["ms"]["384/16"] == 2.5
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, before) == before
+    bres = tune.tune_batch("f64", measure=lambda mk: {0x8: 5.0, 0xC: 4.0, 0xE: 3.0, 0xF: 3.5}[mk])
+    assert bres["mask"] == 0xE
+    old = tune.apply({"panel": {"f64": {"rows": 384, "strip": 16}}, "batch_classes": 0xE})
+    try:
+        assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 384) == 384
+        assert _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, 0xE) == 0xE
+    finally:
+        tune.apply({"panel": old["panel"], "batch_classes": old["batch_classes"]})
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, before) == before
