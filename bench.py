@@ -23,7 +23,15 @@ Also reported on the same line:
                 batched        10,000 lower TRSMs (n log-uniform on [16, 1024],
                                RHS log-uniform on [1, 256]) in one
                                graph-captured persistent launch, against a
-                               loop of plain calls.
+                               loop of plain calls;
+                served         a stream of 600 TRSM / TRMM requests (n on
+                               [32, 4096], RHS on [1, 512], FP32 / FP64
+                               accuracy, cond(A) <= 1e4, 20 % on resident
+                               factors, 64 GB budget) through the
+                               continuous-batching server: p50 / p99 latency
+                               at Poisson rates, saturated solves/s against
+                               one-at-a-time plain calls, and the FP64 TRSMs
+                               served by refinement instead.
 """
 import argparse
 import json
@@ -216,6 +224,72 @@ def batch_leg(nla, torch, dtype, count=10000, seed=0):
             "speedup_vs_loop": round(tl / tb, 1)}
 
 
+def serve_leg(nla, torch, count=600, seed=0):
+    """The serving workload through nextla_amd.serve (see the module)."""
+    import numpy as np
+    from nextla_amd import serve
+    reqs = serve.workload_requests(count, seed=seed)
+    B0 = [r.B.clone() for r in reqs]
+    flops = sum(float(r.A.shape[0]) ** 2 * (r.B.shape[1]) for r in reqs)
+
+    def restore():
+        for r, b in zip(reqs, B0):
+            r.B.copy_(b)
+            r.done = None
+        torch.cuda.synchronize()
+
+    def server(mode):
+        s = serve.Server(budget_bytes=64 << 30, max_inflight=4, max_batch=256, fp64_trsm=mode)
+        for r in reqs:
+            if r.a_key is not None:
+                s.resident_a(r.a_key, r.A)
+        return s
+
+    out = {"requests": count, "workload": "left-lower TRSM/TRMM 50/50, FP32/FP64 accuracy 50/50, n logU[32,4096], "
+                                          "RHS logU[1,512], cond(A) <= 1e4, 20% resident factors, budget 64 GB"}
+    restore()
+    serve.run_stream(server("plain"), reqs[:60], [0.0] * 60)  # warm-up: allocator, kernels
+    restore()
+    t0 = time.perf_counter()
+    serve.one_at_a_time(reqs)
+    w1 = time.perf_counter() - t0
+    out["one_at_a_time_solves_per_s"] = round(count / w1, 1)
+    for mode in ("plain", "mixed"):
+        restore()
+        srv = server(mode)
+        lat, wall = serve.run_stream(srv, reqs, [0.0] * count)
+        srv.close()
+        res = {"saturated_solves_per_s": round(count / wall, 1), "speedup_vs_one_at_a_time": round(w1 / wall, 3),
+               "saturated_tflops": round(flops / wall / 1e12, 3)}
+        rates = (200.0, 1000.0, 5000.0) if mode == "plain" else (1000.0,)
+        rng = np.random.default_rng(seed + 1)
+        for lam in rates:
+            restore()
+            srv = server(mode)
+            arr = np.cumsum(rng.exponential(1.0 / lam, count))
+            lat, wall = serve.run_stream(srv, reqs, arr)
+            srv.close()
+            lat = np.array(lat) * 1e3
+            res[f"rate_{int(lam)}"] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                       "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                                       "solves_per_s": round(count / wall, 1)}
+        if mode == "mixed":  # the refined FP64 TRSMs reached the target
+            worst = 0.0
+            for r, b0 in zip(reqs, B0):
+                if r.func == "S" and r.A.dtype == torch.float64:
+                    T = torch.tril(r.A)
+                    R = b0 - T @ r.B
+                    den = torch.linalg.norm(T) * torch.linalg.norm(r.B) + torch.linalg.norm(b0)
+                    worst = max(worst, (torch.linalg.norm(R) / den).item())
+            res["worst_fp64_trsm_backward_error"] = worst
+        out["fp64_trsm_" + mode] = res
+        log(f"bench: served ({mode} FP64 TRSM): {res['saturated_solves_per_s']} solves/s saturated, "
+            f"{res['speedup_vs_one_at_a_time']}x one-at-a-time")
+    del reqs, B0
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(A_host, n, seconds_target=12.0):
     """The oracle on host cores over a bounded number of RHS columns: a short
     calibration run at 128 columns, then a sample sized for ~seconds_target."""
@@ -322,6 +396,7 @@ def main():
             b = extra["batched"][label]
             log(f"bench: batched {label}: {b['ms']} ms ({b['tflops']} TFLOP/s, roofline {b['roofline_frac']}), "
                 f"loop of plain calls {b['loop_of_plain_calls_ms']} ms")
+        extra["served"] = serve_leg(nla, torch)
     torch.cuda.synchronize()
 
     cpu = None

@@ -1,17 +1,26 @@
 // Batched triangular solves / multiplies: host-side planning for the
-// persistent panel kernel (nla_batch_* in include/nla.h).
+// persistent panel kernel and the grouped GEMM (nla_batch_* in
+// include/nla.h).
 //
 // A plan puts every system into the panel kernel's canonical form (the same
 // transformation nla_rectrxm applies to one system: op(A) as a strided view
-// with flipped uplo for 'T'/'C'; right-side systems solved on the transpose),
-// lists one task per (system, 16-column strip), grouped by size class (the
-// LDS strip of the kernel that solves a class fits its largest system, so
-// small systems run at higher occupancy) and longest first within a class
-// (the cost of a strip grows with the square of the number of 64-blocks), and
-// uploads the descriptors into the caller's workspace.  Executing a plan
-// launches the inversion kernel (TRSM) and, per non-empty class, a counter
-// reset and the persistent solver, on the caller's stream and nothing else,
-// so it can be captured in a hipGraph and replayed.
+// with flipped uplo for 'T'/'C'; right-side systems solved on the transpose).
+// A system of up to BATCH_MAX rows is one panel.  A larger one is cut into
+// blocks of BATCH_STAGE rows and solved in stages, as the recursion would at
+// that block size: per stage one diagonal block (a panel) and one GEMM that
+// applies it to the rest of the system (TRSM: the rows still to solve; TRMM:
+// the block itself from the rows not yet multiplied).  Stage s of every
+// system runs in the same launches, so a plan of any mix of sizes takes
+//   1 inversion launch (TRSM) + per stage: a panel launch per size class
+//   and one grouped GEMM launch,
+// independent of the number of systems.  Panel tasks are one per (panel,
+// 16-column strip), grouped by size class (the LDS strip of the kernel that
+// solves a class fits its largest panel, so small panels run at higher
+// occupancy) and longest first within a class; GEMM tasks are one per 64 x 64
+// output tile, longest k-range first.  Descriptors are uploaded in stream
+// order into the caller's workspace; executing a plan launches kernels on
+// the caller's stream and nothing else, so it can be captured in a hipGraph
+// and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,29 +40,53 @@ int32_t report_bad_arg(int k, const char *what);
 static inline char upc(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
 static inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 
+// Stages and blocks of one system.
+static inline int64_t stages_of(int64_t n) { return n <= BATCH_MAX ? 1 : (n + BATCH_STAGE - 1) / BATCH_STAGE; }
+
 struct Layout {
-    int64_t counter, sys, tasks, blk, inv, total;
-    int64_t ntasks, nblk;
+    int64_t counter, sys, pan, ptasks, gem, gtasks, blk, inv, total;
+    int64_t npan, nptasks, ngem, ngtasks, nblk, nstages;
 };
 
+// Counts and offsets; they depend on func, dtype and the items' n, m only.
 static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, int64_t count) {
     const int64_t es = dtype == NLA_F64 ? 8 : 4;
     const int64_t sys_bytes = dtype == NLA_F64 ? sizeof(BatchSys<double>) : sizeof(BatchSys<float>);
+    const int64_t gem_bytes = dtype == NLA_F64 ? sizeof(GroupGemm<double>) : sizeof(GroupGemm<float>);
     Layout L{};
+    L.nstages = 0;
     for (int64_t i = 0; i < count; ++i) {
-        int64_t n = items[i].n, m = items[i].m;
+        const int64_t n = items[i].n, m = items[i].m;
         if (n <= 0 || m <= 0) continue;
-        L.ntasks += (m + 15) / 16;
+        const int64_t st = stages_of(n), strips = (m + 15) / 16;
+        L.nstages = std::max(L.nstages, st);
+        L.npan += st;
+        L.nptasks += st * strips;
         if (func == 'S') L.nblk += (n + 63) / 64;
+        if (st > 1) {
+            // every stage but one has a GEMM; its output is at most n rows x m
+            const int64_t tn = (m + GROUP_TILE - 1) / GROUP_TILE, tr = (n + GROUP_TILE - 1) / GROUP_TILE;
+            L.ngem += st - 1;
+            L.ngtasks += (st - 1) * tr * tn;
+        }
     }
     L.counter = 0;
-    L.sys = align256(256);
-    L.tasks = align256(L.sys + count * sys_bytes);
-    L.blk = align256(L.tasks + L.ntasks * (int64_t)sizeof(BatchTask));
+    L.sys = align256(std::max<int64_t>(256, L.nstages * 5 * 4));
+    L.pan = align256(L.sys + count * sys_bytes);
+    L.ptasks = align256(L.pan + L.npan * sys_bytes);
+    L.gem = align256(L.ptasks + L.nptasks * (int64_t)sizeof(BatchTask));
+    L.gtasks = align256(L.gem + L.ngem * gem_bytes);
+    L.blk = align256(L.gtasks + L.ngtasks * (int64_t)sizeof(int2));
     L.inv = align256(L.blk + L.nblk * (int64_t)sizeof(int2));
     L.total = align256(L.inv + L.nblk * 64 * 64 * es);
     return L;
 }
+
+// One launch of the plan: a panel launch (cls 0..3) or a grouped GEMM (cls 4).
+struct Launch {
+    int stage, cls, start, count, groups;
+    bool vec;
+};
 
 }  // namespace nla
 
@@ -61,31 +94,41 @@ struct nla_batch_plan_s {
     std::vector<char> host;  // staged descriptors, alive until the upload has run
     int32_t dtype;
     bool trmm;
+    bool tx, ty;  // grouped GEMM operand orientation
     char *ws;
-    int ntasks, nblk;
-    // per size class (nla::BATCH_CLASSES): first task, task count, grid
-    int cls_nbig[4], cls_start[4], cls_count[4], cls_groups[4];
+    int nblk;
+    std::vector<nla::Launch> launches;
     nla::Layout L;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t graph_stream = nullptr;
 };
 
 namespace nla {
+
+static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 template <typename T>
 static int32_t build(char side, char uplo, char trans, char func, const nla_batch_item *items, int64_t count,
                      nla_batch_plan_s *P, hipStream_t stream) {
     const Layout &L = P->L;
-    std::vector<BatchSys<T>> sys((size_t)count);
-    std::vector<BatchTask> tasks;
+    const bool trmm = (func == 'M');
+    std::vector<BatchSys<T>> sys((size_t)count), pan;
+    std::vector<int> pan_stage;
+    std::vector<BatchTask> ptasks;
+    std::vector<GroupGemm<T>> gem;
+    std::vector<int> gem_stage;
     std::vector<int2> blk;
-    tasks.reserve((size_t)L.ntasks);
+    pan.reserve((size_t)L.npan);
+    ptasks.reserve((size_t)L.nptasks);
+    gem.reserve((size_t)L.ngem);
     blk.reserve((size_t)L.nblk);
     T *inv = (T *)(P->ws + L.inv);
     int64_t inv_off = 0;
     const bool tr = (trans != 'N');
     const char ue = tr ? (uplo == 'L' ? 'U' : 'L') : uplo;  // uplo of op(A)
+    // grouped GEMM orientation (see the descriptors below)
+    P->tx = (side == 'L') ? tr : false;
+    P->ty = (side == 'L') ? false : tr;
     for (int64_t i = 0; i < count; ++i) {
         const nla_batch_item &it = items[i];
         BatchSys<T> &S = sys[(size_t)i];
@@ -121,76 +164,186 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             for (int64_t b = 0; b < nb; ++b) blk.push_back(make_int2((int)i, (int)b));
             inv_off += nb * 64 * 64;
         }
-        for (int64_t s = 0; s < (it.m + 15) / 16; ++s) tasks.push_back(BatchTask{(int)i, (int)s});
+        const int64_t strips = (it.m + 15) / 16;
+        const int J = (int)stages_of(it.n);
+        if (J == 1) {  // one panel, alpha at its first (TRSM) / last (TRMM) touch
+            const int pi = (int)pan.size();
+            pan.push_back(S);
+            pan_stage.push_back(0);
+            for (int64_t st = 0; st < strips; ++st) ptasks.push_back(BatchTask{pi, (int)st});
+            continue;
+        }
+        const T alpha = (T)it.alpha;
+        const bool fwd = (S.lower != 0) != trmm;  // block order: first to last
+        for (int s = 0; s < J; ++s) {
+            const int j = fwd ? s : J - 1 - s;
+            const int64_t r0 = (int64_t)j * BATCH_STAGE;
+            const int64_t rows = std::min<int64_t>(BATCH_STAGE, it.n - r0);
+            // the diagonal block as a panel
+            BatchSys<T> D = S;
+            D.n = (int)rows;
+            D.M = S.M + r0 * S.mrs + r0 * S.mcs;
+            D.B = S.B + r0 * S.brs;
+            if (func == 'S') D.W = S.W + (r0 / 64) * 64 * 64;
+            D.scale = trmm ? alpha : (s == 0 ? alpha : T(1));
+            const int pi = (int)pan.size();
+            pan.push_back(D);
+            pan_stage.push_back(s);
+            for (int64_t st = 0; st < strips; ++st) ptasks.push_back(BatchTask{pi, (int)st});
+            // the GEMM of the stage: rows R of the output, columns K of M
+            int64_t R0, RN, K0, KN;
+            T ga, gb;
+            if (!trmm) {  // the solved block updates the rows still to solve
+                if (s == J - 1) continue;
+                K0 = r0;
+                KN = rows;
+                if (S.lower) {
+                    R0 = r0 + rows;
+                    RN = it.n - R0;
+                } else {
+                    R0 = 0;
+                    RN = r0;
+                }
+                ga = T(-1);
+                gb = (s == 0) ? alpha : T(1);
+            } else {  // the block gathers the rows not yet multiplied
+                R0 = r0;
+                RN = rows;
+                if (S.lower) {
+                    K0 = 0;
+                    KN = r0;
+                } else {
+                    K0 = r0 + rows;
+                    KN = it.n - K0;
+                }
+                if (KN == 0) continue;
+                ga = alpha;
+                gb = T(1);
+            }
+            GroupGemm<T> g;
+            const T *Mblk = S.M + R0 * S.mrs + K0 * S.mcs;  // op(M)(r, k), r in R, k in K
+            if (side == 'L') {  // B(R, :) = gb*B(R, :) + ga * M(R, K) * B(K, :)
+                g.M = RN; g.N = it.m; g.K = KN;
+                g.X = Mblk; g.ldx = P->tx ? S.mrs : S.mcs;
+                g.Y = S.B + K0; g.ldy = S.bcs;
+                g.C = S.B + R0; g.ldc = S.bcs;
+            } else {  // transposed: Bt(:, R) = gb*Bt(:, R) + ga * Bt(:, K) * M(R, K)^T
+                g.M = it.m; g.N = RN; g.K = KN;
+                g.X = S.B + K0 * S.brs; g.ldx = S.brs;
+                g.Y = Mblk; g.ldy = P->ty ? S.mcs : S.mrs;
+                g.C = S.B + R0 * S.brs; g.ldc = S.brs;
+            }
+            g.alpha = ga;
+            g.beta = gb;
+            g.tiles_m = (int)((g.M + GROUP_TILE - 1) / GROUP_TILE);
+            gem.push_back(g);
+            gem_stage.push_back(s);
+        }
     }
-    // Size class of a system: the smallest enabled class that holds it (the
-    // largest class is always enabled).  Tasks are grouped by class, largest
-    // class first, and longest first within a class (the cost of a strip
-    // grows with the number of 64-block phases).
+    // Panel tasks: by stage, then size class (largest first), then cost.
     const int mask = batch_class_mask();
     auto cls_of = [&](int n) {
         for (int c = 0; c < 4; ++c)
             if (((mask >> c) & 1 || c == 3) && n <= BATCH_CLASSES[c]) return c;
         return 3;
     };
-    std::stable_sort(tasks.begin(), tasks.end(), [&](const BatchTask &a, const BatchTask &b) {
-        const int na = sys[(size_t)a.sys].n, nb2 = sys[(size_t)b.sys].n;
+    std::stable_sort(ptasks.begin(), ptasks.end(), [&](const BatchTask &a, const BatchTask &b) {
+        const int sa = pan_stage[(size_t)a.sys], sb = pan_stage[(size_t)b.sys];
+        if (sa != sb) return sa < sb;
+        const int na = pan[(size_t)a.sys].n, nb2 = pan[(size_t)b.sys].n;
         const int ca = cls_of(na), cb = cls_of(nb2);
         if (ca != cb) return ca > cb;
         const int pa = (na + 63) / 64, pb = (nb2 + 63) / 64;
         return pa * (pa + 1) > pb * (pb + 1);
     });
-    for (int c = 0; c < 4; ++c) {
-        P->cls_nbig[c] = BATCH_CLASSES[c];
-        P->cls_start[c] = 0;
-        P->cls_count[c] = 0;
-        P->cls_groups[c] = 0;
+    // GEMM tiles: by stage, longest k-range first.
+    std::vector<int> gorder(gem.size());
+    for (size_t k = 0; k < gorder.size(); ++k) gorder[k] = (int)k;
+    std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) {
+        if (gem_stage[(size_t)a] != gem_stage[(size_t)b]) return gem_stage[(size_t)a] < gem_stage[(size_t)b];
+        return gem[(size_t)a].K > gem[(size_t)b].K;
+    });
+    std::vector<int2> gtasks;
+    gtasks.reserve((size_t)L.ngtasks);
+    for (int gi : gorder) {
+        const GroupGemm<T> &g = gem[(size_t)gi];
+        const int tiles = g.tiles_m * (int)((g.N + GROUP_TILE - 1) / GROUP_TILE);
+        for (int t = 0; t < tiles; ++t) gtasks.push_back(make_int2(gi, t));
     }
-    for (size_t k = 0; k < tasks.size(); ++k) {
-        const int c = cls_of(sys[(size_t)tasks[k].sys].n);
-        if (P->cls_count[c]++ == 0) P->cls_start[c] = (int)k;
+    if ((int64_t)pan.size() > L.npan || (int64_t)ptasks.size() > L.nptasks || (int64_t)gem.size() > L.ngem ||
+        (int64_t)gtasks.size() > L.ngtasks || (int64_t)blk.size() > L.nblk)
+        return report_hip(hipErrorInvalidValue, "batch plan sizes");
+
+    // launches, in execution order, with persistent grids from occupancy
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return report_hip(e, "batch occupancy");
+    auto grid = [&](const void *k, int count_) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        return std::max(1, std::min(cus * per_cu, count_));
+    };
+    P->launches.clear();
+    size_t pt = 0, gt = 0;
+    for (int s = 0; s < (int)L.nstages; ++s) {
+        for (int c = 3; c >= 0; --c) {
+            const size_t start = pt;
+            while (pt < ptasks.size() && pan_stage[(size_t)ptasks[pt].sys] == s &&
+                   cls_of(pan[(size_t)ptasks[pt].sys].n) == c)
+                ++pt;
+            if (pt > start)
+                P->launches.push_back(Launch{s, c, (int)start, (int)(pt - start),
+                                             grid(batch_kernel_ptr<T>(trmm, BATCH_CLASSES[c]), (int)(pt - start)),
+                                             false});
+        }
+        const size_t start = gt;
+        bool vec = true;
+        while (gt < gtasks.size() && gem_stage[(size_t)gtasks[gt].x] == s) {
+            const GroupGemm<T> &g = gem[(size_t)gtasks[gt].x];
+            constexpr int V = 16 / sizeof(T);
+            vec = vec && al16(g.X) && al16(g.Y) && g.ldx % V == 0 && g.ldy % V == 0;
+            ++gt;
+        }
+        if (gt > start)
+            P->launches.push_back(Launch{s, 4, (int)start, (int)(gt - start),
+                                         grid(grouped_gemm_ptr<T>(P->tx, P->ty, vec), (int)(gt - start)), vec});
     }
     // one staged image of the descriptor region, copied in stream order
     P->host.assign((size_t)(L.inv - L.sys), 0);
-    std::memcpy(P->host.data(), sys.data(), sys.size() * sizeof(BatchSys<T>));
-    if (!tasks.empty())
-        std::memcpy(P->host.data() + (L.tasks - L.sys), tasks.data(), tasks.size() * sizeof(BatchTask));
-    if (!blk.empty()) std::memcpy(P->host.data() + (L.blk - L.sys), blk.data(), blk.size() * sizeof(int2));
+    char *h = P->host.data() - L.sys;
+    std::memcpy(h + L.sys, sys.data(), sys.size() * sizeof(BatchSys<T>));
+    if (!pan.empty()) std::memcpy(h + L.pan, pan.data(), pan.size() * sizeof(BatchSys<T>));
+    if (!ptasks.empty()) std::memcpy(h + L.ptasks, ptasks.data(), ptasks.size() * sizeof(BatchTask));
+    if (!gem.empty()) std::memcpy(h + L.gem, gem.data(), gem.size() * sizeof(GroupGemm<T>));
+    if (!gtasks.empty()) std::memcpy(h + L.gtasks, gtasks.data(), gtasks.size() * sizeof(int2));
+    if (!blk.empty()) std::memcpy(h + L.blk, blk.data(), blk.size() * sizeof(int2));
     // stream-ordered: no wait on other streams; the image lives in the plan
-    hipError_t e = hipMemcpyAsync(P->ws + L.sys, P->host.data(), P->host.size(), hipMemcpyHostToDevice, stream);
+    e = hipMemcpyAsync(P->ws + L.sys, P->host.data(), P->host.size(), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return report_hip(e, "batch plan upload");
-    P->ntasks = (int)tasks.size();
     P->nblk = (int)blk.size();
-    // persistent grid per class: every CU filled to that kernel's occupancy
-    int dev = 0, cus = 0;
-    e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return report_hip(e, "batch occupancy");
-    for (int c = 0; c < 4; ++c) {
-        if (P->cls_count[c] == 0) continue;
-        int per_cu = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_ptr<T>(func == 'M', BATCH_CLASSES[c]),
-                                                         256, 0);
-        if (e != hipSuccess) return report_hip(e, "batch occupancy");
-        const int g = std::max(1, cus * std::max(1, per_cu));
-        P->cls_groups[c] = std::min(g, P->cls_count[c]);
-    }
     return NLA_OK;
 }
 
 template <typename T>
 static hipError_t enqueue(nla_batch_plan_s *P, hipStream_t s) {
     const Layout &L = P->L;
-    const BatchSys<T> *sys = (const BatchSys<T> *)(P->ws + L.sys);
     hipError_t e = hipSuccess;
-    if (!P->trmm) e = launch_batch_trtri<T>(sys, (const int2 *)(P->ws + L.blk), P->nblk, s);
-    if (e != hipSuccess) return e;
-    const BatchTask *tasks = (const BatchTask *)(P->ws + L.tasks);
+    if (!P->trmm) e = launch_batch_trtri<T>((const BatchSys<T> *)(P->ws + L.sys), (const int2 *)(P->ws + L.blk),
+                                            P->nblk, s);
+    const BatchSys<T> *pan = (const BatchSys<T> *)(P->ws + L.pan);
+    const BatchTask *ptasks = (const BatchTask *)(P->ws + L.ptasks);
+    const GroupGemm<T> *gem = (const GroupGemm<T> *)(P->ws + L.gem);
+    const int2 *gtasks = (const int2 *)(P->ws + L.gtasks);
     int *counters = (int *)(P->ws + L.counter);
-    for (int c = 3; c >= 0 && e == hipSuccess; --c)
-        if (P->cls_count[c] > 0)
-            e = launch_batch<T>(sys, tasks + P->cls_start[c], P->cls_count[c], counters + c, P->trmm, P->cls_nbig[c],
-                                P->cls_groups[c], s);
+    for (const Launch &l : P->launches) {
+        if (e != hipSuccess) break;
+        int *ctr = counters + l.stage * 5 + l.cls;
+        if (l.cls < 4)
+            e = launch_batch<T>(pan, ptasks + l.start, l.count, ctr, P->trmm, BATCH_CLASSES[l.cls], l.groups, s);
+        else
+            e = launch_grouped_gemm<T>(gem, gtasks + l.start, l.count, ctr, P->tx, P->ty, l.vec, l.groups, s);
+    }
     return e;
 }
 
@@ -220,7 +373,7 @@ int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32
     if (count > (int64_t)1 << 30) return report_bad_arg(7, "count");
     for (int64_t i = 0; i < count; ++i) {
         const nla_batch_item &it = items[i];
-        if (it.n < 0 || it.n > BATCH_MAX || it.m < 0) return report_bad_arg(6, "items (n must be in [0, 1024])");
+        if (it.n < 0 || it.n > BATCH_N_MAX || it.m < 0) return report_bad_arg(6, "items (n must be in [0, 65536])");
         if (it.n == 0 || it.m == 0) continue;
         if (!it.A || !it.B || it.lda < it.n || it.ldb < (side == 'L' ? it.n : it.m))
             return report_bad_arg(6, "items (pointer or leading dimension)");
@@ -249,7 +402,7 @@ int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32
 int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_graph) {
     if (plan == nullptr) return report_bad_arg(1, "plan");
     hipStream_t s = (hipStream_t)stream;
-    if (plan->ntasks == 0) return NLA_OK;
+    if (plan->launches.empty()) return NLA_OK;
     auto enq = [&](hipStream_t q) {
         return plan->dtype == NLA_F64 ? enqueue<double>(plan, q) : enqueue<float>(plan, q);
     };

@@ -529,6 +529,89 @@ hipError_t launch_gemm(const GemmArgs<T> &a, hipStream_t s) {
     }
 }
 
+// ---- grouped GEMM of the staged batched solver ----------------------------
+
+// Work-groups take (gemm, tile) tasks from a counter until it passes ntasks;
+// each task is one tile of one GEMM of the group, run by the same tile
+// routine as the single GEMM.
+template <typename T, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY, int VW>
+__global__ __launch_bounds__(WM *WN * 64) void grouped_gemm_kernel(const GroupGemm<T> *__restrict__ g,
+                                                                   const int2 *__restrict__ tasks, int ntasks,
+                                                                   int *counter) {
+    constexpr int NT = WM * WN * 64;
+    typedef TileLoader<T, BM, BK, NT, TX, VW> LX;
+    typedef TileLoader<T, BN, BK, NT, !TY, VW> LY;
+    __shared__ __attribute__((aligned(16))) T lds_x[2 * LX::LDS_ELEMS];
+    __shared__ __attribute__((aligned(16))) T lds_y[2 * LY::LDS_ELEMS];
+    __shared__ int s_task;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = s_task;
+        __syncthreads();
+        if (t >= ntasks) break;
+        const int2 tk = tasks[t];
+        const GroupGemm<T> G = g[tk.x];
+        const int64_t row0 = (int64_t)(tk.y % G.tiles_m) * BM, col0 = (int64_t)(tk.y / G.tiles_m) * BN;
+        const bool interior = (row0 + BM <= G.M) && (col0 + BN <= G.N) && (G.K % BK == 0);
+        if (interior)
+            gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, 0, false>(G.M, G.N, G.K, G.alpha, G.X, G.ldx, G.Y, G.ldy,
+                                                                   G.beta, G.C, G.ldc, row0, col0,
+                                                                   (NLA_LDS T *)lds_x, (NLA_LDS T *)lds_y);
+        else
+            gemm_tile<T, BM, BN, BK, WM, WN, TX, TY, VW, 0, true>(G.M, G.N, G.K, G.alpha, G.X, G.ldx, G.Y, G.ldy,
+                                                                  G.beta, G.C, G.ldc, row0, col0,
+                                                                  (NLA_LDS T *)lds_x, (NLA_LDS T *)lds_y);
+    }
+}
+
+__global__ void zero_counter_kernel(int *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// 64 x 64 tiles: FP64 four waves of 32 x 32 (2 x 2 MFMA blocks), FP32 four
+// waves of one 32 x 32 block each.
+template <typename T, bool TX, bool TY, int VW>
+static const void *gg_ptr() {
+    if constexpr (sizeof(T) == 8)
+        return (const void *)grouped_gemm_kernel<T, GROUP_TILE, GROUP_TILE, 16, 2, 2, TX, TY, VW>;
+    else
+        return (const void *)grouped_gemm_kernel<T, GROUP_TILE, GROUP_TILE, 32, 2, 2, TX, TY, VW>;
+}
+
+template <typename T>
+const void *grouped_gemm_ptr(bool tx, bool ty, bool vec) {
+    constexpr int V = 16 / sizeof(T);
+    if (vec) {
+        if (!tx && !ty) return gg_ptr<T, false, false, V>();
+        if (tx && !ty) return gg_ptr<T, true, false, V>();
+        if (!tx && ty) return gg_ptr<T, false, true, V>();
+        return gg_ptr<T, true, true, V>();
+    }
+    if (!tx && !ty) return gg_ptr<T, false, false, 1>();
+    if (tx && !ty) return gg_ptr<T, true, false, 1>();
+    if (!tx && ty) return gg_ptr<T, false, true, 1>();
+    return gg_ptr<T, true, true, 1>();
+}
+
+template <typename T>
+hipError_t launch_grouped_gemm(const GroupGemm<T> *g, const int2 *tasks, int ntasks, int *counter, bool tx, bool ty,
+                               bool vec, int groups, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, s, counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    void *args[] = {(void *)&g, (void *)&tasks, (void *)&ntasks, (void *)&counter};
+    return hipLaunchKernel(grouped_gemm_ptr<T>(tx, ty, vec), dim3(groups), dim3(256), args, 0, s);
+}
+
+template const void *grouped_gemm_ptr<float>(bool, bool, bool);
+template const void *grouped_gemm_ptr<double>(bool, bool, bool);
+template hipError_t launch_grouped_gemm<float>(const GroupGemm<float> *, const int2 *, int, int *, bool, bool, bool,
+                                               int, hipStream_t);
+template hipError_t launch_grouped_gemm<double>(const GroupGemm<double> *, const int2 *, int, int *, bool, bool,
+                                                bool, int, hipStream_t);
+
 template hipError_t launch_gemm<float>(const GemmArgs<float> &, hipStream_t);
 template hipError_t launch_gemm<double>(const GemmArgs<double> &, hipStream_t);
 

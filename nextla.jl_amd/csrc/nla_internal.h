@@ -78,8 +78,41 @@ struct PanelArgs {
 template <typename T>
 hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s);
 
-// Largest system the persistent batched solver takes (one panel per system).
+// Largest system the persistent batched solver takes as a single panel;
+// larger systems (up to BATCH_N_MAX) are solved in stages of BATCH_STAGE
+// rows: a batched panel launch per stage, then a grouped GEMM that applies
+// the stage's block to the rest of its system.
 constexpr int BATCH_MAX = 1024;
+constexpr int BATCH_STAGE = 512;
+constexpr int BATCH_N_MAX = 65536;
+
+// One GEMM of a grouped launch: C = beta*C + alpha*op(X)*op(Y) with the
+// operand orientation (TX, TY) shared by the launch; tiles_m row tiles.
+template <typename T>
+struct GroupGemm {
+    int64_t M, N, K;
+    const T *X;
+    int64_t ldx;
+    const T *Y;
+    int64_t ldy;
+    T *C;
+    int64_t ldc;
+    T alpha, beta;
+    int tiles_m;
+};
+
+// Tile edge of the grouped GEMM (square, so a GEMM and its transpose give the
+// same tile count).
+constexpr int GROUP_TILE = 64;
+
+// Persistent grouped GEMM over tasks (gemm index, tile index) taken from a
+// counter (cleared by a kernel first); vec: every X and Y of the launch has
+// 16-byte aligned columns and leading dimensions.
+template <typename T>
+hipError_t launch_grouped_gemm(const GroupGemm<T> *g, const int2 *tasks, int ntasks, int *counter, bool tx, bool ty,
+                               bool vec, int groups, hipStream_t s);
+template <typename T>
+const void *grouped_gemm_ptr(bool tx, bool ty, bool vec);
 
 // One system of a batch in the panel's canonical form (see PanelArgs).
 template <typename T>

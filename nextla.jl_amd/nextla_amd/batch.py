@@ -2,10 +2,12 @@
 GPU (the nla_batch_* entry points of include/nla.h).
 
 The reference solves each system with its own unified_rectrxm! call
-(src/rectrxm.jl:43); here a whole list of systems, each with its own n
-(<= 1024), m, A, B and alpha but a shared side / uplo / trans / func and
-dtype, runs as one persistent kernel launch (plus the batched inversion of
-the 64x64 diagonal blocks for TRSM), optionally replayed from a hipGraph.
+(src/rectrxm.jl:43); here a whole list of systems, each with its own n, m,
+A, B and alpha but a shared side / uplo / trans / func and dtype, runs in a
+fixed number of launches: the batched inversion of the 64x64 diagonal blocks
+(TRSM), then per stage a persistent panel launch per size class and one
+grouped GEMM (systems up to 1024 rows are a single panel; larger ones are
+solved in 512-row stages), optionally replayed from a hipGraph.
 """
 import ctypes
 
@@ -14,7 +16,8 @@ import torch
 from . import _lib
 from ._lib import BatchItem
 
-BATCH_MAX = 1024
+BATCH_MAX = 1024     # largest system solved as one panel
+BATCH_N_MAX = 65536  # largest system a plan takes
 
 
 class BatchPlan:

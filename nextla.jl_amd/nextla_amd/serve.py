@@ -7,26 +7,40 @@ A request is one unified_rectrxm!(side, uplo, trans, alpha, func, A, B) call
   * admits requests against an HBM budget (the bytes of A, B and the
     batched workspace they need; A counted once while it is resident),
   * groups admitted requests that share (dtype, side, uplo, trans, func)
-    into one batched launch (nla_batch_*: a persistent kernel over all
-    (system, strip) tasks) and sends systems larger than the batched
-    solver's limit through the recursive path on the same stream,
+    into one batched plan (nla_batch_*: persistent panel launches over all
+    (system, strip) tasks, systems above 1024 rows in 512-row stages with a
+    grouped GEMM per stage), so a batch costs a fixed number of launches
+    whatever its sizes,
   * keeps several batches in flight on separate streams, so requests that
     arrive while others run are admitted as slots and memory free up and
     their work overlaps on the device,
   * records each request's latency from arrival to completion (the
     completion event of its batch).
 
+FP64 requests are FP64-accuracy requests.  Their TRSMs are served either by
+the plain FP64 path (fp64_trsm="plain", batched like the rest) or by
+low-precision correction solves with FP64 refinement to backward error
+<= 1e-14 (fp64_trsm="mixed", nla_rectrxm_mixed): that call reads its stopping
+test on the host once per step, so each such request runs on a worker thread
+with its own stream and workspace while the loop keeps admitting and
+launching.  The plain path is the faster of the two on MI355X (FP64 MFMA runs
+at the FP64 vector rate, so each FP64 residual costs as much as a whole plain
+solve); "mixed" is there to serve and measure the refined form.
+
 A system's result does not depend on what else is in its batch: the batched
 kernel solves each (system, strip) task with the same code path whatever the
-batch holds, so a request gives bit-identical output alone and in the mix.
+batch holds, so a request gives bit-identical output alone and in the mix
+(and a refined request runs the same deterministic kernels either way).
 """
 import collections
+import concurrent.futures
+import threading
 import time
 
 import torch
 
 from . import unified_rectrxm_
-from .batch import BATCH_MAX, BatchPlan
+from .batch import BATCH_N_MAX, BatchPlan
 
 
 class Request:
@@ -49,7 +63,41 @@ class Request:
 
 
 class _InFlight:
-    __slots__ = ("reqs", "plan", "event", "stream", "nbytes", "a_keys")
+    __slots__ = ("reqs", "plan", "event", "stream", "nbytes", "a_keys", "future")
+
+
+class _MixedWorkers:
+    """Worker threads for refined FP64 solves: each has its own stream and a
+    workspace that grows to the largest request it has served."""
+
+    def __init__(self, device, workers, precision, tol):
+        self.device = device
+        self.precision = precision
+        self.tol = tol
+        self.local = threading.local()
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
+
+    def _state(self):
+        st = getattr(self.local, "st", None)
+        if st is None:
+            st = self.local.st = {"stream": torch.cuda.Stream(device=self.device), "ws": None}
+        return st
+
+    def run(self, r):
+        from .mixed import rectrxm_mixed_, workspace_bytes
+        st = self._state()
+        need = workspace_bytes(r.side, r.A.shape[0], r.B.shape[1] if r.side == "L" else r.B.shape[0]) + 256
+        if st["ws"] is None or st["ws"].numel() < need:
+            with torch.cuda.stream(st["stream"]):
+                st["ws"] = torch.empty(need, dtype=torch.uint8, device=self.device)
+        with torch.cuda.stream(st["stream"]):
+            _, info = rectrxm_mixed_(r.side, r.uplo, r.trans, r.alpha, r.A, r.B, precision=self.precision,
+                                     tol=self.tol, workspace=st["ws"])
+        r.done = time.perf_counter()
+        return info
+
+    def close(self):
+        self.pool.shutdown(wait=True)
 
 
 class Server:
@@ -60,7 +108,10 @@ class Server:
     max_batch: most requests per batch.
     """
 
-    def __init__(self, device="cuda", budget_bytes=64 << 30, max_inflight=4, max_batch=512, use_graph=False):
+    def __init__(self, device="cuda", budget_bytes=64 << 30, max_inflight=4, max_batch=512, use_graph=False,
+                 fp64_trsm="plain", mixed_precision="fp16", mixed_tol=1e-14, mixed_workers=4):
+        if fp64_trsm not in ("plain", "mixed"):
+            raise ValueError(f"fp64_trsm must be 'plain' or 'mixed', got {fp64_trsm!r}")
         self.device = torch.device(device)
         self.budget = int(budget_bytes)
         self.max_batch = max_batch
@@ -72,6 +123,10 @@ class Server:
         self.used = 0
         self.resident = collections.OrderedDict()  # a_key -> (A tensor, refcount)
         self.completed = []
+        self.fp64_trsm = fp64_trsm
+        self.mixed = (_MixedWorkers(self.device, mixed_workers, mixed_precision, mixed_tol)
+                      if fp64_trsm == "mixed" else None)
+        self.mixed_slots = mixed_workers
 
     # -- resident factors ----------------------------------------------------
     def resident_a(self, key, A=None):
@@ -102,42 +157,78 @@ class Server:
             req.arrival = time.perf_counter()
         self.queue.append(req)
 
+    def _is_mixed(self, r):
+        return self.mixed is not None and r.func == "S" and r.A.dtype == torch.float64
+
     def _req_bytes(self, r):
         b = r.nbytes()
         if r.a_key is None:
             b += r.A.numel() * r.A.element_size()
-        if r.func == "S" and r.A.shape[0] <= BATCH_MAX:
-            b += ((r.A.shape[0] + 63) // 64) * 64 * 64 * r.A.element_size()
+        n = r.A.shape[0]
+        if self._is_mixed(r):
+            m = r.B.shape[1] if r.side == "L" else r.B.shape[0]
+            b += 4 * n * n + 20 * n * m  # the refinement workspace
+        elif r.func == "S":
+            b += ((n + 63) // 64) * 64 * 64 * r.A.element_size()  # diagonal inverses
         return b
 
-    def _launch(self):
-        if not self.queue or not self.free_streams:
+    def _launch_mixed(self):
+        """Hand the first queued refined request to a free worker."""
+        busy = sum(1 for f in self.inflight if f.future is not None)
+        if busy >= self.mixed_slots:
             return False
-        sig = self.queue[0].signature()
+        for k, r in enumerate(self.queue):
+            if not self._is_mixed(r):
+                continue
+            rb = self._req_bytes(r)
+            if self.used + rb > self.budget:
+                self._evict(rb)
+                if self.used + rb > self.budget:
+                    if not self.inflight:
+                        del self.queue[k]
+                        raise RuntimeError(f"request {r.rid} needs {rb} bytes, more than the HBM budget of "
+                                           f"{self.budget}")
+                    return False
+            del self.queue[k]
+            f = _InFlight()
+            f.reqs, f.stream, f.nbytes, f.plan, f.event = [r], None, rb, None, None
+            f.a_keys = {r.a_key} if r.a_key is not None else set()
+            f.future = self.mixed.pool.submit(self.mixed.run, r)
+            self.used += rb
+            self.inflight.append(f)
+            return True
+        return False
+
+    def _launch(self):
+        if self.mixed is not None and self._launch_mixed():
+            return True
+        head = next((r for r in self.queue if not self._is_mixed(r)), None)
+        if head is None or not self.free_streams:
+            return False
+        sig = head.signature()
         take, keep, nbytes = [], collections.deque(), 0
         while self.queue and len(take) < self.max_batch:
             r = self.queue.popleft()
             rb = self._req_bytes(r)
-            if r.signature() == sig and self.used + nbytes + rb <= self.budget:
+            if not self._is_mixed(r) and r.signature() == sig and self.used + nbytes + rb <= self.budget:
                 take.append(r)
                 nbytes += rb
             else:
                 keep.append(r)
         self.queue.extendleft(reversed(keep))
         if not take:
-            head = self.queue[0]
             self._evict(self._req_bytes(head))
             if not self.inflight and self.used + self._req_bytes(head) > self.budget:
-                self.queue.popleft()
+                self.queue.remove(head)
                 raise RuntimeError(f"request {head.rid} needs {self._req_bytes(head)} bytes, "
                                    f"more than the HBM budget of {self.budget}")
             return False
         stream = self.free_streams.pop()
         f = _InFlight()
-        f.reqs, f.stream, f.nbytes = take, stream, nbytes
+        f.reqs, f.stream, f.nbytes, f.future = take, stream, nbytes, None
         f.a_keys = {r.a_key for r in take if r.a_key is not None}
-        small = [r for r in take if r.A.shape[0] <= BATCH_MAX]
-        big = [r for r in take if r.A.shape[0] > BATCH_MAX]
+        small = [r for r in take if r.A.shape[0] <= BATCH_N_MAX]
+        big = [r for r in take if r.A.shape[0] > BATCH_N_MAX]
         with torch.cuda.stream(stream):
             f.plan = None
             if small:
@@ -154,24 +245,35 @@ class Server:
         return True
 
     def _reap(self):
-        done = [f for f in self.inflight if f.event.query()]
+        done = [f for f in self.inflight if (f.future.done() if f.future is not None else f.event.query())]
         now = time.perf_counter()
         for f in done:
             self.inflight.remove(f)
-            for r in f.reqs:
-                r.done = now
-                self.completed.append(r)
+            if f.future is not None:
+                f.future.result()  # re-raises a worker's error here
+                self.completed.extend(f.reqs)  # done was stamped by the worker
+            else:
+                for r in f.reqs:
+                    r.done = now
+                    self.completed.append(r)
+                self.free_streams.append(f.stream)
             if f.plan is not None:
                 f.plan.close()
             self.used -= f.nbytes
-            self.free_streams.append(f.stream)
         return len(done)
 
     def step(self):
         """One turn of the loop: retire finished batches, launch new ones."""
         n = self._reap()
+        launched = False
         while self._launch():
-            pass
+            launched = True
+        if not n and not launched and self.mixed is not None:
+            # Nothing to do but wait: block on the refinement workers for a
+            # moment instead of polling, so their Python side gets the GIL.
+            pending = [f.future for f in self.inflight if f.future is not None]
+            if pending:
+                concurrent.futures.wait(pending, timeout=1e-4, return_when=concurrent.futures.FIRST_COMPLETED)
         return n
 
     def drain(self):
@@ -180,6 +282,54 @@ class Server:
 
     def idle(self):
         return not self.queue and not self.inflight
+
+    def close(self):
+        if self.mixed is not None:
+            self.mixed.close()
+
+
+def workload_requests(count, seed=0, device="cuda", reuse=0.2, pool=8, n_range=(32, 4096), m_range=(1, 512),
+                      cond_max=1e4):
+    """The serving workload: count left-lower requests, TRSM / TRMM and FP32 /
+    FP64 accuracy half and half, n log-uniform on n_range, right-hand sides
+    log-uniform on m_range, A as the refinement workload (off-diagonal
+    U(-1,1)/sqrt(n), log-spaced diagonal 2 .. 2c with c log-uniform on
+    [1, cond_max / 2], so cond(A) <= cond_max), B N(0,1); a fraction `reuse`
+    of the requests use one of `pool` device-resident factors per dtype
+    (a_key set).  All inputs are generated in HBM and synchronized."""
+    import math
+
+    import numpy as np
+
+    from .inputs import device_precision_tri
+    rng = np.random.default_rng(seed)
+    lg = math.log
+    factors = {}
+    reqs = []
+    for k in range(count):
+        dt = torch.float64 if rng.random() < 0.5 else torch.float32
+        func = "S" if rng.random() < 0.5 else "M"
+        m = int(math.exp(rng.uniform(lg(m_range[0]), lg(m_range[1]))))
+        c = math.exp(rng.uniform(0.0, lg(cond_max / 2.0)))
+        if rng.random() < reuse:
+            key = (int(rng.integers(pool)), str(dt))
+            if key not in factors:
+                n = int(math.exp(rng.uniform(lg(n_range[0]), lg(n_range[1]))))
+                A, _ = device_precision_tri(n, 1, c, seed * 100003 + len(factors), device)
+                factors[key] = A if dt == torch.float64 else A.float()
+            A, a_key = factors[key], key
+            n = A.shape[0]
+        else:
+            n = int(math.exp(rng.uniform(lg(n_range[0]), lg(n_range[1]))))
+            A, _ = device_precision_tri(n, 1, c, seed * 100003 + 50000 + k, device)
+            A = A if dt == torch.float64 else A.float()
+            a_key = None
+        g = torch.Generator(device=device)
+        g.manual_seed(seed * 7919 + k)
+        B = torch.empty((m, n), dtype=dt, device=device).t().normal_(generator=g)
+        reqs.append(Request(k, "L", "L", "N", func, 1.0, A, B, a_key=a_key))
+    torch.cuda.synchronize()
+    return reqs
 
 
 def one_at_a_time(requests, arrivals=None):

@@ -100,7 +100,7 @@ def test_batch_validation(nla):
     lib = _lib.load()
     items = (_lib.BatchItem * 2)()
     items[0].n, items[0].m, items[0].A, items[0].lda, items[0].B, items[0].ldb, items[0].alpha = 8, 4, 64, 8, 64, 8, 1.0
-    items[1].n, items[1].m, items[1].A, items[1].lda, items[1].B, items[1].ldb, items[1].alpha = 2000, 4, 64, 2000, 64, 2000, 1.0
+    items[1].n, items[1].m, items[1].A, items[1].lda, items[1].B, items[1].ldb, items[1].alpha = 70000, 4, 64, 70000, 64, 70000, 1.0
     need = lib.nla_batch_workspace_bytes(b"S", 1, items, 1)
     assert need > 0
     plan = ctypes.c_void_p()

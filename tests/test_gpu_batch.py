@@ -99,12 +99,65 @@ def test_graph_replays_match_eager(nla):
     plan.close()
 
 
-def test_batch_rejects_oversized_system(nla):
+BIG = [(1025, 3), (1100, 40), (1600, 17), (2100, 70)]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_staged_systems_all_combos(nla, dt):
+    """Systems above 1024 rows are solved in 512-row stages (panel launch +
+    grouped GEMM per stage), mixed with single-panel systems in one plan:
+    every combo within the oracle's bounds and agreeing with plain calls."""
     from nextla_amd import batch
-    A = torch.eye(1025, dtype=torch.float64, device=DEV)
-    B = torch.zeros(1025, 2, dtype=torch.float64, device=DEV).t().contiguous().t()
-    with pytest.raises(ValueError):
-        batch.BatchPlan("L", "L", "N", "S", [A], [B])
+    rng = np.random.default_rng(23)
+    eps = np.finfo(dt).eps
+    shapes = BIG + [(64, 16), (700, 9)]
+    for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
+        hostA, hostB, As, Bs, refs = [], [], [], [], []
+        for n, m in shapes:
+            A, B = make_system(rng, n, m, side, uplo, dt)
+            hostA.append(A)
+            hostB.append(B)
+            As.append(dev(A))
+            Bs.append(dev(B))
+            Bp = dev(B)
+            nla.unified_rectrxm_(side, uplo, trans, -0.75, func, dev(A), Bp)
+            refs.append(Bp)
+        batch.batched_rectrxm_(side, uplo, trans, func, As, Bs, alphas=-0.75)
+        torch.cuda.synchronize()
+        for A, B, X, R in zip(hostA, hostB, Bs, refs):
+            n = A.shape[0]
+            Xh = X.cpu().numpy()
+            e = (O.backward_error if func == "S" else O.product_error)(side, uplo, trans, -0.75, A, B, Xh)
+            assert e <= 10 * n * eps, (side, uplo, trans, func, n, e)
+            assert O.rel_error(Xh, R.cpu().numpy()) <= 100 * n * eps, (side, uplo, trans, func, n)
+
+
+def test_staged_alone_equals_in_mix_and_graph(nla):
+    """A staged system gets the same bits alone, inside a mixed batch, and
+    from graph replays of the plan."""
+    from nextla_amd import batch
+    rng = np.random.default_rng(29)
+    for side, uplo, func in (("L", "L", "S"), ("R", "U", "S"), ("L", "U", "M"), ("R", "L", "M")):
+        A, B = make_system(rng, 1600, 33, side, uplo, np.float64)
+        alone = [dev(B)]
+        batch.batched_rectrxm_(side, uplo, "N", func, [dev(A)], alone, alphas=1.5)
+        others = [make_system(rng, n, m, side, uplo, np.float64) for n, m in ((2500, 5), (90, 20), (1030, 64))]
+        As = [dev(a) for a, _ in others] + [dev(A)]
+        B0 = [dev(b) for _, b in others] + [dev(B)]
+        Bs = [b.clone() for b in B0]
+        plan = batch.BatchPlan(side, uplo, "N", func, As, Bs, 1.5)
+        plan.execute()
+        torch.cuda.synchronize()
+        eager = [b.clone() for b in Bs]
+        assert torch.equal(eager[-1], alone[0]), (side, uplo, func)
+        for rep in range(3):
+            for b, b0 in zip(Bs, B0):
+                b.copy_(b0)
+            plan.execute(use_graph=True)
+            torch.cuda.synchronize()
+            for b, e in zip(Bs, eager):
+                assert torch.equal(b, e), (side, uplo, func, rep)
+        plan.close()
 
 
 @pytest.mark.parametrize("mask", [0x8, 0xA, 0xF])

@@ -105,3 +105,38 @@ def test_resident_factor_reuse(nla):
         e = O.backward_error("L", "L", "N", 1.0, Ah, b0.cpu().numpy(), r.B.cpu().numpy())
         assert e < 10 * 300 * 2.2e-16
     assert "factor-0" in srv.resident
+
+
+def test_refined_fp64_requests(nla):
+    """fp64_trsm='mixed': FP64 TRSM requests are served by refinement to
+    backward error <= 1e-14, agree with the plain FP64 solve, and give the
+    same bits alone and in the mixed stream."""
+    from nextla_amd import serve
+    reqs = serve.workload_requests(24, seed=5, n_range=(32, 1500), m_range=(1, 64))
+    B0 = [r.B.clone() for r in reqs]
+    srv = serve.Server(budget_bytes=16 << 30, max_inflight=3, max_batch=8, fp64_trsm="mixed", mixed_workers=2)
+    for r in reqs:
+        srv.submit(r)
+    srv.drain()
+    srv.close()
+    torch.cuda.synchronize()
+    refined = [k for k, r in enumerate(reqs) if r.func == "S" and r.A.dtype == torch.float64]
+    assert refined
+    for k in refined:
+        r, b0 = reqs[k], B0[k]
+        T = torch.tril(r.A)
+        bwd = (torch.linalg.norm(b0 - T @ r.B) / (torch.linalg.norm(T) * torch.linalg.norm(r.B)
+                                                  + torch.linalg.norm(b0))).item()
+        assert bwd <= 1.5e-14, (k, bwd)
+        P = b0.clone()
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", r.A, P)
+        rel = (torch.linalg.norm(r.B - P) / torch.linalg.norm(P)).item()
+        assert rel <= 1e-10, (k, rel)
+    k = refined[0]
+    solo = serve.Request(999, "L", "L", "N", "S", 1.0, reqs[k].A, B0[k].clone())
+    s2 = serve.Server(budget_bytes=16 << 30, fp64_trsm="mixed", mixed_workers=1)
+    s2.submit(solo)
+    s2.drain()
+    s2.close()
+    torch.cuda.synchronize()
+    assert torch.equal(solo.B, reqs[k].B)
