@@ -152,6 +152,24 @@ int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_gr
 int32_t nla_batch_plan_destroy(nla_batch_plan plan);
 
 /*
+ * BLAS-style xTRSM / xTRMM with the diagonal argument: the same operation as
+ * nla_rectrxm with func 'S' / 'M', plus diag:
+ *   'N'  non-unit diagonal (as nla_rectrxm);
+ *   'U'  unit diagonal: A's diagonal is taken as 1 and never read.
+ * Replaces the reference's trsm(side, uplo, transa, diag, alpha, A, B) and
+ * trmm(...) wrappers (src/trsm.jl:186-204, src/trmm.jl:430-447), which accept
+ * transa and diag but ignore them, and gives getrf2!'s unit-lower solve of
+ * the recursive LU (src/lu.jl:277) a GPU path.
+ * Argument positions: 1 side, 2 uplo, 3 transa, 4 diag, 5 dtype, 6 alpha,
+ * 7 n, 8 m, 9 A, 10 lda, 11 B, 12 ldb, 13 stream.
+ */
+int32_t nla_trsm(char side, char uplo, char transa, char diag, int32_t dtype, double alpha, int64_t n, int64_t m,
+                 const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
+
+int32_t nla_trmm(char side, char uplo, char transa, char diag, int32_t dtype, double alpha, int64_t n, int64_t m,
+                 const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
+
+/*
  * Mixed-precision TRSM with FP64 iterative refinement (no counterpart in the
  * reference; the FP64 solution it converges to is that of unified_rectrxm!'s
  * TRSM, src/rectrxm.jl:43-76).  Solves

@@ -163,6 +163,7 @@ struct Recursion {
     int mode = 1;
     bool use_panel = true;  // fuse diagonal blocks <= panel_rows into one launch
     int panel_rows = PANEL_MAX, panel_strip = PANEL_STRIP;
+    bool unit = false;  // unit diagonal: only the panels and the inverses read the diagonal
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
     // -1 plain fp32.  Each update takes two absolute-maximum slots from amax
@@ -231,6 +232,7 @@ struct Recursion {
             a.wrs = nb; a.wcs = 1;
         }
         a.strip = panel_strip;
+        a.unit = unit;
         const double fl = (double)n * (double)n * (double)a.m;
         const double by = ((double)n * (n + 1) / 2 + 2.0 * n * a.m) * sizeof(T);
         err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_panel<T>(a, stream); });
@@ -432,7 +434,7 @@ static hipError_t reserve(StreamWorkspace *w, size_t bytes) {
 
 template <typename T>
 static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double alpha, int64_t n, int64_t m,
-                             const void *Ap, int64_t lda, void *Bp, int64_t ldb, hipStream_t s) {
+                             const void *Ap, int64_t lda, void *Bp, int64_t ldb, hipStream_t s, bool unit = false) {
     View<T> A{(T *)Ap, 1, lda, n, n};
     View<T> B{(T *)Bp, 1, ldb, side == 'L' ? n : m, side == 'L' ? m : n};
     if (alpha == 0.0) return hip_status(launch_zero<T>(B.nr, B.nc, B.p, ldb, s), "zero fill");
@@ -449,6 +451,12 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     r.stream = s;
     r.mode = leaf_mode_default();
     r.use_panel = panel_default();
+    if (unit) {  // every diagonal block goes through a panel (and, for TRSM, the inverses)
+        r.unit = true;
+        r.mode = 1;
+        r.nb = 64;
+        r.use_panel = true;
+    }
     {
         const int di = std::is_same<T, double>::value ? NLA_F64 : NLA_F32;
         r.panel_rows = g_panel_rows[di].load();
@@ -470,7 +478,7 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
         const double fl = (double)blocks * r.nb * r.nb * r.nb / 3.0;
         const double by = (double)blocks * r.nb * r.nb * 1.5 * sizeof(T);
         e = timed(NLA_KIND_LEAF, fl, by, s,
-                  [&] { return launch_trtri_blocks<T>(n, (int)r.nb, A.p, A.rs, A.cs, uplo == 'L', W, s); });
+                  [&] { return launch_trtri_blocks<T>(n, (int)r.nb, A.p, A.rs, A.cs, uplo == 'L', W, s, unit); });
         if (e != hipSuccess) return hip_status(e, "diagonal inverses");
         r.W = W;
     }
@@ -679,6 +687,39 @@ int32_t nla_rectrxm(char side, char uplo, char trans, char func, int32_t dtype, 
     hipStream_t s = (hipStream_t)stream;
     if (dtype == NLA_F64) return rectrxm_typed<double>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s);
     return rectrxm_typed<float>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s);
+}
+
+// BLAS-style xTRSM / xTRMM with the diagonal argument.
+static int32_t trxm_diag(char func, char side, char uplo, char transa, char diag, int32_t dtype, double alpha,
+                         int64_t n, int64_t m, const void *A, int64_t lda, void *B, int64_t ldb, void *stream) {
+    side = up(side); uplo = up(uplo); transa = up(transa); diag = up(diag);
+    if (side != 'L' && side != 'R') return bad_arg(1, "side");
+    if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
+    if (transa != 'N' && transa != 'T' && transa != 'C') return bad_arg(3, "transa");
+    if (diag != 'N' && diag != 'U') return bad_arg(4, "diag");
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(5, "dtype");
+    if (n < 0) return bad_arg(7, "n");
+    if (m < 0) return bad_arg(8, "m");
+    if (n == 0 || m == 0) return NLA_OK;
+    if (A == nullptr) return bad_arg(9, "A");
+    if (lda < n) return bad_arg(10, "lda");
+    if (B == nullptr) return bad_arg(11, "B");
+    if (ldb < (side == 'L' ? n : m)) return bad_arg(12, "ldb");
+    hipStream_t s = (hipStream_t)stream;
+    const bool unit = diag == 'U';
+    if (dtype == NLA_F64)
+        return rectrxm_typed<double>(side, uplo, transa, func, alpha, n, m, A, lda, B, ldb, s, unit);
+    return rectrxm_typed<float>(side, uplo, transa, func, alpha, n, m, A, lda, B, ldb, s, unit);
+}
+
+int32_t nla_trsm(char side, char uplo, char transa, char diag, int32_t dtype, double alpha, int64_t n, int64_t m,
+                 const void *A, int64_t lda, void *B, int64_t ldb, void *stream) {
+    return trxm_diag('S', side, uplo, transa, diag, dtype, alpha, n, m, A, lda, B, ldb, stream);
+}
+
+int32_t nla_trmm(char side, char uplo, char transa, char diag, int32_t dtype, double alpha, int64_t n, int64_t m,
+                 const void *A, int64_t lda, void *B, int64_t ldb, void *stream) {
+    return trxm_diag('M', side, uplo, transa, diag, dtype, alpha, n, m, A, lda, B, ldb, stream);
 }
 
 int32_t nla_gemm(int32_t dtype, char transx, char transy, int64_t M, int64_t N, int64_t K, double alpha,

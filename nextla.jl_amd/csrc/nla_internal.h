@@ -73,6 +73,7 @@ struct PanelArgs {
     int64_t wrs, wcs;
     T scale;
     int strip = PANEL_STRIP;  // RHS columns per work-group: 16 or 32
+    bool unit = false;        // unit diagonal (taken as 1, not read)
 };
 
 template <typename T>
@@ -171,10 +172,10 @@ hipError_t launch_leaf(const LeafArgs<T> &a, hipStream_t s);
 // M(i,j) = A[i*rs + j*cs] (lower if lower, else upper): block b covers rows
 // and columns [b*nb, min(n, (b+1)*nb)); W receives inv(M_bb) as an nb x nb
 // column-major block at W + b*nb*nb (zero outside the triangle and padded
-// with the identity past n).
+// with the identity past n).  unit: the diagonal is taken as 1, not read.
 template <typename T>
 hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,
-                               hipStream_t s);
+                               hipStream_t s, bool unit = false);
 
 template <typename T>
 hipError_t launch_zero(int64_t rows, int64_t cols, T *B, int64_t ldb, hipStream_t s);

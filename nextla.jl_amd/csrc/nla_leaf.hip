@@ -152,7 +152,7 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
 // being built; st: 32 x 32 scratch.  Upper blocks are inverted as J U J.
 template <typename T>
 __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
-                              T *__restrict__ W, T *sl, T *si, T *st) {
+                              bool unit, T *__restrict__ W, T *sl, T *si, T *st) {
     constexpr int NB = 64, NT = 256;
     const int t = threadIdx.x;
     const int64_t off = blk * NB;
@@ -165,7 +165,8 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
     for (int e = t; e < NB * NB; e += NT) {
         int i = arow ? e % NB : e / NB;
         int j = arow ? e / NB : e % NB;
-        bool ok = i < nb && j < nb && j <= i;
+        // a unit diagonal is taken as 1 and not read
+        bool ok = i < nb && j < nb && j <= i && !(unit && i == j);
         T v = *(ok ? base + i * ars + j * acs : base);
         sl[j * NB + i] = ok ? v : (i == j ? T(1) : T(0));
         si[e] = T(0);
@@ -239,7 +240,7 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
 // MFMA products.
 template <typename T, int NB>
 __device__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
-                            T *__restrict__ W, T *sl, T *srinv) {
+                            bool unit, T *__restrict__ W, T *sl, T *srinv) {
     const int t = threadIdx.x;
     const int64_t off = blk * NB;
     const int nb = (int)(n - off < NB ? n - off : NB);
@@ -256,7 +257,7 @@ __device__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int
             int e = e0 + u * LEAF_COLS + t;
             int i = arow ? e % NB : e / NB;
             int j = arow ? e / NB : e % NB;
-            if (e < NB * NB && i < nb && j < nb)
+            if (e < NB * NB && i < nb && j < nb && !(unit && i == j))
                 v[u] = (j <= i) ? base[i * ars + j * acs] : T(0);
             else
                 v[u] = (i == j) ? T(1) : T(0);
@@ -269,7 +270,7 @@ __device__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int
             if (e < NB * NB) sl[j * NB + i] = v[u];
         }
     }
-    for (int i = t; i < NB; i += LEAF_COLS) srinv[i] = (i < nb) ? T(1) / base[i * ars + i * acs] : T(1);
+    for (int i = t; i < NB; i += LEAF_COLS) srinv[i] = (i < nb && !unit) ? T(1) / base[i * ars + i * acs] : T(1);
     __syncthreads();
     if (t >= NB) return;
     T x[NB];
@@ -302,20 +303,20 @@ __device__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int
 // MFMA products.
 template <typename T, int NB>
 __global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__restrict__ A, int64_t rs,
-                                                          int64_t cs, int lower, T *__restrict__ W) {
+                                                          int64_t cs, int lower, int unit, T *__restrict__ W) {
     __shared__ __attribute__((aligned(16))) T sl[NB * NB];
     __shared__ T srinv[NB];
-    trtri_block<T, NB>(n, blockIdx.x, A, rs, cs, lower != 0, W, sl, srinv);
+    trtri_block<T, NB>(n, blockIdx.x, A, rs, cs, lower != 0, unit != 0, W, sl, srinv);
 }
 
 // The 64-block inverse used by the recursion and the batched solver.
 template <typename T>
 __global__ __launch_bounds__(256) void trtri64_kernel(int64_t n, const T *__restrict__ A, int64_t rs, int64_t cs,
-                                                      int lower, T *__restrict__ W) {
+                                                      int lower, int unit, T *__restrict__ W) {
     __shared__ __attribute__((aligned(16))) T sl[64 * 64];
     __shared__ __attribute__((aligned(16))) T si[64 * 64];
     __shared__ __attribute__((aligned(16))) T st[32 * 32];
-    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, W, sl, si, st);
+    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, unit != 0, W, sl, si, st);
 }
 
 // Batched form: one work-group per (system, block) pair.
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(256) void batch_trtri_kernel(const BatchSys<T> *__r
     const bool right = (S.wrs != 1);
     const int64_t rs = right ? S.mcs : S.mrs, cs = right ? S.mrs : S.mcs;
     const bool lower = right ? (S.lower == 0) : (S.lower != 0);
-    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, (T *)S.W, sl, si, st);
+    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, false, (T *)S.W, sl, si, st);
 }
 
 template <typename T>
@@ -348,27 +349,27 @@ template hipError_t launch_batch_trtri<double>(const BatchSys<double> *, const i
 
 template <typename T>
 hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,
-                               hipStream_t s) {
+                               hipStream_t s, bool unit) {
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + nb - 1) / nb;
+    const int lo = lower ? 1 : 0, un = unit ? 1 : 0;
     if (nb == 64)
-        hipLaunchKernelGGL((trtri64_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, n, A, rs, cs, lower ? 1 : 0,
-                           W);
+        hipLaunchKernelGGL((trtri64_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, n, A, rs, cs, lo, un, W);
     else if (nb == 32)
-        hipLaunchKernelGGL((trtri_kernel<T, 32>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
-                           lower ? 1 : 0, W);
+        hipLaunchKernelGGL((trtri_kernel<T, 32>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs, lo, un,
+                           W);
     else if (nb == 16)
-        hipLaunchKernelGGL((trtri_kernel<T, 16>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs,
-                           lower ? 1 : 0, W);
+        hipLaunchKernelGGL((trtri_kernel<T, 16>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs, lo, un,
+                           W);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 template hipError_t launch_trtri_blocks<float>(int64_t, int, const float *, int64_t, int64_t, bool, float *,
-                                               hipStream_t);
+                                               hipStream_t, bool);
 template hipError_t launch_trtri_blocks<double>(int64_t, int, const double *, int64_t, int64_t, bool, double *,
-                                                hipStream_t);
+                                                hipStream_t, bool);
 
 template <typename T, int NB, bool ICONTIG>
 static hipError_t launch_leaf_nb(const LeafArgs<T> &a, hipStream_t s) {

@@ -119,7 +119,8 @@ template <typename T, int NBIG, int WS, bool TRMM>
 // calling one instantiation, the compiler otherwise emits a real call (and
 // the panel ran 50 % slower).
 __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, const T *__restrict__ Mp, int64_t mrs,
-                                            int64_t mcs, bool lower, T *__restrict__ Bp, int64_t brs, int64_t bcs,
+                                            int64_t mcs, bool lower, bool unit, T *__restrict__ Bp, int64_t brs,
+                                            int64_t bcs,
                                             const T *__restrict__ Wp, int64_t wrs, int64_t wcs, T scale,
                                             NLA_LDS T *bs) {
     typedef PanelShape<T, WS> PS;
@@ -208,8 +209,10 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         for (int s = 0; s < NKS; ++s) {
             const int k = s * MF::KD + lk;
             bool ok = i < ni && k < nk;
-            if (tri == 1) ok = ok && k <= i;
-            if (tri == 2) ok = ok && k >= i;
+            // a unit diagonal is not read: the strict triangle here, and the
+            // block itself added when the result is written
+            if (tri == 1) ok = ok && (unit ? k < i : k <= i);
+            if (tri == 2) ok = ok && (unit ? k > i : k >= i);
             af[s] = *(ok ? A + i * ars + k * acs : A);
             bits |= (ok ? 1u : 0u) << s;
         }
@@ -269,7 +272,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
             if (ph.last) __syncthreads();
         } else if (ph.last) {
             __syncthreads();  // every wave has read B_i
-            write_block(ph.i, T(1), T(0));
+            write_block(ph.i, T(1), unit ? T(1) : T(0));
             __syncthreads();
         }
     };
@@ -327,24 +330,25 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
 
 template <typename T, int NBIG, int WS, bool TRMM>
 __global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T *__restrict__ Mp, int64_t mrs,
-                                                    int64_t mcs, int lower, T *__restrict__ Bp, int64_t brs,
-                                                    int64_t bcs, const T *__restrict__ Wp, int64_t wrs,
+                                                    int64_t mcs, int lower, int unit, T *__restrict__ Bp,
+                                                    int64_t brs, int64_t bcs, const T *__restrict__ Wp, int64_t wrs,
                                                     int64_t wcs, T scale) {
     constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
-    panel_strip<T, NBIG, WS, TRMM>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, Bp, brs, bcs, Wp, wrs,
-                                   wcs, scale, (NLA_LDS T *)bs_raw);
+    panel_strip<T, NBIG, WS, TRMM>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0, Bp, brs,
+                                   bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw);
 }
 
 template <typename T, int W>
 static hipError_t launch_panel_w(const PanelArgs<T> &a, hipStream_t s) {
     const int64_t groups = (a.m + W - 1) / W;
+    const int lo = a.lower ? 1 : 0, un = a.unit ? 1 : 0;
     if (a.trmm)
         hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
-                           a.m, a.M, a.mrs, a.mcs, a.lower ? 1 : 0, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+                           a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     else
         hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
-                           a.m, a.M, a.mrs, a.mcs, a.lower ? 1 : 0, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+                           a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     return hipGetLastError();
 }
 
@@ -383,8 +387,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchSys<T> *__restric
         if (idx >= ntasks) break;  // every wave of the group sees the same idx
         const BatchTask tk = tasks[idx];
         const BatchSys<T> &S = sys[tk.sys];
-        panel_strip<T, NBIG, WS, TRMM>(S.n, S.m, (int64_t)tk.strip * WS, S.M, S.mrs, S.mcs, S.lower != 0, S.B, S.brs,
-                                       S.bcs, S.W, S.wrs, S.wcs, S.scale, (NLA_LDS T *)bs_raw);
+        panel_strip<T, NBIG, WS, TRMM>(S.n, S.m, (int64_t)tk.strip * WS, S.M, S.mrs, S.mcs, S.lower != 0, false, S.B,
+                                       S.brs, S.bcs, S.W, S.wrs, S.wcs, S.scale, (NLA_LDS T *)bs_raw);
         __syncthreads();
     }
 }

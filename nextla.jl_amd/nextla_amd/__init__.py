@@ -184,18 +184,29 @@ for _f, _doc in ((LeftLowerTRSM_, "A\\B, A lower (src/trsm.jl:128)"), (LeftUpper
     _f.__doc__ = f"In-place {_doc}; any n (the reference leaf is limited to one block)."
 
 
+def _trxm_diag(func, side, uplo, transa, diag, A, B, alpha):
+    for k, (v, ok) in enumerate(((side, "LRlr"), (uplo, "LUlu"), (transa, "NTCntc"), (diag, "NUnu")), 1):
+        if not isinstance(v, str) or len(v) != 1 or v not in ok:
+            raise ValueError(f"argument {k} has invalid value {v!r}")
+    _check_pair(A, B)
+    side, uplo, tr, n, m, lda, ldb = _canonical(side.upper(), uplo.upper(), transa.upper(), A, B)
+    fn = _lib.load().nla_trsm if func == "S" else _lib.load().nla_trmm
+    st = fn(_lib.ch(side), _lib.ch(uplo), _lib.ch("T" if tr else "N"), _lib.ch(diag.upper()), _DT[A.dtype],
+            float(alpha), n, m, A.data_ptr(), lda, B.data_ptr(), ldb, _stream(B))
+    _lib.check(st, "trsm" if func == "S" else "trmm")
+    return B
+
+
 def trsm(side, uplo, transa, diag, A, B, alpha=1.0):
-    """trsm(side, uplo, transa, diag, A, B, alpha) of src/trsm.jl:186-204.
-    Unlike the reference wrapper, transa is honoured; only diag 'N' is
-    supported (the reference ignores diag)."""
-    if diag not in ("N", "n"):
-        raise ValueError("diag 'U' is not supported; only non-unit triangles")
-    return unified_rectrxm_(side, uplo, transa, alpha, "S", A, B)
+    """trsm(side, uplo, transa, diag, A, B, alpha) of src/trsm.jl:186-204:
+    solve op(A)*X = alpha*B ('L') or X*op(A) = alpha*B ('R') in place.  Unlike
+    the reference wrapper (which ignores both), transa and diag are honoured:
+    diag 'U' takes A's diagonal as 1 without reading it."""
+    return _trxm_diag("S", side, uplo, transa, diag, A, B, alpha)
 
 
 def trmm(side, uplo, transa, diag, A, B, alpha=1.0):
-    """trmm(side, uplo, transa, diag, A, B, alpha) of src/trmm.jl:430-447
-    (transa honoured, diag 'N' only)."""
-    if diag not in ("N", "n"):
-        raise ValueError("diag 'U' is not supported; only non-unit triangles")
-    return unified_rectrxm_(side, uplo, transa, alpha, "M", A, B)
+    """trmm(side, uplo, transa, diag, A, B, alpha) of src/trmm.jl:430-447:
+    B <- alpha*op(A)*B ('L') or alpha*B*op(A) ('R'), transa and diag
+    honoured (diag 'U': unit diagonal, not read)."""
+    return _trxm_diag("M", side, uplo, transa, diag, A, B, alpha)

@@ -39,6 +39,8 @@ SIGNATURES = {
     "nla_abi_version": (_i32, []),
     "nla_last_error": (ctypes.c_char_p, []),
     "nla_rectrxm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_trsm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_trmm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_gemm_add": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_gemm_sub": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_gemm": (_i32, [_i32, _c_char, _c_char, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),

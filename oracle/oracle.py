@@ -125,16 +125,18 @@ def leaf(func, side, uplo, A, B, nthreads=1):
     return X
 
 
-def blas(side, uplo, trans, alpha, func, A, B):
-    """LinearAlgebra.BLAS.trsm!/trmm!(side, uplo, trans, 'N', alpha, A, B) as
-    called by test/unified_rectrxm.jl:36-40, through scipy's OpenBLAS."""
+def blas(side, uplo, trans, alpha, func, A, B, diag="N"):
+    """LinearAlgebra.BLAS.trsm!/trmm!(side, uplo, trans, diag, alpha, A, B) as
+    called by test/unified_rectrxm.jl:36-40 (diag 'N' there), through scipy's
+    OpenBLAS; diag 'U' leaves A's diagonal unreferenced."""
     from scipy.linalg import blas as sb
     A = _f(A)
     B = _f(B)
     pre = "d" if A.dtype == np.float64 else "s"
     fn = getattr(sb, pre + ("trsm" if func == "S" else "trmm"))
     t = {"N": 0, "T": 1, "C": 2}[trans]
-    return fn(alpha, A, B, side=0 if side == "L" else 1, lower=1 if uplo == "L" else 0, trans_a=t, diag=0)
+    return fn(alpha, A, B, side=0 if side == "L" else 1, lower=1 if uplo == "L" else 0, trans_a=t,
+              diag=1 if diag == "U" else 0)
 
 
 def op_matrix(uplo, trans, A):

@@ -91,7 +91,7 @@ def test_python_mirror_validates_before_device_work(nla):
     with pytest.raises(ValueError):  # host tensors are rejected, never computed on the CPU
         nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, B)
     with pytest.raises(ValueError):
-        nla.trsm("L", "L", "N", "U", A, B)
+        nla.trsm("L", "L", "N", "X", A, B)
 
 
 def test_batch_validation(nla):
@@ -139,3 +139,13 @@ def test_mixed_validation(nla):
     assert call(13, 1032) == -14
     assert call(14, need - 1) == -15
     assert call(6, 0) == 0 and info.iterations == 0  # empty: nothing to do
+
+
+def test_trsm_trmm_diag_validation(nla):
+    from nextla_amd import _lib
+    lib = _lib.load()
+    for fn in (lib.nla_trsm, lib.nla_trmm):
+        assert fn(b"L", b"L", b"N", b"Q", 1, 1.0, 8, 4, 16, 8, 32, 8, None) == -4
+        assert fn(b"L", b"L", b"Z", b"U", 1, 1.0, 8, 4, 16, 8, 32, 8, None) == -3
+        assert fn(b"R", b"U", b"T", b"U", 1, 1.0, 8, 10, 16, 8, 32, 9, None) == -12
+        assert fn(b"L", b"L", b"N", b"U", 1, 1.0, 0, 4, None, 1, None, 1, None) == 0

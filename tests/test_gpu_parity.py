@@ -191,6 +191,33 @@ def test_config_rejects_invalid_panel_shapes(nla):
     assert _lib.set_config(99, 1) == -1
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_unit_diagonal(nla, dt):
+    """trsm / trmm with diag 'U' (nla_trsm / nla_trmm): the diagonal holds
+    NaN and must never be read; results match BLAS with diag 'U' and the
+    error bounds of the unit-diagonal matrix, for every combo and sizes that
+    take one panel, several panels and the recursion."""
+    rng = np.random.default_rng(41)
+    for n, m in ((1, 3), (63, 8), (300, 17), (1500, 24)):
+        for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
+            A = np.tril(rng.uniform(-1, 1, (n, n)) / np.sqrt(n))
+            if uplo == "U":
+                A = A.T.copy()
+            A1 = A.copy()
+            np.fill_diagonal(A1, 1.0)
+            An = A.copy()
+            np.fill_diagonal(An, np.nan)
+            A1, An = A1.astype(dt), An.astype(dt)
+            B = rng.standard_normal((n, m) if side == "L" else (m, n)).astype(dt)
+            Bd = dev(B)
+            (nla.trsm if func == "S" else nla.trmm)(side, uplo, trans, "U", dev(An), Bd, alpha=0.5)
+            X = host(Bd)
+            assert np.all(np.isfinite(X)), (n, side, uplo, trans, func)
+            check_bound(side, uplo, trans, 0.5, func, A1, B, X)
+            ref = O.blas(side, uplo, trans, 0.5, func, An, B, diag="U")
+            assert O.rel_error(X, ref) < 50 * n * eps(dt), (n, side, uplo, trans, func, O.rel_error(X, ref))
+
+
 def test_odd_leading_dimensions(nla):
     """Sub-matrix views with odd lda/ldb take the unaligned (scalar) load path."""
     rng = np.random.default_rng(9)

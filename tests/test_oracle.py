@@ -235,3 +235,24 @@ def test_cond1_estimator_against_exact():
         assert 0.5 * exact <= est <= exact * (1 + 1e-12), (c, est, exact)
         # the diagonal scale sets the 2-norm condition number to within a few percent
         assert 0.9 * c <= np.linalg.cond(A, 2) <= 1.2 * c
+
+
+def test_blas_unit_diagonal_is_not_read():
+    """The unit-diagonal oracle: BLAS with diag 'U' on a matrix whose
+    diagonal holds NaN equals BLAS with diag 'N' on the same triangle with
+    ones on the diagonal, for every combo (pins O.blas's diag argument)."""
+    rng = np.random.default_rng(31)
+    for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
+        n, m = 37, 5
+        A = np.tril(rng.uniform(-1, 1, (n, n)) / np.sqrt(n))
+        if uplo == "U":
+            A = A.T.copy()
+        A1 = A.copy()
+        np.fill_diagonal(A1, 1.0)
+        An = A.copy()
+        np.fill_diagonal(An, np.nan)
+        B = rng.standard_normal((n, m) if side == "L" else (m, n))
+        got = O.blas(side, uplo, trans, 0.5, func, An, B, diag="U")
+        ref = O.blas(side, uplo, trans, 0.5, func, A1, B)
+        assert np.all(np.isfinite(got))
+        assert O.rel_error(got, ref) < 1e-14, (side, uplo, trans, func)
