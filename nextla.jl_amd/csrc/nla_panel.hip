@@ -65,24 +65,29 @@ struct Phase {
 __device__ inline Phase phase_of(int p, int nsb, bool lower, bool trmm) {
     Phase ph;
     if (!trmm) {
-        // groups per step: 1 diagonal + (nsb-1-step) updates
+        // Left-looking: output block i (top-down for lower, bottom-up for
+        // upper) gathers its sources k (already solved) in registers, then
+        // its diagonal phase subtracts them from B_i and applies inv(M_ii).
+        // Step s has s source phases and one diagonal phase.
         int step = 0, base = 0;
-        while (p >= base + (nsb - step)) {
-            base += nsb - step;
+        while (p >= base + step + 1) {
+            base += step + 1;
             ++step;
         }
-        int q = p - base;
-        int k = lower ? step : nsb - 1 - step;
-        ph.k = k;
-        if (q == 0) {
-            ph.i = k;
-            ph.kind = 0;
-        } else {
-            ph.i = lower ? k + q : q - 1;
+        const int q = p - base;
+        const int i = lower ? step : nsb - 1 - step;
+        ph.i = i;
+        if (q < step) {
+            ph.k = lower ? q : nsb - 1 - q;
             ph.kind = 1;
+            ph.first = (q == 0);
+            ph.last = false;
+        } else {
+            ph.k = i;
+            ph.kind = 0;
+            ph.first = (step == 0);  // diagonal phase of a block with no sources
+            ph.last = true;
         }
-        ph.first = true;
-        ph.last = (q == nsb - 1 - step);  // last phase of this step
     } else {
         // output block i (bottom-up for lower): diagonal, then its sources
         int step = 0, base = 0;
@@ -252,25 +257,32 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                 *p = (beta == T(0)) ? v : v + beta * *p;
             }
     };
-    auto run = [&](const Phase &ph, const T *af, uint32_t okb) {
-        if (ph.first) {
+    auto zero_acc = [&]() {
 #pragma unroll
-            for (int cb = 0; cb < PS::CBW; ++cb) {
-                acc[cb] = MF::zero();
-                acc2[cb] = MF::zero();
-            }
+        for (int cb = 0; cb < PS::CBW; ++cb) {
+            acc[cb] = MF::zero();
+            acc2[cb] = MF::zero();
         }
-        compute(ph, af, okb);
+    };
+    auto run = [&](const Phase &ph, const T *af, uint32_t okb) {
         if (!TRMM) {
-            if (ph.kind == 0) {  // X_k = inv(M_kk) B_k, in place
+            if (ph.kind == 1) {  // acc += M_ik X_k, kept in registers
+                if (ph.first) zero_acc();
+                compute(ph, af, okb);
+            } else {  // B_i -= acc; X_i = inv(M_ii) B_i, in place
+                if (!ph.first) write_block(ph.i, T(-1), T(1));
                 __syncthreads();
-                write_block(ph.k, T(1), T(0));
-                __syncthreads();
-            } else {
-                write_block(ph.i, T(-1), T(1));
+                zero_acc();
+                compute(ph, af, okb);
+                __syncthreads();  // every wave has read B_i
+                write_block(ph.i, T(1), T(0));
+                __syncthreads();  // X_i complete for the blocks that follow
             }
-            if (ph.last) __syncthreads();
-        } else if (ph.last) {
+            return;
+        }
+        if (ph.first) zero_acc();
+        compute(ph, af, okb);
+        if (ph.last) {
             __syncthreads();  // every wave has read B_i
             write_block(ph.i, T(1), unit ? T(1) : T(0));
             __syncthreads();
