@@ -334,6 +334,8 @@ def main():
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM rates and the mixed and batched legs")
+    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served"),
+                    help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-GPU by design "
@@ -345,6 +347,15 @@ def main():
     from nextla_amd import _lib
 
     nla.load()
+    if args.leg:
+        if args.leg == "refined_mixed":
+            res = mixed_leg(nla, torch)
+        elif args.leg.startswith("batched"):
+            res = batch_leg(nla, torch, torch.float64 if args.leg.endswith("f64") else torch.float32)
+        else:
+            res = serve_leg(nla, torch)
+        print(json.dumps({"leg": args.leg, "result": res}), flush=True)
+        return
     n, m = args.n, args.m
     flops = float(n) * n * m
     log(f"bench: FP64 LL-N TRSM n={n} m={m} steps={args.steps} warmup={args.warmup}")

@@ -166,10 +166,15 @@ struct Recursion {
     bool unit = false;  // unit diagonal: only the panels and the inverses read the diagonal
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
-    // -1 plain fp32.  Each update takes two absolute-maximum slots from amax
-    // (zeroed before the solve).
+    // -1 plain fp32.  fp16 scales each operand block by a power of two from
+    // its absolute maximum: update u uses slot amax_a[u] for its block of A
+    // and amax_b[u] for its block of B (bf16 has fp32's exponent range and is
+    // not scaled).  The k-th update of every solve on one A uses the same
+    // block of A, so with amax_a_ready its maximum from an earlier solve is
+    // reused; amax_b is cleared before each solve.
     int lp = -1;
-    unsigned *amax = nullptr;
+    unsigned *amax_a = nullptr, *amax_b = nullptr;
+    bool amax_a_ready = false;
     int64_t amax_used = 0, amax_cap = 0;
 
     // Leaf as an MFMA product: TRSM applies inv(A_kk) from W, TRMM applies
@@ -276,12 +281,12 @@ struct Recursion {
     // The update with operands rounded to bf16/fp16 after power-of-two
     // scaling per operand block (scales from the blocks' absolute maxima).
     void lp_gemm(View<float> C, View<float> X, View<float> Y, float alpha, float beta) {
-        if (amax_used + 2 > amax_cap) {
+        if (amax_used + 1 > amax_cap) {
             err = hipErrorInvalidValue;
             return;
         }
-        unsigned *slot = amax + amax_used;
-        amax_used += 2;
+        unsigned *sa = amax_a + amax_used, *sb = amax_b + amax_used;
+        ++amax_used;
         LpGemmArgs g;
         g.M = C.nr; g.N = C.nc; g.K = X.nc;
         g.tx = (X.rs != 1); g.ldx = g.tx ? X.rs : X.cs;
@@ -289,10 +294,16 @@ struct Recursion {
         g.X = X.p; g.Y = Y.p; g.C = C.p; g.ldc = C.cs;
         g.alpha = alpha; g.beta = beta;
         g.lp = lp;
-        g.amax = slot;
-        err = launch_absmax(X.nr, X.nc, X.p, X.rs, X.cs, slot, stream);
-        if (err == hipSuccess) err = launch_absmax(Y.nr, Y.nc, Y.p, Y.rs, Y.cs, slot + 1, stream);
-        if (err != hipSuccess) return;
+        if (lp == 1) {
+            // operand roles: left side X = block of A, Y = block of B; right side swapped
+            const bool a_is_x = (side == 'L');
+            g.amax_x = a_is_x ? sa : sb;
+            g.amax_y = a_is_x ? sb : sa;
+            const View<float> &Av = a_is_x ? X : Y, &Bv = a_is_x ? Y : X;
+            if (!amax_a_ready) err = launch_absmax(Av.nr, Av.nc, Av.p, Av.rs, Av.cs, sa, stream);
+            if (err == hipSuccess) err = launch_absmax(Bv.nr, Bv.nc, Bv.p, Bv.rs, Bv.cs, sb, stream);
+            if (err != hipSuccess) return;
+        }
         const double fl = 2.0 * g.M * g.N * g.K;
         const double by = ((double)g.M * g.K + (double)g.K * g.N + 2.0 * g.M * g.N) * sizeof(float);
         err = timed(NLA_KIND_LP_GEMM, fl, by, stream, [&] { return launch_lp_gemm(g, stream); });
@@ -508,7 +519,7 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     L.ld32 = (n + 63) / 64 * 64;
     L.ldr = (L.rows + 63) / 64 * 64;
     const int64_t nblk = (n + 63) / 64;
-    L.amax_slots = 2 * (2 * nblk + 2);  // an update per internal node of the recursion
+    L.amax_slots = 2 * (2 * nblk + 2);  // A and B halves: an update per internal node of the recursion
     int64_t o = 0;
     L.a32 = o;   o = al256(o + L.ld32 * n * 4);
     L.w = o;     o = al256(o + nblk * 64 * 64 * 4);
@@ -569,6 +580,8 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     const double nA = std::sqrt(h[0]), nS = std::sqrt(h[2]);
 
     // D <- op(A)^-1 D (or D op(A)^-1) on the fp32 copy, updates in `prec`
+    bool amax_a_ready = false;  // A-block maxima of the fp16 updates computed
+    const int64_t half = L.amax_slots / 2;
     auto lp_solve = [&](int prec) -> hipError_t {
         Recursion<float> r;
         r.func = 'S';
@@ -581,13 +594,19 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         r.W = W;
         if (prec != NLA_LP_FP32) {
             r.lp = prec;
-            r.amax = amax;
-            r.amax_cap = L.amax_slots;
-            hipError_t e2 = launch_fill_u32(amax, L.amax_slots, 0u, s);
-            if (e2 != hipSuccess) return e2;
+            r.amax_a = amax;
+            r.amax_b = amax + half;
+            r.amax_cap = half;
+            if (prec == NLA_LP_FP16) {
+                r.amax_a_ready = amax_a_ready;
+                hipError_t e2 = launch_fill_u32(amax_a_ready ? amax + half : amax,
+                                                amax_a_ready ? half : 2 * half, 0u, s);
+                if (e2 != hipSuccess) return e2;
+            }
         }
         View<float> Dv{D, 1, L.ldr, L.rows, L.cols};
         r.rec(A32v, Dv, n, 0, 1.0f, 1.0f);
+        if (prec == NLA_LP_FP16 && r.err == hipSuccess) amax_a_ready = true;
         return r.err;
     };
 

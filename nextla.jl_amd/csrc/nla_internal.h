@@ -183,7 +183,7 @@ hipError_t launch_zero(int64_t rows, int64_t cols, T *B, int64_t ldb, hipStream_
 // ---- mixed precision ------------------------------------------------------
 
 // Low-precision GEMM update C = beta*C + alpha*op(X)*op(Y) on fp32 storage:
-// op(X) and op(Y) are scaled by powers of two chosen from amax[0] / amax[1]
+// op(X) and op(Y) are scaled by powers of two chosen from *amax_x / *amax_y
 // (absolute maxima of the two operand blocks as float bit patterns; nullptr
 // for no scaling), rounded to bf16 (lp = 0) or fp16 (lp = 1), multiplied on
 // MFMA with fp32 accumulation and unscaled on the way out.  X, Y need
@@ -199,7 +199,7 @@ struct LpGemmArgs {
     float *C;
     int64_t ldc;
     int lp;
-    const unsigned *amax;
+    const unsigned *amax_x = nullptr, *amax_y = nullptr;
 };
 hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s);
 // amax[0] = max(amax[0], max |P(r,c)|) over P(r,c) = P[r*rs + c*cs]; the slot

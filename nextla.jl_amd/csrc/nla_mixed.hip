@@ -292,7 +292,8 @@ __global__ __launch_bounds__(WM *WN * 64) void lp_gemm_kernel(int64_t M, int64_t
                                                               const float *__restrict__ X, int64_t ldx,
                                                               const float *__restrict__ Y, int64_t ldy, float beta,
                                                               float *C, int64_t ldc, int tiles_m,
-                                                              const unsigned *__restrict__ amax) {
+                                                              const unsigned *__restrict__ amax_x,
+                                                              const unsigned *__restrict__ amax_y) {
     typedef typename LpType<LP>::elem_t E;
     constexpr int NT = WM * WN * 64;
     typedef LpLoader<LP, BM, BK, NT, TX> LX;
@@ -302,8 +303,8 @@ __global__ __launch_bounds__(WM *WN * 64) void lp_gemm_kernel(int64_t M, int64_t
     int tm, tn;
     lp_tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
-    const float sx = amax ? block_scale(amax[0]) : 1.0f;
-    const float sy = amax ? block_scale(amax[1]) : 1.0f;
+    const float sx = amax_x ? block_scale(*amax_x) : 1.0f;
+    const float sy = amax_y ? block_scale(*amax_y) : 1.0f;
     const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0);
     if (interior)
         lp_tile<LP, BM, BN, BK, WM, WN, TX, TY, false>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc, row0, col0,
@@ -319,7 +320,7 @@ static hipError_t lp_launch_cfg(const LpGemmArgs &a, hipStream_t s) {
     int tiles_n = (int)((a.N + BN - 1) / BN);
     hipLaunchKernelGGL((lp_gemm_kernel<LP, BM, BN, BK, WM, WN, TX, TY>), dim3(tiles_m * tiles_n),
                        dim3(WM * WN * 64), 0, s, a.M, a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C,
-                       a.ldc, tiles_m, (const unsigned *)a.amax);
+                       a.ldc, tiles_m, a.amax_x, a.amax_y);
     return hipGetLastError();
 }
 
