@@ -108,9 +108,11 @@ int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int6
  * Batched solves/multiplies of many independent systems (no counterpart in
  * the reference, which makes one unified_rectrxm! call per system).  All
  * systems of a batch share side, uplo, trans, func and dtype; each has its
- * own n (<= 1024), m, A, B and alpha.  Every system is solved as one fused
- * diagonal panel by a persistent kernel that pulls (system, 16-column strip)
- * tasks, longest first, from a device queue.
+ * own n (<= 65536), m, A, B and alpha.  A system of up to 1024 rows is solved
+ * as one fused diagonal panel by a persistent kernel that pulls (system,
+ * 16-column strip) tasks, longest first, from a device queue; a larger one
+ * in 512-row stages, each a panel launch plus one grouped GEMM over all
+ * systems, so a plan takes a fixed number of launches.
  *
  *   nla_batch_workspace_bytes  device workspace the plan needs (descriptors,
  *                              task queue and, for TRSM, the inverted 64x64
@@ -130,7 +132,7 @@ int32_t nla_leaf(char func, char side, char uplo, int32_t dtype, int64_t n, int6
  * 10 stream, 11 plan.
  */
 typedef struct nla_batch_item {
-    int64_t n;       /* order of A_i (<= 1024) */
+    int64_t n;       /* order of A_i (<= 65536) */
     int64_t m;       /* other dimension of B_i */
     const void *A;   /* column-major, leading dimension lda */
     int64_t lda;
@@ -211,6 +213,51 @@ int64_t nla_mixed_workspace_bytes(char side, int64_t n, int64_t m);
 int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, double alpha, int64_t n, int64_t m,
                           const double *A, int64_t lda, double *B, int64_t ldb, double tol, int32_t max_iter,
                           void *workspace, int64_t workspace_bytes, void *stream, nla_mixed_info *info);
+
+/*
+ * Batched FP64-accurate TRSM by fp32 correction solves and FP64 iterative
+ * refinement (no counterpart in the reference; each system converges to the
+ * FP64 solution of its unified_rectrxm! TRSM, src/rectrxm.jl:43-76).  All
+ * systems share side, uplo and trans; each has its own FP64 A_i, B_i, n_i
+ * (<= 65536), m_i and alpha_i (nla_batch_item, func 'S').  Per step, for the
+ * whole batch at once:
+ *   - a batched fp32 solve (nla_batch_* on an fp32 copy of each A_i's
+ *     triangle) of the residual scaled by a power of two;
+ *   - X_i += correction (FP64) for every system not yet done;
+ *   - R_i = alpha_i·B_i - op(A_i)·X_i from a batched FP64 TRMM, and its
+ *     normwise backward error ||R_i||_F / (||A_i||_F ||X_i||_F +
+ *     ||alpha_i·B_i||_F); a system is done (and its X_i frozen) once that is
+ *     <= tol.
+ * The loop ends when every system is done or after max_iter (<=
+ * NLA_MIXED_MAX_ITER) corrections; the stream is synchronized once per step.
+ * A system's result, error and step count do not depend on the rest of the
+ * batch.  X_i overwrites B_i.  backward_error and iterations (host arrays of
+ * count entries, either may be NULL) receive each system's final error and
+ * its number of corrections.  NLA_ERR_NOT_CONVERGED: some system is above
+ * tol after max_iter steps.  A_i's entries must lie in the fp32 range.
+ *   nla_batch_mixed_workspace_bytes  device workspace of a plan (about
+ *     4*n^2 + 20*n*m bytes per system plus the two batched plans);
+ *   nla_batch_mixed_plan_create      validates and uploads descriptors on
+ *     `stream` (A_i, B_i must stay allocated while the plan exists);
+ *   nla_batch_mixed_plan_execute     runs the refinement; use_graph != 0
+ *     replays the batched solve and product from hipGraphs;
+ *   nla_batch_mixed_plan_destroy     releases the plan.
+ * Argument positions for create: 1 side, 2 uplo, 3 trans, 4 items, 5 count,
+ * 6 workspace, 7 workspace_bytes, 8 stream, 9 plan; for execute: 1 plan,
+ * 2 tol, 3 max_iter, 4 stream, 5 use_graph, 6 backward_error, 7 iterations.
+ */
+typedef struct nla_batch_mixed_plan_s *nla_batch_mixed_plan;
+
+int64_t nla_batch_mixed_workspace_bytes(char side, const nla_batch_item *items, int64_t count);
+
+int32_t nla_batch_mixed_plan_create(char side, char uplo, char trans, const nla_batch_item *items, int64_t count,
+                                    void *workspace, int64_t workspace_bytes, void *stream,
+                                    nla_batch_mixed_plan *plan);
+
+int32_t nla_batch_mixed_plan_execute(nla_batch_mixed_plan plan, double tol, int32_t max_iter, void *stream,
+                                     int32_t use_graph, double *backward_error, int32_t *iterations);
+
+int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
 
 /*
  * Process-wide choice of the methods used below the recursion (no

@@ -89,6 +89,10 @@ static hipError_t timed(int kind, double flops, double bytes, hipStream_t s, F f
 
 int32_t report_hip(hipError_t e, const char *where) { return hip_status(e, where); }
 int32_t report_bad_arg(int k, const char *what) { return bad_arg(k, what); }
+int32_t report_status(int32_t code, const char *msg) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s", msg);
+    return code;
+}
 
 static inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
 

@@ -223,4 +223,42 @@ constexpr int NORM_PARTS_MAX = 1024;
 hipError_t launch_norms(int64_t rows, int64_t cols, const double *P, int64_t ld, int tri, double *part,
                         double *out, hipStream_t s);
 
+// ---- batched refinement ----------------------------------------------------
+
+// One FP64 system of a refined batch and its work arrays (column-major; the
+// B-shaped ones rows x cols with leading dimension ldw).  Tasks cover bcb
+// columns of the B-shaped arrays or acb columns of A.
+struct MixSys {
+    const double *A;  // stored triangle (lower or upper) read
+    int64_t lda;
+    float *A32;  // fp32 copy of that triangle
+    int64_t lda32;
+    double *B;  // right-hand side in, solution out
+    int64_t ldb;
+    double *S, *T;  // alpha*B; op(A)*X, then the residual
+    float *D;       // scaled residual, then the correction
+    int64_t ldw;
+    int64_t rows, cols, n;
+    double alpha;
+    int lower;
+    int bcb, acb;
+    int bt0, bnt, at0, ant;  // task ranges [bt0, bt0 + bnt), [at0, at0 + ant)
+};
+
+// Per-system state of the refinement loop.
+struct MixState {
+    double anorm, snorm, sigma, bwd;
+    int done, pad;
+};
+
+hipError_t launch_mix_prep_a(const MixSys *sys, const int2 *atasks, int nat, double *apart, hipStream_t s);
+hipError_t launch_mix_prep_b(const MixSys *sys, const int2 *btasks, int nbt, double *bpart, hipStream_t s);
+hipError_t launch_mix_down(const MixSys *sys, const int2 *btasks, int nbt, const MixState *st, bool from_t,
+                           hipStream_t s);
+hipError_t launch_mix_up(const MixSys *sys, const int2 *btasks, int nbt, const MixState *st, bool first,
+                         hipStream_t s);
+hipError_t launch_mix_resid(const MixSys *sys, const int2 *btasks, int nbt, double *bpart, hipStream_t s);
+hipError_t launch_mix_final(const MixSys *sys, int64_t count, const double *apart, const double *bpart,
+                            MixState *st, bool init, double tol, hipStream_t s);
+
 }  // namespace nla

@@ -18,19 +18,22 @@ A request is one unified_rectrxm!(side, uplo, trans, alpha, func, A, B) call
     completion event of its batch).
 
 FP64 requests are FP64-accuracy requests.  Their TRSMs are served either by
-the plain FP64 path (fp64_trsm="plain", batched like the rest) or by
-low-precision correction solves with FP64 refinement to backward error
-<= 1e-14 (fp64_trsm="mixed", nla_rectrxm_mixed): that call reads its stopping
-test on the host once per step, so each such request runs on a worker thread
-with its own stream and workspace while the loop keeps admitting and
-launching.  The plain path is the faster of the two on MI355X (FP64 MFMA runs
-at the FP64 vector rate, so each FP64 residual costs as much as a whole plain
-solve); "mixed" is there to serve and measure the refined form.
+the plain FP64 path (fp64_trsm="plain", batched like the rest) or by fp32
+correction solves with FP64 refinement to backward error <= 1e-14
+(fp64_trsm="mixed"): same-signature requests are batched into one
+MixedBatchPlan (nla_batch_mixed_*: batched fp32 solves, batched FP64 TRMM
+residuals, each system frozen once it reaches the target).  The refinement
+reads its stopping test on the host once per step, so each refined batch
+runs on a worker thread with its own stream while the loop keeps admitting
+and launching.  The plain path is the faster of the two on MI355X (FP64 MFMA
+runs at the FP64 vector rate, so each FP64 residual costs about as much as a
+whole plain solve); "mixed" is there to serve and measure the refined form.
 
 A system's result does not depend on what else is in its batch: the batched
 kernel solves each (system, strip) task with the same code path whatever the
 batch holds, so a request gives bit-identical output alone and in the mix
-(and a refined request runs the same deterministic kernels either way).
+(a refined request likewise: its steps, error and result do not depend on
+the rest of its batch).
 """
 import collections
 import concurrent.futures
@@ -40,7 +43,7 @@ import time
 import torch
 
 from . import unified_rectrxm_
-from .batch import BATCH_N_MAX, BatchPlan
+from .batch import BATCH_N_MAX, BatchPlan, MixedBatchPlan
 
 
 class Request:
@@ -67,34 +70,46 @@ class _InFlight:
 
 
 class _MixedWorkers:
-    """Worker threads for refined FP64 solves: each has its own stream and a
-    workspace that grows to the largest request it has served."""
+    """Worker threads for refined FP64 batches, each with its own stream."""
 
-    def __init__(self, device, workers, precision, tol):
+    def __init__(self, device, workers, precision, tol, max_iter=10):
         self.device = device
         self.precision = precision
         self.tol = tol
+        self.max_iter = max_iter
         self.local = threading.local()
         self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
 
-    def _state(self):
-        st = getattr(self.local, "st", None)
+    def _stream(self):
+        st = getattr(self.local, "stream", None)
         if st is None:
-            st = self.local.st = {"stream": torch.cuda.Stream(device=self.device), "ws": None}
+            st = self.local.stream = torch.cuda.Stream(device=self.device)
         return st
 
-    def run(self, r):
-        from .mixed import rectrxm_mixed_, workspace_bytes
-        st = self._state()
-        need = workspace_bytes(r.side, r.A.shape[0], r.B.shape[1] if r.side == "L" else r.B.shape[0]) + 256
-        if st["ws"] is None or st["ws"].numel() < need:
-            with torch.cuda.stream(st["stream"]):
-                st["ws"] = torch.empty(need, dtype=torch.uint8, device=self.device)
-        with torch.cuda.stream(st["stream"]):
-            _, info = rectrxm_mixed_(r.side, r.uplo, r.trans, r.alpha, r.A, r.B, precision=self.precision,
-                                     tol=self.tol, workspace=st["ws"])
-        r.done = time.perf_counter()
-        return info
+    def run(self, reqs):
+        """Refine a batch of same-signature FP64 TRSM requests; returns the
+        backward error of each."""
+        from .mixed import rectrxm_mixed_
+        stream = self._stream()
+        small = [r for r in reqs if r.A.shape[0] <= BATCH_N_MAX]
+        big = [r for r in reqs if r.A.shape[0] > BATCH_N_MAX]
+        errors = {}
+        with torch.cuda.stream(stream):
+            if small:
+                r0 = small[0]
+                plan = MixedBatchPlan(r0.side, r0.uplo, r0.trans, [r.A for r in small], [r.B for r in small],
+                                      [r.alpha for r in small], stream=stream)
+                bwd, _, _ = plan.execute(tol=self.tol, max_iter=self.max_iter, stream=stream)
+                plan.close()
+                errors.update({r.rid: b for r, b in zip(small, bwd)})
+            for r in big:  # beyond the batched planner's size limit: one refined call each
+                _, info = rectrxm_mixed_(r.side, r.uplo, r.trans, r.alpha, r.A, r.B, precision=self.precision,
+                                         tol=self.tol)
+                errors[r.rid] = info["backward_error"]
+        now = time.perf_counter()
+        for r in reqs:
+            r.done = now
+        return errors
 
     def close(self):
         self.pool.shutdown(wait=True)
@@ -167,37 +182,46 @@ class Server:
         n = r.A.shape[0]
         if self._is_mixed(r):
             m = r.B.shape[1] if r.side == "L" else r.B.shape[0]
-            b += 4 * n * n + 20 * n * m  # the refinement workspace
+            b += 4 * n * n + 20 * n * m + 256 * n + 4096  # refinement arrays, fp32 inverses, descriptors
         elif r.func == "S":
             b += ((n + 63) // 64) * 64 * 64 * r.A.element_size()  # diagonal inverses
         return b
 
     def _launch_mixed(self):
-        """Hand the first queued refined request to a free worker."""
+        """Hand the queued refined requests that share the first one's
+        signature (within the budget) to a free worker as one batch."""
         busy = sum(1 for f in self.inflight if f.future is not None)
         if busy >= self.mixed_slots:
             return False
-        for k, r in enumerate(self.queue):
-            if not self._is_mixed(r):
-                continue
+        head = next((r for r in self.queue if self._is_mixed(r)), None)
+        if head is None:
+            return False
+        sig = head.signature()
+        take, keep, nbytes = [], collections.deque(), 0
+        while self.queue and len(take) < self.max_batch:
+            r = self.queue.popleft()
             rb = self._req_bytes(r)
-            if self.used + rb > self.budget:
-                self._evict(rb)
-                if self.used + rb > self.budget:
-                    if not self.inflight:
-                        del self.queue[k]
-                        raise RuntimeError(f"request {r.rid} needs {rb} bytes, more than the HBM budget of "
-                                           f"{self.budget}")
-                    return False
-            del self.queue[k]
-            f = _InFlight()
-            f.reqs, f.stream, f.nbytes, f.plan, f.event = [r], None, rb, None, None
-            f.a_keys = {r.a_key} if r.a_key is not None else set()
-            f.future = self.mixed.pool.submit(self.mixed.run, r)
-            self.used += rb
-            self.inflight.append(f)
-            return True
-        return False
+            if self._is_mixed(r) and r.signature() == sig and self.used + nbytes + rb <= self.budget:
+                take.append(r)
+                nbytes += rb
+            else:
+                keep.append(r)
+        self.queue.extendleft(reversed(keep))
+        if not take:
+            need = self._req_bytes(head)
+            self._evict(need)
+            if not self.inflight and self.used + need > self.budget:
+                self.queue.remove(head)
+                raise RuntimeError(f"request {head.rid} needs {need} bytes, more than the HBM budget of "
+                                   f"{self.budget}")
+            return False
+        f = _InFlight()
+        f.reqs, f.stream, f.nbytes, f.plan, f.event = take, None, nbytes, None, None
+        f.a_keys = {r.a_key for r in take if r.a_key is not None}
+        f.future = self.mixed.pool.submit(self.mixed.run, take)
+        self.used += nbytes
+        self.inflight.append(f)
+        return True
 
     def _launch(self):
         if self.mixed is not None and self._launch_mixed():

@@ -149,3 +149,37 @@ def test_trsm_trmm_diag_validation(nla):
         assert fn(b"L", b"L", b"Z", b"U", 1, 1.0, 8, 4, 16, 8, 32, 8, None) == -3
         assert fn(b"R", b"U", b"T", b"U", 1, 1.0, 8, 10, 16, 8, 32, 9, None) == -12
         assert fn(b"L", b"L", b"N", b"U", 1, 1.0, 0, 4, None, 1, None, 1, None) == 0
+
+
+def test_batch_mixed_validation(nla):
+    import ctypes
+    from nextla_amd import _lib
+    lib = _lib.load()
+    items = (_lib.BatchItem * 2)()
+    items[0].n, items[0].m, items[0].A, items[0].lda, items[0].B, items[0].ldb, items[0].alpha = 100, 8, 256, 100, 512, 100, 1.0
+    items[1].n, items[1].m, items[1].A, items[1].lda, items[1].B, items[1].ldb, items[1].alpha = 70000, 4, 64, 70000, 64, 70000, 1.0
+    need = lib.nla_batch_mixed_workspace_bytes(b"L", items, 1)
+    assert need >= 4 * 100 * 100 + 20 * 100 * 8
+    assert lib.nla_batch_mixed_workspace_bytes(b"Q", items, 1) == -1
+    assert lib.nla_batch_mixed_workspace_bytes(b"L", items, 2) == -1
+    plan = ctypes.c_void_p()
+    good = [b"L", b"L", b"N", items, 1, 256, need, None, ctypes.byref(plan)]
+
+    def call(pos, val):
+        args = list(good)
+        args[pos] = val
+        return lib.nla_batch_mixed_plan_create(*args)
+
+    assert call(0, b"X") == -1
+    assert call(1, b"Q") == -2
+    assert call(2, b"Z") == -3
+    assert call(4, 2) == -4          # n = 70000
+    assert call(5, 264) == -6        # alignment
+    assert call(6, need - 1) == -7
+    assert call(8, None) == -9
+    items[0].ldb = 99
+    assert call(4, 1) == -4          # ldb < n
+    bwd = (ctypes.c_double * 1)()
+    its = (ctypes.c_int32 * 1)()
+    assert lib.nla_batch_mixed_plan_execute(None, 1e-14, 10, None, 0, bwd, its) == -1
+    assert lib.nla_batch_mixed_plan_destroy(None) == 0
