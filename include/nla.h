@@ -270,7 +270,8 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      one fused panel launch (default, needs leaf mode 1),
  *                      0 = recurse to 64-leaves;
  *   NLA_CFG_PANEL_ROWS_F32 / _F64: panel size for that dtype, a multiple of
- *                      64 in [64, 512] (defaults 256 for FP32, 512 for FP64);
+ *                      64 in [64, 1024] (defaults 256 for FP32, 512 for FP64;
+ *                      above 512 rows FP64 panels use 16 columns per group);
  *   NLA_CFG_PANEL_STRIP_F32 / _F64: right-hand-side columns per panel
  *                      work-group, 16 (default) or 32;
  *   NLA_CFG_BATCH_CLASSES: size classes new batch plans may use, bit c for

@@ -115,7 +115,7 @@ static std::atomic<int> g_panel{-1};
 // and its RHS columns per work-group.  Defaults from a sweep on MI355X at
 // n = 8192 (rows 256/384/512 x strips 16/32): FP64 512 rows, FP32 256 rows
 // (3-5 % faster than 512 there), 16 columns for both.
-static std::atomic<int> g_panel_rows[2] = {{256}, {PANEL_MAX}};
+static std::atomic<int> g_panel_rows[2] = {{256}, {512}};
 static std::atomic<int> g_panel_strip[2] = {{PANEL_STRIP}, {PANEL_STRIP}};
 static std::atomic<int> g_batch_classes{0xF};
 
@@ -166,7 +166,7 @@ struct Recursion {
     const T *W = nullptr;   // inverted diagonal blocks (TRSM, leaf mode 1)
     int mode = 1;
     bool use_panel = true;  // fuse diagonal blocks <= panel_rows into one launch
-    int panel_rows = PANEL_MAX, panel_strip = PANEL_STRIP;
+    int panel_rows = 512, panel_strip = PANEL_STRIP;
     bool unit = false;  // unit diagonal: only the panels and the inverses read the diagonal
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,

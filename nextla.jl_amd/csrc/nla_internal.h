@@ -54,7 +54,7 @@ struct LeafArgs {
 
 // Largest diagonal block the fused panel kernel solves in one launch, and the
 // default RHS columns per work-group (16 or 32 per launch).
-constexpr int PANEL_MAX = 512;
+constexpr int PANEL_MAX = 1024;
 constexpr int PANEL_STRIP = 16;
 
 // Fused diagonal panel: M(i,j) = M[i*mrs + j*mcs] (nbig x nbig, lower or

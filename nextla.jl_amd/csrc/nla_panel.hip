@@ -351,26 +351,32 @@ __global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T
                                    bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw);
 }
 
-template <typename T, int W>
+template <typename T, int NBIG, int W>
 static hipError_t launch_panel_w(const PanelArgs<T> &a, hipStream_t s) {
     const int64_t groups = (a.m + W - 1) / W;
     const int lo = a.lower ? 1 : 0, un = a.unit ? 1 : 0;
     if (a.trmm)
-        hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
+        hipLaunchKernelGGL((panel_kernel<T, NBIG, W, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
                            a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     else
-        hipLaunchKernelGGL((panel_kernel<T, PANEL_MAX, W, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
+        hipLaunchKernelGGL((panel_kernel<T, NBIG, W, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
                            a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     return hipGetLastError();
 }
 
+// Panels up to 512 rows use the 512-row LDS strip; larger ones (up to
+// PANEL_MAX) the 1024-row strip, whose FP64 form only fits LDS with 16
+// columns per work-group.
 template <typename T>
 hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s) {
     if (a.m <= 0 || a.nbig <= 0) return hipSuccess;
     if (a.nbig > PANEL_MAX) return hipErrorInvalidValue;
-    if (a.strip == 32) return launch_panel_w<T, 32>(a, s);
-    if (a.strip != 16) return hipErrorInvalidValue;
-    return launch_panel_w<T, 16>(a, s);
+    if (a.strip != 16 && a.strip != 32) return hipErrorInvalidValue;
+    if (a.nbig <= 512) return a.strip == 32 ? launch_panel_w<T, 512, 32>(a, s) : launch_panel_w<T, 512, 16>(a, s);
+    if constexpr (sizeof(T) == 4) {
+        if (a.strip == 32) return launch_panel_w<T, 1024, 32>(a, s);
+    }
+    return launch_panel_w<T, 1024, 16>(a, s);
 }
 
 template hipError_t launch_panel<float>(const PanelArgs<float> &, hipStream_t);

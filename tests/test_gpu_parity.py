@@ -153,7 +153,7 @@ def test_recursion_all_combos(nla, method, dt, n, m):
         assert O.rel_error(X, ref) < tol, (side, uplo, trans, func, O.rel_error(X, ref))
 
 
-PANEL_SHAPES = [(256, 16), (512, 32), (192, 32)]
+PANEL_SHAPES = [(256, 16), (512, 32), (192, 32), (1024, 16), (768, 32)]
 
 
 @pytest.mark.parametrize("rows,strip", PANEL_SHAPES)
@@ -186,7 +186,7 @@ def test_panel_shapes(nla, rows, strip, dt):
 def test_config_rejects_invalid_panel_shapes(nla):
     from nextla_amd import _lib
     assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 100) == -1
-    assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 1024) == -1
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 1088) == -1
     assert _lib.set_config(_lib.NLA_CFG_PANEL_STRIP_F32, 24) == -1
     assert _lib.set_config(99, 1) == -1
 
