@@ -160,15 +160,26 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
     const T *base = lower ? A + off * (rs + cs) : A + (off + nb - 1) * (rs + cs);
     const int64_t ars = lower ? rs : -rs, acs = lower ? cs : -cs;
     const bool arow = (ars == 1 || ars == -1);
-    // stage L (identity padding past nb); zero the inverse
-#pragma unroll 4
-    for (int e = t; e < NB * NB; e += NT) {
-        int i = arow ? e % NB : e / NB;
-        int j = arow ? e / NB : e % NB;
+    // stage L (identity padding past nb); zero the inverse.  All 16 loads of
+    // a thread are issued before the first LDS store.
+    constexpr int PER = NB * NB / NT;
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = t + u * NT;
+        const int i = arow ? e % NB : e / NB;
+        const int j = arow ? e / NB : e % NB;
         // a unit diagonal is taken as 1 and not read
-        bool ok = i < nb && j < nb && j <= i && !(unit && i == j);
-        T v = *(ok ? base + i * ars + j * acs : base);
-        sl[j * NB + i] = ok ? v : (i == j ? T(1) : T(0));
+        const bool ok = i < nb && j < nb && j <= i && !(unit && i == j);
+        v[u] = *(ok ? base + i * ars + j * acs : base);
+        v[u] = ok ? v[u] : (i == j ? T(1) : T(0));
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int e = t + u * NT;
+        const int i = arow ? e % NB : e / NB;
+        const int j = arow ? e / NB : e % NB;
+        sl[j * NB + i] = v[u];
         si[e] = T(0);
     }
     __syncthreads();
@@ -188,35 +199,61 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
         for (int i = 0; i < 16; ++i) si[(o + c) * NB + o + i] = x[i];
     }
     __syncthreads();
+    // The products below run over full, unrolled k ranges: the inverse
+    // blocks in si are zero outside their lower triangles (si starts zeroed),
+    // so the extra terms add exact zeros, and fixed trip counts let every
+    // LDS read of a product be issued before its FMAs.
     // 2. two 32-blocks p: st_p = C_p * inv(D_2p) (16 x 16), then
-    //    si[32p+16.., 32p..] = -inv(D_2p+1) * st_p
-    for (int e = t; e < 2 * 256; e += NT) {
-        int p = e >> 8, q = e & 255, i = q & 15, j = q >> 4, o = 32 * p;
-        T acc = T(0);
-        for (int k = j; k < 16; ++k) acc += sl[(o + k) * NB + o + 16 + i] * si[(o + j) * NB + o + k];
-        st[p * 256 + j * 16 + i] = acc;
+    //    si[32p+16.., 32p..] = -inv(D_2p+1) * st_p.  Thread t: block p = t>>7,
+    //    row i, columns j and j+8.
+    {
+        const int p = t >> 7, q = t & 127, i = q & 15, j = q >> 4, o = 32 * p;
+        T a0 = T(0), a1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const T c = sl[(o + k) * NB + o + 16 + i];
+            a0 += c * si[(o + j) * NB + o + k];
+            a1 += c * si[(o + j + 8) * NB + o + k];
+        }
+        st[p * 256 + j * 16 + i] = a0;
+        st[p * 256 + (j + 8) * 16 + i] = a1;
+        __syncthreads();
+        a0 = T(0);
+        a1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const T c = si[(o + 16 + k) * NB + o + 16 + i];
+            a0 += c * st[p * 256 + j * 16 + k];
+            a1 += c * st[p * 256 + (j + 8) * 16 + k];
+        }
+        si[(o + j) * NB + o + 16 + i] = -a0;
+        si[(o + j + 8) * NB + o + 16 + i] = -a1;
     }
     __syncthreads();
-    for (int e = t; e < 2 * 256; e += NT) {
-        int p = e >> 8, q = e & 255, i = q & 15, j = q >> 4, o = 32 * p;
-        T acc = T(0);
-        for (int k = 0; k <= i; ++k) acc += si[(o + 16 + k) * NB + o + 16 + i] * st[p * 256 + j * 16 + k];
-        si[(o + j) * NB + o + 16 + i] = -acc;
-    }
-    __syncthreads();
-    // 3. st = Q * inv(P) (32 x 32, inv(P) lower), si[32.., 0..32] = -inv(R) * st
-    for (int e = t; e < 1024; e += NT) {
-        int i = e & 31, j = e >> 5;
-        T acc = T(0);
-        for (int k = j; k < 32; ++k) acc += sl[k * NB + 32 + i] * si[j * NB + k];
-        st[j * 32 + i] = acc;
-    }
-    __syncthreads();
-    for (int e = t; e < 1024; e += NT) {
-        int i = e & 31, j = e >> 5;
-        T acc = T(0);
-        for (int k = 0; k <= i; ++k) acc += si[(32 + k) * NB + 32 + i] * st[j * 32 + k];
-        si[j * NB + 32 + i] = -acc;
+    // 3. st = Q * inv(P) (32 x 32, inv(P) lower), si[32.., 0..32] = -inv(R) * st.
+    //    Thread t: row i = t & 31, columns j0 + 8u.
+    {
+        const int i = t & 31, j0 = t >> 5;
+        T a[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const T c = sl[k * NB + 32 + i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += c * si[(j0 + 8 * u) * NB + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) st[(j0 + 8 * u) * 32 + i] = a[u];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = T(0);
+#pragma unroll
+        for (int k = 0; k < 32; ++k) {
+            const T c = si[(32 + k) * NB + 32 + i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += c * st[(j0 + 8 * u) * 32 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) si[(j0 + 8 * u) * NB + 32 + i] = -a[u];
     }
     __syncthreads();
     // write: column-major 64 x 64; an upper block's inverse is J inv(L) J
