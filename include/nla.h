@@ -276,9 +276,12 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      work-group, 16 (default) or 32;
  *   NLA_CFG_BATCH_CLASSES: size classes new batch plans may use, bit c for
  *                      systems up to 128 << c rows (c = 0..3; the 1024 class
- *                      is always on; default all).  A system goes to the
+ *                      is always on; default 0x1F: all, with bit 4).  A system goes to the
  *                      smallest enabled class that holds it; smaller classes
  *                      use less LDS per work-group, so more of them run per CU.
+ *                      Bit 4 (0x10): systems above 512 rows are solved in
+ *                      512-row stages (panels of the 512 class plus a grouped
+ *                      GEMM) instead of as one 1024-row panel.
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {

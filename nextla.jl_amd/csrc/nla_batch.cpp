@@ -5,7 +5,8 @@
 // A plan puts every system into the panel kernel's canonical form (the same
 // transformation nla_rectrxm applies to one system: op(A) as a strided view
 // with flipped uplo for 'T'/'C'; right-side systems solved on the transpose).
-// A system of up to BATCH_MAX rows is one panel.  A larger one is cut into
+// A system of up to BATCH_MAX rows (512 when the class mask has
+// BATCH_STAGE_ABOVE_512 set) is one panel.  A larger one is cut into
 // blocks of BATCH_STAGE rows and solved in stages, as the recursion would at
 // that block size: per stage one diagonal block (a panel) and one GEMM that
 // applies it to the rest of the system (TRSM: the rows still to solve; TRMM:
@@ -40,25 +41,30 @@ int32_t report_bad_arg(int k, const char *what);
 static inline char upc(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
 static inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 
-// Stages and blocks of one system.
-static inline int64_t stages_of(int64_t n) { return n <= BATCH_MAX ? 1 : (n + BATCH_STAGE - 1) / BATCH_STAGE; }
+// Stages of one system: one up to the single-panel limit, else 512-row blocks.
+static inline int64_t panel_limit(int mask) { return (mask & BATCH_STAGE_ABOVE_512) ? BATCH_STAGE : BATCH_MAX; }
+static inline int64_t stages_of(int64_t n, int mask) {
+    return n <= panel_limit(mask) ? 1 : (n + BATCH_STAGE - 1) / BATCH_STAGE;
+}
 
 struct Layout {
     int64_t counter, sys, pan, ptasks, gem, gtasks, blk, inv, total;
     int64_t npan, nptasks, ngem, ngtasks, nblk, nstages;
+    int mask;  // size-class mask the plan was laid out for
 };
 
 // Counts and offsets; they depend on func, dtype and the items' n, m only.
-static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, int64_t count) {
+static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, int64_t count, int mask) {
     const int64_t es = dtype == NLA_F64 ? 8 : 4;
     const int64_t sys_bytes = dtype == NLA_F64 ? sizeof(BatchSys<double>) : sizeof(BatchSys<float>);
     const int64_t gem_bytes = dtype == NLA_F64 ? sizeof(GroupGemm<double>) : sizeof(GroupGemm<float>);
     Layout L{};
+    L.mask = mask;
     L.nstages = 0;
     for (int64_t i = 0; i < count; ++i) {
         const int64_t n = items[i].n, m = items[i].m;
         if (n <= 0 || m <= 0) continue;
-        const int64_t st = stages_of(n), strips = (m + 15) / 16;
+        const int64_t st = stages_of(n, mask), strips = (m + 15) / 16;
         L.nstages = std::max(L.nstages, st);
         L.npan += st;
         L.nptasks += st * strips;
@@ -165,7 +171,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             inv_off += nb * 64 * 64;
         }
         const int64_t strips = (it.m + 15) / 16;
-        const int J = (int)stages_of(it.n);
+        const int J = (int)stages_of(it.n, L.mask);
         if (J == 1) {  // one panel, alpha at its first (TRSM) / last (TRMM) touch
             const int pi = (int)pan.size();
             pan.push_back(S);
@@ -241,7 +247,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
         }
     }
     // Panel tasks: by stage, then size class (largest first), then cost.
-    const int mask = batch_class_mask();
+    const int mask = L.mask;
     auto cls_of = [&](int n) {
         for (int c = 0; c < 4; ++c)
             if (((mask >> c) & 1 || c == 3) && n <= BATCH_CLASSES[c]) return c;
@@ -357,7 +363,7 @@ int64_t nla_batch_workspace_bytes(char func, int32_t dtype, const nla_batch_item
     func = upc(func);
     if ((func != 'S' && func != 'M') || (dtype != NLA_F32 && dtype != NLA_F64) || count < 0 || (count && !items))
         return -1;
-    return layout_of(func, dtype, items, count).total;
+    return layout_of(func, dtype, items, count, batch_class_mask()).total;
 }
 
 int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32_t dtype, const nla_batch_item *items,
@@ -379,7 +385,7 @@ int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32
             return report_bad_arg(6, "items (pointer or leading dimension)");
     }
     if (plan == nullptr) return report_bad_arg(11, "plan");
-    Layout L = layout_of(func, dtype, items, count);
+    Layout L = layout_of(func, dtype, items, count, batch_class_mask());
     if (workspace == nullptr || workspace_bytes < L.total) return report_bad_arg(9, "workspace_bytes");
     if (((uintptr_t)workspace & 255) != 0) return report_bad_arg(8, "workspace (needs 256-byte alignment)");
     nla_batch_plan_s *P = new (std::nothrow) nla_batch_plan_s();

@@ -84,6 +84,9 @@ hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s);
 // rows: a batched panel launch per stage, then a grouped GEMM that applies
 // the stage's block to the rest of its system.
 constexpr int BATCH_MAX = 1024;
+// Class-mask bit: systems above 512 rows are solved in 512-row stages instead
+// of as one 1024-row panel.
+constexpr int BATCH_STAGE_ABOVE_512 = 0x10;
 constexpr int BATCH_STAGE = 512;
 constexpr int BATCH_N_MAX = 65536;
 

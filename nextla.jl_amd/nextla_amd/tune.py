@@ -5,11 +5,14 @@ src/rectrxm.jl:52,63); here the corresponding choices are measured on the
 device at hand and kept per device:
 
   panel rows / strip  the largest diagonal block one fused panel launch
-                      solves (a multiple of 64 up to 512) and the right-hand
+                      solves (a multiple of 64 up to 1024; 256-512 are
+                      measured) and the right-hand
                       sides per work-group (16 or 32), per dtype
                       (NLA_CFG_PANEL_ROWS_*, NLA_CFG_PANEL_STRIP_*);
   batch classes       which LDS-sized variants of the persistent batched
-                      solver a plan may use (NLA_CFG_BATCH_CLASSES).
+                      solver a plan may use, and whether systems above 512
+                      rows are one 1024-row panel or two 512-row stages
+                      (NLA_CFG_BATCH_CLASSES).
 
 Each candidate is timed with HIP events on a representative problem (the
 8192 x 8192 drop-in case for panels; a 10,000-system-style mixed batch for
@@ -29,7 +32,7 @@ from . import _lib
 
 PANEL_ROWS = (256, 384, 512)
 PANEL_STRIPS = (16, 32)
-BATCH_MASKS = (0x8, 0xC, 0xE, 0xF)
+BATCH_MASKS = (0x8, 0xC, 0xE, 0xF, 0x1E, 0x1F)  # 0x10: stage systems above 512 rows
 _KEYS = {"f32": (_lib.NLA_CFG_PANEL_ROWS_F32, _lib.NLA_CFG_PANEL_STRIP_F32),
          "f64": (_lib.NLA_CFG_PANEL_ROWS_F64, _lib.NLA_CFG_PANEL_STRIP_F64)}
 

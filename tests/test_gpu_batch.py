@@ -160,14 +160,15 @@ def test_staged_alone_equals_in_mix_and_graph(nla):
         plan.close()
 
 
-@pytest.mark.parametrize("mask", [0x8, 0xA, 0xF])
+@pytest.mark.parametrize("mask", [0x8, 0xA, 0xF, 0x18, 0x1A])
 def test_size_classes_bitwise_identical(nla, mask):
     """Whatever size classes a plan may use (NLA_CFG_BATCH_CLASSES), every
-    system gets the same bits: the classes differ only in LDS stride."""
+    system gets the same bits: the classes differ only in LDS stride (with
+    and without bit 0x10, which changes the algorithm above 512 rows)."""
     from nextla_amd import _lib, batch
     shapes = [(16, 3), (128, 20), (129, 7), (256, 33), (300, 16), (512, 5), (513, 40), (1024, 17)]
     results = {}
-    for cur in (0xF, mask):
+    for cur in (0xF | (mask & 0x10), mask):
         old = _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, cur)
         try:
             for side, uplo, func in (("L", "L", "S"), ("R", "U", "M")):
@@ -185,3 +186,39 @@ def test_size_classes_bitwise_identical(nla, mask):
     for key, (ref, got) in results.items():
         for r, g in zip(ref, got):
             assert np.array_equal(r, g), key
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_staged_above_512_option(nla, dt):
+    """Class-mask bit 0x10: systems of 513..1024 rows solved in two 512-row
+    stages (panel, grouped GEMM, panel) instead of one 1024-row panel.  Every
+    combo stays within the oracle's bounds, and a system alone and in a
+    mixed batch gets the same bits under that option."""
+    from nextla_amd import _lib, batch
+    rng = np.random.default_rng(17)
+    eps = np.finfo(dt).eps
+    shapes = [(513, 40), (700, 9), (1024, 17), (300, 5), (64, 1)]
+    old = _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, 0x1F)
+    try:
+        for side, uplo, trans, func in itertools.product("LR", "LU", "NT", "SM"):
+            hostA, hostB, As, Bs = [], [], [], []
+            for n, m in shapes:
+                A, B = make_system(rng, n, m, side, uplo, dt)
+                hostA.append(A)
+                hostB.append(B)
+                As.append(dev(A))
+                Bs.append(dev(B))
+            batch.batched_rectrxm_(side, uplo, trans, func, As, Bs, alphas=-0.75)
+            torch.cuda.synchronize()
+            for A, B, X in zip(hostA, hostB, Bs):
+                n = A.shape[0]
+                e = (O.backward_error if func == "S" else O.product_error)(side, uplo, trans, -0.75, A, B,
+                                                                          X.cpu().numpy())
+                assert e <= 10 * n * eps, (side, uplo, trans, func, n, e)
+            if (side, uplo, trans) == ("L", "L", "N"):
+                alone = [dev(hostB[2])]
+                batch.batched_rectrxm_(side, uplo, trans, func, [As[2]], alone, alphas=-0.75)
+                torch.cuda.synchronize()
+                assert torch.equal(alone[0], Bs[2]), func
+    finally:
+        _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, old)

@@ -126,8 +126,9 @@ def test_tuner_picks_fastest_and_restores(nla):
     assert (res["rows"], res["strip"]) == (384, 16)
     assert res["ms"]["384/16"] == 2.5
     assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, before) == before
-    bres = tune.tune_batch("f64", measure=lambda mk: {0x8: 5.0, 0xC: 4.0, 0xE: 3.0, 0xF: 3.5}[mk])
-    assert bres["mask"] == 0xE
+    bres = tune.tune_batch("f64", measure=lambda mk: {0x8: 5.0, 0xC: 4.0, 0xE: 3.0, 0xF: 3.5, 0x1E: 3.2,
+                                                      0x1F: 3.1}[mk])
+    assert bres["mask"] == 0xE and len(bres["ms"]) == len(tune.BATCH_MASKS)
     old = tune.apply({"panel": {"f64": {"rows": 384, "strip": 16}}, "batch_classes": 0xE})
     try:
         assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 384) == 384
