@@ -15,6 +15,12 @@ with the reference's argument order and return values.  Work is queued on
 `AMDGPU.stream()`; call `AMDGPU.synchronize()` (what
 `KernelAbstractions.synchronize` does for a ROCBackend) before reading B.
 
+It also exposes the forms the reference does not have: `batched_rectrxm!`
+(many systems in one batched launch sequence), `rectrxm_mixed!` (FP64-accurate
+TRSM by low-precision correction solves and FP64 refinement) and
+`batched_rectrxm_mixed!` (the same for many systems at once), and methods of
+NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`.
+
 Differences from the reference, by design: arguments are validated before any
 device work (`ArgumentError`, B untouched), lower-case characters are
 accepted, and `alpha == 0` zeroes B without reading it (BLAS semantics).
@@ -87,6 +93,148 @@ for (f, func, side, uplo) in ((:LeftLowerTRSM!, 'S', 'L', 'L'), (:LeftUpperTRSM!
     @eval function NextLA.$f(A::DeviceMat{T}, B::DeviceMat{T}) where {T <: Union{Float32, Float64}}
         NextLA.unified_rectrxm!($side, $uplo, 'N', one(T), $func, A, B)
     end
+end
+
+# trsm / trmm wrappers (src/trsm.jl:186, src/trmm.jl:430): transa and diag honoured
+for (f, sym) in ((:trsm, :nla_trsm), (:trmm, :nla_trmm))
+    @eval function NextLA.$f(side::Char, uplo::Char, transa::Char, diag::Char, A::DeviceMat{T}, B::DeviceMat{T},
+                             alpha=one(T)) where {T <: Union{Float32, Float64}}
+        n = size(A, 1)
+        m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
+        pa, lda = ptr_ld(A)
+        pb, ldb = ptr_ld(B)
+        st = GC.@preserve A B ccall(($(QuoteNode(sym)), libnlahip), Int32,
+            (Cchar, Cchar, Cchar, Cchar, Int32, Float64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64,
+             Ptr{Cvoid}),
+            Cchar(side), Cchar(uplo), Cchar(transa), Cchar(diag), dtype_code(T), Float64(alpha), Int64(n),
+            Int64(m), pa, lda, pb, ldb, hip_stream())
+        check(st, $(string(f)))
+        return B
+    end
+end
+
+# ---- forms without a reference counterpart ---------------------------------
+
+struct NlaBatchItem
+    n::Int64
+    m::Int64
+    A::Ptr{Cvoid}
+    lda::Int64
+    B::Ptr{Cvoid}
+    ldb::Int64
+    alpha::Float64
+end
+
+function batch_items(side::Char, As, Bs, alphas)
+    length(As) == length(Bs) || throw(DimensionMismatch("As and Bs differ in length"))
+    items = Vector{NlaBatchItem}(undef, length(As))
+    for (i, (A, B)) in enumerate(zip(As, Bs))
+        pa, lda = ptr_ld(A)
+        pb, ldb = ptr_ld(B)
+        m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
+        items[i] = NlaBatchItem(size(A, 1), m, pa, lda, pb, ldb, Float64(alphas isa Number ? alphas : alphas[i]))
+    end
+    return items
+end
+
+# 256-byte aligned device workspace of at least `bytes` bytes
+function workspace(bytes::Integer)
+    buf = AMDGPU.ROCVector{UInt8}(undef, bytes + 256)
+    p = pointer(buf)
+    off = (256 - (UInt(p) % 256)) % 256
+    return buf, p + off
+end
+
+"""
+    batched_rectrxm!(side, uplo, transpose, func, As, Bs; alphas=1, use_graph=false)
+
+`unified_rectrxm!` applied to every (A_i, B_i) pair in one persistent batched
+launch sequence (nla_batch_*).  Returns Bs; work is queued on AMDGPU.stream().
+"""
+function batched_rectrxm!(side::Char, uplo::Char, transpose::Char, func::Char, As::AbstractVector,
+                          Bs::AbstractVector; alphas=1, use_graph::Bool=false)
+    T = eltype(first(As))
+    items = batch_items(side, As, Bs, alphas)
+    GC.@preserve As Bs items begin
+        bytes = ccall((:nla_batch_workspace_bytes, libnlahip), Int64, (Cchar, Int32, Ptr{NlaBatchItem}, Int64),
+                      Cchar(func), dtype_code(T), items, length(items))
+        bytes >= 0 || throw(ArgumentError("invalid batch description"))
+        buf, ws = workspace(bytes)
+        plan = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve buf begin
+            check(ccall((:nla_batch_plan_create, libnlahip), Int32,
+                        (Cchar, Cchar, Cchar, Cchar, Int32, Ptr{NlaBatchItem}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid},
+                         Ptr{Ptr{Cvoid}}),
+                        Cchar(side), Cchar(uplo), Cchar(transpose), Cchar(func), dtype_code(T), items, length(items),
+                        ws, bytes, hip_stream(), plan), "nla_batch_plan_create")
+            st = ccall((:nla_batch_plan_execute, libnlahip), Int32, (Ptr{Cvoid}, Ptr{Cvoid}, Int32), plan[],
+                       hip_stream(), Int32(use_graph))
+            AMDGPU.synchronize()
+            ccall((:nla_batch_plan_destroy, libnlahip), Int32, (Ptr{Cvoid},), plan[])
+            check(st, "nla_batch_plan_execute")
+        end
+    end
+    return Bs
+end
+
+"""
+    rectrxm_mixed!(side, uplo, transpose, alpha, A, B; precision=:fp16, tol=1e-14, max_iter=20)
+
+FP64-accurate TRSM by bf16 / fp16 / fp32 correction solves and FP64 iterative
+refinement (nla_rectrxm_mixed).  Returns (B, steps, backward_error).
+"""
+function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, A::DeviceMat{Float64},
+                        B::DeviceMat{Float64}; precision::Symbol=:fp16, tol::Real=1e-14, max_iter::Integer=20)
+    prec = Dict(:bf16 => Int32(0), :fp16 => Int32(1), :fp32 => Int32(2))[precision]
+    n = size(A, 1)
+    m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
+    pa, lda = ptr_ld(A)
+    pb, ldb = ptr_ld(B)
+    bytes = ccall((:nla_mixed_workspace_bytes, libnlahip), Int64, (Cchar, Int64, Int64), Cchar(side), n, m)
+    buf, ws = workspace(bytes)
+    info = zeros(UInt8, 4 + 4 + 8 * 32)  # nla_mixed_info: int32 iterations, int32 precision, double[1 + 31]
+    st = GC.@preserve A B buf info ccall((:nla_rectrxm_mixed, libnlahip), Int32,
+        (Cchar, Cchar, Cchar, Int32, Float64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Float64, Int32,
+         Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{UInt8}),
+        Cchar(side), Cchar(uplo), Cchar(transpose), prec, Float64(alpha), Int64(n), Int64(m), pa, lda, pb, ldb,
+        Float64(tol), Int32(max_iter), ws, bytes, hip_stream(), info)
+    check(st, "rectrxm_mixed!")
+    steps = reinterpret(Int32, info[1:4])[1]
+    bwd = reinterpret(Float64, info[9:16])[1]
+    return B, steps, bwd
+end
+
+"""
+    batched_rectrxm_mixed!(side, uplo, transpose, As, Bs; alphas=1, tol=1e-14, max_iter=10)
+
+FP64-accurate TRSM of every (A_i, B_i) pair by batched fp32 correction solves
+and FP64 refinement (nla_batch_mixed_*).  Returns (Bs, backward errors, steps).
+"""
+function batched_rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, As::AbstractVector, Bs::AbstractVector;
+                                alphas=1, tol::Real=1e-14, max_iter::Integer=10)
+    items = batch_items(side, As, Bs, alphas)
+    count = length(items)
+    bwd = zeros(Float64, count)
+    its = zeros(Int32, count)
+    GC.@preserve As Bs items begin
+        bytes = ccall((:nla_batch_mixed_workspace_bytes, libnlahip), Int64, (Cchar, Ptr{NlaBatchItem}, Int64),
+                      Cchar(side), items, count)
+        bytes >= 0 || throw(ArgumentError("invalid batch description"))
+        buf, ws = workspace(bytes)
+        plan = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve buf begin
+            check(ccall((:nla_batch_mixed_plan_create, libnlahip), Int32,
+                        (Cchar, Cchar, Cchar, Ptr{NlaBatchItem}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}),
+                        Cchar(side), Cchar(uplo), Cchar(transpose), items, count, ws, bytes, hip_stream(), plan),
+                  "nla_batch_mixed_plan_create")
+            st = ccall((:nla_batch_mixed_plan_execute, libnlahip), Int32,
+                       (Ptr{Cvoid}, Float64, Int32, Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Int32}),
+                       plan[], Float64(tol), Int32(max_iter), hip_stream(), Int32(1), bwd, its)
+            ccall((:nla_batch_mixed_plan_destroy, libnlahip), Int32, (Ptr{Cvoid},), plan[])
+            check(st, "nla_batch_mixed_plan_execute")
+        end
+    end
+    return Bs, bwd, its
 end
 
 end # module

@@ -124,7 +124,7 @@ class Server:
     """
 
     def __init__(self, device="cuda", budget_bytes=64 << 30, max_inflight=4, max_batch=512, use_graph=False,
-                 fp64_trsm="plain", mixed_precision="fp16", mixed_tol=1e-14, mixed_workers=4):
+                 fp64_trsm="plain", mixed_precision="fp16", mixed_tol=1e-14, mixed_workers=4, direct_rows=None):
         if fp64_trsm not in ("plain", "mixed"):
             raise ValueError(f"fp64_trsm must be 'plain' or 'mixed', got {fp64_trsm!r}")
         self.device = torch.device(device)
@@ -142,6 +142,9 @@ class Server:
         self.mixed = (_MixedWorkers(self.device, mixed_workers, mixed_precision, mixed_tol)
                       if fp64_trsm == "mixed" else None)
         self.mixed_slots = mixed_workers
+        # requests with more rows than this go through the recursion one call
+        # each (same stream, no wait) instead of into the batched plan
+        self.direct_rows = BATCH_N_MAX if direct_rows is None else min(int(direct_rows), BATCH_N_MAX)
 
     # -- resident factors ----------------------------------------------------
     def resident_a(self, key, A=None):
@@ -251,8 +254,8 @@ class Server:
         f = _InFlight()
         f.reqs, f.stream, f.nbytes, f.future = take, stream, nbytes, None
         f.a_keys = {r.a_key for r in take if r.a_key is not None}
-        small = [r for r in take if r.A.shape[0] <= BATCH_N_MAX]
-        big = [r for r in take if r.A.shape[0] > BATCH_N_MAX]
+        small = [r for r in take if r.A.shape[0] <= self.direct_rows]
+        big = [r for r in take if r.A.shape[0] > self.direct_rows]
         with torch.cuda.stream(stream):
             f.plan = None
             if small:
