@@ -118,7 +118,8 @@ def event_ms(torch, fn, reps):
 
 
 def mixed_leg(nla, torch):
-    """FP64-accurate TRSM by fp16 / bf16 correction solves and FP64
+    """FP64-accurate TRSM by fp16 / bf16 correction solves (and the
+    automatic per-step choice among bf16 / fp16 / fp32) with FP64
     refinement vs the plain FP64 solve on the same system."""
     from nextla_amd import inputs
     from nextla_amd.mixed import rectrxm_mixed_, workspace_bytes
@@ -145,7 +146,7 @@ def mixed_leg(nla, torch):
            "cond1_measured": kappa, "plain_fp64_ms": round(tp, 3),
            "plain_fp64_tflops": round(float(n) * n * m / tp / 1e9, 3)}
     X = B0.clone()
-    for prec in ("fp16", "bf16"):
+    for prec in ("fp16", "bf16", "auto"):
         X.copy_(B0)
         rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws, raise_on_stall=False)
         torch.cuda.synchronize()

@@ -186,7 +186,10 @@ int32_t nla_trmm(char side, char uplo, char transa, char diag, int32_t dtype, do
  *     error ||R||_F / (||A||_F ||X||_F + ||alpha·B||_F); the loop stops once
  *     that is <= tol, otherwise X += the correction solve of R;
  *   - a step that less than halves the backward error raises the update
- *     precision for the following steps (bf16 -> fp16 -> fp32).
+ *     precision for the following steps (bf16 -> fp16 -> fp32);
+ *   - NLA_LP_AUTO: X0 from fp32 updates, then each correction in the
+ *     precision (bf16, fp16 or fp32) with the least predicted cost to reach
+ *     tol, from the measured error and the contraction of the last step.
  * X overwrites B.  The call synchronizes `stream` once per step (the
  * stopping test reads two norms on the host).  NLA_ERR_NOT_CONVERGED: after
  * max_iter (<= NLA_MIXED_MAX_ITER) correction steps the error is still above
@@ -197,7 +200,7 @@ int32_t nla_trmm(char side, char uplo, char transa, char diag, int32_t dtype, do
  * 7 m, 8 A, 9 lda, 10 B, 11 ldb, 12 tol, 13 max_iter, 14 workspace,
  * 15 workspace_bytes, 16 stream, 17 info.
  */
-enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2 };
+enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2, NLA_LP_AUTO = 3 };
 
 #define NLA_MIXED_MAX_ITER 30
 

@@ -617,9 +617,43 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         return r.err;
     };
 
+    // NLA_LP_AUTO: X0 in fp32 (a bf16 / fp16 X0 starts so far from the
+    // target at small cond that the saved solve time is lost in an extra
+    // residual), then each correction in the precision whose
+    // predicted cost to reach tol is least.  Costs are in units of one FP64
+    // residual (the refinement's dominant kernel): a correction solve costs
+    // about 0.30 (bf16), 0.34 (fp16) or 0.60 (fp32) of one on MI355X at the
+    // sizes where refinement is used (n >= 4096).  A step in precision p
+    // shrinks the backward error by a factor rho_p ~ 0.03 u_p for bf16 / fp16
+    // (u_p the unit roundoff; fitted over cond 1e2..1e6) and ~1e-7 for fp32;
+    // once a step has been observed, its factor rescaled by u_p / u_q
+    // replaces the prior (never below the fp32 floor).
+    const bool autop = (precision == NLA_LP_AUTO);
+    const double unit_round[3] = {0x1p-8, 0x1p-11, 0x1p-24};
+    const double solve_cost[3] = {0.30, 0.34, 0.60};
+    double rho_obs = -1.0;
+    int rho_prec = NLA_LP_FP32;
+    auto choose = [&](double err) {
+        int best = NLA_LP_FP32;
+        double best_cost = INFINITY;
+        for (int p = NLA_LP_BF16; p <= NLA_LP_FP32; ++p) {
+            double rho = p == NLA_LP_FP32 ? 1e-7 : 0.03 * unit_round[p];
+            if (rho_obs > 0.0) rho = std::max(rho_obs * unit_round[p] / unit_round[rho_prec], 1e-7);
+            if (!(rho < 1.0)) continue;
+            const double need = std::log(tol / err) / std::log(rho);
+            const double steps = std::max(1.0, std::ceil(need - 1e-9));
+            const double cost = steps * (solve_cost[p] + 1.0);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best = p;
+            }
+        }
+        return best;
+    };
+
     // X0: the low-precision solve of S, scaled by a power of two so the fp32
     // buffer holds O(1) values
-    int prec = precision;
+    int prec = autop ? NLA_LP_FP32 : precision;
     double sig = pow2_above(h[3]);
     NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, S, L.ldr, D, L.ldr, s));
     NLA_CK(lp_solve(prec));
@@ -663,8 +697,16 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         bwd = den > 0.0 ? nR / den : (nR > 0.0 ? INFINITY : 0.0);
         if (info && it <= NLA_MIXED_MAX_ITER) info->history[it] = bwd;
         if (bwd <= tol || it >= max_iter || !std::isfinite(bwd)) break;
-        // too little progress: raise the update precision for the next steps
-        if (it > 0 && bwd > 0.5 * prev && prec != NLA_LP_FP32) prec = (prec == NLA_LP_BF16) ? NLA_LP_FP16 : NLA_LP_FP32;
+        if (autop) {
+            if (it > 0 && prev > 0.0 && bwd > 0.0) {
+                rho_obs = bwd / prev;
+                rho_prec = prec;
+            }
+            prec = choose(bwd);
+        } else if (it > 0 && bwd > 0.5 * prev && prec != NLA_LP_FP32) {
+            // too little progress: raise the update precision for the next steps
+            prec = (prec == NLA_LP_BF16) ? NLA_LP_FP16 : NLA_LP_FP32;
+        }
         prev = bwd;
         sig = pow2_above(h[1]);
         NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, R, L.ldr, D, L.ldr, s));
@@ -820,7 +862,7 @@ int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, d
     if (side != 'L' && side != 'R') return bad_arg(1, "side");
     if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
     if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
-    if (precision != NLA_LP_BF16 && precision != NLA_LP_FP16 && precision != NLA_LP_FP32)
+    if (precision != NLA_LP_BF16 && precision != NLA_LP_FP16 && precision != NLA_LP_FP32 && precision != NLA_LP_AUTO)
         return bad_arg(4, "precision");
     if (!std::isfinite(alpha)) return bad_arg(5, "alpha");
     if (n < 0) return bad_arg(6, "n");

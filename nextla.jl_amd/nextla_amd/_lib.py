@@ -72,6 +72,7 @@ NLA_KIND_RESIDUAL = 3
 NLA_LP_BF16 = 0
 NLA_LP_FP16 = 1
 NLA_LP_FP32 = 2
+NLA_LP_AUTO = 3
 NLA_ERR_NOT_CONVERGED = 5
 NLA_CFG_LEAF_MODE = 0
 NLA_CFG_PANEL = 1

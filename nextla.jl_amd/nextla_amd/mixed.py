@@ -13,7 +13,8 @@ import torch
 
 from . import _lib, _canonical, _require_device, _stream
 
-PRECISIONS = {"bf16": _lib.NLA_LP_BF16, "fp16": _lib.NLA_LP_FP16, "fp32": _lib.NLA_LP_FP32}
+PRECISIONS = {"bf16": _lib.NLA_LP_BF16, "fp16": _lib.NLA_LP_FP16, "fp32": _lib.NLA_LP_FP32,
+              "auto": _lib.NLA_LP_AUTO}
 _NAMES = {v: k for k, v in PRECISIONS.items()}
 
 
