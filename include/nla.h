@@ -277,6 +277,10 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      above 512 rows FP64 panels use 16 columns per group);
  *   NLA_CFG_PANEL_STRIP_F32 / _F64: right-hand-side columns per panel
  *                      work-group, 16 (default) or 32;
+ *   NLA_CFG_PANEL_WAVES: waves per panel work-group (panels up to 512 rows):
+ *                      4, or 8, where two waves share each 16-row output
+ *                      tile and split its k range; 0 (default) = 4 for FP64
+ *                      TRSM, 8 otherwise;
  *   NLA_CFG_BATCH_CLASSES: size classes new batch plans may use, bit c for
  *                      systems up to 128 << c rows (c = 0..3; the 1024 class
  *                      is always on; default 0x1F: all, with bit 4).  A system goes to the
@@ -294,7 +298,8 @@ enum nla_config_key {
     NLA_CFG_PANEL_ROWS_F64 = 3,
     NLA_CFG_PANEL_STRIP_F32 = 4,
     NLA_CFG_PANEL_STRIP_F64 = 5,
-    NLA_CFG_BATCH_CLASSES = 6
+    NLA_CFG_BATCH_CLASSES = 6,
+    NLA_CFG_PANEL_WAVES = 7
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);

@@ -117,6 +117,11 @@ static std::atomic<int> g_panel{-1};
 // (3-5 % faster than 512 there), 16 columns for both.
 static std::atomic<int> g_panel_rows[2] = {{256}, {512}};
 static std::atomic<int> g_panel_strip[2] = {{PANEL_STRIP}, {PANEL_STRIP}};
+// Waves per panel work-group: 0 = by operation (measured on MI355X at
+// n = 8192: 8 waves, splitting each phase's k range, are 1-3 % faster for
+// FP32 TRSM and for TRMM; FP64 TRSM is 1.4 % faster with 4, its diagonal
+// steps paying for the extra barriers), else 4 or 8 for every panel.
+static std::atomic<int> g_panel_waves{0};
 // Batch size classes: all four, systems above 512 rows staged (measured on
 // MI355X over 10,000 systems of 16..1024 rows: 5.5 ms vs 6.9 ms with
 // 1024-row panels in FP64, 3.4 vs 3.6 ms in FP32).
@@ -169,7 +174,7 @@ struct Recursion {
     const T *W = nullptr;   // inverted diagonal blocks (TRSM, leaf mode 1)
     int mode = 1;
     bool use_panel = true;  // fuse diagonal blocks <= panel_rows into one launch
-    int panel_rows = 512, panel_strip = PANEL_STRIP;
+    int panel_rows = 512, panel_strip = PANEL_STRIP, panel_waves = 4;
     bool unit = false;  // unit diagonal: only the panels and the inverses read the diagonal
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
@@ -244,6 +249,7 @@ struct Recursion {
             a.wrs = nb; a.wcs = 1;
         }
         a.strip = panel_strip;
+        a.waves = n <= 512 ? panel_waves : 4;
         a.unit = unit;
         const double fl = (double)n * (double)n * (double)a.m;
         const double by = ((double)n * (n + 1) / 2 + 2.0 * n * a.m) * sizeof(T);
@@ -479,6 +485,8 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
         const int di = std::is_same<T, double>::value ? NLA_F64 : NLA_F32;
         r.panel_rows = g_panel_rows[di].load();
         r.panel_strip = g_panel_strip[di].load();
+        const int pw = g_panel_waves.load();
+        r.panel_waves = pw ? pw : ((std::is_same<T, double>::value && func == 'S') ? 4 : 8);
     }
     T *W = nullptr;
     std::unique_lock<std::mutex> hold;
@@ -599,6 +607,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         r.mode = 1;
         r.use_panel = true;
         r.W = W;
+        r.panel_waves = g_panel_waves.load() ? g_panel_waves.load() : 8;
         if (prec != NLA_LP_FP32) {
             r.lp = prec;
             r.amax_a = amax;
@@ -904,6 +913,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_BATCH_CLASSES) {
         if (value < 0 || value > 0x1F) return -1;
         return g_batch_classes.exchange(value | 0x8);
+    }
+    if (key == NLA_CFG_PANEL_WAVES) {
+        if (value != 0 && value != 4 && value != 8) return -1;
+        return g_panel_waves.exchange(value);
     }
     if (key == NLA_CFG_PANEL_STRIP_F32 || key == NLA_CFG_PANEL_STRIP_F64) {
         if (value != 16 && value != 32) return -1;

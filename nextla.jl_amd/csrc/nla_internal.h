@@ -74,6 +74,7 @@ struct PanelArgs {
     T scale;
     int strip = PANEL_STRIP;  // RHS columns per work-group: 16 or 32
     bool unit = false;        // unit diagonal (taken as 1, not read)
+    int waves = 4;            // waves per work-group: 4, or 8 splitting each phase's k range (<= 512 rows)
 };
 
 template <typename T>

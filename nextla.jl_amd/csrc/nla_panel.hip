@@ -118,8 +118,12 @@ __device__ inline int phase_count(int nsb, bool trmm) {
 }
 
 // Solve / multiply one strip of W columns (starting at c0) of one diagonal
-// panel; bs is the LDS strip (W x LDB).
-template <typename T, int NBIG, int WS, bool TRMM>
+// panel; bs is the LDS strip (W x LDB).  NW waves: with 4, wave w computes
+// rows 16w..16w+15 of every 64-row phase over its whole 64-deep k range;
+// with 8, waves w and w+4 share those rows and take one half of the k range
+// each (twice the waves to hide latency with the same work), their partial
+// sums meeting in the 64 x W LDS scratch sc when a block is written.
+template <typename T, int NBIG, int WS, bool TRMM, int NW = 4>
 // Always inlined: with the single-panel kernel and the batched kernel both
 // calling one instantiation, the compiler otherwise emits a real call (and
 // the panel ran 50 % slower).
@@ -127,14 +131,16 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                                             int64_t mcs, bool lower, bool unit, T *__restrict__ Bp, int64_t brs,
                                             int64_t bcs,
                                             const T *__restrict__ Wp, int64_t wrs, int64_t wcs, T scale,
-                                            NLA_LDS T *bs) {
+                                            NLA_LDS T *bs, NLA_LDS T *sc = nullptr) {
     typedef PanelShape<T, WS> PS;
     typedef typename PS::MF MF;
     constexpr int W = PS::W;
     constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     constexpr int NKS = 64 / MF::KD;  // k-steps per 64-deep phase
+    static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+    constexpr int NT = NW * 64, KSPLIT = NW / 4, NKW = NKS / KSPLIT;  // k-steps per wave
 
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, wave = (t >> 6) & 3, kh = (t >> 6) >> 2;
     const int nsb = (nbig + 63) / 64;
 
     // ---- load the strip (coalesced along B's contiguous direction) ----
@@ -145,10 +151,10 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
     //   t / W, t / W + 256 / W, ...
     const bool rowc = (brs == 1 || brs == -1);
     const int nrow = nsb * 64;  // rows past nbig up to the 64-boundary are zeroed
-    static_assert(256 % W == 0, "strip width must divide the work-group");
-    constexpr int RSTEP = 256 / W;
+    static_assert(NT % W == 0, "strip width must divide the work-group");
+    constexpr int RSTEP = NT / W;
     if (rowc) {
-        for (int r = t; r < nrow; r += 256) {
+        for (int r = t; r < nrow; r += NT) {
             T v[W];
 #pragma unroll
             for (int c = 0; c < W; ++c) {
@@ -211,41 +217,68 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         const int i = rb * MF::BD + lo;
         uint32_t bits = 0;
 #pragma unroll
-        for (int s = 0; s < NKS; ++s) {
+        for (int sw = 0; sw < NKW; ++sw) {
+            const int s = kh * NKW + sw;
             const int k = s * MF::KD + lk;
             bool ok = i < ni && k < nk;
             // a unit diagonal is not read: the strict triangle here, and the
             // block itself added when the result is written
             if (tri == 1) ok = ok && (unit ? k < i : k <= i);
             if (tri == 2) ok = ok && (unit ? k > i : k >= i);
-            af[s] = *(ok ? A + i * ars + k * acs : A);
-            bits |= (ok ? 1u : 0u) << s;
+            af[sw] = *(ok ? A + i * ars + k * acs : A);
+            bits |= (ok ? 1u : 0u) << sw;
         }
         *okb = bits;
     };
     auto compute = [&](const Phase &ph, const T *af, uint32_t okb) {
         const NLA_LDS T *src = bs + ph.k * 64;
         // all LDS operands of the phase are requested before the first MFMA
-        T bv[NKS][PS::CBW];
+        T bv[NKW][PS::CBW];
 #pragma unroll
-        for (int s = 0; s < NKS; ++s)
+        for (int sw = 0; sw < NKW; ++sw)
 #pragma unroll
             for (int cb = 0; cb < PS::CBW; ++cb)
-                bv[s][cb] = src[((cbase + cb) * MF::BD + lo) * LDB + s * MF::KD + lk];
+                bv[sw][cb] = src[((cbase + cb) * MF::BD + lo) * LDB + (kh * NKW + sw) * MF::KD + lk];
         // keep the scheduler from sinking the reads back next to their MFMAs
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int s = 0; s < NKS; ++s) {
-            const T a = ((okb >> s) & 1u) ? af[s] : T(0);
+        for (int sw = 0; sw < NKW; ++sw) {
+            const T a = ((okb >> sw) & 1u) ? af[sw] : T(0);
 #pragma unroll
             for (int cb = 0; cb < PS::CBW; ++cb) {
-                if (s & 1) acc2[cb] = MF::mma(a, bv[s][cb], acc2[cb]);
-                else acc[cb] = MF::mma(a, bv[s][cb], acc[cb]);
+                if (sw & 1) acc2[cb] = MF::mma(a, bv[sw][cb], acc2[cb]);
+                else acc[cb] = MF::mma(a, bv[sw][cb], acc[cb]);
             }
         }
     };
     auto write_block = [&](int ib, T alpha, T beta) {
         NLA_LDS T *dst = bs + ib * 64;
+        if constexpr (KSPLIT == 2) {
+            // the second k half hands its partial sums to the first
+            if (kh == 1) {
+#pragma unroll
+                for (int cb = 0; cb < PS::CBW; ++cb)
+#pragma unroll
+                    for (int r = 0; r < MF::NR; ++r) {
+                        const int row = rb * MF::BD + MF::di(lane, r);
+                        const int col = (cbase + cb) * MF::BD + MF::dj(lane, r);
+                        sc[col * 64 + row] = acc[cb][r] + acc2[cb][r];
+                    }
+            }
+            __syncthreads();
+            if (kh == 1) return;
+#pragma unroll
+            for (int cb = 0; cb < PS::CBW; ++cb)
+#pragma unroll
+                for (int r = 0; r < MF::NR; ++r) {
+                    const int row = rb * MF::BD + MF::di(lane, r);
+                    const int col = (cbase + cb) * MF::BD + MF::dj(lane, r);
+                    NLA_LDS T *p = dst + col * LDB + row;
+                    T v = alpha * ((acc[cb][r] + acc2[cb][r]) + sc[col * 64 + row]);
+                    *p = (beta == T(0)) ? v : v + beta * *p;
+                }
+            return;
+        }
 #pragma unroll
         for (int cb = 0; cb < PS::CBW; ++cb)
 #pragma unroll
@@ -297,7 +330,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
     // that did not need it.
     constexpr int PD = 2;
     const int np = phase_count(nsb, TRMM);
-    T fr[PD][NKS];
+    T fr[PD][NKW];
     uint32_t okr[PD];
     Phase phr[PD];
 #pragma unroll
@@ -322,7 +355,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
 
     // ---- write the strip back ----
     if (rowc) {
-        for (int r = t; r < nbig; r += 256) {
+        for (int r = t; r < nbig; r += NT) {
 #pragma unroll
             for (int c = 0; c < W; ++c) {
                 const T v = bs[c * LDB + r];
@@ -340,43 +373,55 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
     }
 }
 
-template <typename T, int NBIG, int WS, bool TRMM>
-__global__ __launch_bounds__(256) void panel_kernel(int nbig, int64_t m, const T *__restrict__ Mp, int64_t mrs,
-                                                    int64_t mcs, int lower, int unit, T *__restrict__ Bp,
-                                                    int64_t brs, int64_t bcs, const T *__restrict__ Wp, int64_t wrs,
-                                                    int64_t wcs, T scale) {
+template <typename T, int NBIG, int WS, bool TRMM, int NW>
+__global__ __launch_bounds__(NW * 64) void panel_kernel(int nbig, int64_t m, const T *__restrict__ Mp, int64_t mrs,
+                                                        int64_t mcs, int lower, int unit, T *__restrict__ Bp,
+                                                        int64_t brs, int64_t bcs, const T *__restrict__ Wp,
+                                                        int64_t wrs, int64_t wcs, T scale) {
     constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
-    panel_strip<T, NBIG, WS, TRMM>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0, Bp, brs,
-                                   bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw);
+    if constexpr (NW == 8) {
+        __shared__ __attribute__((aligned(16))) T sc_raw[64 * WS];
+        panel_strip<T, NBIG, WS, TRMM, NW>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0,
+                                           Bp, brs, bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw,
+                                           (NLA_LDS T *)sc_raw);
+    } else {
+        panel_strip<T, NBIG, WS, TRMM, NW>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0,
+                                           Bp, brs, bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw);
+    }
 }
 
-template <typename T, int NBIG, int W>
+template <typename T, int NBIG, int W, int NW>
 static hipError_t launch_panel_w(const PanelArgs<T> &a, hipStream_t s) {
     const int64_t groups = (a.m + W - 1) / W;
     const int lo = a.lower ? 1 : 0, un = a.unit ? 1 : 0;
     if (a.trmm)
-        hipLaunchKernelGGL((panel_kernel<T, NBIG, W, true>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
-                           a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+        hipLaunchKernelGGL((panel_kernel<T, NBIG, W, true, NW>), dim3((unsigned)groups), dim3(NW * 64), 0, s,
+                           a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     else
-        hipLaunchKernelGGL((panel_kernel<T, NBIG, W, false>), dim3((unsigned)groups), dim3(256), 0, s, a.nbig,
-                           a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+        hipLaunchKernelGGL((panel_kernel<T, NBIG, W, false, NW>), dim3((unsigned)groups), dim3(NW * 64), 0, s,
+                           a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
     return hipGetLastError();
 }
 
-// Panels up to 512 rows use the 512-row LDS strip; larger ones (up to
-// PANEL_MAX) the 1024-row strip, whose FP64 form only fits LDS with 16
-// columns per work-group.
+// Panels up to 512 rows use the 512-row LDS strip (4 or 8 waves); larger
+// ones (up to PANEL_MAX) the 1024-row strip with 4 waves, whose FP64 form
+// only fits LDS with 16 columns per work-group.
 template <typename T>
 hipError_t launch_panel(const PanelArgs<T> &a, hipStream_t s) {
     if (a.m <= 0 || a.nbig <= 0) return hipSuccess;
     if (a.nbig > PANEL_MAX) return hipErrorInvalidValue;
     if (a.strip != 16 && a.strip != 32) return hipErrorInvalidValue;
-    if (a.nbig <= 512) return a.strip == 32 ? launch_panel_w<T, 512, 32>(a, s) : launch_panel_w<T, 512, 16>(a, s);
-    if constexpr (sizeof(T) == 4) {
-        if (a.strip == 32) return launch_panel_w<T, 1024, 32>(a, s);
+    if (a.waves != 4 && a.waves != 8) return hipErrorInvalidValue;
+    if (a.nbig <= 512) {
+        if (a.waves == 8)
+            return a.strip == 32 ? launch_panel_w<T, 512, 32, 8>(a, s) : launch_panel_w<T, 512, 16, 8>(a, s);
+        return a.strip == 32 ? launch_panel_w<T, 512, 32, 4>(a, s) : launch_panel_w<T, 512, 16, 4>(a, s);
     }
-    return launch_panel_w<T, 1024, 16>(a, s);
+    if constexpr (sizeof(T) == 4) {
+        if (a.strip == 32) return launch_panel_w<T, 1024, 32, 4>(a, s);
+    }
+    return launch_panel_w<T, 1024, 16, 4>(a, s);
 }
 
 template hipError_t launch_panel<float>(const PanelArgs<float> &, hipStream_t);

@@ -81,6 +81,7 @@ NLA_CFG_PANEL_ROWS_F64 = 3
 NLA_CFG_PANEL_STRIP_F32 = 4
 NLA_CFG_PANEL_STRIP_F64 = 5
 NLA_CFG_BATCH_CLASSES = 6
+NLA_CFG_PANEL_WAVES = 7
 
 
 def set_config(key, value):

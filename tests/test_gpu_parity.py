@@ -156,17 +156,19 @@ def test_recursion_all_combos(nla, method, dt, n, m):
 PANEL_SHAPES = [(256, 16), (512, 32), (192, 32), (1024, 16), (768, 32)]
 
 
-@pytest.mark.parametrize("rows,strip", PANEL_SHAPES)
+@pytest.mark.parametrize("rows,strip,waves", [s + (4,) for s in PANEL_SHAPES] + [(512, 16, 8), (512, 32, 8),
+                                                                                  (192, 16, 8)])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_panel_shapes(nla, rows, strip, dt):
-    """Every panel size / strip width the configuration (and a tuner) may
-    pick gives the same results as the oracle: all combos at sizes around
-    the panel boundaries."""
+def test_panel_shapes(nla, rows, strip, waves, dt):
+    """Every panel size / strip width / wave count the configuration (and a
+    tuner) may pick gives the same results as the oracle: all combos at sizes
+    around the panel boundaries."""
     from nextla_amd import _lib
     keys = ((_lib.NLA_CFG_PANEL_ROWS_F64, _lib.NLA_CFG_PANEL_STRIP_F64) if dt == np.float64
             else (_lib.NLA_CFG_PANEL_ROWS_F32, _lib.NLA_CFG_PANEL_STRIP_F32))
     old = (_lib.set_config(keys[0], rows), _lib.set_config(keys[1], strip))
-    assert old[0] > 0 and old[1] > 0
+    old_w = _lib.set_config(_lib.NLA_CFG_PANEL_WAVES, waves)
+    assert old[0] > 0 and old[1] > 0 and old_w >= 0
     try:
         rng = np.random.default_rng(rows + strip)
         for n, m in ((rows, 33), (rows + 70, 50), (3 * rows + 5, 40)):
@@ -181,6 +183,7 @@ def test_panel_shapes(nla, rows, strip, dt):
     finally:
         _lib.set_config(keys[0], old[0])
         _lib.set_config(keys[1], old[1])
+        _lib.set_config(_lib.NLA_CFG_PANEL_WAVES, old_w)
 
 
 def test_config_rejects_invalid_panel_shapes(nla):
@@ -188,6 +191,7 @@ def test_config_rejects_invalid_panel_shapes(nla):
     assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 100) == -1
     assert _lib.set_config(_lib.NLA_CFG_PANEL_ROWS_F64, 1088) == -1
     assert _lib.set_config(_lib.NLA_CFG_PANEL_STRIP_F32, 24) == -1
+    assert _lib.set_config(_lib.NLA_CFG_PANEL_WAVES, 6) == -1
     assert _lib.set_config(99, 1) == -1
 
 
