@@ -31,7 +31,11 @@ Also reported on the same line:
                                continuous-batching server: p50 / p99 latency
                                at Poisson rates, saturated solves/s against
                                one-at-a-time plain calls, and the FP64 TRSMs
-                               served by refinement instead.
+                               served by refinement instead;
+                lu             getrf2! (recursive LU with partial pivoting,
+                               built on the unit-lower TRSM and GEMM update)
+                               at n = 4096 and 8192 in FP64, against LAPACK
+                               dgetrf on the host's cores.
 """
 import argparse
 import json
@@ -225,6 +229,53 @@ def batch_leg(nla, torch, dtype, count=10000, seed=0):
             "speedup_vs_loop": round(tl / tb, 1)}
 
 
+def lu_leg(nla, torch, sizes=(4096, 8192), seed=1):
+    """getrf2! (recursive LU with partial pivoting, the reference's caller of
+    the unit-lower TRSM) on the device, FP64 square N(0,1) matrices: time,
+    (2/3) n^3 flops rate and the relative residual ||P*L*U - A|| / ||A||
+    checked on the device; beside it LAPACK dgetrf (OpenBLAS through scipy)
+    on the host's cores on the same matrix."""
+    import numpy as np
+    import scipy.linalg
+    out = {"workload": "FP64 square A with N(0,1) entries, partial pivoting"}
+    for n in sizes:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        A0 = torch.empty((n, n), dtype=torch.float64, device="cuda").t().normal_(generator=g)
+        A = A0.clone()
+        nla.getrf2_(A)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            A.copy_(A0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, ipiv, info = nla.getrf2_(A)  # reads info: synchronizes
+            ts.append((time.perf_counter() - t0) * 1e3)
+        ms = min(ts)
+        L = torch.tril(A, -1) + torch.eye(n, dtype=A.dtype, device=A.device)
+        R = L @ torch.triu(A)
+        p = np.arange(n)
+        for i, j in enumerate(ipiv.cpu().numpy() - 1):
+            p[i], p[j] = p[j], p[i]
+        res = (torch.linalg.norm(R - A0[torch.from_numpy(p).to(A.device)]) / torch.linalg.norm(A0)).item()
+        Ah = A0.cpu().numpy()
+        t0 = time.perf_counter()
+        scipy.linalg.lapack.dgetrf(Ah)
+        tc = (time.perf_counter() - t0) * 1e3
+        fl = 2.0 / 3.0 * n ** 3
+        out[f"n{n}"] = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3),
+                        "roofline_frac": round(fl / ms / 1e9 / PEAK["f64"], 4), "info": info,
+                        "rel_residual": res, "host_lapack_ms": round(tc, 2),
+                        "speedup_vs_host_lapack": round(tc / ms, 3)}
+        log(f"bench: getrf2 n={n}: {ms:.2f} ms ({fl / ms / 1e9:.2f} TFLOP/s), host LAPACK {tc:.1f} ms, "
+            f"residual {res:.1e}")
+        del A0, A, L, R
+        torch.cuda.empty_cache()
+    out["host_threads"] = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return out
+
+
 def serve_leg(nla, torch, count=600, seed=0):
     """The serving workload through nextla_amd.serve (see the module)."""
     import numpy as np
@@ -335,7 +386,7 @@ def main():
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM rates and the mixed and batched legs")
-    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served"),
+    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served", "lu"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -353,6 +404,8 @@ def main():
             res = mixed_leg(nla, torch)
         elif args.leg.startswith("batched"):
             res = batch_leg(nla, torch, torch.float64 if args.leg.endswith("f64") else torch.float32)
+        elif args.leg == "lu":
+            res = lu_leg(nla, torch)
         else:
             res = serve_leg(nla, torch)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
@@ -409,6 +462,7 @@ def main():
             log(f"bench: batched {label}: {b['ms']} ms ({b['tflops']} TFLOP/s, roofline {b['roofline_frac']}), "
                 f"loop of plain calls {b['loop_of_plain_calls_ms']} ms")
         extra["served"] = serve_leg(nla, torch)
+        extra["lu"] = lu_leg(nla, torch)
     torch.cuda.synchronize()
 
     cpu = None

@@ -172,6 +172,29 @@ int32_t nla_trmm(char side, char uplo, char transa, char diag, int32_t dtype, do
                  const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
 
 /*
+ * Recursive LU factorization with partial pivoting, A = P·L·U, in place
+ * (replaces getrf2!(A, ipiv, info), src/lu.jl:185-300: LAPACK dgetrf2's
+ * recursion).  Splits the columns at n1 = min(m,n)/2, factors the left
+ * block, applies its interchanges to the right block, solves
+ * A12 = L11^-1·A12 with the unit-lower TRSM of this library (nla_trsm,
+ * diag 'U'), updates A22 -= A21·A12 (nla_gemm), factors A22 and applies its
+ * interchanges to the left block; a single column is one kernel (pivot =
+ * first index of the largest |a|, swap, scale by the reciprocal).
+ *   A     m x n column-major (leading dimension lda), FP32 or FP64;
+ *   ipiv  device array of min(m,n) int64: row i was interchanged with row
+ *         ipiv[i] (1-based, as the reference);
+ *   info  device int64: 0, or the 1-based index of the first exactly zero
+ *         pivot (the factorization still completes, as LAPACK's).
+ * Enqueued on `stream`, no host synchronization.  Differences from the
+ * reference: info is returned (the reference's Int argument cannot carry
+ * it out), and the factorization does not stop at a zero pivot.
+ * Argument positions: 1 dtype, 2 m, 3 n, 4 A, 5 lda, 6 ipiv, 7 info,
+ * 8 stream.
+ */
+int32_t nla_getrf2(int32_t dtype, int64_t m, int64_t n, void *A, int64_t lda, int64_t *ipiv, int64_t *info,
+                   void *stream);
+
+/*
  * Mixed-precision TRSM with FP64 iterative refinement (no counterpart in the
  * reference; the FP64 solution it converges to is that of unified_rectrxm!'s
  * TRSM, src/rectrxm.jl:43-76).  Solves

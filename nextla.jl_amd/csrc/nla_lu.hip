@@ -1,0 +1,292 @@
+// Recursive LU factorization with partial pivoting (nla_getrf2 in
+// include/nla.h): getrf2! of the reference (src/lu.jl:185-300, LAPACK
+// dgetrf2's recursion) on the device, built on this library's unit-lower
+// TRSM and GEMM update.
+//
+// The host walks the same recursion as the reference: split the columns at
+// n1 = min(m, n) / 2, factor the left block, apply its row interchanges to
+// the right block, solve A12 = L11^-1 A12 (unit lower), update
+// A22 -= A21 A12, factor A22 and apply its interchanges to the left block.
+// Blocks of up to 8 columns are one kernel (the same eliminations with
+// rank-1 updates in place of the GEMMs of that width), which keeps the
+// launch count near 5 per 8 columns instead of 5 per column.  Pivots are
+// kept as global rows throughout (the reference shifts them by n1 after
+// each recursive call).  Everything is enqueued on the caller's stream;
+// pivots and info stay on the device.
+#include <hip/hip_runtime.h>
+
+#include <limits>
+
+#include "../../include/nla.h"
+#include "nla_internal.h"
+
+namespace nla {
+
+int32_t report_hip(hipError_t e, const char *where);
+int32_t report_bad_arg(int k, const char *what);
+
+namespace {
+
+constexpr int PIV_THREADS = 1024;
+
+// Leaf of up to LEAF_COLS columns, one work-group: for each column j the
+// pivot search (src/lu.jl:223-253: first index of the largest |a|, as
+// idamax), the row interchange across the leaf's columns, the scaling of the
+// column below the pivot (by the reciprocal; a division below the smallest
+// normal number) and the rank-1 update of the leaf's remaining columns (what
+// the recursion's GEMM updates do at this width).  m == 1 is the reference's
+// row case (src/lu.jl:216-221).  Pivots are written 1-based with the leaf's
+// row offset r0 added (its rows start on the diagonal, r0 = its first
+// column); info takes the first zero pivot's global column.
+constexpr int LEAF_COLS = 8;
+constexpr int LEAF_WAVES = PIV_THREADS / 64;
+
+// (|a|, signed a, row) of the larger |a|, ties to the smaller row.
+template <typename T>
+__device__ inline void better(T &bv, T &bs, int64_t &bi, T v, T sv, int64_t i) {
+    if (v > bv || (v == bv && i < bi)) {
+        bv = v;
+        bs = sv;
+        bi = i;
+    }
+}
+
+template <typename T>
+__device__ inline void wave_argmax(T &bv, T &bs, int64_t &bi) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T v = __shfl_xor(bv, o, 64), sv = __shfl_xor(bs, o, 64);
+        const int64_t i = __shfl_xor(bi, o, 64);
+        better(bv, bs, bi, v, sv, i);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, T *__restrict__ A, int64_t lda,
+                                                              int64_t *__restrict__ ipiv, int64_t *__restrict__ info,
+                                                              int64_t r0) {
+    __shared__ T s_val[2][LEAF_WAVES], s_sgn[2][LEAF_WAVES];
+    __shared__ int64_t s_row[2][LEAF_WAVES];
+    __shared__ T s_u[LEAF_COLS];  // row j of the leaf after the interchange
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int kmax = (int)(m < n ? m : n);
+    // partial maxima of column 0 (each later column's are taken in the
+    // update pass of the column before it); the two LDS buffers alternate
+    // between consecutive columns
+    T bv = T(-1), bs = T(0);
+    int64_t bi = m;
+    for (int64_t i = t; i < m; i += PIV_THREADS) {
+        const T a = A[i];
+        better(bv, bs, bi, (T)fabs(a), a, i);
+    }
+    for (int j = 0; j < kmax; ++j) {
+        const int buf = j & 1;
+        T *cj = A + (int64_t)j * lda;
+        wave_argmax(bv, bs, bi);
+        if (lane == 0) {
+            s_val[buf][wv] = bv;
+            s_sgn[buf][wv] = bs;
+            s_row[buf][wv] = bi;
+        }
+        __syncthreads();
+        T pv = T(-1), a = T(0);
+        int64_t p = m;
+#pragma unroll
+        for (int w = 0; w < LEAF_WAVES; ++w) better(pv, a, p, s_val[buf][w], s_sgn[buf][w], s_row[buf][w]);
+        if (p >= m) {  // no finite maximum (all NaN): keep row j
+            p = j;
+            a = cj[j];
+        }
+        if (t == 0) {
+            ipiv[j] = r0 + p + 1;
+            if (a == T(0) && *info == 0) *info = r0 + j + 1;
+        }
+        // interchange rows j and p of the leaf's columns; row j's new values
+        // go to LDS for the update
+        if (t < n) {
+            T *ct = A + (int64_t)t * lda;
+            const T xj = ct[j], xp = ct[p];
+            if (a != T(0) && p != j) {
+                ct[j] = xp;
+                ct[p] = xj;
+                s_u[t] = xp;
+            } else {
+                s_u[t] = xj;
+            }
+        }
+        __syncthreads();
+        bv = T(-1);
+        bs = T(0);
+        bi = m;
+        if (a != T(0)) {
+            // scale column j below the pivot, update the columns to its right
+            // and take the partial maxima of column j + 1.  Rows go in chunks
+            // of RU per thread; every load of a chunk (column j and all the
+            // columns to its right) is issued before its first store, with
+            // addresses clamped to row / column j and results masked.
+            constexpr int RU = 4, NC = LEAF_COLS - 1;
+            const bool recip = fabs(a) >= std::numeric_limits<T>::min();
+            const T r = T(1) / a;
+            for (int64_t i0 = j + 1 + t; i0 < m; i0 += RU * PIV_THREADS) {
+                T l[RU], x[NC][RU];
+#pragma unroll
+                for (int u = 0; u < RU; ++u) {
+                    const int64_t i = i0 + (int64_t)u * PIV_THREADS;
+                    l[u] = cj[i < m ? i : j];
+                }
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    const int c = j + 1 + q;
+                    const T *cc = A + (int64_t)(c < n ? c : j) * lda;
+#pragma unroll
+                    for (int u = 0; u < RU; ++u) {
+                        const int64_t i = i0 + (int64_t)u * PIV_THREADS;
+                        x[q][u] = cc[i < m ? i : j];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < RU; ++u) {
+                    const int64_t i = i0 + (int64_t)u * PIV_THREADS;
+                    l[u] = recip ? l[u] * r : l[u] / a;
+                    if (i < m) cj[i] = l[u];
+                }
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    const int c = j + 1 + q;
+                    if (c >= n) break;
+                    T *cc = A + (int64_t)c * lda;
+                    const T uc = s_u[c];
+#pragma unroll
+                    for (int u = 0; u < RU; ++u) {
+                        const int64_t i = i0 + (int64_t)u * PIV_THREADS;
+                        if (i < m) {
+                            const T y = x[q][u] - l[u] * uc;
+                            cc[i] = y;
+                            if (q == 0) better(bv, bs, bi, (T)fabs(y), y, i);
+                        }
+                    }
+                }
+            }
+        } else if (j + 1 < kmax) {
+            // a zero column (the max |a| is 0): nothing to eliminate
+            T *cn = A + (int64_t)(j + 1) * lda;
+            for (int64_t i = j + 1 + t; i < m; i += PIV_THREADS) {
+                const T x = cn[i];
+                better(bv, bs, bi, (T)fabs(x), x, i);
+            }
+        }
+        // no barrier here: the next column's first barrier follows every
+        // thread's updates, and s_u is rewritten only after it
+    }
+}
+
+// laswp (src/lu.jl:470): for k = k1..k2 (1-based) swap rows k and
+// ipiv[k] - r0 of each of the ncols columns (ipiv holds global rows, the
+// block's row 1 is global row r0 + 1); one thread per column walks the
+// interchanges in order.
+template <typename T>
+__global__ void laswp_kernel(int64_t ncols, T *__restrict__ A, int64_t lda, int64_t k1, int64_t k2,
+                             const int64_t *__restrict__ ipiv, int64_t r0) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    T *col = A + c * lda;
+    for (int64_t k = k1; k <= k2; ++k) {
+        const int64_t p = ipiv[k - 1] - 1 - r0;
+        if (p != k - 1) {
+            const T a = col[k - 1];
+            col[k - 1] = col[p];
+            col[p] = a;
+        }
+    }
+}
+
+__global__ void info_zero_kernel(int64_t *info) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *info = 0;
+}
+
+template <typename T>
+struct LuRec {
+    int32_t dtype;
+    int64_t lda;
+    int64_t *info;
+    hipStream_t s;
+    int32_t st = NLA_OK;
+
+    bool ok(hipError_t e, const char *where) {
+        if (e != hipSuccess && st == NLA_OK) st = report_hip(e, where);
+        return st == NLA_OK;
+    }
+
+    void laswp(T *A, int64_t ncols, int64_t k1, int64_t k2, const int64_t *ipiv, int64_t r0) {
+        if (ncols <= 0 || k2 < k1 || st != NLA_OK) return;
+        hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, s, ncols, A, lda, k1,
+                           k2, ipiv, r0);
+        ok(hipGetLastError(), "laswp");
+    }
+
+    // Factor the m x n block at A, whose first row and column are the global
+    // row / column r0 (blocks start on the diagonal); its pivots go to
+    // ipiv[0 .. min(m,n)) as global 1-based rows, which is what the
+    // reference's "ipiv[i] += n1" (src/lu.jl:289-291) makes of them.
+    void rec(int64_t m, int64_t n, T *A, int64_t *ipiv, int64_t r0) {
+        if (st != NLA_OK || m == 0 || n == 0) return;
+        if (n <= LEAF_COLS || m == 1) {
+            const int nl = (int)(n <= LEAF_COLS ? n : 1);  // m == 1: only the first column is examined
+            hipLaunchKernelGGL(lu_leaf_kernel<T>, dim3(1), dim3(PIV_THREADS), 0, s, m, nl, A, lda, ipiv, info, r0);
+            ok(hipGetLastError(), "lu leaf");
+            return;
+        }
+        const int64_t mn = m < n ? m : n, n1 = mn / 2, n2 = n - n1;
+        rec(m, n1, A, ipiv, r0);
+        laswp(A + n1 * lda, n2, 1, n1, ipiv, r0);
+        if (st != NLA_OK) return;
+        // A12 = L11^-1 A12 (unit lower; the identity for one column)
+        if (n1 > 1) {
+            const int32_t r = nla_trsm('L', 'L', 'N', 'U', dtype, 1.0, n1, n2, A, lda, A + n1 * lda, lda, s);
+            if (r != NLA_OK) {
+                st = r;
+                return;
+            }
+        }
+        // A22 -= A21 A12
+        if (m > n1) {
+            const int32_t r = nla_gemm(dtype, 'N', 'N', m - n1, n2, n1, -1.0, A + n1, lda, A + n1 * lda, lda, 1.0,
+                                       A + n1 + n1 * lda, lda, s);
+            if (r != NLA_OK) {
+                st = r;
+                return;
+            }
+        }
+        rec(m - n1, n2, A + n1 + n1 * lda, ipiv + n1, r0 + n1);
+        laswp(A, n1, n1 + 1, mn, ipiv, r0);
+    }
+};
+
+}  // namespace
+}  // namespace nla
+
+using namespace nla;
+
+extern "C" int32_t nla_getrf2(int32_t dtype, int64_t m, int64_t n, void *A, int64_t lda, int64_t *ipiv,
+                              int64_t *info, void *stream) {
+    if (dtype != NLA_F32 && dtype != NLA_F64) return report_bad_arg(1, "dtype");
+    if (m < 0) return report_bad_arg(2, "m");
+    if (n < 0) return report_bad_arg(3, "n");
+    if (lda < (m > 1 ? m : 1)) return report_bad_arg(5, "lda");
+    if (m > 0 && n > 0 && A == nullptr) return report_bad_arg(4, "A");
+    if (m > 0 && n > 0 && ipiv == nullptr) return report_bad_arg(6, "ipiv");
+    if (info == nullptr) return report_bad_arg(7, "info");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(info_zero_kernel, dim3(1), dim3(64), 0, s, info);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return report_hip(e, "getrf2 info");
+    if (m == 0 || n == 0) return NLA_OK;
+    if (dtype == NLA_F64) {
+        LuRec<double> r{dtype, lda, info, s};
+        r.rec(m, n, (double *)A, ipiv, 0);
+        return r.st;
+    }
+    LuRec<float> r{dtype, lda, info, s};
+    r.rec(m, n, (float *)A, ipiv, 0);
+    return r.st;
+}

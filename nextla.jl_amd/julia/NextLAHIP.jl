@@ -19,7 +19,8 @@ It also exposes the forms the reference does not have: `batched_rectrxm!`
 (many systems in one batched launch sequence), `rectrxm_mixed!` (FP64-accurate
 TRSM by low-precision correction solves and FP64 refinement) and
 `batched_rectrxm_mixed!` (the same for many systems at once), and methods of
-NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`.
+NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`, and
+`getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM).
 
 Differences from the reference, by design: arguments are validated before any
 device work (`ArgumentError`, B untouched), lower-case characters are
@@ -111,6 +112,23 @@ for (f, sym) in ((:trsm, :nla_trsm), (:trmm, :nla_trmm))
         check(st, $(string(f)))
         return B
     end
+end
+
+# getrf2!(A, ipiv, info) (src/lu.jl:185-300): recursive LU with partial pivoting on the device.
+# Returns (A, ipiv, info) with info read back (0 or the first zero pivot).
+function NextLA.getrf2!(A::DeviceMat{T}, ipiv::AbstractVector{Int}, info::Int) where {T <: Union{Float32, Float64}}
+    m, n = size(A)
+    pa, lda = ptr_ld(A)
+    dpiv = AMDGPU.ROCVector{Int64}(undef, max(min(m, n), 1))
+    dinfo = AMDGPU.zeros(Int64, 1)
+    st = GC.@preserve A dpiv dinfo ccall((:nla_getrf2, libnlahip), Int32,
+        (Int32, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}),
+        dtype_code(T), Int64(m), Int64(n), pa, lda, pointer(dpiv), pointer(dinfo), hip_stream())
+    check(st, "getrf2!")
+    AMDGPU.synchronize()
+    k = min(m, n)
+    k > 0 && copyto!(ipiv, 1, Array(dpiv), 1, k)
+    return A, ipiv, Int(Array(dinfo)[1])
 end
 
 # ---- forms without a reference counterpart ---------------------------------

@@ -8,6 +8,8 @@ Names follow the reference with Julia's `!` spelled `_`:
     GEMM_SUB_(A, B, C)   A -= B*C                               src/matmul.jl:76
     LeftLowerTRSM_(A, B) ... RightUpperTRSM_(A, B)              src/trsm.jl:128-150
     LeftLowerTRMM_(A, B) ... RightUpperTRMM_(A, B)              src/trmm.jl:332-393
+    trsm / trmm(side, uplo, transa, diag, A, B, alpha)          src/trsm.jl:186, src/trmm.jl:430
+    getrf2_(A, ipiv=None) -> (A, ipiv, info)                    src/lu.jl:185-300
 
 A and B are 2-D torch tensors on a ROCm device, float32 or float64.  Julia
 matrices are column-major; a tensor with stride (1, ld) is used as is, and a
@@ -31,7 +33,7 @@ __all__ = [
     "unified_rectrxm_", "GEMM_ADD_", "GEMM_SUB_",
     "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_",
-    "trsm", "trmm", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
+    "trsm", "trmm", "getrf2_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
 ]
 
 _DT = {torch.float32: NLA_F32, torch.float64: NLA_F64}
@@ -210,3 +212,30 @@ def trmm(side, uplo, transa, diag, A, B, alpha=1.0):
     B <- alpha*op(A)*B ('L') or alpha*B*op(A) ('R'), transa and diag
     honoured (diag 'U': unit diagonal, not read)."""
     return _trxm_diag("M", side, uplo, transa, diag, A, B, alpha)
+
+
+def getrf2_(A, ipiv=None):
+    """getrf2!(A, ipiv, info) of src/lu.jl:185-300: LU factorization with
+    partial pivoting, A = P*L*U, in place (unit-diagonal L below the
+    diagonal, U on and above it).  A: column-major float32/float64 tensor on
+    a ROCm device (Julia layout; a row-major tensor would be factored as its
+    transpose, so it is rejected).  Returns (A, ipiv, info): ipiv an int64
+    device tensor of min(m, n) 1-based interchanges, info 0 or the 1-based
+    index of the first exactly zero pivot (read after synchronizing the
+    stream; the factorization completes either way, as LAPACK's)."""
+    if A.dtype not in _DT:
+        raise ValueError(f"unsupported element type {A.dtype} (Float32/Float64 only)")
+    _require_device(A)
+    tr, lda = _layout(A, "A")
+    m, n = A.shape
+    if tr and m > 1 and n > 1:
+        raise ValueError("A must be column-major (stride (1, ld)); use colmajor(A)")
+    k = min(m, n)
+    if ipiv is None:
+        ipiv = torch.empty(max(k, 1), dtype=torch.int64, device=A.device)
+    elif ipiv.dtype != torch.int64 or ipiv.device != A.device or ipiv.numel() < k or not ipiv.is_contiguous():
+        raise ValueError("ipiv must be a contiguous int64 tensor of min(m, n) entries on A's device")
+    info = torch.zeros(1, dtype=torch.int64, device=A.device)
+    st = load().nla_getrf2(_DT[A.dtype], m, n, A.data_ptr(), lda, ipiv.data_ptr(), info.data_ptr(), _stream(A))
+    _lib.check(st, "getrf2!")
+    return A, ipiv[:k], int(info.item())

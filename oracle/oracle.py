@@ -12,6 +12,11 @@ Two oracles, as in the reference's own tests (test/unified_rectrxm.jl:36-40):
 
 Error measures: `backward_error` (TRSM, needs no reference solution) and
 `product_error` (TRMM), both normwise in Frobenius norm and computed in FP64.
+
+`getrf2` restates the reference's recursive LU with partial pivoting
+(src/lu.jl:185-300, laswp src/lu.jl:470-520) in numpy; it is pinned against
+LAPACK's dgetrf / sgetrf through scipy (same pivots and factors) and by
+P·L·U = A (tests/test_oracle.py).
 """
 import ctypes
 import os
@@ -174,3 +179,78 @@ def rel_error(X, Y):
     Y = np.asarray(Y, dtype=np.float64)
     d = np.linalg.norm(Y)
     return float(np.linalg.norm(X - Y) / d) if d > 0 else float(np.linalg.norm(X - Y))
+
+
+def _laswp(A, k1, k2, ipiv):
+    """Rows k and ipiv[k] swapped in order for k = k1..k2 (1-based),
+    src/lu.jl:470-520 (incx = 1)."""
+    for k in range(k1, k2 + 1):
+        p = int(ipiv[k - 1])
+        if p != k:
+            A[[k - 1, p - 1], :] = A[[p - 1, k - 1], :]
+
+
+def _getrf2_rec(A, ipiv):
+    """One call of getrf2! on the view A (src/lu.jl:185-300); returns info
+    (0 or the 1-based first zero pivot, LAPACK dgetrf2's meaning)."""
+    m, n = A.shape
+    if m == 0 or n == 0:
+        return 0
+    if m == 1:  # src/lu.jl:216-221
+        ipiv[0] = 1
+        return 1 if A[0, 0] == 0 else 0
+    if n == 1:  # src/lu.jl:223-253
+        sfmin = np.finfo(A.dtype).tiny
+        col = A[:, 0]
+        i = int(np.argmax(np.abs(col)))  # first index of the maximum, as idamax
+        ipiv[0] = i + 1
+        if col[i] == 0:
+            return 1
+        if i != 0:
+            col[0], col[i] = col[i], col[0]
+        if abs(col[0]) >= sfmin:
+            col[1:] *= A.dtype.type(1) / col[0]
+        else:
+            col[1:] /= col[0]
+        return 0
+    n1 = min(m, n) // 2  # src/lu.jl:255-296
+    n2 = n - n1
+    info = _getrf2_rec(A[:, :n1], ipiv)
+    _laswp(A[:, n1:], 1, n1, ipiv)
+    L11 = np.tril(A[:n1, :n1], -1) + np.eye(n1, dtype=A.dtype)
+    A[:n1, n1:] = _unit_lower_solve(L11, A[:n1, n1:])
+    A[n1:, n1:] -= A[n1:, :n1] @ A[:n1, n1:]
+    iinfo = _getrf2_rec(A[n1:, n1:], ipiv[n1:])
+    if info == 0 and iinfo > 0:
+        info = iinfo + n1
+    ipiv[n1:min(m, n)] += n1
+    _laswp(A[:, :n1], n1 + 1, min(m, n), ipiv)
+    return info
+
+
+def _unit_lower_solve(L, B):
+    import scipy.linalg
+    return scipy.linalg.solve_triangular(L, B, lower=True, unit_diagonal=True)
+
+
+def getrf2(A):
+    """Recursive LU with partial pivoting of a copy of A: returns (LU, ipiv
+    (1-based int64), info) as getrf2!(A, ipiv, info) of the reference."""
+    A = np.array(A, order="F", copy=True)
+    ipiv = np.zeros(min(A.shape), dtype=np.int64)
+    info = _getrf2_rec(A, ipiv)
+    return A, ipiv, info
+
+
+def lu_reconstruct(LU, ipiv):
+    """P·L·U from packed factors and 1-based interchanges (FP64)."""
+    m, n = LU.shape
+    k = min(m, n)
+    L = np.tril(LU[:, :k].astype(np.float64), -1) + np.eye(m, k)
+    U = np.triu(LU[:k, :].astype(np.float64))
+    R = L @ U
+    for i in range(k - 1, -1, -1):  # undo the interchanges, last first
+        p = int(ipiv[i]) - 1
+        if p != i:
+            R[[i, p], :] = R[[p, i], :]
+    return R

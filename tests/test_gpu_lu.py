@@ -1,0 +1,109 @@
+"""Recursive LU with partial pivoting on the device (nla_getrf2, the
+reference's getrf2!, src/lu.jl:185-300) against the oracle's restatement,
+which is itself pinned to LAPACK dgetrf (tests/test_oracle.py).
+
+FP64: the same pivots as the oracle and factors equal to rounding.  FP32:
+P*L*U = A to the bound of the reference tests' check (test/lu.jl:25-28,
+relative error of the reconstruction) and |L| <= 1; pivots are not compared
+in FP32, where a different summation order can flip a near tie.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+SHAPES = [(1, 1), (1, 7), (7, 1), (2, 2), (3, 5), (64, 64), (100, 90), (90, 100), (257, 300), (513, 513),
+          (1000, 700)]
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(DEV).t()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("m,n", SHAPES)
+def test_fp64_matches_oracle(nla, m, n):
+    rng = np.random.default_rng(m * 1009 + n)
+    A = rng.standard_normal((m, n))
+    LU, ipiv, info = nla.getrf2_(dev(A))
+    ref, ref_piv, ref_info = O.getrf2(A)
+    assert info == ref_info == 0
+    assert np.array_equal(host(ipiv), ref_piv), (m, n)
+    got = host(LU)
+    assert np.max(np.abs(got - ref)) <= 1e-11 * max(1.0, np.max(np.abs(ref))), (m, n)
+    assert np.linalg.norm(O.lu_reconstruct(got, host(ipiv)) - A) <= 20 * max(m, n) * 2.2e-16 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("m,n", [(64, 64), (300, 257), (257, 300), (1000, 1000)])
+def test_fp32_reconstructs(nla, m, n):
+    rng = np.random.default_rng(7 + m + n)
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    LU, ipiv, info = nla.getrf2_(dev(A))
+    got, piv = host(LU), host(ipiv)
+    assert info == 0
+    k = min(m, n)
+    assert np.all((piv >= np.arange(1, k + 1)) & (piv <= m))
+    assert np.max(np.abs(np.tril(got[:, :k], -1))) <= 1.0 + 1e-6
+    err = np.linalg.norm(O.lu_reconstruct(got, piv) - A) / np.linalg.norm(A)
+    assert err <= 20 * max(m, n) * 1.2e-7, err
+
+
+def test_singular_info_and_padded_lda(nla):
+    """A zero third column: info = 3 (U(3,3) exactly zero) and the rest of
+    the factorization as the oracle's; A inside a larger allocation (lda > m)
+    with the rows past m untouched."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((40, 30))
+    A[:, 2] = 0.0
+    big = torch.full((30, 48), 7.0, dtype=torch.float64, device=DEV).t()  # 48 x 30, lda 48
+    big[:40].copy_(torch.from_numpy(A).to(DEV))
+    view = big[:40]
+    LU, ipiv, info = nla.getrf2_(view)
+    ref, ref_piv, ref_info = O.getrf2(A)
+    assert info == ref_info == 3
+    assert np.array_equal(host(ipiv), ref_piv)
+    assert np.max(np.abs(host(LU) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert torch.all(big[40:] == 7.0)
+
+
+def test_empty_and_invalid(nla):
+    E = torch.empty((0, 5), dtype=torch.float64, device=DEV)
+    _, ipiv, info = nla.getrf2_(E)
+    assert info == 0 and ipiv.numel() == 0
+    with pytest.raises(ValueError):
+        nla.getrf2_(torch.ones(4, 5, dtype=torch.float64, device=DEV))  # row-major
+    with pytest.raises(ValueError):
+        nla.getrf2_(dev(np.eye(4)), ipiv=torch.empty(2, dtype=torch.int64, device=DEV))
+    with pytest.raises(ValueError):
+        nla.getrf2_(dev(np.eye(4)).to(torch.int32))
+
+
+def test_large_fp64_self_certified(nla):
+    """n = 4096: P*L*U = A checked on the device in FP64."""
+    n = 4096
+    g = torch.Generator(device=DEV)
+    g.manual_seed(5)
+    A0 = torch.empty((n, n), dtype=torch.float64, device=DEV).t().normal_(generator=g)
+    A = A0.clone()
+    LU, ipiv, info = nla.getrf2_(A)
+    assert info == 0
+    L = torch.tril(LU, -1) + torch.eye(n, dtype=torch.float64, device=DEV)
+    U = torch.triu(LU)
+    R = L @ U
+    perm = torch.arange(n, device=DEV)
+    piv = ipiv.cpu().numpy() - 1
+    p = perm.cpu().numpy()
+    for i in range(n):  # row i of P*A came from row p[i]
+        j = piv[i]
+        p[i], p[j] = p[j], p[i]
+    PA = A0[torch.from_numpy(p).to(DEV)]
+    err = (torch.linalg.norm(R - PA) / torch.linalg.norm(A0)).item()
+    assert err <= 1e-13, err
+    assert torch.max(torch.abs(torch.tril(LU, -1))).item() <= 1.0

@@ -256,3 +256,30 @@ def test_blas_unit_diagonal_is_not_read():
         ref = O.blas(side, uplo, trans, 0.5, func, A1, B)
         assert np.all(np.isfinite(got))
         assert O.rel_error(got, ref) < 1e-14, (side, uplo, trans, func)
+
+
+def test_getrf2_oracle_matches_lapack():
+    """The numpy restatement of getrf2! (src/lu.jl:185-300) against LAPACK
+    dgetrf / sgetrf: identical pivots, factors equal to rounding, P*L*U = A,
+    and LAPACK's info for exactly singular inputs."""
+    import scipy.linalg
+    rng = np.random.default_rng(12)
+    for dt, f, tol in ((np.float64, scipy.linalg.lapack.dgetrf, 1e-12), (np.float32, scipy.linalg.lapack.sgetrf, 1e-4)):
+        for m, n in ((1, 1), (1, 6), (6, 1), (2, 2), (9, 9), (40, 33), (33, 40), (128, 128), (150, 90)):
+            A = rng.standard_normal((m, n)).astype(dt)
+            LU, ipiv, info = O.getrf2(A)
+            lu, piv, inf = f(A)
+            assert info == inf == 0
+            assert np.array_equal(ipiv, piv + 1), (m, n)
+            assert np.max(np.abs(LU - lu)) <= tol * max(1.0, np.max(np.abs(lu))), (m, n)
+            assert np.linalg.norm(O.lu_reconstruct(LU, ipiv) - A) <= 50 * max(m, n) * np.finfo(dt).eps * np.linalg.norm(A)
+    # a zero third column stays zero under elimination: U(3,3) is exactly zero
+    A = rng.standard_normal((6, 6))
+    A[:, 2] = 0.0
+    LU, ipiv, info = O.getrf2(A)
+    assert info == scipy.linalg.lapack.dgetrf(A)[2] == 3 and LU[2, 2] == 0.0
+    LU, ipiv, info = O.getrf2(np.zeros((4, 3)))
+    assert info == 1 and list(ipiv) == [1, 2, 3]
+    # 2 x 2 by hand: pivot row 2, l21 = 1/3, u22 = 2 - 4/3
+    LU, ipiv, info = O.getrf2(np.array([[1.0, 2.0], [3.0, 4.0]]))
+    assert list(ipiv) == [2, 2] and np.allclose(LU, [[3.0, 4.0], [1 / 3, 2 - 4 / 3]])
