@@ -200,6 +200,35 @@ __global__ void laswp_kernel(int64_t ncols, T *__restrict__ A, int64_t lda, int6
     }
 }
 
+// A12 = L11^-1 A12 for a unit lower L11 of n1 <= SMALL_TRSM rows (the
+// reference's trsm!('L','L','N','U'), src/lu.jl:277): forward substitution,
+// one thread per column of A12 holding it in registers; L11 is read through
+// the cache (every thread reads the same entries).  Replaces the inverse and
+// panel launches of the general TRSM at the bottom of the recursion.
+constexpr int SMALL_TRSM = 32;
+
+template <typename T>
+__global__ __launch_bounds__(64) void lu_trsm_small_kernel(int n1, int64_t n2, const T *__restrict__ L, int64_t lda,
+                                                           T *__restrict__ B) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n2) return;
+    T *b = B + c * lda;
+    T x[SMALL_TRSM];
+#pragma unroll
+    for (int i = 0; i < SMALL_TRSM; ++i) x[i] = b[i < n1 ? i : 0];
+#pragma unroll
+    for (int k = 0; k < SMALL_TRSM - 1; ++k) {
+        const T xk = x[k];
+        const T *lk = L + (int64_t)(k < n1 ? k : 0) * lda;
+#pragma unroll
+        for (int i = k + 1; i < SMALL_TRSM; ++i)
+            if (i < n1) x[i] -= lk[i] * xk;
+    }
+#pragma unroll
+    for (int i = 0; i < SMALL_TRSM; ++i)
+        if (i < n1) b[i] = x[i];
+}
+
 __global__ void info_zero_kernel(int64_t *info) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *info = 0;
 }
@@ -241,7 +270,11 @@ struct LuRec {
         laswp(A + n1 * lda, n2, 1, n1, ipiv, r0);
         if (st != NLA_OK) return;
         // A12 = L11^-1 A12 (unit lower; the identity for one column)
-        if (n1 > 1) {
+        if (n1 > 1 && n1 <= SMALL_TRSM) {
+            hipLaunchKernelGGL(lu_trsm_small_kernel<T>, dim3((unsigned)((n2 + 63) / 64)), dim3(64), 0, s, (int)n1, n2,
+                               A, lda, A + n1 * lda);
+            if (!ok(hipGetLastError(), "lu small trsm")) return;
+        } else if (n1 > 1) {
             const int32_t r = nla_trsm('L', 'L', 'N', 'U', dtype, 1.0, n1, n2, A, lda, A + n1 * lda, lda, s);
             if (r != NLA_OK) {
                 st = r;
