@@ -35,7 +35,10 @@ Also reported on the same line:
                 lu             getrf2! (recursive LU with partial pivoting,
                                built on the unit-lower TRSM and GEMM update)
                                at n = 4096 and 8192 in FP64, against LAPACK
-                               dgetrf on the host's cores.
+                               dgetrf on the host's cores;
+                lauum          lauum! (U*U^T, L^T*L in place, built on the
+                               TRMM and the GEMM update) at n = 4096 and
+                               8192 in FP64, against LAPACK dlauum.
 """
 import argparse
 import json
@@ -276,6 +279,46 @@ def lu_leg(nla, torch, sizes=(4096, 8192), seed=1):
     return out
 
 
+def lauum_leg(nla, torch, sizes=(4096, 8192), seed=1):
+    """lauum! (U*U^T / L^T*L in place, built on the TRMM of the hot path and
+    the GEMM update) in FP64 on U(-0.5, 0.5) triangles, against LAPACK
+    dlauum (OpenBLAS through scipy) on the host's cores."""
+    import scipy.linalg
+    out = {"workload": "FP64 triangle with U(-0.5,0.5) entries; n^3/3 flops"}
+    lib = nla.load()
+    for n in sizes:
+        for uplo in ("L", "U"):
+            g = torch.Generator(device="cuda")
+            g.manual_seed(seed)
+            A0 = torch.empty((n, n), dtype=torch.float64, device="cuda").t().uniform_(-0.5, 0.5, generator=g)
+            A = A0.clone()
+            ws = torch.empty(int(lib.nla_lauum_workspace_bytes(1, n)), dtype=torch.uint8, device="cuda")
+            nla.lauum_(uplo, n, A, workspace=ws)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(3):
+                A.copy_(A0)
+                t, _ = event_ms(torch, lambda: nla.lauum_(uplo, n, A, workspace=ws), 1)
+                ts.append(t)
+            ms = min(ts)
+            T = torch.tril(A0) if uplo == "L" else torch.triu(A0)
+            R = torch.tril(T.t() @ T) if uplo == "L" else torch.triu(T @ T.t())
+            got = torch.tril(A) if uplo == "L" else torch.triu(A)
+            rel = (torch.linalg.norm(got - R) / torch.linalg.norm(R)).item()
+            Ah = A0.cpu().numpy()
+            t0 = time.perf_counter()
+            scipy.linalg.lapack.dlauum(Ah, lower=1 if uplo == "L" else 0)
+            tc = (time.perf_counter() - t0) * 1e3
+            fl = n ** 3 / 3.0
+            out[f"{uplo}_n{n}"] = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3),
+                                   "roofline_frac": round(fl / ms / 1e9 / PEAK["f64"], 4), "rel_error": rel,
+                                   "host_lapack_ms": round(tc, 2), "speedup_vs_host_lapack": round(tc / ms, 2)}
+            log(f"bench: lauum {uplo} n={n}: {ms:.2f} ms ({fl / ms / 1e9:.2f} TFLOP/s), host LAPACK {tc:.1f} ms")
+            del A0, A, T, R, got, ws
+            torch.cuda.empty_cache()
+    return out
+
+
 def serve_leg(nla, torch, count=600, seed=0):
     """The serving workload through nextla_amd.serve (see the module)."""
     import numpy as np
@@ -386,7 +429,7 @@ def main():
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM rates and the mixed and batched legs")
-    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served", "lu"),
+    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served", "lu", "lauum"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -406,6 +449,8 @@ def main():
             res = batch_leg(nla, torch, torch.float64 if args.leg.endswith("f64") else torch.float32)
         elif args.leg == "lu":
             res = lu_leg(nla, torch)
+        elif args.leg == "lauum":
+            res = lauum_leg(nla, torch)
         else:
             res = serve_leg(nla, torch)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
@@ -463,6 +508,7 @@ def main():
                 f"loop of plain calls {b['loop_of_plain_calls_ms']} ms")
         extra["served"] = serve_leg(nla, torch)
         extra["lu"] = lu_leg(nla, torch)
+        extra["lauum"] = lauum_leg(nla, torch)
     torch.cuda.synchronize()
 
     cpu = None

@@ -195,6 +195,24 @@ int32_t nla_getrf2(int32_t dtype, int64_t m, int64_t n, void *A, int64_t lda, in
                    void *stream);
 
 /*
+ * LAUUM: A := U·U^T (uplo 'U') or L^T·L (uplo 'L') in place, reading and
+ * writing only the uplo triangle (replaces lauum!(uplo, n, A, ib),
+ * src/lauum.jl:52-186; the block size ib is internal here).  Recursion on a
+ * 2 x 2 split: the diagonal blocks recursively (64-row blocks in one
+ * work-group each), the off-diagonal product into the diagonal block by the
+ * GEMM update (through a scratch square, then added to the triangle), and
+ * the off-diagonal block by this library's TRMM (nla_rectrxm, trans 'T').
+ * The workspace (nla_lauum_workspace_bytes: (n/2 rounded up to 64)^2
+ * elements) is the caller's; none is needed for n <= 64.
+ * Argument positions: 1 uplo, 2 dtype, 3 n, 4 A, 5 lda, 6 workspace,
+ * 7 workspace_bytes, 8 stream.
+ */
+int64_t nla_lauum_workspace_bytes(int32_t dtype, int64_t n);
+
+int32_t nla_lauum(char uplo, int32_t dtype, int64_t n, void *A, int64_t lda, void *workspace, int64_t workspace_bytes,
+                  void *stream);
+
+/*
  * Mixed-precision TRSM with FP64 iterative refinement (no counterpart in the
  * reference; the FP64 solution it converges to is that of unified_rectrxm!'s
  * TRSM, src/rectrxm.jl:43-76).  Solves

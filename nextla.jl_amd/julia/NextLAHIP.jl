@@ -20,7 +20,8 @@ It also exposes the forms the reference does not have: `batched_rectrxm!`
 TRSM by low-precision correction solves and FP64 refinement) and
 `batched_rectrxm_mixed!` (the same for many systems at once), and methods of
 NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`, and
-`getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM).
+`getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM) and
+`lauum!` (U·U^T / L^T·L on the TRMM).
 
 Differences from the reference, by design: arguments are validated before any
 device work (`ArgumentError`, B untouched), lower-case characters are
@@ -129,6 +130,21 @@ function NextLA.getrf2!(A::DeviceMat{T}, ipiv::AbstractVector{Int}, info::Int) w
     k = min(m, n)
     k > 0 && copyto!(ipiv, 1, Array(dpiv), 1, k)
     return A, ipiv, Int(Array(dinfo)[1])
+end
+
+# lauum!(uplo, n, A, ib) (src/lauum.jl:52): A := U*U^T or L^T*L on the device; ib is ignored.
+function NextLA.lauum!(uplo::Char, n::Integer, A::DeviceMat{T}, ib::Integer) where {T <: Union{Float32, Float64}}
+    uplo in ('U', 'L') || throw(ArgumentError("uplo must be 'U' or 'L', got '$uplo'"))
+    n >= 0 || throw(ArgumentError("n must be non-negative, got $n"))
+    pa, lda = ptr_ld(A)
+    bytes = ccall((:nla_lauum_workspace_bytes, libnlahip), Int64, (Int32, Int64), dtype_code(T), Int64(n))
+    buf, ws = workspace(max(bytes, 1))
+    st = GC.@preserve A buf ccall((:nla_lauum, libnlahip), Int32,
+        (Cchar, Int32, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+        Cchar(uplo), dtype_code(T), Int64(n), pa, lda, ws, bytes, hip_stream())
+    check(st, "lauum!")
+    AMDGPU.synchronize()  # the workspace is released when this returns
+    return A
 end
 
 # ---- forms without a reference counterpart ---------------------------------

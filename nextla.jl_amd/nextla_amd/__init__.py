@@ -10,6 +10,7 @@ Names follow the reference with Julia's `!` spelled `_`:
     LeftLowerTRMM_(A, B) ... RightUpperTRMM_(A, B)              src/trmm.jl:332-393
     trsm / trmm(side, uplo, transa, diag, A, B, alpha)          src/trsm.jl:186, src/trmm.jl:430
     getrf2_(A, ipiv=None) -> (A, ipiv, info)                    src/lu.jl:185-300
+    lauum_(uplo, n, A, ib=None)                                 src/lauum.jl:52
 
 A and B are 2-D torch tensors on a ROCm device, float32 or float64.  Julia
 matrices are column-major; a tensor with stride (1, ld) is used as is, and a
@@ -33,7 +34,7 @@ __all__ = [
     "unified_rectrxm_", "GEMM_ADD_", "GEMM_SUB_",
     "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_",
-    "trsm", "trmm", "getrf2_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
+    "trsm", "trmm", "getrf2_", "lauum_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
 ]
 
 _DT = {torch.float32: NLA_F32, torch.float64: NLA_F64}
@@ -239,3 +240,35 @@ def getrf2_(A, ipiv=None):
     st = load().nla_getrf2(_DT[A.dtype], m, n, A.data_ptr(), lda, ipiv.data_ptr(), info.data_ptr(), _stream(A))
     _lib.check(st, "getrf2!")
     return A, ipiv[:k], int(info.item())
+
+
+def lauum_(uplo, n, A, ib=None, workspace=None):
+    """lauum!(uplo, n, A, ib) of src/lauum.jl:52: A := U*U^T (uplo 'U') or
+    L^T*L (uplo 'L') on the leading n x n block, in place, only that
+    triangle read and written.  A: column-major float32/float64 tensor on a
+    ROCm device.  ib (the reference's block size) is accepted and ignored:
+    the blocking is internal.  workspace: optional uint8 device tensor of at
+    least nla_lauum_workspace_bytes bytes (allocated here when omitted).
+    Returns A."""
+    if not isinstance(uplo, str) or uplo.upper() not in ("U", "L"):
+        raise ValueError(f"uplo must be 'U' or 'L', got {uplo!r}")
+    if not isinstance(n, int) or n < 0:
+        raise ValueError(f"n must be non-negative, got {n!r}")
+    if A.dtype not in _DT:
+        raise ValueError(f"unsupported element type {A.dtype} (Float32/Float64 only)")
+    _require_device(A)
+    if A.dim() != 2 or A.shape[0] < n or A.shape[1] < n:
+        raise ValueError(f"A {tuple(A.shape)} is smaller than n = {n}")
+    tr, lda = _layout(A, "A")
+    if tr and n > 1:
+        raise ValueError("A must be column-major (stride (1, ld)); use colmajor(A)")
+    lib = load()
+    need = int(lib.nla_lauum_workspace_bytes(_DT[A.dtype], n))
+    if workspace is None:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=A.device)
+    elif workspace.numel() < need:
+        raise ValueError(f"workspace holds {workspace.numel()} bytes, {need} needed")
+    st = lib.nla_lauum(_lib.ch(uplo.upper()), _DT[A.dtype], n, A.data_ptr(), lda, workspace.data_ptr(), need,
+                       _stream(A))
+    _lib.check(st, "lauum!")
+    return A

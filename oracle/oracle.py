@@ -13,6 +13,10 @@ Two oracles, as in the reference's own tests (test/unified_rectrxm.jl:36-40):
 Error measures: `backward_error` (TRSM, needs no reference solution) and
 `product_error` (TRMM), both normwise in Frobenius norm and computed in FP64.
 
+`lauum` restates lauum!(uplo, n, A, ib) (src/lauum.jl:52-186, lauu2!
+src/lauu2.jl:48-107): U*U^T or L^T*L on the uplo triangle, the other
+triangle untouched; pinned against LAPACK dlauum / slauum.
+
 `getrf2` restates the reference's recursive LU with partial pivoting
 (src/lu.jl:185-300, laswp src/lu.jl:470-520) in numpy; it is pinned against
 LAPACK's dgetrf / sgetrf through scipy (same pivots and factors) and by
@@ -254,3 +258,22 @@ def lu_reconstruct(LU, ipiv):
         if p != i:
             R[[i, p], :] = R[[p, i], :]
     return R
+
+
+def lauum(uplo, n, A):
+    """lauum!(uplo, n, A, ib) of src/lauum.jl:52-186 on a copy of A: the
+    leading n x n triangle replaced by U*U^T (uplo 'U') or L^T*L (uplo 'L'),
+    computed in FP64; everything else unchanged."""
+    A = np.array(A, order="F", copy=True)
+    T = A[:n, :n].astype(np.float64)
+    if uplo == "U":
+        U = np.triu(T)
+        R = np.triu(U @ U.T)
+        mask = np.triu(np.ones((n, n), dtype=bool))
+    else:
+        L = np.tril(T)
+        R = np.tril(L.T @ L)
+        mask = np.tril(np.ones((n, n), dtype=bool))
+    blk = A[:n, :n]
+    blk[mask] = R[mask].astype(A.dtype)
+    return A

@@ -283,3 +283,25 @@ def test_getrf2_oracle_matches_lapack():
     # 2 x 2 by hand: pivot row 2, l21 = 1/3, u22 = 2 - 4/3
     LU, ipiv, info = O.getrf2(np.array([[1.0, 2.0], [3.0, 4.0]]))
     assert list(ipiv) == [2, 2] and np.allclose(LU, [[3.0, 4.0], [1 / 3, 2 - 4 / 3]])
+
+
+def test_lauum_oracle_matches_lapack():
+    """lauum (src/lauum.jl:52) against LAPACK dlauum / slauum on both
+    triangles, with the other triangle untouched, and the reference test's
+    own check (test/lauum.jl: the triangle of A*A' or A'*A)."""
+    import scipy.linalg
+    rng = np.random.default_rng(21)
+    for dt, f, tol in ((np.float64, scipy.linalg.lapack.dlauum, 1e-13), (np.float32, scipy.linalg.lapack.slauum, 1e-5)):
+        for n in (1, 2, 16, 65, 130):
+            for uplo in "UL":
+                A = rng.uniform(-0.5, 0.5, (n, n)).astype(dt)
+                got = O.lauum(uplo, n, A)
+                ref, info = f(A, lower=1 if uplo == "L" else 0)
+                assert info == 0
+                assert np.max(np.abs(got - ref)) <= tol * max(1.0, np.max(np.abs(ref))) * n, (uplo, n)
+                other = np.tril(np.ones((n, n), bool), -1) if uplo == "U" else np.triu(np.ones((n, n), bool), 1)
+                assert np.array_equal(got[other], A[other])
+                T = np.triu(A.astype(np.float64)) if uplo == "U" else np.tril(A.astype(np.float64))
+                closed = np.triu(T @ T.T) if uplo == "U" else np.tril(T.T @ T)
+                tri = ~other
+                assert np.max(np.abs(got[tri] - closed[tri])) <= tol * n * max(1.0, np.max(np.abs(closed)))
