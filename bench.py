@@ -38,7 +38,10 @@ Also reported on the same line:
                                dgetrf on the host's cores;
                 lauum          lauum! (U*U^T, L^T*L in place, built on the
                                TRMM and the GEMM update) at n = 4096 and
-                               8192 in FP64, against LAPACK dlauum.
+                               8192 in FP64, against LAPACK dlauum;
+                larfb          larfb! (block reflector of tile QR, its
+                               triangular multiplies on the TRMM) at
+                               m = n = 8192, k = 64 and 256 in FP64.
 """
 import argparse
 import json
@@ -319,6 +322,36 @@ def lauum_leg(nla, torch, sizes=(4096, 8192), seed=1):
     return out
 
 
+def larfb_leg(nla, torch, shapes=((8192, 8192, 64), (8192, 8192, 256)), seed=1):
+    """larfb! (apply H^T = (I - V T V^T)^T from the left, forward columnwise
+    reflectors) in FP64: 4*m*n*k flops over the device time; its three
+    triangular multiplies are this library's TRMM, its two products the GEMM
+    update."""
+    out = {"workload": "C (m x n) := H^T C, V m x k, T k x k upper, FP64 N(0,1)"}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    for m, n, k in shapes:
+        V = torch.empty((k, m), dtype=torch.float64, device="cuda").t().normal_(generator=g)
+        T = torch.triu(torch.empty((k, k), dtype=torch.float64, device="cuda").normal_(generator=g)).t().contiguous().t()
+        C0 = torch.empty((n, m), dtype=torch.float64, device="cuda").t().normal_(generator=g)
+        C = C0.clone()
+        nla.larfb_("L", "T", "F", "C", V, T, C)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(5):
+            C.copy_(C0)
+            t, _ = event_ms(torch, lambda: nla.larfb_("L", "T", "F", "C", V, T, C), 1)
+            ts.append(t)
+        ms = min(ts)
+        fl = 4.0 * m * n * k
+        out[f"m{m}_n{n}_k{k}"] = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3),
+                                  "roofline_frac": round(fl / ms / 1e9 / PEAK["f64"], 4)}
+        log(f"bench: larfb m={m} n={n} k={k}: {ms:.3f} ms ({fl / ms / 1e9:.1f} TFLOP/s)")
+        del V, T, C0, C
+    torch.cuda.empty_cache()
+    return out
+
+
 def serve_leg(nla, torch, count=600, seed=0):
     """The serving workload through nextla_amd.serve (see the module)."""
     import numpy as np
@@ -429,7 +462,7 @@ def main():
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM rates and the mixed and batched legs")
-    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served", "lu", "lauum"),
+    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served", "lu", "lauum", "larfb"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -451,6 +484,8 @@ def main():
             res = lu_leg(nla, torch)
         elif args.leg == "lauum":
             res = lauum_leg(nla, torch)
+        elif args.leg == "larfb":
+            res = larfb_leg(nla, torch)
         else:
             res = serve_leg(nla, torch)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
@@ -509,6 +544,7 @@ def main():
         extra["served"] = serve_leg(nla, torch)
         extra["lu"] = lu_leg(nla, torch)
         extra["lauum"] = lauum_leg(nla, torch)
+        extra["larfb"] = larfb_leg(nla, torch)
     torch.cuda.synchronize()
 
     cpu = None

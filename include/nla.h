@@ -213,6 +213,25 @@ int32_t nla_lauum(char uplo, int32_t dtype, int64_t n, void *A, int64_t lda, voi
                   void *stream);
 
 /*
+ * Block reflector application (replaces larfb!(side, trans, direct, storev,
+ * m, n, k, V, ldv, T, C, work), src/larfb.jl:37-142, for forward,
+ * columnwise reflectors, the only form the reference supports):
+ * H = I - V·T·V^T with V unit lower (m x k for side 'L', n x k for 'R'; its
+ * diagonal and upper part are not read) and T k x k upper triangular;
+ * C (m x n) := H·C, H^T·C, C·H or C·H^T (trans 'N' applies H, 'T' / 'C'
+ * its transpose).  The three triangular multiplies go through this
+ * library's TRMM (nla_trmm, V1 with diag 'U') and the products with V2
+ * through the GEMM update.  work: device block of at least (side 'L' ? n : m)
+ * rows by k columns, leading dimension ldwork.
+ * Argument positions: 1 side, 2 trans, 3 direct, 4 storev, 5 dtype, 6 m,
+ * 7 n, 8 k, 9 V, 10 ldv, 11 T, 12 ldt, 13 C, 14 ldc, 15 work, 16 ldwork,
+ * 17 stream.
+ */
+int32_t nla_larfb(char side, char trans, char direct, char storev, int32_t dtype, int64_t m, int64_t n, int64_t k,
+                  const void *V, int64_t ldv, const void *T, int64_t ldt, void *C, int64_t ldc, void *work,
+                  int64_t ldwork, void *stream);
+
+/*
  * Mixed-precision TRSM with FP64 iterative refinement (no counterpart in the
  * reference; the FP64 solution it converges to is that of unified_rectrxm!'s
  * TRSM, src/rectrxm.jl:43-76).  Solves

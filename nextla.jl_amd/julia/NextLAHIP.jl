@@ -20,8 +20,9 @@ It also exposes the forms the reference does not have: `batched_rectrxm!`
 TRSM by low-precision correction solves and FP64 refinement) and
 `batched_rectrxm_mixed!` (the same for many systems at once), and methods of
 NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`, and
-`getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM) and
-`lauum!` (U·U^T / L^T·L on the TRMM).
+`getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM),
+`lauum!` (U·U^T / L^T·L on the TRMM) and `larfb!` (tile QR's block reflector,
+its triangular multiplies on the TRMM).
 
 Differences from the reference, by design: arguments are validated before any
 device work (`ArgumentError`, B untouched), lower-case characters are
@@ -145,6 +146,25 @@ function NextLA.lauum!(uplo::Char, n::Integer, A::DeviceMat{T}, ib::Integer) whe
     check(st, "lauum!")
     AMDGPU.synchronize()  # the workspace is released when this returns
     return A
+end
+
+# larfb!(side, trans, direct, storev, V, T, C) (src/larfb.jl:166): forward columnwise block
+# reflector applied on the device; its triangular multiplies are the library's TRMM.
+function NextLA.larfb!(side::Char, trans::Char, direct::Char, storev::Char, V::DeviceMat{T}, Tm::DeviceMat{T},
+                       C::DeviceMat{T}) where {T <: Union{Float32, Float64}}
+    m, n = size(C)
+    k = size(Tm, 1)
+    pv, ldv = ptr_ld(V); pt, ldt = ptr_ld(Tm); pc, ldc = ptr_ld(C)
+    wr = side in ('L', 'l') ? n : m
+    work = AMDGPU.ROCMatrix{T}(undef, max(wr, 1), max(k, 1))
+    st = GC.@preserve V Tm C work ccall((:nla_larfb, libnlahip), Int32,
+        (Cchar, Cchar, Cchar, Cchar, Int32, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid},
+         Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+        Cchar(side), Cchar(trans), Cchar(direct), Cchar(storev), dtype_code(T), Int64(m), Int64(n), Int64(k),
+        pv, ldv, pt, ldt, pc, ldc, pointer(work), Int64(max(wr, 1)), hip_stream())
+    check(st, "larfb!")
+    AMDGPU.synchronize()  # work is released when this returns
+    return C
 end
 
 # ---- forms without a reference counterpart ---------------------------------

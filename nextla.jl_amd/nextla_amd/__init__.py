@@ -11,6 +11,7 @@ Names follow the reference with Julia's `!` spelled `_`:
     trsm / trmm(side, uplo, transa, diag, A, B, alpha)          src/trsm.jl:186, src/trmm.jl:430
     getrf2_(A, ipiv=None) -> (A, ipiv, info)                    src/lu.jl:185-300
     lauum_(uplo, n, A, ib=None)                                 src/lauum.jl:52
+    larfb_(side, trans, direct, storev, V, T, C, work=None)     src/larfb.jl:37,166
 
 A and B are 2-D torch tensors on a ROCm device, float32 or float64.  Julia
 matrices are column-major; a tensor with stride (1, ld) is used as is, and a
@@ -34,7 +35,7 @@ __all__ = [
     "unified_rectrxm_", "GEMM_ADD_", "GEMM_SUB_",
     "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_",
-    "trsm", "trmm", "getrf2_", "lauum_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
+    "trsm", "trmm", "getrf2_", "lauum_", "larfb_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
 ]
 
 _DT = {torch.float32: NLA_F32, torch.float64: NLA_F64}
@@ -272,3 +273,37 @@ def lauum_(uplo, n, A, ib=None, workspace=None):
                        _stream(A))
     _lib.check(st, "lauum!")
     return A
+
+
+def larfb_(side, trans, direct, storev, V, T, C, work=None):
+    """larfb!(side, trans, direct, storev, V, T, C) of src/larfb.jl:166 (the
+    sized form, src/larfb.jl:37): C := H*C, H^T*C, C*H or C*H^T with
+    H = I - V*T*V^T, forward columnwise reflectors (the reference's only
+    form).  V (m x k for side 'L', n x k for 'R'; unit lower, diagonal and
+    upper part not read), T (k x k upper) and C (m x n): column-major
+    float32/float64 tensors on one ROCm device.  Returns C."""
+    for k_, (v, ok) in enumerate(((side, "LRlr"), (trans, "NTCntc"), (direct, "FBfb"), (storev, "CRcr")), 1):
+        if not isinstance(v, str) or len(v) != 1 or v not in ok:
+            raise ValueError(f"argument {k_} has invalid value {v!r}")
+    if direct.upper() != "F" or storev.upper() != "C":
+        raise ValueError("larfb! supports only forward, columnwise reflectors (as the reference)")
+    if C.dtype not in _DT or V.dtype != C.dtype or T.dtype != C.dtype:
+        raise ValueError("V, T and C must share one element type, float32 or float64")
+    _require_device(V, T, C)
+    m, n = C.shape
+    k = T.shape[0]
+    lds = []
+    for t, nm in ((V, "V"), (T, "T"), (C, "C")):
+        tr, ld = _layout(t, nm)
+        if tr and t.shape[0] > 1 and t.shape[1] > 1:
+            raise ValueError(f"{nm} must be column-major (stride (1, ld)); use colmajor({nm})")
+        lds.append(ld)
+    wr = n if side.upper() == "L" else m
+    if work is None:
+        work = torch.empty((k, max(wr, 1)), dtype=C.dtype, device=C.device).t()
+    tr, ldw = _layout(work, "work")
+    st = load().nla_larfb(_lib.ch(side.upper()), _lib.ch(trans.upper()), b"F", b"C", _DT[C.dtype], m, n, k,
+                          V.data_ptr(), lds[0], T.data_ptr(), lds[1], C.data_ptr(), lds[2], work.data_ptr(),
+                          max(ldw, wr), _stream(C))
+    _lib.check(st, "larfb!")
+    return C

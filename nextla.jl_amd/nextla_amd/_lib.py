@@ -58,6 +58,8 @@ SIGNATURES = {
     "nla_batch_mixed_plan_destroy": (_i32, [_vp]),
     "nla_getrf2": (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _vp]),
     "nla_lauum_workspace_bytes": (_i64, [_i32, _i64]),
+    "nla_larfb": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                         _vp, _i64, _vp]),
     "nla_lauum": (_i32, [_c_char, _i32, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_mixed_workspace_bytes": (_i64, [_c_char, _i64, _i64]),
     "nla_rectrxm_mixed": (_i32, [_c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _dbl, _i32,

@@ -277,3 +277,29 @@ def lauum(uplo, n, A):
     blk = A[:n, :n]
     blk[mask] = R[mask].astype(A.dtype)
     return A
+
+
+def larft(V, tau):
+    """T of H(1)...H(k) = I - V T V^T (forward, columnwise; LAPACK dlarft),
+    as the reference builds it with larft! for its tests."""
+    k = V.shape[1]
+    T = np.zeros((k, k), dtype=np.float64)
+    Vd = V.astype(np.float64)
+    for i in range(k):
+        T[i, i] = tau[i]
+        if i:
+            T[:i, i] = -tau[i] * (T[:i, :i] @ (Vd[:, :i].T @ Vd[:, i]))
+    return T.astype(V.dtype)
+
+
+def larfb(side, trans, V, T, C):
+    """larfb! (src/larfb.jl:37-142) from the closed form H = I - V T V^T (V
+    unit lower), in FP64: H C, H^T C, C H or C H^T."""
+    Vd = np.tril(V.astype(np.float64), -1)
+    k = V.shape[1]
+    Vd[np.arange(k), np.arange(k)] = 1.0
+    Td = np.triu(T.astype(np.float64))
+    H = np.eye(Vd.shape[0]) - Vd @ Td @ Vd.T
+    Hop = H if trans.upper() == "N" else H.T
+    Cd = C.astype(np.float64)
+    return (Hop @ Cd if side.upper() == "L" else Cd @ Hop).astype(C.dtype)

@@ -305,3 +305,23 @@ def test_lauum_oracle_matches_lapack():
                 closed = np.triu(T @ T.T) if uplo == "U" else np.tril(T.T @ T)
                 tri = ~other
                 assert np.max(np.abs(got[tri] - closed[tri])) <= tol * n * max(1.0, np.max(np.abs(closed)))
+
+
+def test_larfb_oracle_is_the_qr_q():
+    """larft + larfb from a LAPACK Householder QR (dgeqrf through scipy):
+    H = I - V T V^T applied to C equals Q C with Q from dorgqr, for every
+    side / trans (the reference's own test builds V and T the same way,
+    test/larfb.jl:14-27)."""
+    import scipy.linalg
+    rng = np.random.default_rng(2)
+    for m, k in ((5, 4), (20, 4), (64, 16), (130, 33)):
+        qr, tau, _, _ = scipy.linalg.lapack.dgeqrf(rng.standard_normal((m, k)))
+        V = np.tril(qr, -1) + np.eye(m, k)
+        T = O.larft(V, tau)
+        Qf, _, _ = scipy.linalg.lapack.dorgqr(np.hstack([qr, np.zeros((m, m - k))]), np.concatenate([tau, np.zeros(m - k)]))
+        C = rng.standard_normal((m, 7))
+        assert np.allclose(O.larfb("L", "N", V, T, C), Qf @ C, atol=1e-12)
+        assert np.allclose(O.larfb("L", "T", V, T, C), Qf.T @ C, atol=1e-12)
+        D = rng.standard_normal((7, m))
+        assert np.allclose(O.larfb("R", "N", V, T, D), D @ Qf, atol=1e-12)
+        assert np.allclose(O.larfb("R", "C", V, T, D), D @ Qf.T, atol=1e-12)
