@@ -184,3 +184,47 @@ def test_batch_mixed_validation(nla):
     its = (ctypes.c_int32 * 1)()
     assert lib.nla_batch_mixed_plan_execute(None, 1e-14, 10, None, 0, bwd, its) == -1
     assert lib.nla_batch_mixed_plan_destroy(None) == 0
+
+
+def test_lu_lauum_larfb_validation(nla):
+    """Argument checks of the widened entry points return -k before any
+    device work."""
+    import ctypes
+    from nextla_amd import _lib
+    lib = _lib.load()
+    p = 4096  # never dereferenced: every call below fails validation first
+    assert lib.nla_getrf2(3, 4, 4, p, 4, p, p, None) == -1
+    assert lib.nla_getrf2(1, -1, 4, p, 4, p, p, None) == -2
+    assert lib.nla_getrf2(1, 4, -1, p, 4, p, p, None) == -3
+    assert lib.nla_getrf2(1, 4, 4, None, 4, p, p, None) == -4
+    assert lib.nla_getrf2(1, 4, 4, p, 3, p, p, None) == -5
+    assert lib.nla_getrf2(1, 4, 4, p, 4, None, p, None) == -6
+    assert lib.nla_getrf2(1, 4, 4, p, 4, p, None, None) == -7
+    need = lib.nla_lauum_workspace_bytes(1, 200)
+    assert need == 128 * 128 * 8 and lib.nla_lauum_workspace_bytes(1, 64) == 8
+    assert lib.nla_lauum_workspace_bytes(7, 10) == -1
+    assert lib.nla_lauum(b"X", 1, 4, p, 4, None, 0, None) == -1
+    assert lib.nla_lauum(b"L", 9, 4, p, 4, None, 0, None) == -2
+    assert lib.nla_lauum(b"L", 1, -1, p, 4, None, 0, None) == -3
+    assert lib.nla_lauum(b"L", 1, 4, None, 4, None, 0, None) == -4
+    assert lib.nla_lauum(b"L", 1, 4, p, 3, None, 0, None) == -5
+    assert lib.nla_lauum(b"L", 1, 200, p, 200, p, need - 1, None) == -7
+    assert lib.nla_lauum(b"U", 1, 0, None, 1, None, 0, None) == 0
+    good = [b"L", b"N", b"F", b"C", 1, 10, 8, 4, p, 10, p, 4, p, 10, p, 8, None]
+
+    def larfb(pos, val):
+        args = list(good)
+        args[pos] = val
+        return lib.nla_larfb(*args)
+
+    assert larfb(0, b"Q") == -1
+    assert larfb(1, b"Q") == -2
+    assert larfb(2, b"B") == -3      # backward: not supported, as the reference
+    assert larfb(3, b"R") == -4      # rowwise: not supported, as the reference
+    assert larfb(4, 5) == -5
+    assert larfb(7, 11) == -8        # more reflectors than rows of V
+    assert larfb(9, 9) == -10        # ldv < m
+    assert larfb(11, 3) == -12       # ldt < k
+    assert larfb(13, 9) == -14       # ldc < m
+    assert larfb(15, 7) == -16       # ldwork < n
+    assert larfb(7, 0) == 0          # k = 0: nothing to apply
