@@ -121,6 +121,14 @@ class Server:
     budget_bytes: HBM admission budget for in-flight requests.
     max_inflight: batches in flight at once (one stream each).
     max_batch: most requests per batch.
+    use_graph: replay batched plans from hipGraphs.
+    fp64_trsm: "plain" (FP64 TRSMs batched like the rest) or "mixed" (batched
+        fp32 correction solves with FP64 refinement to mixed_tol on
+        mixed_workers worker threads; mixed_precision is the update precision
+        of the single-system refinement used above the batched size limit).
+    direct_rows: requests with more rows go through the recursion one call
+        each instead of into the batched plan (default: none; measured slower
+        in aggregate, see the serving notes).
     """
 
     def __init__(self, device="cuda", budget_bytes=64 << 30, max_inflight=4, max_batch=512, use_graph=False,
