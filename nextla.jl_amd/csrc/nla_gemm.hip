@@ -244,6 +244,7 @@ __device__ __forceinline__ void gemm_tile(int64_t M, int64_t N, int64_t K, T alp
         }
         const NLA_LDS T *sx = lds_x[cur];
         const NLA_LDS T *sy = lds_y[cur];
+        __builtin_amdgcn_s_setprio(1);  // issue priority to waves in their MFMA cluster
 #pragma unroll
         for (int ks = 0; ks < BK; ks += MF::KD) {
             T a[CB], b[RB];
@@ -256,6 +257,7 @@ __device__ __forceinline__ void gemm_tile(int64_t M, int64_t N, int64_t K, T alp
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = MF::mma(a[cb], b[rb], acc[cb][rb]);
         }
+        __builtin_amdgcn_s_setprio(0);
         if (more) {
             masks(knext);
             lx.store(lds_x[cur ^ 1], t);
