@@ -241,6 +241,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                 bv[sw][cb] = src[((cbase + cb) * MF::BD + lo) * LDB + (kh * NKW + sw) * MF::KD + lk];
         // keep the scheduler from sinking the reads back next to their MFMAs
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(1);  // issue priority while in the MFMA cluster
 #pragma unroll
         for (int sw = 0; sw < NKW; ++sw) {
             const T a = ((okb >> sw) & 1u) ? af[sw] : T(0);
@@ -250,6 +251,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                 else acc[cb] = MF::mma(a, bv[sw][cb], acc[cb]);
             }
         }
+        __builtin_amdgcn_s_setprio(0);
     };
     auto write_block = [&](int ib, T alpha, T beta) {
         NLA_LDS T *dst = bs + ib * 64;
