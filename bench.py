@@ -12,7 +12,7 @@ Also reported on the same line:
                 `traffic` is its HBM bytes per launch from the committed
                 rocprofv3 FETCH_SIZE / WRITE_SIZE profile of this workload;
   cpu_baseline  the oracle (C restatement of the reference algorithm, OpenMP
-                over the box's host cores) on a bounded sample: the same A with
+                over the host's cores) on a bounded sample: the same A with
                 a subset of the right-hand sides;
   extra         FP32 and TRMM rates, the whole-operation roofline fraction, the
                 backward error and p99 of the timed solve, and two more legs:
@@ -448,7 +448,7 @@ def cpu_baseline(A_host, n, seconds_target=12.0):
         dtb = time.perf_counter() - t0
         out["openblas_tflops"] = n * n * m_s / dtb / 1e12
         out["openblas_vs_port_rel_diff"] = O.rel_error(X, Y)
-    except Exception as e:  # scipy missing on a box: report the port only
+    except Exception as e:  # scipy missing on this host: report the port only
         out["openblas_error"] = str(e)
     return out
 
@@ -466,8 +466,8 @@ def main():
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-GPU by design "
-              f"(sharding plan in DESIGN.md), so no multi-GPU figure is produced.", flush=True)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is single-GPU by design, "
+              f"so no multi-GPU figure is produced.", flush=True)
         sys.exit(2)
 
     import torch

@@ -11,7 +11,7 @@ for p in (ROOT, PKG):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (runs on the MI355X box)")
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (an MI355X)")
 
 
 def gpu_available():
@@ -24,7 +24,7 @@ def gpu_available():
 
 @pytest.fixture(scope="session")
 def nla():
-    """The product package; on a GPU box its library must load (no fallback)."""
+    """The product package; with a GPU present its library must load (no fallback)."""
     import nextla_amd
     nextla_amd.load()
     return nextla_amd
