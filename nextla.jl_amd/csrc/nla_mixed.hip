@@ -213,6 +213,7 @@ __device__ __forceinline__ void lp_tile(int64_t M, int64_t N, int64_t K, float a
         }
         const NLA_LDS E *sx_ = lds_x[cur];
         const NLA_LDS E *sy_ = lds_y[cur];
+        __builtin_amdgcn_s_setprio(1);  // issue priority while in the MFMA cluster
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 32) {
             OP a[CB], b[RB];
@@ -227,6 +228,7 @@ __device__ __forceinline__ void lp_tile(int64_t M, int64_t N, int64_t K, float a
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) acc[cb][rb] = LT::mma(a[cb], b[rb], acc[cb][rb]);
         }
+        __builtin_amdgcn_s_setprio(0);
         if (more) {
             if (CHECK) {
                 lx.apply_bounds();
