@@ -200,6 +200,70 @@ __global__ void laswp_kernel(int64_t ncols, T *__restrict__ A, int64_t lda, int6
     }
 }
 
+// The same interchanges as one row permutation, for blocks whose rows fit
+// the work-group's LDS.  Positions are taken relative to b = k1 - 1; swap k
+// (0-based) exchanges positions k and p_k >= k.  Every work-group composes
+// the K swaps into src (the final content of position x is the old content
+// of position src[x]) with one thread walking them in LDS, lists the
+// positions past K that the swaps moved, and then permutes its columns: each
+// thread loads up to LASWP_R moved entries of a column, a barrier, and the
+// stores.  The sequential part runs at LDS latency instead of one dependent
+// round trip to memory per swap and column.
+constexpr int LASWP_THREADS = 1024, LASWP_R = 16;
+constexpr int LASWP_LDS = 40960 - 64;  // ints: src map (span) + pivots / moved list (K)
+
+template <typename T>
+__global__ __launch_bounds__(LASWP_THREADS) void laswp_perm_kernel(int64_t ncols, T *__restrict__ A, int64_t lda,
+                                                                   int64_t b, int K, int span,
+                                                                   const int64_t *__restrict__ ipiv, int64_t r0) {
+    __shared__ int s_map[LASWP_LDS];
+    __shared__ int s_moved;
+    int *src = s_map, *lst = s_map + span;
+    const int t = threadIdx.x;
+    for (int x = t; x < span; x += LASWP_THREADS) src[x] = x;
+    for (int k = t; k < K; k += LASWP_THREADS) {
+        const int64_t p = ipiv[b + k] - 1 - r0 - b;
+        lst[k] = (p >= k && p < span) ? (int)p : k;  // pivots lie in [k, span)
+    }
+    if (t == 0) s_moved = 0;
+    __syncthreads();
+    if (t == 0) {
+        for (int k = 0; k < K; ++k) {
+            const int p = lst[k];
+            if (p != k) {
+                const int u = src[k];
+                src[k] = src[p];
+                src[p] = u;
+            }
+        }
+    }
+    __syncthreads();
+    // positions past K hold their own rows unless a swap moved them
+    for (int x = K + t; x < span; x += LASWP_THREADS)
+        if (src[x] != x) lst[atomicAdd(&s_moved, 1)] = x;
+    __syncthreads();
+    const int tot = K + s_moved;
+    for (int64_t c = blockIdx.x; c < ncols; c += gridDim.x) {
+        T *col = A + c * lda + b;
+        T v[LASWP_R];
+#pragma unroll
+        for (int r = 0; r < LASWP_R; ++r) {
+            const int i = t + r * LASWP_THREADS;
+            const int x = i < K ? i : (i < tot ? lst[i - K] : 0);
+            v[r] = (i < tot) ? col[src[x]] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < LASWP_R; ++r) {
+            const int i = t + r * LASWP_THREADS;
+            if (i < tot) {
+                const int x = i < K ? i : lst[i - K];
+                if (src[x] != x) col[x] = v[r];
+            }
+        }
+    }
+}
+
 // A12 = L11^-1 A12 for a unit lower L11 of n1 <= SMALL_TRSM rows (the
 // reference's trsm!('L','L','N','U'), src/lu.jl:277): forward substitution,
 // one thread per column of A12 holding it in registers; L11 is read through
@@ -246,10 +310,19 @@ struct LuRec {
         return st == NLA_OK;
     }
 
-    void laswp(T *A, int64_t ncols, int64_t k1, int64_t k2, const int64_t *ipiv, int64_t r0) {
+    // interchanges k1..k2 of the m-row block at A on ncols columns: the LDS
+    // permutation when its map fits, else the sequential walk
+    void laswp(T *A, int64_t m, int64_t ncols, int64_t k1, int64_t k2, const int64_t *ipiv, int64_t r0) {
         if (ncols <= 0 || k2 < k1 || st != NLA_OK) return;
-        hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, s, ncols, A, lda, k1,
-                           k2, ipiv, r0);
+        const int64_t K = k2 - k1 + 1, b = k1 - 1, span = m - b;
+        if (2 * K <= (int64_t)LASWP_R * LASWP_THREADS && span + K <= LASWP_LDS) {
+            const unsigned g = (unsigned)(ncols < 256 ? ncols : 256);
+            hipLaunchKernelGGL(laswp_perm_kernel<T>, dim3(g), dim3(LASWP_THREADS), 0, s, ncols, A, lda, b, (int)K,
+                               (int)span, ipiv, r0);
+        } else {
+            hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, s, ncols, A, lda, k1,
+                               k2, ipiv, r0);
+        }
         ok(hipGetLastError(), "laswp");
     }
 
@@ -267,7 +340,7 @@ struct LuRec {
         }
         const int64_t mn = m < n ? m : n, n1 = mn / 2, n2 = n - n1;
         rec(m, n1, A, ipiv, r0);
-        laswp(A + n1 * lda, n2, 1, n1, ipiv, r0);
+        laswp(A + n1 * lda, m, n2, 1, n1, ipiv, r0);
         if (st != NLA_OK) return;
         // A12 = L11^-1 A12 (unit lower; the identity for one column)
         if (n1 > 1 && n1 <= SMALL_TRSM) {
@@ -291,7 +364,7 @@ struct LuRec {
             }
         }
         rec(m - n1, n2, A + n1 + n1 * lda, ipiv + n1, r0 + n1);
-        laswp(A, n1, n1 + 1, mn, ipiv, r0);
+        laswp(A, m, n1, n1 + 1, mn, ipiv, r0);
     }
 };
 
