@@ -184,7 +184,9 @@ int32_t nla_trmm(char side, char uplo, char transa, char diag, int32_t dtype, do
  *   ipiv  device array of min(m,n) int64: row i was interchanged with row
  *         ipiv[i] (1-based, as the reference);
  *   info  device int64: 0, or the 1-based index of the first exactly zero
- *         pivot (the factorization still completes, as LAPACK's).
+ *         pivot (the factorization still completes, as LAPACK's); -1 if a
+ *         leaf spread over several work-groups (NLA_CFG_LU_SPLIT_ROWS) timed
+ *         out waiting for the others, when the factors are not valid.
  * Enqueued on `stream`, no host synchronization.  Differences from the
  * reference: info is returned (the reference's Int argument cannot carry
  * it out), and the factorization does not stop at a zero pivot.
@@ -349,6 +351,10 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      Bit 4 (0x10): systems above 512 rows are solved in
  *                      512-row stages (panels of the 512 class plus a grouped
  *                      GEMM) instead of as one 1024-row panel.
+ *   NLA_CFG_LU_SPLIT_ROWS: nla_getrf2 leaves (blocks of up to 8 columns) of at
+ *                      least this many rows are spread over one work-group
+ *                      per 2048 rows, up to 131072 rows (default 4096;
+ *                      0 = one work-group per leaf).
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {
@@ -359,7 +365,8 @@ enum nla_config_key {
     NLA_CFG_PANEL_STRIP_F32 = 4,
     NLA_CFG_PANEL_STRIP_F64 = 5,
     NLA_CFG_BATCH_CLASSES = 6,
-    NLA_CFG_PANEL_WAVES = 7
+    NLA_CFG_PANEL_WAVES = 7,
+    NLA_CFG_LU_SPLIT_ROWS = 8
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);

@@ -128,6 +128,9 @@ static std::atomic<int> g_panel_waves{0};
 static std::atomic<int> g_batch_classes{0x1F};
 
 int batch_class_mask() { return g_batch_classes.load(); }
+// Rows from which an LU leaf spreads over several work-groups (0 = never).
+static std::atomic<int> g_lu_split_rows{4096};
+int lu_split_rows() { return g_lu_split_rows.load(); }
 
 // Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),
 // 0 = register/LDS substitution kernel.
@@ -913,6 +916,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_BATCH_CLASSES) {
         if (value < 0 || value > 0x1F) return -1;
         return g_batch_classes.exchange(value | 0x8);
+    }
+    if (key == NLA_CFG_LU_SPLIT_ROWS) {
+        if (value < 0) return -1;
+        return g_lu_split_rows.exchange(value);
     }
     if (key == NLA_CFG_PANEL_WAVES) {
         if (value != 0 && value != 4 && value != 8) return -1;

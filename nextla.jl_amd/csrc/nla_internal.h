@@ -150,6 +150,7 @@ constexpr int BATCH_CLASSES[4] = {128, 256, 512, 1024};
 // Classes a new batch plan may use: bit c enables BATCH_CLASSES[c] (the
 // largest is always used); set with nla_set_config(NLA_CFG_BATCH_CLASSES).
 int batch_class_mask();
+int lu_split_rows();
 
 // Kernel entry of the persistent batched solver for one size class (for
 // occupancy queries); nullptr for an unknown class.

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <limits>
+#include <mutex>
+#include <vector>
 
 #include "../../include/nla.h"
 #include "nla_internal.h"
@@ -180,6 +182,222 @@ __global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, 
     }
 }
 
+// The same leaf for tall blocks, spread over G work-groups of SPLIT_ROWS
+// rows each, the block held in registers from the first load to the final
+// store.  Per column: each work-group finds its candidate (first index of
+// the largest |a| among its rows at or below the diagonal) and publishes it
+// with its row, the group holding row j publishes that row, and after one
+// grid barrier every work-group takes the same winner, makes the same
+// interchange in its registers and eliminates its own rows.  The published
+// values alternate between two buffers by column parity, so a work-group
+// one column ahead never overwrites what a slower one still reads.  The
+// launch is cooperative (all G groups resident); each wait is bounded, and
+// a wait that runs out sets info to -1 instead of hanging.  The last group
+// to finish resets the counters for the next launch on the stream.
+constexpr int SPLIT_THREADS = 256, SPLIT_RPT = 8, SPLIT_ROWS = SPLIT_THREADS * SPLIT_RPT;
+constexpr int SPLIT_MAX_WG = 64, SPLIT_SLOT = 16;  // 64-bit words per published candidate
+constexpr int SYNC_SLOTS = 64, SYNC_DIAG = SYNC_SLOTS + 2 * SPLIT_MAX_WG * SPLIT_SLOT;
+constexpr int SYNC_WORDS = SYNC_DIAG + 2 * SPLIT_SLOT;
+constexpr int64_t SPLIT_SPIN_LIMIT = int64_t(1) << 20;  // about a second of polling
+
+__device__ inline void st_agent(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t ld_agent(uint64_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline uint64_t dbits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ inline double bitsd(uint64_t v) { return __longlong_as_double((long long)v); }
+
+template <typename T>
+__global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m, int n, T *__restrict__ A,
+                                                                      int64_t lda, int64_t *__restrict__ ipiv,
+                                                                      int64_t *__restrict__ info, int64_t r0,
+                                                                      uint64_t *__restrict__ sync) {
+    constexpr int W = SPLIT_THREADS / 64;
+    __shared__ T s_val[W], s_sgn[W];
+    __shared__ int64_t s_row[W];
+    __shared__ T s_u[LEAF_COLS], s_o[LEAF_COLS];  // candidate / pivot row, row j before the interchange
+    __shared__ T s_a;
+    __shared__ int64_t s_p;
+    __shared__ int s_fail;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int G = (int)gridDim.x, g = (int)blockIdx.x;
+    const int64_t first = (int64_t)g * SPLIT_ROWS;  // this group's rows: [first, first + SPLIT_ROWS)
+    const int64_t base = first + t;
+    const int kmax = (int)(m < n ? m : n);
+    uint64_t *arrive = sync, *leave = sync + 1;
+    T a[SPLIT_RPT][LEAF_COLS];
+#pragma unroll
+    for (int r = 0; r < SPLIT_RPT; ++r) {
+        const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+#pragma unroll
+        for (int c = 0; c < LEAF_COLS; ++c) a[r][c] = (i < m && c < n) ? A[i + (int64_t)c * lda] : T(0);
+    }
+    if (t == 0) s_fail = 0;
+    bool failed = false;  // lane 0 of wave 0: a wait ran out; later ones are skipped
+#pragma unroll
+    for (int j = 0; j < LEAF_COLS; ++j) {
+        if (j >= kmax) break;
+        const int par = j & 1;
+        T bv = T(-1), bs = T(0);
+        int64_t bi = m;
+#pragma unroll
+        for (int r = 0; r < SPLIT_RPT; ++r) {
+            const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+            if (i >= j && i < m) better(bv, bs, bi, (T)fabs(a[r][j]), a[r][j], i);
+        }
+        wave_argmax(bv, bs, bi);
+        if (lane == 0) {
+            s_val[wv] = bv;
+            s_sgn[wv] = bs;
+            s_row[wv] = bi;
+        }
+        __syncthreads();
+        T pv = T(-1), ps = T(0);
+        int64_t pi = m;
+#pragma unroll
+        for (int w = 0; w < W; ++w) better(pv, ps, pi, s_val[w], s_sgn[w], s_row[w]);
+#pragma unroll
+        for (int r = 0; r < SPLIT_RPT; ++r) {
+            const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+            if (i == pi) {
+#pragma unroll
+                for (int c = 0; c < LEAF_COLS; ++c) s_u[c] = a[r][c];
+            }
+            if (i == j) {
+#pragma unroll
+                for (int c = 0; c < LEAF_COLS; ++c) s_o[c] = a[r][c];
+            }
+        }
+        __syncthreads();
+        if (wv == 0) {
+            uint64_t *slot = sync + SYNC_SLOTS + (par * SPLIT_MAX_WG + g) * SPLIT_SLOT;
+            if (lane == 0) {
+                st_agent(slot + 0, dbits((double)pv));
+                st_agent(slot + 1, dbits((double)ps));
+                st_agent(slot + 2, (uint64_t)pi);
+            }
+            if (lane < LEAF_COLS) {
+                st_agent(slot + 4 + lane, dbits(pi < m ? (double)s_u[lane] : 0.0));
+                if (j >= first && j < first + SPLIT_ROWS)
+                    st_agent(sync + SYNC_DIAG + par * SPLIT_SLOT + lane, dbits((double)s_o[lane]));
+            }
+            if (lane == 0) {
+                // grid barrier: arrival count reaches G * (j + 1)
+                __hip_atomic_fetch_add(arrive, (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t target = (uint64_t)G * (uint64_t)(j + 1);
+                int64_t spins = 0;
+                while (!failed && ld_agent(arrive) < target) {
+                    if (++spins > SPLIT_SPIN_LIMIT) {
+                        failed = true;
+                        s_fail = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            // every lane reads one group's candidate (and lanes < LEAF_COLS
+            // row j), then the wave reduces to the winner; ties go to the
+            // smaller row, as within a group
+            double cv = -1.0, cs = 0.0, crow[LEAF_COLS];
+            int64_t ci = m;
+            if (lane < G) {
+                uint64_t *sl = sync + SYNC_SLOTS + (par * SPLIT_MAX_WG + lane) * SPLIT_SLOT;
+                cv = bitsd(ld_agent(sl + 0));
+                cs = bitsd(ld_agent(sl + 1));
+                ci = (int64_t)ld_agent(sl + 2);
+#pragma unroll
+                for (int c = 0; c < LEAF_COLS; ++c) crow[c] = bitsd(ld_agent(sl + 4 + c));
+            } else {
+#pragma unroll
+                for (int c = 0; c < LEAF_COLS; ++c) crow[c] = 0.0;
+            }
+            const double rowj = lane < LEAF_COLS ? bitsd(ld_agent(sync + SYNC_DIAG + par * SPLIT_SLOT + lane)) : 0.0;
+            wave_argmax(cv, cs, ci);
+            // the winning candidate's lane is the group holding row ci
+            const int wl = ci < m ? (int)(ci / SPLIT_ROWS) : 0;
+            double u = 0.0;
+#pragma unroll
+            for (int c = 0; c < LEAF_COLS; ++c) {
+                const double x = __shfl(crow[c], wl, 64);
+                if (lane == c) u = x;
+            }
+            if (lane == 0) {
+                s_p = ci;
+                s_a = (T)cs;
+            }
+            if (lane < LEAF_COLS) {
+                s_u[lane] = (T)u;
+                s_o[lane] = (T)rowj;
+            }
+        }
+        __syncthreads();
+        int64_t p = s_p;
+        T av = s_a, u[LEAF_COLS], o[LEAF_COLS];
+#pragma unroll
+        for (int c = 0; c < LEAF_COLS; ++c) {
+            u[c] = s_u[c];
+            o[c] = s_o[c];
+        }
+        if (p >= m) {  // no finite maximum (all NaN): keep row j
+            p = j;
+            av = o[j];
+#pragma unroll
+            for (int c = 0; c < LEAF_COLS; ++c) u[c] = o[c];
+        }
+        if (g == 0 && t == 0) {
+            ipiv[j] = r0 + p + 1;
+            if (av == T(0) && *info == 0) *info = r0 + j + 1;
+        }
+        if (av != T(0)) {
+            const bool swap = p != j;
+            const bool recip = fabs(av) >= std::numeric_limits<T>::min();
+            const T rc = T(1) / av;
+#pragma unroll
+            for (int r = 0; r < SPLIT_RPT; ++r) {
+                const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+                if (swap && i == j) {
+#pragma unroll
+                    for (int c = 0; c < LEAF_COLS; ++c) a[r][c] = u[c];
+                } else if (swap && i == p) {
+#pragma unroll
+                    for (int c = 0; c < LEAF_COLS; ++c) a[r][c] = o[c];
+                }
+                if (i > j && i < m) {
+                    const T l = recip ? a[r][j] * rc : a[r][j] / av;
+                    a[r][j] = l;
+#pragma unroll
+                    for (int c = j + 1; c < LEAF_COLS; ++c) a[r][c] = a[r][c] - l * u[c];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < SPLIT_RPT; ++r) {
+        const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+        if (i < m) {
+#pragma unroll
+            for (int c = 0; c < LEAF_COLS; ++c)
+                if (c < n) A[i + (int64_t)c * lda] = a[r][c];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (s_fail) *info = -1;
+        // the last group out resets the counters (every group has passed
+        // every barrier by then)
+        const uint64_t prev = __hip_atomic_fetch_add(leave, (uint64_t)1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == (uint64_t)G - 1) {
+            st_agent(arrive, 0);
+            st_agent(leave, 0);
+        }
+    }
+}
+
+__global__ void sync_zero_kernel(uint64_t *p, int words) {
+    for (int i = threadIdx.x; i < words; i += blockDim.x) p[i] = 0;
+}
+
 // laswp (src/lu.jl:470): for k = k1..k2 (1-based) swap rows k and
 // ipiv[k] - r0 of each of the ncols columns (ipiv holds global rows, the
 // block's row 1 is global row r0 + 1); one thread per column walks the
@@ -297,6 +515,40 @@ __global__ void info_zero_kernel(int64_t *info) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *info = 0;
 }
 
+// Counters and published candidates of the split leaf, one buffer per
+// (device, stream): launches on one stream run in order, so they share it.
+struct SplitSync {
+    int dev;
+    hipStream_t stream;
+    uint64_t *ptr;
+};
+
+uint64_t *split_sync(hipStream_t s, int32_t *st) {
+    static std::mutex mu;
+    static std::vector<SplitSync> cache;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *st = report_hip(e, "lu split leaf device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> g(mu);
+    for (auto &c : cache)
+        if (c.dev == dev && c.stream == s) return c.ptr;
+    uint64_t *p = nullptr;
+    e = hipMalloc((void **)&p, SYNC_WORDS * sizeof(uint64_t));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(sync_zero_kernel, dim3(1), dim3(256), 0, s, p, SYNC_WORDS);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        *st = report_hip(e, "lu split leaf sync buffer");
+        return nullptr;
+    }
+    cache.push_back({dev, s, p});
+    return p;
+}
+
 template <typename T>
 struct LuRec {
     int32_t dtype;
@@ -304,6 +556,18 @@ struct LuRec {
     int64_t *info;
     hipStream_t s;
     int32_t st = NLA_OK;
+    int64_t split_rows = lu_split_rows();
+
+    // an m x n leaf (n <= LEAF_COLS) over ceil(m / SPLIT_ROWS) work-groups
+    void split_leaf(int64_t m, int n, T *A, int64_t *ipiv, int64_t r0) {
+        uint64_t *sync = split_sync(s, &st);
+        if (!sync) return;
+        int64_t lda_ = lda, *info_ = info;
+        void *args[] = {&m, &n, &A, &lda_, &ipiv, &info_, &r0, &sync};
+        const unsigned g = (unsigned)((m + SPLIT_ROWS - 1) / SPLIT_ROWS);
+        ok(hipLaunchCooperativeKernel((const void *)lu_leaf_split_kernel<T>, dim3(g), dim3(SPLIT_THREADS), args, 0, s),
+           "lu split leaf");
+    }
 
     bool ok(hipError_t e, const char *where) {
         if (e != hipSuccess && st == NLA_OK) st = report_hip(e, where);
@@ -332,6 +596,10 @@ struct LuRec {
     // reference's "ipiv[i] += n1" (src/lu.jl:289-291) makes of them.
     void rec(int64_t m, int64_t n, T *A, int64_t *ipiv, int64_t r0) {
         if (st != NLA_OK || m == 0 || n == 0) return;
+        if (n <= LEAF_COLS && split_rows > 0 && m >= split_rows && m <= (int64_t)SPLIT_ROWS * SPLIT_MAX_WG) {
+            split_leaf(m, (int)n, A, ipiv, r0);
+            return;
+        }
         if (n <= LEAF_COLS || m == 1) {
             const int nl = (int)(n <= LEAF_COLS ? n : 1);  // m == 1: only the first column is examined
             hipLaunchKernelGGL(lu_leaf_kernel<T>, dim3(1), dim3(PIV_THREADS), 0, s, m, nl, A, lda, ipiv, info, r0);

@@ -240,7 +240,10 @@ def getrf2_(A, ipiv=None):
     info = torch.zeros(1, dtype=torch.int64, device=A.device)
     st = load().nla_getrf2(_DT[A.dtype], m, n, A.data_ptr(), lda, ipiv.data_ptr(), info.data_ptr(), _stream(A))
     _lib.check(st, "getrf2!")
-    return A, ipiv[:k], int(info.item())
+    inf = int(info.item())
+    if inf < 0:
+        raise _lib.NLAError("getrf2!: a multi-work-group leaf timed out waiting at its grid barrier")
+    return A, ipiv[:k], inf
 
 
 def lauum_(uplo, n, A, ib=None, workspace=None):

@@ -87,6 +87,7 @@ NLA_CFG_PANEL_STRIP_F32 = 4
 NLA_CFG_PANEL_STRIP_F64 = 5
 NLA_CFG_BATCH_CLASSES = 6
 NLA_CFG_PANEL_WAVES = 7
+NLA_CFG_LU_SPLIT_ROWS = 8
 
 
 def set_config(key, value):

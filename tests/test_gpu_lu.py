@@ -107,3 +107,68 @@ def test_large_fp64_self_certified(nla):
     err = (torch.linalg.norm(R - PA) / torch.linalg.norm(A0)).item()
     assert err <= 1e-13, err
     assert torch.max(torch.abs(torch.tril(LU, -1))).item() <= 1.0
+
+
+@pytest.fixture
+def split_rows():
+    """Set the row count from which leaves spread over several work-groups;
+    restores the previous value."""
+    from nextla_amd import _lib
+    olds = []
+
+    def set_(v):
+        olds.append(_lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, v))
+    yield set_
+    if olds:
+        _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, olds[0])
+
+
+@pytest.mark.parametrize("m,n", SHAPES + [(2049, 8), (4097, 20)])
+def test_split_leaf_matches_oracle(nla, split_rows, m, n):
+    """Every leaf through the multi-work-group kernel (threshold 1 row): the
+    same pivots as the oracle and factors equal to rounding, for the shapes
+    above and blocks with a partial last work-group."""
+    split_rows(1)
+    rng = np.random.default_rng(m * 7919 + n)
+    A = rng.standard_normal((m, n))
+    LU, ipiv, info = nla.getrf2_(dev(A))
+    ref, ref_piv, ref_info = O.getrf2(A)
+    assert info == ref_info == 0
+    assert np.array_equal(host(ipiv), ref_piv), (m, n)
+    assert np.max(np.abs(host(LU) - ref)) <= 1e-11 * max(1.0, np.max(np.abs(ref))), (m, n)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("m,n", [(131072, 8), (131073, 8), (6000, 8), (8192, 64)])
+def test_split_leaf_equals_single_group_leaf(nla, split_rows, dtype, m, n):
+    """Tall leaves up to the 64-work-group limit (131072 rows; one more row
+    takes the one-work-group kernel): the split kernel gives the same pivots
+    and the same factors as the one-work-group leaf on the same input."""
+    g = torch.Generator(device=DEV)
+    g.manual_seed(m + n)
+    A0 = torch.empty((n, m), dtype=dtype, device=DEV).normal_(generator=g).t()
+    split_rows(0)
+    L1, p1, i1 = nla.getrf2_(A0.clone())
+    split_rows(1)
+    L2, p2, i2 = nla.getrf2_(A0.clone())
+    assert i1 == i2 == 0
+    assert torch.equal(p1, p2)
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert torch.max(torch.abs(L1 - L2)).item() <= tol * torch.max(torch.abs(L1)).item()
+
+
+def test_split_leaf_singular_and_padded(nla, split_rows):
+    """A zero column and lda > m through the split kernel: info, pivots and
+    factors as the oracle's, the rows past m untouched."""
+    split_rows(1)
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((3000, 12))
+    A[:, 2] = 0.0
+    big = torch.full((12, 3100), 7.0, dtype=torch.float64, device=DEV).t()
+    big[:3000].copy_(torch.from_numpy(A).to(DEV))
+    LU, ipiv, info = nla.getrf2_(big[:3000])
+    ref, ref_piv, ref_info = O.getrf2(A)
+    assert info == ref_info == 3
+    assert np.array_equal(host(ipiv), ref_piv)
+    assert np.max(np.abs(host(LU) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert torch.all(big[3000:] == 7.0)
