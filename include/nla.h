@@ -352,9 +352,11 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      512-row stages (panels of the 512 class plus a grouped
  *                      GEMM) instead of as one 1024-row panel.
  *   NLA_CFG_LU_SPLIT_ROWS: nla_getrf2 leaves (blocks of up to 8 columns) of at
- *                      least this many rows are spread over one work-group
- *                      per 2048 rows, up to 131072 rows (default 4096;
- *                      0 = one work-group per leaf).
+ *                      least this many rows are held in registers: one
+ *                      work-group up to 4096 rows, above that one per 2048
+ *                      rows up to 131072 rows (default 1: every leaf up to
+ *                      that size; 0 = the one-work-group leaf that works in
+ *                      global memory, for every size).
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {

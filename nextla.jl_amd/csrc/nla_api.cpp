@@ -128,8 +128,9 @@ static std::atomic<int> g_panel_waves{0};
 static std::atomic<int> g_batch_classes{0x1F};
 
 int batch_class_mask() { return g_batch_classes.load(); }
-// Rows from which an LU leaf spreads over several work-groups (0 = never).
-static std::atomic<int> g_lu_split_rows{4096};
+// Rows from which an LU leaf is held in registers (one work-group up to 4096
+// rows, several above); 0 = the global-memory leaf for every size.
+static std::atomic<int> g_lu_split_rows{1};
 int lu_split_rows() { return g_lu_split_rows.load(); }
 
 // Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),

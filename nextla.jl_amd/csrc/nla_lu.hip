@@ -9,7 +9,9 @@
 // A22 -= A21 A12, factor A22 and apply its interchanges to the left block.
 // Blocks of up to 8 columns are one kernel (the same eliminations with
 // rank-1 updates in place of the GEMMs of that width), which keeps the
-// launch count near 5 per 8 columns instead of 5 per column.  Pivots are
+// launch count near 5 per 8 columns instead of 5 per column.  Such a leaf
+// is held in registers: by one work-group up to 4096 rows, by one
+// work-group per 2048 rows with a grid barrier per column above that.  Pivots are
 // kept as global rows throughout (the reference shifts them by n1 after
 // each recursive call).  Everything is enqueued on the caller's stream;
 // pivots and info stay on the device.
@@ -194,7 +196,11 @@ __global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, 
 // launch is cooperative (all G groups resident); each wait is bounded, and
 // a wait that runs out sets info to -1 instead of hanging.  The last group
 // to finish resets the counters for the next launch on the stream.
+// With ONE (a single work-group) the group's own winner is the pivot and
+// nothing is published: the whole leaf stays in registers with two
+// work-group barriers per column and no memory round trip.
 constexpr int SPLIT_THREADS = 256, SPLIT_RPT = 8, SPLIT_ROWS = SPLIT_THREADS * SPLIT_RPT;
+constexpr int ONE_THREADS = 512, ONE_RPT = 8, ONE_ROWS = ONE_THREADS * ONE_RPT;
 constexpr int SPLIT_MAX_WG = 64, SPLIT_SLOT = 16;  // 64-bit words per published candidate
 constexpr int SYNC_SLOTS = 64, SYNC_DIAG = SYNC_SLOTS + 2 * SPLIT_MAX_WG * SPLIT_SLOT;
 constexpr int SYNC_WORDS = SYNC_DIAG + 2 * SPLIT_SLOT;
@@ -207,12 +213,11 @@ __device__ inline uint64_t ld_agent(uint64_t *p) { return __hip_atomic_load(p, _
 __device__ inline uint64_t dbits(double v) { return (uint64_t)__double_as_longlong(v); }
 __device__ inline double bitsd(uint64_t v) { return __longlong_as_double((long long)v); }
 
-template <typename T>
-__global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m, int n, T *__restrict__ A,
-                                                                      int64_t lda, int64_t *__restrict__ ipiv,
-                                                                      int64_t *__restrict__ info, int64_t r0,
-                                                                      uint64_t *__restrict__ sync) {
-    constexpr int W = SPLIT_THREADS / 64;
+template <typename T, int NT, int RPT, bool ONE>
+__global__ __launch_bounds__(NT) void lu_leaf_reg_kernel(int64_t m, int n, T *__restrict__ A, int64_t lda,
+                                                         int64_t *__restrict__ ipiv, int64_t *__restrict__ info,
+                                                         int64_t r0, uint64_t *__restrict__ sync) {
+    constexpr int W = NT / 64, ROWS = NT * RPT;
     __shared__ T s_val[W], s_sgn[W];
     __shared__ int64_t s_row[W];
     __shared__ T s_u[LEAF_COLS], s_o[LEAF_COLS];  // candidate / pivot row, row j before the interchange
@@ -221,14 +226,14 @@ __global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m,
     __shared__ int s_fail;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int G = (int)gridDim.x, g = (int)blockIdx.x;
-    const int64_t first = (int64_t)g * SPLIT_ROWS;  // this group's rows: [first, first + SPLIT_ROWS)
+    const int64_t first = (int64_t)g * ROWS;  // this group's rows: [first, first + ROWS)
     const int64_t base = first + t;
     const int kmax = (int)(m < n ? m : n);
     uint64_t *arrive = sync, *leave = sync + 1;
-    T a[SPLIT_RPT][LEAF_COLS];
+    T a[RPT][LEAF_COLS];
 #pragma unroll
-    for (int r = 0; r < SPLIT_RPT; ++r) {
-        const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+    for (int r = 0; r < RPT; ++r) {
+        const int64_t i = base + (int64_t)r * NT;
 #pragma unroll
         for (int c = 0; c < LEAF_COLS; ++c) a[r][c] = (i < m && c < n) ? A[i + (int64_t)c * lda] : T(0);
     }
@@ -241,8 +246,8 @@ __global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m,
         T bv = T(-1), bs = T(0);
         int64_t bi = m;
 #pragma unroll
-        for (int r = 0; r < SPLIT_RPT; ++r) {
-            const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+        for (int r = 0; r < RPT; ++r) {
+            const int64_t i = base + (int64_t)r * NT;
             if (i >= j && i < m) better(bv, bs, bi, (T)fabs(a[r][j]), a[r][j], i);
         }
         wave_argmax(bv, bs, bi);
@@ -257,8 +262,8 @@ __global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m,
 #pragma unroll
         for (int w = 0; w < W; ++w) better(pv, ps, pi, s_val[w], s_sgn[w], s_row[w]);
 #pragma unroll
-        for (int r = 0; r < SPLIT_RPT; ++r) {
-            const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+        for (int r = 0; r < RPT; ++r) {
+            const int64_t i = base + (int64_t)r * NT;
             if (i == pi) {
 #pragma unroll
                 for (int c = 0; c < LEAF_COLS; ++c) s_u[c] = a[r][c];
@@ -269,71 +274,76 @@ __global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m,
             }
         }
         __syncthreads();
-        if (wv == 0) {
-            uint64_t *slot = sync + SYNC_SLOTS + (par * SPLIT_MAX_WG + g) * SPLIT_SLOT;
-            if (lane == 0) {
-                st_agent(slot + 0, dbits((double)pv));
-                st_agent(slot + 1, dbits((double)ps));
-                st_agent(slot + 2, (uint64_t)pi);
-            }
-            if (lane < LEAF_COLS) {
-                st_agent(slot + 4 + lane, dbits(pi < m ? (double)s_u[lane] : 0.0));
-                if (j >= first && j < first + SPLIT_ROWS)
-                    st_agent(sync + SYNC_DIAG + par * SPLIT_SLOT + lane, dbits((double)s_o[lane]));
-            }
-            if (lane == 0) {
-                // grid barrier: arrival count reaches G * (j + 1)
-                __hip_atomic_fetch_add(arrive, (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t target = (uint64_t)G * (uint64_t)(j + 1);
-                int64_t spins = 0;
-                while (!failed && ld_agent(arrive) < target) {
-                    if (++spins > SPLIT_SPIN_LIMIT) {
-                        failed = true;
-                        s_fail = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
+        int64_t p = pi;
+        T av = ps;
+        if constexpr (!ONE) {
+            if (wv == 0) {
+                uint64_t *slot = sync + SYNC_SLOTS + (par * SPLIT_MAX_WG + g) * SPLIT_SLOT;
+                if (lane == 0) {
+                    st_agent(slot + 0, dbits((double)pv));
+                    st_agent(slot + 1, dbits((double)ps));
+                    st_agent(slot + 2, (uint64_t)pi);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            // every lane reads one group's candidate (and lanes < LEAF_COLS
-            // row j), then the wave reduces to the winner; ties go to the
-            // smaller row, as within a group
-            double cv = -1.0, cs = 0.0, crow[LEAF_COLS];
-            int64_t ci = m;
-            if (lane < G) {
-                uint64_t *sl = sync + SYNC_SLOTS + (par * SPLIT_MAX_WG + lane) * SPLIT_SLOT;
-                cv = bitsd(ld_agent(sl + 0));
-                cs = bitsd(ld_agent(sl + 1));
-                ci = (int64_t)ld_agent(sl + 2);
+                if (lane < LEAF_COLS) {
+                    st_agent(slot + 4 + lane, dbits(pi < m ? (double)s_u[lane] : 0.0));
+                    if (j >= first && j < first + ROWS)
+                        st_agent(sync + SYNC_DIAG + par * SPLIT_SLOT + lane, dbits((double)s_o[lane]));
+                }
+                if (lane == 0) {
+                    // grid barrier: arrival count reaches G * (j + 1)
+                    __hip_atomic_fetch_add(arrive, (uint64_t)1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t target = (uint64_t)G * (uint64_t)(j + 1);
+                    int64_t spins = 0;
+                    while (!failed && ld_agent(arrive) < target) {
+                        if (++spins > SPLIT_SPIN_LIMIT) {
+                            failed = true;
+                            s_fail = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                }
+                // every lane reads one group's candidate (and lanes < LEAF_COLS
+                // row j), then the wave reduces to the winner; ties go to the
+                // smaller row, as within a group
+                double cv = -1.0, cs = 0.0, crow[LEAF_COLS];
+                int64_t ci = m;
+                if (lane < G) {
+                    uint64_t *sl = sync + SYNC_SLOTS + (par * SPLIT_MAX_WG + lane) * SPLIT_SLOT;
+                    cv = bitsd(ld_agent(sl + 0));
+                    cs = bitsd(ld_agent(sl + 1));
+                    ci = (int64_t)ld_agent(sl + 2);
 #pragma unroll
-                for (int c = 0; c < LEAF_COLS; ++c) crow[c] = bitsd(ld_agent(sl + 4 + c));
-            } else {
+                    for (int c = 0; c < LEAF_COLS; ++c) crow[c] = bitsd(ld_agent(sl + 4 + c));
+                } else {
 #pragma unroll
-                for (int c = 0; c < LEAF_COLS; ++c) crow[c] = 0.0;
-            }
-            const double rowj = lane < LEAF_COLS ? bitsd(ld_agent(sync + SYNC_DIAG + par * SPLIT_SLOT + lane)) : 0.0;
-            wave_argmax(cv, cs, ci);
-            // the winning candidate's lane is the group holding row ci
-            const int wl = ci < m ? (int)(ci / SPLIT_ROWS) : 0;
-            double u = 0.0;
+                    for (int c = 0; c < LEAF_COLS; ++c) crow[c] = 0.0;
+                }
+                const double rowj = lane < LEAF_COLS ? bitsd(ld_agent(sync + SYNC_DIAG + par * SPLIT_SLOT + lane)) : 0.0;
+                wave_argmax(cv, cs, ci);
+                // the winning candidate's lane is the group holding row ci
+                const int wl = ci < m ? (int)(ci / ROWS) : 0;
+                double u = 0.0;
 #pragma unroll
-            for (int c = 0; c < LEAF_COLS; ++c) {
-                const double x = __shfl(crow[c], wl, 64);
-                if (lane == c) u = x;
+                for (int c = 0; c < LEAF_COLS; ++c) {
+                    const double x = __shfl(crow[c], wl, 64);
+                    if (lane == c) u = x;
+                }
+                if (lane == 0) {
+                    s_p = ci;
+                    s_a = (T)cs;
+                }
+                if (lane < LEAF_COLS) {
+                    s_u[lane] = (T)u;
+                    s_o[lane] = (T)rowj;
+                }
             }
-            if (lane == 0) {
-                s_p = ci;
-                s_a = (T)cs;
-            }
-            if (lane < LEAF_COLS) {
-                s_u[lane] = (T)u;
-                s_o[lane] = (T)rowj;
-            }
+            __syncthreads();
+            p = s_p;
+            av = s_a;
         }
-        __syncthreads();
-        int64_t p = s_p;
-        T av = s_a, u[LEAF_COLS], o[LEAF_COLS];
+        T u[LEAF_COLS], o[LEAF_COLS];
 #pragma unroll
         for (int c = 0; c < LEAF_COLS; ++c) {
             u[c] = s_u[c];
@@ -354,8 +364,8 @@ __global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m,
             const bool recip = fabs(av) >= std::numeric_limits<T>::min();
             const T rc = T(1) / av;
 #pragma unroll
-            for (int r = 0; r < SPLIT_RPT; ++r) {
-                const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+            for (int r = 0; r < RPT; ++r) {
+                const int64_t i = base + (int64_t)r * NT;
                 if (swap && i == j) {
 #pragma unroll
                     for (int c = 0; c < LEAF_COLS; ++c) a[r][c] = u[c];
@@ -373,14 +383,15 @@ __global__ __launch_bounds__(SPLIT_THREADS) void lu_leaf_split_kernel(int64_t m,
         }
     }
 #pragma unroll
-    for (int r = 0; r < SPLIT_RPT; ++r) {
-        const int64_t i = base + (int64_t)r * SPLIT_THREADS;
+    for (int r = 0; r < RPT; ++r) {
+        const int64_t i = base + (int64_t)r * NT;
         if (i < m) {
 #pragma unroll
             for (int c = 0; c < LEAF_COLS; ++c)
                 if (c < n) A[i + (int64_t)c * lda] = a[r][c];
         }
     }
+    if constexpr (ONE) return;
     __syncthreads();
     if (t == 0) {
         if (s_fail) *info = -1;
@@ -565,7 +576,8 @@ struct LuRec {
         int64_t lda_ = lda, *info_ = info;
         void *args[] = {&m, &n, &A, &lda_, &ipiv, &info_, &r0, &sync};
         const unsigned g = (unsigned)((m + SPLIT_ROWS - 1) / SPLIT_ROWS);
-        ok(hipLaunchCooperativeKernel((const void *)lu_leaf_split_kernel<T>, dim3(g), dim3(SPLIT_THREADS), args, 0, s),
+        ok(hipLaunchCooperativeKernel((const void *)lu_leaf_reg_kernel<T, SPLIT_THREADS, SPLIT_RPT, false>, dim3(g),
+                                      dim3(SPLIT_THREADS), args, 0, s),
            "lu split leaf");
     }
 
@@ -597,7 +609,13 @@ struct LuRec {
     void rec(int64_t m, int64_t n, T *A, int64_t *ipiv, int64_t r0) {
         if (st != NLA_OK || m == 0 || n == 0) return;
         if (n <= LEAF_COLS && split_rows > 0 && m >= split_rows && m <= (int64_t)SPLIT_ROWS * SPLIT_MAX_WG) {
-            split_leaf(m, (int)n, A, ipiv, r0);
+            if (m <= ONE_ROWS) {
+                hipLaunchKernelGGL((lu_leaf_reg_kernel<T, ONE_THREADS, ONE_RPT, true>), dim3(1), dim3(ONE_THREADS), 0, s,
+                                   m, (int)n, A, lda, ipiv, info, r0, (uint64_t *)nullptr);
+                ok(hipGetLastError(), "lu register leaf");
+            } else {
+                split_leaf(m, (int)n, A, ipiv, r0);
+            }
             return;
         }
         if (n <= LEAF_COLS || m == 1) {
