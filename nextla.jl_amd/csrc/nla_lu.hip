@@ -192,10 +192,13 @@ __global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, 
 // grid barrier every work-group takes the same winner, makes the same
 // interchange in its registers and eliminates its own rows.  The published
 // values alternate between two buffers by column parity, so a work-group
-// one column ahead never overwrites what a slower one still reads.  The
-// launch is cooperative (all G groups resident); each wait is bounded, and
-// a wait that runs out sets info to -1 instead of hanging.  The last group
-// to finish resets the counters for the next launch on the stream.
+// one column ahead never overwrites what a slower one still reads.  At most
+// 64 groups of one CU each are launched, which an idle part of the chip
+// holds at once; a cooperative launch would guarantee it, but costs about
+// 150 us once other streams have been used.  Each wait is bounded, so a
+// group that is not resident cannot hang the others: a wait that runs out
+// sets info to -1.  The last group to finish resets the counters for the
+// next launch on the stream.
 // With ONE (a single work-group) the group's own winner is the pivot and
 // nothing is published: the whole leaf stays in registers with two
 // work-group barriers per column and no memory round trip.
@@ -573,12 +576,10 @@ struct LuRec {
     void split_leaf(int64_t m, int n, T *A, int64_t *ipiv, int64_t r0) {
         uint64_t *sync = split_sync(s, &st);
         if (!sync) return;
-        int64_t lda_ = lda, *info_ = info;
-        void *args[] = {&m, &n, &A, &lda_, &ipiv, &info_, &r0, &sync};
         const unsigned g = (unsigned)((m + SPLIT_ROWS - 1) / SPLIT_ROWS);
-        ok(hipLaunchCooperativeKernel((const void *)lu_leaf_reg_kernel<T, SPLIT_THREADS, SPLIT_RPT, false>, dim3(g),
-                                      dim3(SPLIT_THREADS), args, 0, s),
-           "lu split leaf");
+        hipLaunchKernelGGL((lu_leaf_reg_kernel<T, SPLIT_THREADS, SPLIT_RPT, false>), dim3(g), dim3(SPLIT_THREADS), 0, s,
+                           m, n, A, lda, ipiv, info, r0, sync);
+        ok(hipGetLastError(), "lu split leaf");
     }
 
     bool ok(hipError_t e, const char *where) {
