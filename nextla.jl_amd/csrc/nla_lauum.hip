@@ -25,7 +25,12 @@ namespace {
 constexpr int LAUUM_LEAF = 64;
 
 // One diagonal block of nb <= 64 rows: lower R(i,j) = sum_{k>=i} L(k,i) L(k,j)
-// for i >= j; upper R(i,j) = sum_{k>=j} U(i,k) U(j,k) for i <= j.
+// for i >= j; upper R(i,j) = sum_{k>=j} U(i,k) U(j,k) for i <= j.  The
+// triangle goes to LDS with zeros elsewhere, so every sum can run over all k
+// from the block's first row; each thread owns a 4 x 4 block of R (the
+// threads whose block lies outside the triangle only help load).
+constexpr int LAUUM_RB = 4, LAUUM_NB = LAUUM_LEAF / LAUUM_RB;  // 16 x 16 register blocks
+
 template <typename T>
 __global__ __launch_bounds__(256) void lauum_leaf_kernel(int nb, T *__restrict__ A, int64_t lda, int lower) {
     __shared__ T s[LAUUM_LEAF * (LAUUM_LEAF + 1)];
@@ -37,17 +42,36 @@ __global__ __launch_bounds__(256) void lauum_leaf_kernel(int nb, T *__restrict__
         s[j * LD + i] = in ? A[i + (int64_t)j * lda] : T(0);
     }
     __syncthreads();
-    for (int e = t; e < LAUUM_LEAF * LAUUM_LEAF; e += 256) {
-        const int i = e % LAUUM_LEAF, j = e / LAUUM_LEAF;
-        if (i >= nb || j >= nb || (lower ? i < j : i > j)) continue;
-        T acc = T(0);
-        if (lower) {
-            for (int k = i; k < nb; ++k) acc += s[i * LD + k] * s[j * LD + k];  // L(k,i) L(k,j)
-        } else {
-            for (int k = j; k < nb; ++k) acc += s[k * LD + i] * s[k * LD + j];  // U(i,k) U(j,k)
+    const int bi = t % LAUUM_NB, bj = t / LAUUM_NB;  // rows 4 bi.., columns 4 bj..
+    const int i0 = bi * LAUUM_RB, j0 = bj * LAUUM_RB;
+    if ((lower ? bi < bj : bi > bj) || i0 >= nb || j0 >= nb) return;
+    T acc[LAUUM_RB][LAUUM_RB];
+#pragma unroll
+    for (int a = 0; a < LAUUM_RB; ++a)
+#pragma unroll
+        for (int b = 0; b < LAUUM_RB; ++b) acc[a][b] = T(0);
+    // lower: L(k, i) = s[i * LD + k], zero for k < i; upper: U(i, k) =
+    // s[k * LD + i], zero for k < i
+    const int k0 = lower ? i0 : j0;
+    for (int k = k0; k < nb; ++k) {
+        T x[LAUUM_RB], y[LAUUM_RB];
+#pragma unroll
+        for (int a = 0; a < LAUUM_RB; ++a) {
+            x[a] = lower ? s[(i0 + a) * LD + k] : s[k * LD + i0 + a];
+            y[a] = lower ? s[(j0 + a) * LD + k] : s[k * LD + j0 + a];
         }
-        A[i + (int64_t)j * lda] = acc;
+#pragma unroll
+        for (int a = 0; a < LAUUM_RB; ++a)
+#pragma unroll
+            for (int b = 0; b < LAUUM_RB; ++b) acc[a][b] += x[a] * y[b];
     }
+#pragma unroll
+    for (int a = 0; a < LAUUM_RB; ++a)
+#pragma unroll
+        for (int b = 0; b < LAUUM_RB; ++b) {
+            const int i = i0 + a, j = j0 + b;
+            if (i < nb && j < nb && (lower ? i >= j : i <= j)) A[i + (int64_t)j * lda] = acc[a][b];
+        }
 }
 
 // C(i,j) += W(i,j) on C's lower (or upper) triangle; W n x n, leading dim n.
