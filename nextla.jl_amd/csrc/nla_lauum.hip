@@ -9,7 +9,7 @@
 //   L21 := L22^T L21             TRMM of the hot path (nla_rectrxm 'L','L','T')
 //   L22 := L22^T L22             (recursively)
 // and the mirror image for upper (U11 += U12 U12^T; U12 := U12 U22^T).
-// Diagonal blocks of up to 64 rows are one work-group each in LDS.
+// Diagonal blocks of up to 128 rows are one work-group each in LDS.
 #include <hip/hip_runtime.h>
 
 #include "../../include/nla.h"
@@ -22,27 +22,29 @@ int32_t report_bad_arg(int k, const char *what);
 
 namespace {
 
-constexpr int LAUUM_LEAF = 64;
+constexpr int LAUUM_LEAF = 128;  // largest diagonal block done in one work-group
 
-// One diagonal block of nb <= 64 rows: lower R(i,j) = sum_{k>=i} L(k,i) L(k,j)
+// One diagonal block of nb <= NBL rows: lower R(i,j) = sum_{k>=i} L(k,i) L(k,j)
 // for i >= j; upper R(i,j) = sum_{k>=j} U(i,k) U(j,k) for i <= j.  The
 // triangle goes to LDS with zeros elsewhere, so every sum can run over all k
 // from the block's first row; each thread owns a 4 x 4 block of R (the
-// threads whose block lies outside the triangle only help load).
-constexpr int LAUUM_RB = 4, LAUUM_NB = LAUUM_LEAF / LAUUM_RB;  // 16 x 16 register blocks
+// threads whose block lies outside the triangle only help load).  NBL = 64
+// (256 threads) or 128 (1024 threads, 132 KB of LDS in FP64).
+constexpr int LAUUM_RB = 4;
 
-template <typename T>
-__global__ __launch_bounds__(256) void lauum_leaf_kernel(int nb, T *__restrict__ A, int64_t lda, int lower) {
-    __shared__ T s[LAUUM_LEAF * (LAUUM_LEAF + 1)];
-    constexpr int LD = LAUUM_LEAF + 1;
+template <typename T, int NBL>
+__global__ __launch_bounds__((NBL / 4) * (NBL / 4)) void lauum_leaf_kernel(int nb, T *__restrict__ A, int64_t lda,
+                                                                        int lower) {
+    constexpr int NB = NBL / LAUUM_RB, NT = NB * NB, LD = NBL + 1;
+    __shared__ T s[NBL * LD];
     const int t = threadIdx.x;
-    for (int e = t; e < LAUUM_LEAF * LAUUM_LEAF; e += 256) {
-        const int i = e % LAUUM_LEAF, j = e / LAUUM_LEAF;
+    for (int e = t; e < NBL * NBL; e += NT) {
+        const int i = e % NBL, j = e / NBL;
         const bool in = i < nb && j < nb && (lower ? i >= j : i <= j);
         s[j * LD + i] = in ? A[i + (int64_t)j * lda] : T(0);
     }
     __syncthreads();
-    const int bi = t % LAUUM_NB, bj = t / LAUUM_NB;  // rows 4 bi.., columns 4 bj..
+    const int bi = t % NB, bj = t / NB;  // rows 4 bi.., columns 4 bj..
     const int i0 = bi * LAUUM_RB, j0 = bj * LAUUM_RB;
     if ((lower ? bi < bj : bi > bj) || i0 >= nb || j0 >= nb) return;
     T acc[LAUUM_RB][LAUUM_RB];
@@ -101,7 +103,11 @@ struct Lauum {
     void rec(T *A, int64_t n) {
         if (st != NLA_OK || n <= 0) return;
         if (n <= LAUUM_LEAF) {
-            hipLaunchKernelGGL(lauum_leaf_kernel<T>, dim3(1), dim3(256), 0, s, (int)n, A, lda, lower ? 1 : 0);
+            if (n <= 64)
+                hipLaunchKernelGGL((lauum_leaf_kernel<T, 64>), dim3(1), dim3(256), 0, s, (int)n, A, lda, lower ? 1 : 0);
+            else
+                hipLaunchKernelGGL((lauum_leaf_kernel<T, 128>), dim3(1), dim3(1024), 0, s, (int)n, A, lda,
+                                   lower ? 1 : 0);
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) st = report_hip(e, "lauum leaf");
             return;

@@ -27,7 +27,7 @@ def host(t):
 def test_matches_oracle(nla, uplo, dt):
     rng = np.random.default_rng(3 if uplo == "U" else 4)
     eps = np.finfo(dt).eps
-    for n in (1, 16, 63, 64, 65, 128, 200, 513, 1000):
+    for n in (1, 16, 63, 64, 65, 127, 128, 129, 192, 200, 257, 513, 1000):  # around the 64- and 128-row leaves
         A = rng.uniform(-0.5, 0.5, (n + 3, n)).astype(dt)  # 3 extra rows: lda > n
         X = dev(A)
         nla.lauum_(uplo, n, X)
