@@ -29,19 +29,24 @@ constexpr int LAUUM_LEAF = 128;  // largest diagonal block done in one work-grou
 // triangle goes to LDS with zeros elsewhere, so every sum can run over all k
 // from the block's first row; each thread owns a 4 x 4 block of R (the
 // threads whose block lies outside the triangle only help load).  NBL = 64
-// (256 threads) or 128 (1024 threads, 132 KB of LDS in FP64).
+// (256 threads) or 128 (1024 threads, 131 KB of LDS in FP64).  Both
+// triangles are stored as rows of the factor the thread's index runs over
+// (L(., i) for lower, U(i, .) for upper), 4 rows to a block with a stride
+// of 4 NBL + 1: the 32 blocks one wave reads at a time start in 32
+// different banks.
 constexpr int LAUUM_RB = 4;
 
 template <typename T, int NBL>
 __global__ __launch_bounds__((NBL / 4) * (NBL / 4)) void lauum_leaf_kernel(int nb, T *__restrict__ A, int64_t lda,
                                                                         int lower) {
-    constexpr int NB = NBL / LAUUM_RB, NT = NB * NB, LD = NBL + 1;
-    __shared__ T s[NBL * LD];
+    constexpr int NB = NBL / LAUUM_RB, NT = NB * NB, BS = LAUUM_RB * NBL + 1;
+    __shared__ T s[NB * BS];
+    auto at = [](int i, int k) { return (i / LAUUM_RB) * BS + (i % LAUUM_RB) * NBL + k; };
     const int t = threadIdx.x;
     for (int e = t; e < NBL * NBL; e += NT) {
-        const int i = e % NBL, j = e / NBL;
-        const bool in = i < nb && j < nb && (lower ? i >= j : i <= j);
-        s[j * LD + i] = in ? A[i + (int64_t)j * lda] : T(0);
+        const int r = e % NBL, c = e / NBL;
+        const bool in = r < nb && c < nb && (lower ? r >= c : r <= c);
+        s[lower ? at(c, r) : at(r, c)] = in ? A[r + (int64_t)c * lda] : T(0);
     }
     __syncthreads();
     const int bi = t % NB, bj = t / NB;  // rows 4 bi.., columns 4 bj..
@@ -52,15 +57,15 @@ __global__ __launch_bounds__((NBL / 4) * (NBL / 4)) void lauum_leaf_kernel(int n
     for (int a = 0; a < LAUUM_RB; ++a)
 #pragma unroll
         for (int b = 0; b < LAUUM_RB; ++b) acc[a][b] = T(0);
-    // lower: L(k, i) = s[i * LD + k], zero for k < i; upper: U(i, k) =
-    // s[k * LD + i], zero for k < i
+    // x[a] = L(k, i0 + a) (lower) or U(i0 + a, k) (upper), zero for k below
+    // the block's first row / column; y likewise for j0
     const int k0 = lower ? i0 : j0;
     for (int k = k0; k < nb; ++k) {
         T x[LAUUM_RB], y[LAUUM_RB];
 #pragma unroll
         for (int a = 0; a < LAUUM_RB; ++a) {
-            x[a] = lower ? s[(i0 + a) * LD + k] : s[k * LD + i0 + a];
-            y[a] = lower ? s[(j0 + a) * LD + k] : s[k * LD + j0 + a];
+            x[a] = s[at(i0 + a, k)];
+            y[a] = s[at(j0 + a, k)];
         }
 #pragma unroll
         for (int a = 0; a < LAUUM_RB; ++a)
