@@ -128,9 +128,12 @@ function NextLA.getrf2!(A::DeviceMat{T}, ipiv::AbstractVector{Int}, info::Int) w
         dtype_code(T), Int64(m), Int64(n), pa, lda, pointer(dpiv), pointer(dinfo), hip_stream())
     check(st, "getrf2!")
     AMDGPU.synchronize()
+    inf = Int(Array(dinfo)[1])
+    # -1: a multi-work-group leaf timed out at its grid barrier (factors not valid)
+    inf < 0 && error("getrf2!: a multi-work-group leaf timed out waiting at its grid barrier")
     k = min(m, n)
     k > 0 && copyto!(ipiv, 1, Array(dpiv), 1, k)
-    return A, ipiv, Int(Array(dinfo)[1])
+    return A, ipiv, inf
 end
 
 # lauum!(uplo, n, A, ib) (src/lauum.jl:52): A := U*U^T or L^T*L on the device; ib is ignored.
