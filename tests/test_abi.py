@@ -186,6 +186,22 @@ def test_batch_mixed_validation(nla):
     assert lib.nla_batch_mixed_plan_destroy(None) == 0
 
 
+def test_lu_split_rows_config(nla):
+    """NLA_CFG_LU_SPLIT_ROWS: register leaves for every LU leaf by default
+    (1), 0 selects the global-memory leaf, negative values are rejected; the
+    setter returns the previous value.  Unknown keys return -1."""
+    from nextla_amd import _lib
+    old = _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, 4096)
+    try:
+        assert old == 1
+        assert _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, -1) == -1
+        assert _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, 0) == 4096
+        assert _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, 1) == 0
+    finally:
+        _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, old)
+    assert _lib.set_config(99, 1) == -1
+
+
 def test_lu_lauum_larfb_validation(nla):
     """Argument checks of the widened entry points return -k before any
     device work."""
