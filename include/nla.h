@@ -244,9 +244,14 @@ int32_t nla_larfb(char side, char trans, char direct, char storev, int32_t dtype
  *     scaling per operand block and accumulate in fp32 on MFMA;
  *     NLA_LP_FP32 is the plain fp32 update;
  *   - every step forms R = alpha·B - op(A)·X (or alpha·B - X·op(A)) in FP64
- *     (MFMA GEMM restricted to the triangle), and its normwise backward
- *     error ||R||_F / (||A||_F ||X||_F + ||alpha·B||_F); the loop stops once
- *     that is <= tol, otherwise X += the correction solve of R;
+ *     (MFMA GEMM restricted to the triangle) and the normwise backward error
+ *     of every system (a column of B for side 'L', a row for side 'R'):
+ *     max_j ||r_j||_2 / (||A||_lb ||x_j||_2 + ||alpha·b_j||_2), where
+ *     ||A||_lb is the largest column 2-norm of A's triangle, a lower bound of
+ *     ||A||_2 (so the value bounds the error with the exact 2-norm from
+ *     above); the loop stops once that is <= tol, otherwise X += the
+ *     correction solve of R.  The whole-matrix form ||R||_F / (||A||_F
+ *     ||X||_F + ||alpha·B||_F), never larger, is reported beside it;
  *   - a step that less than halves the backward error raises the update
  *     precision for the following steps (bf16 -> fp16 -> fp32);
  *   - NLA_LP_AUTO: X0 from fp32 updates, then each correction in the
@@ -269,8 +274,9 @@ enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2, NLA_LP_A
 typedef struct nla_mixed_info {
     int32_t iterations;     /* correction steps applied */
     int32_t precision;      /* update precision of the last step (nla_precision) */
-    double backward_error;  /* of the X returned in B */
-    double history[NLA_MIXED_MAX_ITER + 1]; /* backward error after 0, 1, ... steps */
+    double backward_error;  /* per-system form (the stopping test) of the X returned in B */
+    double history[NLA_MIXED_MAX_ITER + 1]; /* backward_error after 0, 1, ... steps */
+    double backward_error_frobenius; /* whole-matrix (Frobenius) form of the same X */
 } nla_mixed_info;
 
 int64_t nla_mixed_workspace_bytes(char side, int64_t n, int64_t m);

@@ -176,6 +176,7 @@ struct Recursion {
     int64_t nb;             // leaf size
     hipStream_t stream;
     const T *W = nullptr;   // inverted diagonal blocks (TRSM, leaf mode 1)
+    const int *F = nullptr; // their ill-conditioning flags (refined in the panels)
     int mode = 1;
     bool use_panel = true;  // fuse diagonal blocks <= panel_rows into one launch
     int panel_rows = 512, panel_strip = PANEL_STRIP, panel_waves = 4;
@@ -239,6 +240,7 @@ struct Recursion {
         a.nbig = (int)n;
         a.scale = scale;
         a.W = (func == 'S') ? W + (off / nb) * nb * nb : nullptr;
+        a.F = (func == 'S' && F) ? F + off / nb : nullptr;
         if (side == 'L') {
             a.lower = (uplo == 'L');
             a.m = B.nc;
@@ -460,9 +462,22 @@ static hipError_t reserve(StreamWorkspace *w, size_t bytes) {
     return hipSuccess;
 }
 
+// The diagonal-inverse workspace a TRSM call needs: the inverted leaf
+// blocks (nb x nb each) followed by one int flag per block; none for TRMM
+// or for the substitution leaf.
+int64_t rectrxm_workspace_bytes(char func, int32_t dtype, int64_t n) {
+    if (func != 'S' || leaf_mode_default() != 1 || n <= 0) return 0;
+    const int64_t nb = leaf_nb_default(), blocks = (n + nb - 1) / nb;
+    return blocks * nb * nb * (dtype == NLA_F64 ? 8 : 4) + blocks * (int64_t)sizeof(int);
+}
+
+// ws: a workspace of ws_bytes for the diagonal inverses (at least
+// rectrxm_workspace_bytes for a TRSM with leaf mode 1), or nullptr to take
+// this stream's cached buffer.
 template <typename T>
 static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double alpha, int64_t n, int64_t m,
-                             const void *Ap, int64_t lda, void *Bp, int64_t ldb, hipStream_t s, bool unit = false) {
+                             const void *Ap, int64_t lda, void *Bp, int64_t ldb, hipStream_t s, bool unit = false,
+                             void *ws = nullptr, int64_t ws_bytes = 0) {
     View<T> A{(T *)Ap, 1, lda, n, n};
     View<T> B{(T *)Bp, 1, ldb, side == 'L' ? n : m, side == 'L' ? m : n};
     if (alpha == 0.0) return hip_status(launch_zero<T>(B.nr, B.nc, B.p, ldb, s), "zero fill");
@@ -499,18 +514,31 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
         // this stream's cached workspace
         const int64_t blocks = (n + r.nb - 1) / r.nb;
         hipError_t e = hipSuccess;
-        StreamWorkspace *w = stream_workspace(s, &e);
-        if (!w) return hip_status(e, "workspace");
-        hold = std::unique_lock<std::mutex>(w->mu);
-        e = reserve(w, (size_t)(blocks * r.nb * r.nb) * sizeof(T));
-        if (e != hipSuccess) return hip_status(e, "workspace");
-        W = (T *)w->ptr;
+        const size_t wbytes = (size_t)(blocks * r.nb * r.nb) * sizeof(T);
+        if (ws != nullptr && (int64_t)(wbytes + (size_t)blocks * sizeof(int)) > ws_bytes) {
+            std::snprintf(g_last_error, sizeof(g_last_error),
+                          "n = %lld needs %lld bytes of diagonal-inverse workspace, %lld given", (long long)n,
+                          (long long)(wbytes + (size_t)blocks * sizeof(int)), (long long)ws_bytes);
+            return NLA_ERR_WORKSPACE;
+        }
+        if (ws == nullptr) {
+            StreamWorkspace *w = stream_workspace(s, &e);
+            if (!w) return hip_status(e, "workspace");
+            hold = std::unique_lock<std::mutex>(w->mu);
+            e = reserve(w, wbytes + (size_t)blocks * sizeof(int));
+            if (e != hipSuccess) return hip_status(e, "workspace");
+            ws = w->ptr;
+        }
+        W = (T *)ws;
+        int *F = (r.nb == 64) ? (int *)((char *)ws + wbytes) : nullptr;
         const double fl = (double)blocks * r.nb * r.nb * r.nb / 3.0;
         const double by = (double)blocks * r.nb * r.nb * 1.5 * sizeof(T);
-        e = timed(NLA_KIND_LEAF, fl, by, s,
-                  [&] { return launch_trtri_blocks<T>(n, (int)r.nb, A.p, A.rs, A.cs, uplo == 'L', W, s, unit); });
+        e = timed(NLA_KIND_LEAF, fl, by, s, [&] {
+            return launch_trtri_blocks<T>(n, (int)r.nb, A.p, A.rs, A.cs, uplo == 'L', W, s, unit, F);
+        });
         if (e != hipSuccess) return hip_status(e, "diagonal inverses");
         r.W = W;
+        r.F = F;
     }
     r.rec(A, B, n, 0, (T)alpha, (T)alpha);
     return hip_status(r.err, "kernel launch");
@@ -526,7 +554,8 @@ struct MixedLayout {
     int64_t rows, cols;  // shape of B
     int64_t ld32, ldr;   // leading dimension of the fp32 A copy / of S, R, D
     int64_t amax_slots;
-    int64_t a32, w, s, r, d, amax, part, norms, queue, total;
+    int64_t nsys;  // systems: columns of B (side L) or rows (side R)
+    int64_t a32, w, wf, s, r, d, amax, part, norms, queue, spart, xpart, s2, total;
 };
 
 static inline int64_t al256(int64_t x) { return (x + 255) & ~(int64_t)255; }
@@ -542,13 +571,19 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     int64_t o = 0;
     L.a32 = o;   o = al256(o + L.ld32 * n * 4);
     L.w = o;     o = al256(o + nblk * 64 * 64 * 4);
+    L.wf = o;    o = al256(o + nblk * 4);
     L.s = o;     o = al256(o + L.ldr * L.cols * 8);
     L.r = o;     o = al256(o + L.ldr * L.cols * 8);
     L.d = o;     o = al256(o + L.ldr * L.cols * 4);
     L.amax = o;  o = al256(o + L.amax_slots * 4);
     L.part = o;  o = al256(o + NORM_PARTS_MAX * 2 * 8);
-    L.norms = o; o = al256(o + 8 * 8);
+    L.norms = o; o = al256(o + 16 * 8);
     L.queue = o; o = al256(o + 4);
+    L.nsys = m;
+    const int64_t np = (int64_t)SYS_CHUNKS_MAX * 2 * (n > m ? n : m);
+    L.spart = o; o = al256(o + np * 8);
+    L.xpart = o; o = al256(o + np * 8);
+    L.s2 = o;    o = al256(o + (m > 0 ? m : 1) * 8);
     L.total = o;
     return L;
 }
@@ -588,15 +623,25 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         A32v.rs = L.ld32;
         A32v.cs = 1;
     }
-    NLA_CK(launch_trtri_blocks<float>(n, 64, A32v.p, A32v.rs, A32v.cs, ue == 'L', W, s));
+    int *WF = (int *)(ws + L.wf);
+    NLA_CK(launch_trtri_blocks<float>(n, 64, A32v.p, A32v.rs, A32v.cs, ue == 'L', W, s, false, WF));
     // S = alpha*B; the norms of A and S enter every stopping test
     NLA_CK(launch_scale_copy(L.rows, L.cols, alpha, B, ldb, S, L.ldr, s));
-    NLA_CK(launch_norms(n, n, A, lda, uplo == 'L' ? 1 : 2, part, nrm + 0, s));
-    NLA_CK(launch_norms(L.rows, L.cols, S, L.ldr, 0, part, nrm + 2, s));
-    double h[4];
-    NLA_CK(hipMemcpyAsync(h, nrm, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    // ||A||_F and its largest column 2-norm (the lower bound of ||A||_2 in
+    // the per-system test), ||S||_F, max |S| and every system's ||s_j||^2
+    const bool by_col = (side == 'L');
+    double *spart = (double *)(ws + L.spart), *xpart = (double *)(ws + L.xpart), *s2 = (double *)(ws + L.s2);
+    const int ncha = sys_chunks(n, n), nchb = sys_chunks(L.nsys, n);
+    NLA_CK(launch_sys_sumsq(n, n, A, lda, true, uplo == 'L' ? 1 : 2, ncha, spart, s));
+    NLA_CK(launch_sys_fold(ncha, n, spart, nullptr, nrm + 0, s));
+    NLA_CK(launch_sys_sumsq(L.rows, L.cols, S, L.ldr, by_col, 0, nchb, spart, s));
+    NLA_CK(launch_sys_fold(nchb, L.nsys, spart, s2, nrm + 3, s));
+    double h[6];
+    NLA_CK(hipMemcpyAsync(h, nrm, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
     NLA_CK(hipStreamSynchronize(s));
-    const double nA = std::sqrt(h[0]), nS = std::sqrt(h[2]);
+    const double nA = std::sqrt(h[0]), nS = std::sqrt(h[3]);
+    const double maxS = h[4];
+    (void)part;
 
     // D <- op(A)^-1 D (or D op(A)^-1) on the fp32 copy, updates in `prec`
     bool amax_a_ready = false;  // A-block maxima of the fp16 updates computed
@@ -611,6 +656,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         r.mode = 1;
         r.use_panel = true;
         r.W = W;
+        r.F = WF;
         r.panel_waves = g_panel_waves.load() ? g_panel_waves.load() : 8;
         if (prec != NLA_LP_FP32) {
             r.lp = prec;
@@ -667,7 +713,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     // X0: the low-precision solve of S, scaled by a power of two so the fp32
     // buffer holds O(1) values
     int prec = autop ? NLA_LP_FP32 : precision;
-    double sig = pow2_above(h[3]);
+    double sig = pow2_above(maxS);
     NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, S, L.ldr, D, L.ldr, s));
     NLA_CK(lp_solve(prec));
     NLA_CK(launch_up(L.rows, L.cols, sig, D, L.ldr, 0.0, B, ldb, s));
@@ -695,19 +741,22 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     const double res_fl = (double)n * n * m;
     const double res_by = ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8;
 
-    double prev = INFINITY, bwd = INFINITY;
+    double prev = INFINITY, bwd = INFINITY, bwd_f = INFINITY;
     int it = 0;
     for (;; ++it) {
         NLA_CK(launch_scale_copy(L.rows, L.cols, 1.0, S, L.ldr, R, L.ldr, s));
         NLA_CK(launch_fill_u32((unsigned *)g.queue, 1, 0u, s));
         NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, [&] { return launch_gemm<double>(g, s); }));
-        NLA_CK(launch_norms(L.rows, L.cols, R, L.ldr, 0, part, nrm + 4, s));
-        NLA_CK(launch_norms(L.rows, L.cols, B, ldb, 0, part, nrm + 6, s));
-        NLA_CK(hipMemcpyAsync(h, nrm + 4, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+        // per-system test: max_j ||r_j|| / (||A||_lb ||x_j|| + ||s_j||)
+        NLA_CK(launch_sys_sumsq(L.rows, L.cols, R, L.ldr, by_col, 0, nchb, spart, s));
+        NLA_CK(launch_sys_sumsq(L.rows, L.cols, B, ldb, by_col, 0, nchb, xpart, s));
+        NLA_CK(launch_sys_eta(nchb, L.nsys, spart, xpart, s2, nrm + 2, nrm + 6, s));
+        NLA_CK(hipMemcpyAsync(h, nrm + 6, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
         NLA_CK(hipStreamSynchronize(s));
         const double nR = std::sqrt(h[0]), nX = std::sqrt(h[2]);
         const double den = nA * nX + nS;
-        bwd = den > 0.0 ? nR / den : (nR > 0.0 ? INFINITY : 0.0);
+        bwd_f = den > 0.0 ? nR / den : (nR > 0.0 ? INFINITY : 0.0);
+        bwd = h[3];
         if (info && it <= NLA_MIXED_MAX_ITER) info->history[it] = bwd;
         if (bwd <= tol || it >= max_iter || !std::isfinite(bwd)) break;
         if (autop) {
@@ -731,12 +780,43 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         info->iterations = it;
         info->precision = prec;
         info->backward_error = bwd;
+        info->backward_error_frobenius = bwd_f;
     }
     if (!(bwd <= tol)) {
         std::snprintf(g_last_error, sizeof(g_last_error),
                       "refinement stopped at backward error %.3e after %d steps (target %.3e)", bwd, it, tol);
         return NLA_ERR_NOT_CONVERGED;
     }
+    return NLA_OK;
+}
+
+// nla_rectrxm's work with the caller's workspace (rectrxm_workspace_bytes)
+// instead of the per-stream buffer; arguments already validated.
+int32_t rectrxm_in_workspace(char side, char uplo, char trans, char func, int32_t dtype, double alpha, int64_t n,
+                             int64_t m, const void *A, int64_t lda, void *B, int64_t ldb, hipStream_t s, void *ws,
+                             int64_t ws_bytes) {
+    if (ws == nullptr) return report_status(NLA_ERR_WORKSPACE, "no workspace given");
+    if (dtype == NLA_F64)
+        return rectrxm_typed<double>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s, false, ws, ws_bytes);
+    return rectrxm_typed<float>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s, false, ws, ws_bytes);
+}
+
+// Argument checks of nla_rectrxm (0 or -k); characters upper-cased in place.
+int32_t rectrxm_check(char &side, char &uplo, char &trans, char &func, int32_t dtype, int64_t n, int64_t m,
+                      const void *A, int64_t lda, const void *B, int64_t ldb) {
+    side = up(side); uplo = up(uplo); trans = up(trans); func = up(func);
+    if (side != 'L' && side != 'R') return bad_arg(1, "side");
+    if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
+    if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
+    if (func != 'S' && func != 'M') return bad_arg(4, "func");
+    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(5, "dtype");
+    if (n < 0) return bad_arg(7, "n");
+    if (m < 0) return bad_arg(8, "m");
+    if (n == 0 || m == 0) return NLA_OK;
+    if (A == nullptr) return bad_arg(9, "A");
+    if (lda < n) return bad_arg(10, "lda");
+    if (B == nullptr) return bad_arg(11, "B");
+    if (ldb < (side == 'L' ? n : m)) return bad_arg(12, "ldb");
     return NLA_OK;
 }
 
@@ -752,19 +832,8 @@ const char *nla_last_error(void) { return g_last_error; }
 
 int32_t nla_rectrxm(char side, char uplo, char trans, char func, int32_t dtype, double alpha, int64_t n, int64_t m,
                     const void *A, int64_t lda, void *B, int64_t ldb, void *stream) {
-    side = up(side); uplo = up(uplo); trans = up(trans); func = up(func);
-    if (side != 'L' && side != 'R') return bad_arg(1, "side");
-    if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
-    if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
-    if (func != 'S' && func != 'M') return bad_arg(4, "func");
-    if (dtype != NLA_F32 && dtype != NLA_F64) return bad_arg(5, "dtype");
-    if (n < 0) return bad_arg(7, "n");
-    if (m < 0) return bad_arg(8, "m");
-    if (n == 0 || m == 0) return NLA_OK;
-    if (A == nullptr) return bad_arg(9, "A");
-    if (lda < n) return bad_arg(10, "lda");
-    if (B == nullptr) return bad_arg(11, "B");
-    if (ldb < (side == 'L' ? n : m)) return bad_arg(12, "ldb");
+    const int32_t chk = rectrxm_check(side, uplo, trans, func, dtype, n, m, A, lda, B, ldb);
+    if (chk != NLA_OK || n == 0 || m == 0) return chk;
     hipStream_t s = (hipStream_t)stream;
     if (dtype == NLA_F64) return rectrxm_typed<double>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s);
     return rectrxm_typed<float>(side, uplo, trans, func, alpha, n, m, A, lda, B, ldb, s);

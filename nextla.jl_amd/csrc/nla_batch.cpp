@@ -48,7 +48,7 @@ static inline int64_t stages_of(int64_t n, int mask) {
 }
 
 struct Layout {
-    int64_t counter, sys, pan, ptasks, gem, gtasks, blk, inv, total;
+    int64_t counter, sys, pan, ptasks, gem, gtasks, blk, inv, flg, total;
     int64_t npan, nptasks, ngem, ngtasks, nblk, nstages;
     int mask;  // size-class mask the plan was laid out for
 };
@@ -84,7 +84,8 @@ static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, i
     L.gtasks = align256(L.gem + L.ngem * gem_bytes);
     L.blk = align256(L.gtasks + L.ngtasks * (int64_t)sizeof(int2));
     L.inv = align256(L.blk + L.nblk * (int64_t)sizeof(int2));
-    L.total = align256(L.inv + L.nblk * 64 * 64 * es);
+    L.flg = align256(L.inv + L.nblk * 64 * 64 * es);
+    L.total = align256(L.flg + L.nblk * (int64_t)sizeof(int));
     return L;
 }
 
@@ -129,6 +130,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
     gem.reserve((size_t)L.ngem);
     blk.reserve((size_t)L.nblk);
     T *inv = (T *)(P->ws + L.inv);
+    int *flg = (int *)(P->ws + L.flg);
     int64_t inv_off = 0;
     const bool tr = (trans != 'N');
     const char ue = tr ? (uplo == 'L' ? 'U' : 'L') : uplo;  // uplo of op(A)
@@ -166,6 +168,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
         if (it.n <= 0 || it.m <= 0) continue;
         if (func == 'S') {
             S.W = inv + inv_off;
+            S.F = flg + inv_off / (64 * 64);
             const int64_t nb = (it.n + 63) / 64;
             for (int64_t b = 0; b < nb; ++b) blk.push_back(make_int2((int)i, (int)b));
             inv_off += nb * 64 * 64;
@@ -190,7 +193,10 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             D.n = (int)rows;
             D.M = S.M + r0 * S.mrs + r0 * S.mcs;
             D.B = S.B + r0 * S.brs;
-            if (func == 'S') D.W = S.W + (r0 / 64) * 64 * 64;
+            if (func == 'S') {
+                D.W = S.W + (r0 / 64) * 64 * 64;
+                D.F = S.F + r0 / 64;
+            }
             D.scale = trmm ? alpha : (s == 0 ? alpha : T(1));
             const int pi = (int)pan.size();
             pan.push_back(D);

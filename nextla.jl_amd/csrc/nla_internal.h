@@ -72,6 +72,9 @@ struct PanelArgs {
     const T *W;
     int64_t wrs, wcs;
     T scale;
+    // TRSM: per 64-block flags from launch_trtri_blocks (nullable): a flagged
+    // block's product with its inverse is followed by one refinement step
+    const int *F = nullptr;
     int strip = PANEL_STRIP;  // RHS columns per work-group: 16 or 32
     bool unit = false;        // unit diagonal (taken as 1, not read)
     int waves = 4;            // waves per work-group: 4, or 8 splitting each phase's k range (<= 512 rows)
@@ -131,6 +134,7 @@ struct BatchSys {
     const T *W;  // this system's inverse 64-blocks (TRSM)
     int64_t wrs, wcs;
     T scale;
+    const int *F;  // their ill-conditioning flags (TRSM)
 };
 
 struct BatchTask {
@@ -180,10 +184,32 @@ hipError_t launch_leaf(const LeafArgs<T> &a, hipStream_t s);
 // with the identity past n).  unit: the diagonal is taken as 1, not read.
 template <typename T>
 hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,
-                               hipStream_t s, bool unit = false);
+                               hipStream_t s, bool unit = false, int *flags = nullptr);
+// nb = 64 with flags: flags[b] = 1 when block b's 1-norm condition number
+// ||M_bb||_1 ||inv(M_bb)||_1 exceeds LEAF_REFINE_KAPPA (or is not finite),
+// else 0.  Applying the inverse of such a block loses accuracy in
+// proportion to its condition number; the panels refine those blocks.
+constexpr double LEAF_REFINE_KAPPA = 256.0;
 
 template <typename T>
 hipError_t launch_zero(int64_t rows, int64_t cols, T *B, int64_t ldb, hipStream_t s);
+
+// ---- plain path entry helpers (nla_api.cpp) ---------------------------------
+
+// Bytes of the diagonal-inverse workspace of a plain call (TRSM only).
+int64_t rectrxm_workspace_bytes(char func, int32_t dtype, int64_t n);
+// nla_rectrxm's argument checks (0 or -k; characters upper-cased in place).
+int32_t rectrxm_check(char &side, char &uplo, char &trans, char &func, int32_t dtype, int64_t n, int64_t m,
+                      const void *A, int64_t lda, const void *B, int64_t ldb);
+// nla_rectrxm with the caller's workspace ws of ws_bytes bytes (checked
+// against rectrxm_workspace_bytes: NLA_ERR_WORKSPACE when too small); never
+// allocates.
+int32_t rectrxm_in_workspace(char side, char uplo, char trans, char func, int32_t dtype, double alpha, int64_t n,
+                             int64_t m, const void *A, int64_t lda, void *B, int64_t ldb, hipStream_t s, void *ws,
+                             int64_t ws_bytes);
+int32_t report_status(int32_t code, const char *msg);
+int32_t report_hip(hipError_t e, const char *where);
+int32_t report_bad_arg(int k, const char *what);
 
 // ---- mixed precision ------------------------------------------------------
 
@@ -227,6 +253,23 @@ hipError_t launch_up(int64_t rows, int64_t cols, double sigma, const float *D, i
 constexpr int NORM_PARTS_MAX = 1024;
 hipError_t launch_norms(int64_t rows, int64_t cols, const double *P, int64_t ld, int tri, double *part,
                         double *out, hipStream_t s);
+// Per-system norms of the stopping test (a system is a column of the
+// matrix when by_col, else a row).  launch_sys_sumsq writes, per (chunk c,
+// system j), the sum of squares and the absolute maximum over chunk c of the
+// reduction dimension to part[2*(c*nsys + j) + {0, 1}]; nch chunks (at most
+// SYS_CHUNKS_MAX; sys_chunks() picks a count).  tri (by_col only): 1 / 2
+// restricts column c to rows >= c / <= c.
+constexpr int SYS_CHUNKS_MAX = 64;
+int sys_chunks(int64_t nsys, int64_t len);
+hipError_t launch_sys_sumsq(int64_t rows, int64_t cols, const double *P, int64_t ld, bool by_col, int tri, int nch,
+                            double *part, hipStream_t s);
+// sys2[j] (if not null) = system j's sum of squares; out[0] = total sum of
+// squares, out[1] = max |.|, out[2] = largest system 2-norm.
+hipError_t launch_sys_fold(int nch, int64_t nsys, const double *part, double *sys2, double *out, hipStream_t s);
+// out[0] = ||R||_F^2, out[1] = max |R|, out[2] = ||X||_F^2, out[3] =
+// max_j ||r_j|| / (*anorm_lb ||x_j|| + sqrt(s2[j])).
+hipError_t launch_sys_eta(int nch, int64_t nsys, const double *pr, const double *px, const double *s2,
+                          const double *anorm_lb, double *out, hipStream_t s);
 
 // ---- batched refinement ----------------------------------------------------
 

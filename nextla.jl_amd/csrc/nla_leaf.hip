@@ -152,7 +152,7 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
 // being built; st: 32 x 32 scratch.  Upper blocks are inverted as J U J.
 template <typename T>
 __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
-                              bool unit, T *__restrict__ W, T *sl, T *si, T *st) {
+                              bool unit, T *__restrict__ W, T *sl, T *si, T *st, int *__restrict__ F) {
     constexpr int NB = 64, NT = 256;
     const int t = threadIdx.x;
     const int64_t off = blk * NB;
@@ -256,6 +256,30 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
         for (int u = 0; u < 4; ++u) si[(j0 + 8 * u) * NB + 32 + i] = -a[u];
     }
     __syncthreads();
+    // Condition of the block, kappa_1 = ||L||_1 ||inv(L)||_1 (both exact, the
+    // inverse being in LDS): an explicitly inverted block applied as a
+    // product has backward error growing with kappa * u, so a block above
+    // LEAF_REFINE_KAPPA is flagged and the panels follow its product with
+    // one step of refinement (X += inv(L) (B - L X)).
+    if (F != nullptr) {
+        T cs1 = T(0);
+        if (t < 2 * NB) {
+            const T *src = (t < NB) ? sl + t * NB : si + (t - NB) * NB;
+#pragma unroll 8
+            for (int i = 0; i < NB; ++i) cs1 += src[i] < T(0) ? -src[i] : src[i];
+        }
+        // maxima over the 64 columns of each: waves 0 (L) and 1 (inverse)
+        for (int o = 32; o > 0; o >>= 1) {
+            const T other = __shfl_xor(cs1, o);
+            cs1 = other > cs1 ? other : cs1;
+        }
+        if (t == NB) st[0] = cs1;
+        __syncthreads();
+        if (t == 0) {
+            const T kappa = cs1 * st[0];
+            F[blk] = (kappa > T(LEAF_REFINE_KAPPA) || !(kappa == kappa)) ? 1 : 0;
+        }
+    }
     // write: column-major 64 x 64; an upper block's inverse is J inv(L) J
     T *w = W + blk * NB * NB;
 #pragma unroll 4
@@ -349,11 +373,11 @@ __global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__
 // The 64-block inverse used by the recursion and the batched solver.
 template <typename T>
 __global__ __launch_bounds__(256) void trtri64_kernel(int64_t n, const T *__restrict__ A, int64_t rs, int64_t cs,
-                                                      int lower, int unit, T *__restrict__ W) {
+                                                      int lower, int unit, T *__restrict__ W, int *__restrict__ F) {
     __shared__ __attribute__((aligned(16))) T sl[64 * 64];
     __shared__ __attribute__((aligned(16))) T si[64 * 64];
     __shared__ __attribute__((aligned(16))) T st[32 * 32];
-    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, unit != 0, W, sl, si, st);
+    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, unit != 0, W, sl, si, st, F);
 }
 
 // Batched form: one work-group per (system, block) pair.
@@ -371,7 +395,7 @@ __global__ __launch_bounds__(256) void batch_trtri_kernel(const BatchSys<T> *__r
     const bool right = (S.wrs != 1);
     const int64_t rs = right ? S.mcs : S.mrs, cs = right ? S.mrs : S.mcs;
     const bool lower = right ? (S.lower == 0) : (S.lower != 0);
-    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, false, (T *)S.W, sl, si, st);
+    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, false, (T *)S.W, sl, si, st, (int *)S.F);
 }
 
 template <typename T>
@@ -386,12 +410,13 @@ template hipError_t launch_batch_trtri<double>(const BatchSys<double> *, const i
 
 template <typename T>
 hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_t cs, bool lower, T *W,
-                               hipStream_t s, bool unit) {
+                               hipStream_t s, bool unit, int *flags) {
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + nb - 1) / nb;
     const int lo = lower ? 1 : 0, un = unit ? 1 : 0;
     if (nb == 64)
-        hipLaunchKernelGGL((trtri64_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, n, A, rs, cs, lo, un, W);
+        hipLaunchKernelGGL((trtri64_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, n, A, rs, cs, lo, un, W,
+                           flags);
     else if (nb == 32)
         hipLaunchKernelGGL((trtri_kernel<T, 32>), dim3((unsigned)blocks), dim3(LEAF_COLS), 0, s, n, A, rs, cs, lo, un,
                            W);
@@ -404,9 +429,9 @@ hipError_t launch_trtri_blocks(int64_t n, int nb, const T *A, int64_t rs, int64_
 }
 
 template hipError_t launch_trtri_blocks<float>(int64_t, int, const float *, int64_t, int64_t, bool, float *,
-                                               hipStream_t, bool);
+                                               hipStream_t, bool, int *);
 template hipError_t launch_trtri_blocks<double>(int64_t, int, const double *, int64_t, int64_t, bool, double *,
-                                                hipStream_t, bool);
+                                                hipStream_t, bool, int *);
 
 template <typename T, int NB, bool ICONTIG>
 static hipError_t launch_leaf_nb(const LeafArgs<T> &a, hipStream_t s) {

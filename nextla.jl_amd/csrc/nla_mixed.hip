@@ -539,4 +539,177 @@ hipError_t launch_norms(int64_t rows, int64_t cols, const double *P, int64_t ld,
     return hipGetLastError();
 }
 
+
+// ---- per-system norms of the stopping test ---------------------------------
+//
+// A right-hand side and its solution form one system: a column of B for the
+// left side, a row for the right side.  The stopping test is normwise per
+// system, max_j ||r_j|| / (||A||_lb ||x_j|| + ||s_j||), with ||A||_lb the
+// largest column 2-norm of A's stored triangle (a lower bound of ||A||_2, so
+// the value bounds the error with the exact 2-norm from above).  Sums of
+// squares and absolute maxima are taken per (system, chunk of the reduction
+// dimension) and folded in a fixed order, so every result is deterministic.
+
+// Systems are columns (unit stride along the reduction): one wave per column,
+// lanes over the rows of chunk blockIdx.y.  tri: 1 lower / 2 upper triangle
+// only (r >= c / r <= c), 0 the whole column.
+__global__ __launch_bounds__(256) void sys_sumsq_col_kernel(int64_t rows, int64_t cols, const double *__restrict__ P,
+                                                            int64_t ld, int tri, int64_t chunk,
+                                                            double *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= cols) return;
+    int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = r0 + chunk < rows ? r0 + chunk : rows;
+    if (tri == 1 && r0 < c) r0 = c;
+    if (tri == 2 && r1 > c + 1) r1 = c + 1;
+    double ss = 0.0, mx = 0.0;
+    const double *col = P + c * ld;
+    for (int64_t r = r0 + lane; r < r1; r += 64) {
+        const double v = col[r];
+        ss += v * v;
+        mx = fmax(mx, fabs(v));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ss += __shfl_xor(ss, o);
+        mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    if (lane == 0) {
+        double *q = part + 2 * ((int64_t)blockIdx.y * cols + c);
+        q[0] = ss;
+        q[1] = mx;
+    }
+}
+
+// Systems are rows (unit stride across the systems): one thread per row, over
+// the columns of chunk blockIdx.y.
+__global__ __launch_bounds__(256) void sys_sumsq_row_kernel(int64_t rows, int64_t cols, const double *__restrict__ P,
+                                                            int64_t ld, int64_t chunk, double *__restrict__ part) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    const int64_t c0 = (int64_t)blockIdx.y * chunk, c1 = c0 + chunk < cols ? c0 + chunk : cols;
+    double ss = 0.0, mx = 0.0;
+    for (int64_t c = c0; c < c1; ++c) {
+        const double v = P[r + c * ld];
+        ss += v * v;
+        mx = fmax(mx, fabs(v));
+    }
+    double *q = part + 2 * ((int64_t)blockIdx.y * rows + r);
+    q[0] = ss;
+    q[1] = mx;
+}
+
+int sys_chunks(int64_t nsys, int64_t len) {
+    // enough work-groups to cover the chip, chunks of at least 64 elements
+    const int64_t groups = nsys / 4 + 1;
+    int64_t nch = (2048 + groups - 1) / groups;
+    const int64_t cap = (len + 63) / 64;
+    if (nch > cap) nch = cap;
+    if (nch > SYS_CHUNKS_MAX) nch = SYS_CHUNKS_MAX;
+    return nch < 1 ? 1 : (int)nch;
+}
+
+hipError_t launch_sys_sumsq(int64_t rows, int64_t cols, const double *P, int64_t ld, bool by_col, int tri, int nch,
+                            double *part, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    if (by_col) {
+        const int64_t chunk = (rows + nch - 1) / nch;
+        hipLaunchKernelGGL(sys_sumsq_col_kernel, dim3((unsigned)((cols + 3) / 4), (unsigned)nch), dim3(256), 0, s,
+                           rows, cols, P, ld, tri, chunk, part);
+    } else {
+        const int64_t chunk = (cols + nch - 1) / nch;
+        hipLaunchKernelGGL(sys_sumsq_row_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nch), dim3(256), 0,
+                           s, rows, cols, P, ld, chunk, part);
+    }
+    return hipGetLastError();
+}
+
+__device__ inline void fold_block4(double *v, double *red, int nt) {
+    // v[0], v[2] summed; v[1], v[3] max-reduced over the work-group (fixed order)
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k * 256 + t] = v[k];
+    __syncthreads();
+    for (int w = nt / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            red[t] += red[t + w];
+            red[256 + t] = fmax(red[256 + t], red[256 + t + w]);
+            red[512 + t] += red[512 + t + w];
+            red[768 + t] = fmax(red[768 + t], red[768 + t + w]);
+        }
+        __syncthreads();
+    }
+}
+
+// Folds the partials of one matrix: per-system sums of squares into sys2
+// (when not null) and out[0] = total sum of squares, out[1] = max |.|,
+// out[2] = max_j sqrt(system j's sum of squares).
+__global__ __launch_bounds__(256) void sys_fold_kernel(int nch, int64_t nsys, const double *__restrict__ part,
+                                                       double *__restrict__ sys2, double *__restrict__ out) {
+    __shared__ double red[4 * 256];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t j = threadIdx.x; j < nsys; j += 256) {
+        double ss = 0.0, mx = 0.0;
+        for (int c = 0; c < nch; ++c) {
+            ss += part[2 * ((int64_t)c * nsys + j)];
+            mx = fmax(mx, part[2 * ((int64_t)c * nsys + j) + 1]);
+        }
+        if (sys2) sys2[j] = ss;
+        v[0] += ss;
+        v[1] = fmax(v[1], mx);
+        v[3] = fmax(v[3], sqrt(ss));
+    }
+    fold_block4(v, red, 256);
+    if (threadIdx.x == 0) {
+        out[0] = red[0];
+        out[1] = red[256];
+        out[2] = red[768];
+    }
+}
+
+// The stopping test from the partials of R and X: out[0] = ||R||_F^2,
+// out[1] = max |R|, out[2] = ||X||_F^2, out[3] = max_j ||r_j|| /
+// (anorm_lb ||x_j|| + ||s_j||) (0 for a system with r_j = 0; +inf if a
+// norm is not finite).
+__global__ __launch_bounds__(256) void sys_eta_kernel(int nch, int64_t nsys, const double *__restrict__ pr,
+                                                      const double *__restrict__ px, const double *__restrict__ s2,
+                                                      const double *__restrict__ anorm_lb, double *__restrict__ out) {
+    __shared__ double red[4 * 256];
+    const double alb = *anorm_lb;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t j = threadIdx.x; j < nsys; j += 256) {
+        double r2 = 0.0, rmx = 0.0, x2 = 0.0;
+        for (int c = 0; c < nch; ++c) {
+            r2 += pr[2 * ((int64_t)c * nsys + j)];
+            rmx = fmax(rmx, pr[2 * ((int64_t)c * nsys + j) + 1]);
+            x2 += px[2 * ((int64_t)c * nsys + j)];
+        }
+        const double num = sqrt(r2), den = alb * sqrt(x2) + sqrt(s2[j]);
+        double eta = num == 0.0 ? 0.0 : (den > 0.0 ? num / den : INFINITY);
+        if (!(eta == eta) || !(r2 < INFINITY) || !(x2 < INFINITY)) eta = INFINITY;
+        v[0] += r2;
+        v[1] = fmax(v[1], rmx);
+        v[2] += x2;
+        v[3] = fmax(v[3], eta);
+    }
+    fold_block4(v, red, 256);
+    if (threadIdx.x == 0) {
+        out[0] = red[0];
+        out[1] = red[256];
+        out[2] = red[512];
+        out[3] = red[768];
+    }
+}
+
+hipError_t launch_sys_fold(int nch, int64_t nsys, const double *part, double *sys2, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sys_fold_kernel, dim3(1), dim3(256), 0, s, nch, nsys, part, sys2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sys_eta(int nch, int64_t nsys, const double *pr, const double *px, const double *s2,
+                          const double *anorm_lb, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sys_eta_kernel, dim3(1), dim3(256), 0, s, nch, nsys, pr, px, s2, anorm_lb, out);
+    return hipGetLastError();
+}
+
 }  // namespace nla

@@ -131,7 +131,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                                             int64_t mcs, bool lower, bool unit, T *__restrict__ Bp, int64_t brs,
                                             int64_t bcs,
                                             const T *__restrict__ Wp, int64_t wrs, int64_t wcs, T scale,
-                                            NLA_LDS T *bs, NLA_LDS T *sc = nullptr) {
+                                            const int *__restrict__ Fp, NLA_LDS T *bs, NLA_LDS T *sc = nullptr) {
     typedef PanelShape<T, WS> PS;
     typedef typename PS::MF MF;
     constexpr int W = PS::W;
@@ -299,6 +299,60 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
             acc2[cb] = MF::zero();
         }
     };
+    // An ill-conditioned diagonal block (flag set by the inverse kernel):
+    // X_i = inv(M_ii) B_i followed by one refinement step X_i += inv(M_ii)
+    // (B_i - M_ii X_i) (backward stable while kappa(M_ii) * u << 1).  A
+    // rare path, in plain FMAs: thread t owns row t % 64 of the block and
+    // columns t / 64 + NW * j; X goes through the 64 x W scratch sc.
+    auto solve_refined = [&](int ib) {
+        constexpr int NC = W / NW;
+        const int r = t & 63, cg = t >> 6;
+        const int nb = bsize(ib);
+        const T *Wk = Wp + (int64_t)ib * 64 * 64;
+        const T *Mk = Mp + (int64_t)(64 * ib) * (mrs + mcs);
+        NLA_LDS T *blk = bs + ib * 64;
+        T x[NC], v[NC];
+        // X = inv(M_ii) B_i (the inverse is zero outside its triangle and
+        // the identity past nb)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) x[j] = T(0);
+        for (int k = 0; k < 64; ++k) {
+            const T w = Wk[r * wrs + k * wcs];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) x[j] += w * blk[(cg + NW * j) * LDB + k];
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) sc[(cg + NW * j) * 64 + r] = x[j];
+        __syncthreads();
+        // R = B_i - M_ii X
+#pragma unroll
+        for (int j = 0; j < NC; ++j) v[j] = unit ? sc[(cg + NW * j) * 64 + r] : T(0);
+        if (r < nb) {
+            const int k0 = lower ? 0 : (unit ? r + 1 : r);
+            const int k1 = lower ? (unit ? r : r + 1) : nb;
+            for (int k = k0; k < k1; ++k) {
+                const T a = Mk[r * mrs + k * mcs];
+#pragma unroll
+                for (int j = 0; j < NC; ++j) v[j] += a * sc[(cg + NW * j) * 64 + k];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NC; ++j) v[j] = blk[(cg + NW * j) * LDB + r] - v[j];
+        __syncthreads();  // every thread has read B_i
+#pragma unroll
+        for (int j = 0; j < NC; ++j) blk[(cg + NW * j) * LDB + r] = v[j];
+        __syncthreads();
+        // X += inv(M_ii) R
+        for (int k = 0; k < 64; ++k) {
+            const T w = Wk[r * wrs + k * wcs];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) x[j] += w * blk[(cg + NW * j) * LDB + k];
+        }
+        __syncthreads();  // every thread has read R
+#pragma unroll
+        for (int j = 0; j < NC; ++j) blk[(cg + NW * j) * LDB + r] = x[j];
+        __syncthreads();  // X_i complete for the blocks that follow
+    };
     auto run = [&](const Phase &ph, const T *af, uint32_t okb) {
         if (!TRMM) {
             if (ph.kind == 1) {  // acc += M_ik X_k, kept in registers
@@ -307,6 +361,10 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
             } else {  // B_i -= acc; X_i = inv(M_ii) B_i, in place
                 if (!ph.first) write_block(ph.i, T(-1), T(1));
                 __syncthreads();
+                if (Fp != nullptr && Fp[ph.i] != 0) {
+                    solve_refined(ph.i);
+                    return;
+                }
                 zero_acc();
                 compute(ph, af, okb);
                 __syncthreads();  // every wave has read B_i
@@ -379,18 +437,15 @@ template <typename T, int NBIG, int WS, bool TRMM, int NW>
 __global__ __launch_bounds__(NW * 64) void panel_kernel(int nbig, int64_t m, const T *__restrict__ Mp, int64_t mrs,
                                                         int64_t mcs, int lower, int unit, T *__restrict__ Bp,
                                                         int64_t brs, int64_t bcs, const T *__restrict__ Wp,
-                                                        int64_t wrs, int64_t wcs, T scale) {
+                                                        int64_t wrs, int64_t wcs, T scale,
+                                                        const int *__restrict__ Fp) {
     constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
-    if constexpr (NW == 8) {
-        __shared__ __attribute__((aligned(16))) T sc_raw[64 * WS];
-        panel_strip<T, NBIG, WS, TRMM, NW>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0,
-                                           Bp, brs, bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw,
-                                           (NLA_LDS T *)sc_raw);
-    } else {
-        panel_strip<T, NBIG, WS, TRMM, NW>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0,
-                                           Bp, brs, bcs, Wp, wrs, wcs, scale, (NLA_LDS T *)bs_raw);
-    }
+    // k-split partial sums (8 waves) and the refinement step's scratch
+    __shared__ __attribute__((aligned(16))) T sc_raw[(NW == 8 || !TRMM) ? 64 * WS : 1];
+    panel_strip<T, NBIG, WS, TRMM, NW>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0, Bp,
+                                       brs, bcs, Wp, wrs, wcs, scale, Fp, (NLA_LDS T *)bs_raw,
+                                       (NLA_LDS T *)sc_raw);
 }
 
 template <typename T, int NBIG, int W, int NW>
@@ -399,10 +454,12 @@ static hipError_t launch_panel_w(const PanelArgs<T> &a, hipStream_t s) {
     const int lo = a.lower ? 1 : 0, un = a.unit ? 1 : 0;
     if (a.trmm)
         hipLaunchKernelGGL((panel_kernel<T, NBIG, W, true, NW>), dim3((unsigned)groups), dim3(NW * 64), 0, s,
-                           a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+                           a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale,
+                           (const int *)nullptr);
     else
         hipLaunchKernelGGL((panel_kernel<T, NBIG, W, false, NW>), dim3((unsigned)groups), dim3(NW * 64), 0, s,
-                           a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale);
+                           a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale,
+                           a.F);
     return hipGetLastError();
 }
 
@@ -443,6 +500,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchSys<T> *__restric
     constexpr int WS = BatchStrip<T>::W;
     constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
+    __shared__ __attribute__((aligned(16))) T sc_raw[TRMM ? 1 : 64 * WS];  // refinement scratch
     __shared__ int s_idx;
     for (;;) {
         if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1);
@@ -453,7 +511,8 @@ __global__ __launch_bounds__(256) void batch_kernel(const BatchSys<T> *__restric
         const BatchTask tk = tasks[idx];
         const BatchSys<T> &S = sys[tk.sys];
         panel_strip<T, NBIG, WS, TRMM>(S.n, S.m, (int64_t)tk.strip * WS, S.M, S.mrs, S.mcs, S.lower != 0, false, S.B,
-                                       S.brs, S.bcs, S.W, S.wrs, S.wcs, S.scale, (NLA_LDS T *)bs_raw);
+                                       S.brs, S.bcs, S.W, S.wrs, S.wcs, S.scale, TRMM ? nullptr : S.F,
+                                       (NLA_LDS T *)bs_raw, (NLA_LDS T *)sc_raw);
         __syncthreads();
     }
 }

@@ -249,7 +249,9 @@ function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, 
     pb, ldb = ptr_ld(B)
     bytes = ccall((:nla_mixed_workspace_bytes, libnlahip), Int64, (Cchar, Int64, Int64), Cchar(side), n, m)
     buf, ws = workspace(bytes)
-    info = zeros(UInt8, 4 + 4 + 8 * 32)  # nla_mixed_info: int32 iterations, int32 precision, double[1 + 31]
+    # nla_mixed_info: int32 iterations, int32 precision, double backward_error, double history[31],
+    # double backward_error_frobenius
+    info = zeros(UInt8, 4 + 4 + 8 + 8 * 31 + 8)
     st = GC.@preserve A B buf info ccall((:nla_rectrxm_mixed, libnlahip), Int32,
         (Cchar, Cchar, Cchar, Int32, Float64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Float64, Int32,
          Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{UInt8}),

@@ -31,7 +31,8 @@ NLA_MIXED_MAX_ITER = 30
 class MixedInfo(ctypes.Structure):
     """nla_mixed_info of include/nla.h."""
     _fields_ = [("iterations", ctypes.c_int32), ("precision", ctypes.c_int32), ("backward_error", ctypes.c_double),
-                ("history", ctypes.c_double * (NLA_MIXED_MAX_ITER + 1))]
+                ("history", ctypes.c_double * (NLA_MIXED_MAX_ITER + 1)),
+                ("backward_error_frobenius", ctypes.c_double)]
 
 
 # name -> (restype, argtypes); must list every function of include/nla.h

@@ -5,7 +5,10 @@ The FP64 solution it converges to is the one unified_rectrxm!(side, uplo,
 transpose, alpha, 'S', A, B) computes (src/rectrxm.jl:43-76); the correction
 solves use the same recursion on an fp32 copy of A with bf16 / fp16 / fp32
 GEMM updates, and each step's FP64 residual alpha*B - op(A)*X decides when to
-stop (normwise backward error ||R||_F / (||A||_F ||X||_F + ||alpha*B||_F)).
+stop: the normwise backward error of every system (column of B for side 'L',
+row for 'R'), max_j ||r_j|| / (||A||_lb ||x_j|| + ||alpha*b_j||) with ||A||_lb
+the largest column 2-norm of A (<= ||A||_2); the whole-matrix Frobenius form
+is reported beside it as backward_error_frobenius.
 """
 import ctypes
 
@@ -55,7 +58,7 @@ def rectrxm_mixed_(side, uplo, transpose, alpha, A, B, precision="fp16", tol=1e-
                                        ldb, float(tol), int(max_iter), base + off, need, _stream(B),
                                        ctypes.byref(info))
     out = {"iterations": info.iterations, "precision": _NAMES.get(info.precision, info.precision),
-           "backward_error": info.backward_error,
+           "backward_error": info.backward_error, "backward_error_frobenius": info.backward_error_frobenius,
            "history": [info.history[i] for i in range(min(info.iterations, _lib.NLA_MIXED_MAX_ITER) + 1)]}
     if st == _lib.NLA_ERR_NOT_CONVERGED and not raise_on_stall:
         out["converged"] = False
