@@ -243,8 +243,11 @@ int32_t nla_larfb(char side, char trans, char direct, char storev, int32_t dtype
  *     NLA_LP_BF16 / NLA_LP_FP16 round both operands after a power-of-two
  *     scaling per operand block and accumulate in fp32 on MFMA;
  *     NLA_LP_FP32 is the plain fp32 update;
- *   - every step forms R = alpha·B - op(A)·X (or alpha·B - X·op(A)) in FP64
- *     (MFMA GEMM restricted to the triangle) and the normwise backward error
+ *   - every step forms R = alpha·B - op(A)·X (or alpha·B - X·op(A)) to FP64
+ *     accuracy (nla_residual: int8 modular products for the shapes
+ *     NLA_CFG_RESIDUAL allows, which round the iterate X in B to >= 55 bits
+ *     of each system's largest entry first; else the FP64 MFMA GEMM
+ *     restricted to the triangle) and the normwise backward error
  *     of every system (a column of B for side 'L', a row for side 'R'):
  *     max_j ||r_j||_2 / (||A||_lb ||x_j||_2 + ||alpha·b_j||_2), where
  *     ||A||_lb is the largest column 2-norm of A's triangle, a lower bound of
@@ -262,7 +265,8 @@ int32_t nla_larfb(char side, char trans, char direct, char storev, int32_t dtype
  * max_iter (<= NLA_MIXED_MAX_ITER) correction steps the error is still above
  * tol; B then holds the last iterate and info its error.  The workspace
  * (nla_mixed_workspace_bytes, 256-byte aligned) holds about
- * 4*n^2 + 20*n*m bytes.
+ * 4*n^2 + 20*n*m bytes, plus nla_residual_workspace_bytes(NLA_RES_INT8)
+ * when the int8 residual is used (about 8.5*n^2 + 43*n*m at n = 16384).
  * Argument positions: 1 side, 2 uplo, 3 trans, 4 precision, 5 alpha, 6 n,
  * 7 m, 8 A, 9 lda, 10 B, 11 ldb, 12 tol, 13 max_iter, 14 workspace,
  * 15 workspace_bytes, 16 stream, 17 info.
@@ -277,6 +281,7 @@ typedef struct nla_mixed_info {
     double backward_error;  /* per-system form (the stopping test) of the X returned in B */
     double history[NLA_MIXED_MAX_ITER + 1]; /* backward_error after 0, 1, ... steps */
     double backward_error_frobenius; /* whole-matrix (Frobenius) form of the same X */
+    int32_t residual;       /* nla_residual_method of the last residual */
 } nla_mixed_info;
 
 int64_t nla_mixed_workspace_bytes(char side, int64_t n, int64_t m);
@@ -284,6 +289,40 @@ int64_t nla_mixed_workspace_bytes(char side, int64_t n, int64_t m);
 int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, double alpha, int64_t n, int64_t m,
                           const double *A, int64_t lda, double *B, int64_t ldb, double tol, int32_t max_iter,
                           void *workspace, int64_t workspace_bytes, void *stream, nla_mixed_info *info);
+
+/*
+ * FP64-accurate residual of a triangular system (no counterpart in the
+ * reference; the residual step of nla_rectrxm_mixed on its own):
+ *   R = alpha·B - op(A)·X (side 'L': X, B, R are n x m) or
+ *   R = alpha·B - X·op(A) (side 'R': m x n), FP64 throughout.
+ *   NLA_RES_FP64: the FP64 MFMA GEMM restricted to A's triangle; X is read
+ *                 only.
+ *   NLA_RES_INT8: Ozaki's scheme II on the int8 MFMA.  op(A)'s rows are
+ *                 rounded to bA and X's systems (columns for side 'L', rows
+ *                 for 'R') to bX fixed-point bits of their largest entry
+ *                 (bA, bX >= 55, from the moduli count the order needs), X
+ *                 is overwritten by its rounded value, and R is the residual
+ *                 of that X for the rounded A computed exactly (as
+ *                 integers modulo up to 20 moduli <= 256, then combined by
+ *                 the Chinese remainder theorem) and rounded once to FP64.
+ *                 n <= 65536 (NLA_ERR_UNSUPPORTED above).  The call
+ *                 synchronizes `stream` once to read its range check; when
+ *                 an input is not finite or alpha·B lies outside the
+ *                 moduli's range, R is recomputed by the FP64 method.
+ * R may not alias B or X.  The workspace (nla_residual_workspace_bytes,
+ * 256-byte aligned): about (T/2 + 2T)·n·m + T·n^2/2 bytes for int8 with
+ * T ~ 17 moduli, none beyond 256 bytes for FP64.
+ * Argument positions: 1 side, 2 uplo, 3 trans, 4 method, 5 alpha, 6 n, 7 m,
+ * 8 A, 9 lda, 10 X, 11 ldx, 12 B, 13 ldb, 14 R, 15 ldr, 16 workspace,
+ * 17 workspace_bytes, 18 stream.
+ */
+enum nla_residual_method { NLA_RES_FP64 = 0, NLA_RES_INT8 = 1 };
+
+int64_t nla_residual_workspace_bytes(char side, int32_t method, int64_t n, int64_t m);
+
+int32_t nla_residual(char side, char uplo, char trans, int32_t method, double alpha, int64_t n, int64_t m,
+                     const double *A, int64_t lda, double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
+                     int64_t ldr, void *workspace, int64_t workspace_bytes, void *stream);
 
 /*
  * Batched FP64-accurate TRSM by fp32 correction solves and FP64 iterative
@@ -363,6 +402,11 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      rows up to 131072 rows (default 1: every leaf up to
  *                      that size; 0 = the one-work-group leaf that works in
  *                      global memory, for every size).
+ *   NLA_CFG_RESIDUAL:  residual of nla_rectrxm_mixed: NLA_RES_INT8 (default)
+ *                      for n in [2048, 65536] and m >= 64, FP64 otherwise;
+ *                      NLA_RES_FP64 for every size.  Read when the
+ *                      workspace size is queried and when the solve starts
+ *                      (the int8 residual needs the larger workspace).
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {
@@ -374,7 +418,8 @@ enum nla_config_key {
     NLA_CFG_PANEL_STRIP_F64 = 5,
     NLA_CFG_BATCH_CLASSES = 6,
     NLA_CFG_PANEL_WAVES = 7,
-    NLA_CFG_LU_SPLIT_ROWS = 8
+    NLA_CFG_LU_SPLIT_ROWS = 8,
+    NLA_CFG_RESIDUAL = 9
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);

@@ -128,6 +128,12 @@ static std::atomic<int> g_panel_waves{0};
 static std::atomic<int> g_batch_classes{0x1F};
 
 int batch_class_mask() { return g_batch_classes.load(); }
+// Residual of the refined solve: 1 = int8 modular products (Ozaki scheme II)
+// where the shape allows, 0 = FP64 MFMA GEMM.
+static std::atomic<int> g_residual{1};
+// Smallest shape given the int8 residual: below it the FP64 product is a
+// small part of the solve and the residue conversions cost as much.
+constexpr int64_t OZ_MIN_N = 2048, OZ_MIN_M = 64;
 // Rows from which an LU leaf is held in registers (one work-group up to 4096
 // rows, several above); 0 = the global-memory leaf for every size.
 static std::atomic<int> g_lu_split_rows{1};
@@ -556,6 +562,10 @@ struct MixedLayout {
     int64_t amax_slots;
     int64_t nsys;  // systems: columns of B (side L) or rows (side R)
     int64_t a32, w, wf, s, r, d, amax, part, norms, queue, spart, xpart, s2, total;
+    // int8 residual: its plan and workspace (after the fields above)
+    bool oz;
+    OzPlan op;
+    OzLayout ol;
 };
 
 static inline int64_t al256(int64_t x) { return (x + 255) & ~(int64_t)255; }
@@ -585,6 +595,12 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     L.xpart = o; o = al256(o + np * 8);
     L.s2 = o;    o = al256(o + (m > 0 ? m : 1) * 8);
     L.total = o;
+    L.oz = g_residual.load() == 1 && n >= OZ_MIN_N && n <= OZ_N_MAX && m >= OZ_MIN_M;
+    if (L.oz) {
+        L.op = oz_plan(n, m);
+        L.ol = oz_layout(L.op, o);
+        L.total = L.ol.total;
+    }
     return L;
 }
 
@@ -636,9 +652,20 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     NLA_CK(launch_sys_fold(ncha, n, spart, nullptr, nrm + 0, s));
     NLA_CK(launch_sys_sumsq(L.rows, L.cols, S, L.ldr, by_col, 0, nchb, spart, s));
     NLA_CK(launch_sys_fold(nchb, L.nsys, spart, s2, nrm + 3, s));
+    // int8 residual: A's row scales and residue planes (Ahat = op(A) for side
+    // L, op(A)^T for side R, so that the residual is Ahat * Xhat or its
+    // transpose), kept for every step
+    bool use_oz = L.oz;
+    const bool ahat_is_a = (side == 'L') != tr;
+    const bool ahat_lower = ahat_is_a ? (uplo == 'L') : (uplo == 'U');
+    int hflag = 0;
+    if (use_oz)
+        NLA_CK(oz_prepare(L.op, L.ol, ws, A, ahat_is_a ? 1 : lda, ahat_is_a ? lda : 1, ahat_lower, n, s));
     double h[6];
     NLA_CK(hipMemcpyAsync(h, nrm, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (use_oz) NLA_CK(hipMemcpyAsync(&hflag, ws + L.ol.flag, sizeof(int), hipMemcpyDeviceToHost, s));
     NLA_CK(hipStreamSynchronize(s));
+    if (hflag) use_oz = false;  // A not finite: FP64 residuals
     const double nA = std::sqrt(h[0]), nS = std::sqrt(h[3]);
     const double maxS = h[4];
     (void)part;
@@ -741,18 +768,44 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     const double res_fl = (double)n * n * m;
     const double res_by = ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8;
 
+    auto fp64_residual = [&]() -> hipError_t {
+        hipError_t e2 = launch_scale_copy(L.rows, L.cols, 1.0, S, L.ldr, R, L.ldr, s);
+        if (e2 == hipSuccess) e2 = launch_fill_u32((unsigned *)g.queue, 1, 0u, s);
+        if (e2 == hipSuccess) e2 = launch_gemm<double>(g, s);
+        return e2;
+    };
+    // Xhat(k, j) = X[j*xso + k*xsk]: a system is a column of X (side L) or a
+    // row (side R); X is rounded in place to bX bits per system
+    const int64_t xso = (side == 'L') ? ldb : 1, xsk = (side == 'L') ? 1 : ldb;
+    auto int8_residual = [&]() -> hipError_t {
+        return oz_residual(L.op, L.ol, ws, ahat_lower, side == 'R', B, xso, xsk, n, m, S, L.ldr, R, L.ldr, s);
+    };
+    // per-system test: max_j ||r_j|| / (||A||_lb ||x_j|| + ||s_j||), with the
+    // int8 residual's range flag read in the same synchronization
+    auto stop_norms = [&](bool with_flag) -> hipError_t {
+        hipError_t e2 = launch_sys_sumsq(L.rows, L.cols, R, L.ldr, by_col, 0, nchb, spart, s);
+        if (e2 == hipSuccess) e2 = launch_sys_sumsq(L.rows, L.cols, B, ldb, by_col, 0, nchb, xpart, s);
+        if (e2 == hipSuccess) e2 = launch_sys_eta(nchb, L.nsys, spart, xpart, s2, nrm + 2, nrm + 6, s);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(h, nrm + 6, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+        hflag = 0;
+        if (e2 == hipSuccess && with_flag)
+            e2 = hipMemcpyAsync(&hflag, ws + L.ol.flag, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(s);
+        return e2;
+    };
+
     double prev = INFINITY, bwd = INFINITY, bwd_f = INFINITY;
     int it = 0;
     for (;; ++it) {
-        NLA_CK(launch_scale_copy(L.rows, L.cols, 1.0, S, L.ldr, R, L.ldr, s));
-        NLA_CK(launch_fill_u32((unsigned *)g.queue, 1, 0u, s));
-        NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, [&] { return launch_gemm<double>(g, s); }));
-        // per-system test: max_j ||r_j|| / (||A||_lb ||x_j|| + ||s_j||)
-        NLA_CK(launch_sys_sumsq(L.rows, L.cols, R, L.ldr, by_col, 0, nchb, spart, s));
-        NLA_CK(launch_sys_sumsq(L.rows, L.cols, B, ldb, by_col, 0, nchb, xpart, s));
-        NLA_CK(launch_sys_eta(nchb, L.nsys, spart, xpart, s2, nrm + 2, nrm + 6, s));
-        NLA_CK(hipMemcpyAsync(h, nrm + 6, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-        NLA_CK(hipStreamSynchronize(s));
+        NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, [&] { return use_oz ? int8_residual() : fp64_residual(); }));
+        NLA_CK(stop_norms(use_oz));
+        if (use_oz && hflag) {
+            // S or X outside the moduli's range (or not finite): this and
+            // every later residual in FP64
+            use_oz = false;
+            NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, fp64_residual));
+            NLA_CK(stop_norms(false));
+        }
         const double nR = std::sqrt(h[0]), nX = std::sqrt(h[2]);
         const double den = nA * nX + nS;
         bwd_f = den > 0.0 ? nR / den : (nR > 0.0 ? INFINITY : 0.0);
@@ -781,6 +834,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         info->precision = prec;
         info->backward_error = bwd;
         info->backward_error_frobenius = bwd_f;
+        info->residual = use_oz ? NLA_RES_INT8 : NLA_RES_FP64;
     }
     if (!(bwd <= tol)) {
         std::snprintf(g_last_error, sizeof(g_last_error),
@@ -968,6 +1022,100 @@ int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, d
                               (char *)workspace, L, (hipStream_t)stream, info);
 }
 
+int64_t nla_residual_workspace_bytes(char side, int32_t method, int64_t n, int64_t m) {
+    side = up(side);
+    if ((side != 'L' && side != 'R') || n < 0 || m < 0) return -1;
+    if (method == NLA_RES_FP64) return 256;
+    if (method != NLA_RES_INT8 || n > OZ_N_MAX) return -1;
+    if (n == 0 || m == 0) return 256;
+    return oz_layout(oz_plan(n, m), 0).total;
+}
+
+int32_t nla_residual(char side, char uplo, char trans, int32_t method, double alpha, int64_t n, int64_t m,
+                     const double *A, int64_t lda, double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
+                     int64_t ldr, void *workspace, int64_t workspace_bytes, void *stream) {
+    side = up(side); uplo = up(uplo); trans = up(trans);
+    if (side != 'L' && side != 'R') return bad_arg(1, "side");
+    if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
+    if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
+    if (method != NLA_RES_FP64 && method != NLA_RES_INT8) return bad_arg(4, "method");
+    if (!std::isfinite(alpha)) return bad_arg(5, "alpha");
+    if (n < 0) return bad_arg(6, "n");
+    if (m < 0) return bad_arg(7, "m");
+    if (n == 0 || m == 0) return NLA_OK;
+    const int64_t rows = side == 'L' ? n : m;
+    if (A == nullptr) return bad_arg(8, "A");
+    if (lda < n) return bad_arg(9, "lda");
+    if (X == nullptr) return bad_arg(10, "X");
+    if (ldx < rows) return bad_arg(11, "ldx");
+    if (B == nullptr) return bad_arg(12, "B");
+    if (ldb < rows) return bad_arg(13, "ldb");
+    if (R == nullptr) return bad_arg(14, "R");
+    if (ldr < rows) return bad_arg(15, "ldr");
+    if (method == NLA_RES_INT8 && n > OZ_N_MAX) {
+        std::snprintf(g_last_error, sizeof(g_last_error), "int8 residual: n = %lld above %lld", (long long)n,
+                      (long long)OZ_N_MAX);
+        return NLA_ERR_UNSUPPORTED;
+    }
+    const int64_t need = nla_residual_workspace_bytes(side, method, n, m);
+    if (workspace == nullptr || ((uintptr_t)workspace & 255) != 0)
+        return bad_arg(16, "workspace (needs 256-byte alignment)");
+    if (workspace_bytes < need) return bad_arg(17, "workspace_bytes");
+    hipStream_t s = (hipStream_t)stream;
+    char *ws = (char *)workspace;
+    const bool tr = (trans != 'N');
+    const char ue = tr ? (uplo == 'L' ? 'U' : 'L') : uplo;
+    const int64_t cols = side == 'L' ? m : n;
+    hipError_t e = launch_scale_copy(rows, cols, alpha, B, ldb, R, ldr, s);
+    if (e != hipSuccess) return hip_status(e, "residual");
+    GemmArgs<double> g;
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.C = R;
+    g.ldc = ldr;
+    g.K = n;
+    if (side == 'L') {
+        g.M = n; g.N = m;
+        g.X = A; g.tx = tr; g.ldx = lda;
+        g.Y = X; g.ty = false; g.ldy = ldx;
+        g.tri = (ue == 'L') ? 1 : 2;
+    } else {
+        g.M = m; g.N = n;
+        g.X = X; g.tx = false; g.ldx = ldx;
+        g.Y = A; g.ty = tr; g.ldy = lda;
+        g.tri = (ue == 'L') ? 3 : 4;
+    }
+    auto fp64 = [&]() -> hipError_t {
+        g.queue = nullptr;
+        return timed(NLA_KIND_RESIDUAL, (double)n * n * m, ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8, s,
+                     [&] { return launch_gemm<double>(g, s); });
+    };
+    if (method == NLA_RES_FP64) return hip_status(fp64(), "residual");
+    const OzPlan P = oz_plan(n, m);
+    const OzLayout L = oz_layout(P, 0);
+    const bool ahat_is_a = (side == 'L') != tr;
+    const bool ahat_lower = ahat_is_a ? (uplo == 'L') : (uplo == 'U');
+    const int64_t xso = (side == 'L') ? ldx : 1, xsk = (side == 'L') ? 1 : ldx;
+    e = timed(NLA_KIND_RESIDUAL, (double)n * n * m, ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8, s, [&] {
+        hipError_t e2 = oz_prepare(P, L, ws, A, ahat_is_a ? 1 : lda, ahat_is_a ? lda : 1, ahat_lower, n, s);
+        if (e2 == hipSuccess)
+            e2 = oz_residual(P, L, ws, ahat_lower, side == 'R', X, xso, xsk, n, m, R, ldr, R, ldr, s);
+        return e2;
+    });
+    if (e != hipSuccess) return hip_status(e, "residual");
+    int flag = 0;
+    e = hipMemcpyAsync(&flag, ws + L.flag, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e, "residual");
+    if (flag) {
+        // an input outside the moduli's range or not finite: FP64 product
+        e = launch_scale_copy(rows, cols, alpha, B, ldb, R, ldr, s);
+        if (e == hipSuccess) e = fp64();
+        if (e != hipSuccess) return hip_status(e, "residual");
+    }
+    return NLA_OK;
+}
+
 int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_LEAF_MODE) {
         int prev = leaf_mode_default();
@@ -994,6 +1142,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_PANEL_WAVES) {
         if (value != 0 && value != 4 && value != 8) return -1;
         return g_panel_waves.exchange(value);
+    }
+    if (key == NLA_CFG_RESIDUAL) {
+        if (value != NLA_RES_FP64 && value != NLA_RES_INT8) return -1;
+        return g_residual.exchange(value);
     }
     if (key == NLA_CFG_PANEL_STRIP_F32 || key == NLA_CFG_PANEL_STRIP_F64) {
         if (value != 16 && value != 32) return -1;

@@ -271,6 +271,61 @@ hipError_t launch_sys_fold(int nch, int64_t nsys, const double *part, double *sy
 hipError_t launch_sys_eta(int nch, int64_t nsys, const double *pr, const double *px, const double *s2,
                           const double *anorm_lb, double *out, hipStream_t s);
 
+// ---- FP64-accurate residual on the int8 MFMA (nla_ozaki.hip) -------------
+
+// Pairwise coprime moduli, largest first; a residual uses the first T.
+constexpr int OZ_TMAX = 20;
+constexpr int OZ_MODULI[OZ_TMAX] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                    223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+// Entries of the 2^s mod p tables (s < OZ_PW).
+constexpr int OZ_PW = 128;
+// Largest order: int32 sums of K products of residues (|r| <= 128) stay
+// exact for K <= 2^17; 2^16 also keeps the grids of every launch in range.
+constexpr int64_t OZ_N_MAX = 65536;
+// Fewest bits kept of A's rows and X's columns (rounding error 2^-55 of a
+// row's / column's largest entry).
+constexpr int OZ_BITS_MIN = 55;
+
+// Constants of the modular products and the reconstruction (kernel argument).
+struct OzConst {
+    int T;
+    int p[OZ_TMAX];
+    int y[OZ_TMAX];  // (M / p_t)^-1 mod p_t
+    float invpf[OZ_TMAX];
+    double invpd[OZ_TMAX];
+    double c1[OZ_TMAX], c2[OZ_TMAX], c3[OZ_TMAX];  // 1/p_t on grids 2^-45, 2^-86, 2^-127
+    double M;     // product of the T moduli
+    double smax;  // largest |S_int| the moduli cover (M / 8)
+};
+
+// Moduli count and bit split for an order-n residual, and the residue planes'
+// sizes (256-padded orders npad, mpad).  T = 0: n outside (0, OZ_N_MAX].
+struct OzPlan {
+    int T, bA, bX;
+    int64_t npad, mpad, aplane, xplane, cplane;
+    OzConst C;
+};
+OzPlan oz_plan(int64_t n, int64_t m);
+
+// Workspace offsets (bytes from the workspace start) of an OzPlan.
+struct OzLayout {
+    int64_t a, x, c, ea, fx, mxa, mxx, pw, flag, queue, total;
+};
+OzLayout oz_layout(const OzPlan &P, int64_t base);
+
+// Once per solve: A's row scales and residue planes.  Ahat(i,k) =
+// A[i*ars + k*acs] (the triangle of op(A), or of its transpose for side R,
+// lower or upper).  Clears the flag word; a non-finite A sets it.
+hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double *A, int64_t ars, int64_t acs,
+                      bool lower, int64_t n, hipStream_t s);
+// R = S - Ahat * Xhat (out_t = false: R n x m, Xhat(k,j) = X[j*xso + k*xsk])
+// or its transpose (out_t = true: R m x n, side R).  X is replaced by its
+// value rounded to bX bits per system.  Sets the flag word when S or X is
+// not finite or S exceeds the moduli's range (R is then not valid).
+hipError_t oz_residual(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, double *X,
+                       int64_t xso, int64_t xsk, int64_t n, int64_t m, const double *S, int64_t lds, double *R,
+                       int64_t ldr, hipStream_t s);
+
 // ---- batched refinement ----------------------------------------------------
 
 // One FP64 system of a refined batch and its work arrays (column-major; the

@@ -1,0 +1,642 @@
+// FP64-accurate residual R = S - A*X of the refined solve on the int8 MFMA
+// (Ozaki's scheme II: products of integers modulo small moduli, combined by
+// the Chinese remainder theorem).
+//
+// The residual the refinement needs (src/rectrxm.jl:62-68 is the solve it
+// converges to) costs n^2 m flops, as much as the TRSM itself, and FP64 MFMA
+// runs at the FP64 vector rate on MI355X; the int8 MFMA runs at 64 times
+// that rate.  With A's rows and X's columns scaled to integers,
+//     A_int(i,k) = round(A(i,k) 2^e_i),  |A_int| < 2^bA,
+//     X_int(k,j) = round(X(k,j) 2^f_j),  |X_int| < 2^bX,
+//     S_int(i,j) = round(S(i,j) 2^(e_i + f_j)),
+// the integer R_int = S_int - sum_k A_int X_int is known exactly once it is
+// known modulo T pairwise coprime moduli p_t <= 256 whose product M exceeds
+// 2 |R_int|: each residue plane sum_k (A_int mod p_t)(X_int mod p_t) is an
+// exact int32 sum of int8 products (the int8 MFMA), and the reconstruction
+// recovers R_int in (-M/2, M/2] and returns R = R_int 2^-(e_i + f_j).
+//
+// Accuracy: the only rounding is that of A to bA bits per row and of X to
+// bX bits per column.  X is the refinement's own iterate, so its rounded
+// value is written back (the residual is then that of the returned X); the
+// rounding of A is a backward perturbation of at most 2^-bA of each row's
+// largest entry.  With bA, bX >= 55 both stay far below the 1e-14 target.
+//
+// Layout (all int8 / uint8, 256-aligned dimensions npad, mpad):
+//   A residues: the triangle of op(A) (or its transpose for side R), in row
+//     panels of 256 rows holding only the k range of the triangle, k
+//     contiguous;
+//   X residues: (system j, k) at j*npad + k;
+//   product residues: column-major like R.
+#include "nla_device.h"
+#include "nla_internal.h"
+
+namespace nla {
+
+#define NLA_LDS __attribute__((address_space(3)))
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// v mod p in [0, p) for an integer-valued double |v| < 2^53; invp = 1/p.
+// floor(v/p) from the product is off by at most one (its relative error is
+// 2^-52), and v - p*q is an integer below 2p in magnitude, so the fma is
+// exact.
+__device__ __forceinline__ int oz_mod53(double v, double p, double invp) {
+    const double q = floor(v * invp);
+    int r = (int)fma(-p, q, v);
+    const int pi = (int)p;
+    r += (r < 0) ? pi : 0;
+    r -= (r >= pi) ? pi : 0;
+    return r;
+}
+
+// x mod p in [0, p) for 0 <= x < 2^24 (exact in float); invp = 1/p.
+__device__ __forceinline__ int oz_mod24(int x, int p, float invp) {
+    const int q = (int)((float)x * invp);
+    int r = x - q * p;
+    r += (r < 0) ? p : 0;
+    r -= (r >= p) ? p : 0;
+    return r;
+}
+
+// An integer-valued double v as m * 2^sh with |m| < 2^53 integer, sh >= 0.
+__device__ __forceinline__ void oz_split(double v, double &m, int &sh) {
+    sh = 0;
+    m = v;
+    if (fabs(v) >= 0x1p53) {
+        sh = ilogb(v) - 52;
+        m = ldexp(v, -sh);
+    }
+}
+
+// Residue of m * 2^sh modulo p_t: pw = the 2^s mod p_t table of modulus t.
+__device__ __forceinline__ int oz_residue(double m, int sh, const OzConst &C, int t, const uint8_t *pw) {
+    const int r = oz_mod53(m, (double)C.p[t], C.invpd[t]);
+    if (sh == 0) return r;
+    return oz_mod24(r * (int)pw[sh], C.p[t], C.invpf[t]);
+}
+
+// ---- tables -----------------------------------------------------------------
+
+// pw[t*OZ_PW + s] = 2^s mod p_t, s < OZ_PW.
+__global__ void oz_pow2_kernel(OzConst C, uint8_t *pw) {
+    const int t = blockIdx.x, s = threadIdx.x;
+    if (t >= C.T || s >= OZ_PW) return;
+    const int p = C.p[t];
+    int v = 1 % p;
+    for (int i = 0; i < s; ++i) v = (2 * v) % p;
+    pw[t * OZ_PW + s] = (uint8_t)v;
+}
+
+// ---- scales -----------------------------------------------------------------
+
+// Source matrix of a conversion: element (o, k) at P[o*so + k*sk], o < no,
+// k < nk; tri = 1 keeps k <= o, 2 keeps k >= o (the triangle of op(A)).
+struct OzSrc {
+    double *P;
+    int64_t so, sk, no, nk;
+    int tri;
+};
+
+__device__ __forceinline__ bool oz_keep(const OzSrc &s, int64_t o, int64_t k) {
+    if (o >= s.no || k >= s.nk) return false;
+    if (s.tri == 1) return k <= o;
+    if (s.tri == 2) return k >= o;
+    return true;
+}
+
+// Stages the 64 x 64 tile (o0, k0) of the source into LDS (zero outside the
+// kept region), loading along whichever stride is 1.
+__device__ __forceinline__ void oz_load_tile(const OzSrc &s, int64_t o0, int64_t k0, double (*tile)[65]) {
+    const bool kfast = (s.sk == 1);
+    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+        const int a = idx & 63, b = idx >> 6;
+        const int oo = kfast ? b : a, ko = kfast ? a : b;
+        const int64_t o = o0 + oo, k = k0 + ko;
+        const bool ok = oz_keep(s, o, k);
+        tile[oo][ko] = ok ? s.P[o * s.so + k * s.sk] : 0.0;
+    }
+}
+
+// mx[o] = max(mx[o], max_k |src(o, k)|) as the bit pattern of a
+// non-negative double (which orders like the value; NaN orders above
+// infinity).  Grid: (ceil(nk/64), ceil(no/64)).
+__global__ __launch_bounds__(256) void oz_absmax_kernel(OzSrc s, unsigned long long *mx) {
+    __shared__ double tile[64][65];
+    const int64_t o0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
+    if (s.tri == 1 && k0 > o0 + 63) return;
+    if (s.tri == 2 && k0 + 63 < o0) return;
+    oz_load_tile(s, o0, k0, tile);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int oo = threadIdx.x;
+        unsigned long long best = 0;
+        for (int k = 0; k < 64; ++k) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(tile[oo][k]));
+            best = b > best ? b : best;
+        }
+        if (o0 + oo < s.no && best != 0) atomicMax(mx + o0 + oo, best);
+    }
+}
+
+// e[o] = bits - 1 - floor(log2 mx[o]) (so that max |src(o,.)| 2^e[o] lies in
+// [2^(bits-1), 2^bits)), 0 for an all-zero row; a non-finite maximum sets
+// *flag.
+__global__ void oz_exponent_kernel(int64_t no, const unsigned long long *mx, int bits, int *e, int *flag) {
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= no) return;
+    const double v = __longlong_as_double((long long)mx[o]);
+    int r = 0;
+    if (!(v <= 1.7976931348623157e308)) {
+        *flag = 1;
+    } else if (v > 0.0) {
+        r = bits - 1 - ilogb(v);
+    }
+    e[o] = r;
+}
+
+// ---- residues ---------------------------------------------------------------
+
+// Destination of the residue planes: plane t at base + t*plane; element
+// (o, k) at o*ld + k (panel = 0) or in the row-panel layout of a lower
+// (panel = 1) / upper (panel = 2) triangle of order npad.
+struct OzDst {
+    int8_t *base;
+    int64_t plane, ld, npad;
+    int panel;
+};
+
+__device__ __forceinline__ int64_t oz_panel_off(int panel, int64_t npad, int64_t o, int64_t k) {
+    const int64_t tm = o >> 8, il = o & 255;
+    if (panel == 1) return (int64_t)65536 * (tm * (tm + 1) / 2) + il * ((tm + 1) * 256) + k;
+    return (int64_t)256 * (tm * npad - 256 * (tm * (tm - 1) / 2)) + il * (npad - tm * 256) + (k - tm * 256);
+}
+
+// Whether the 64 x 64 tile (o0, k0) lies in the stored region of a panel
+// layout (row panel tm = o0/256 holds k < (tm+1)*256 (lower) or k >= tm*256).
+__device__ __forceinline__ bool oz_tile_stored(int panel, int64_t o0, int64_t k0) {
+    const int64_t tm = o0 >> 8;
+    if (panel == 1) return k0 < (tm + 1) * 256;
+    if (panel == 2) return k0 >= tm * 256;
+    return true;
+}
+
+// Residues of round(src(o,k) 2^e[o]) modulo every p_t, as int8 in the
+// symmetric range, written to the T planes of dst; with writeback the
+// rounded value round(src 2^e) 2^-e replaces src (the refinement's iterate).
+// Grid: (npad_k/64, npad_o/64) over the whole padded destination, so the
+// padding holds zeros.
+__global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, OzDst d, int writeback, OzConst C,
+                                                         const uint8_t *pw_g) {
+    __shared__ double tile[64][65];
+    __shared__ uint8_t pw[OZ_TMAX * 16];
+    const int64_t o0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
+    if (!oz_tile_stored(d.panel, o0, k0)) return;
+    for (int i = threadIdx.x; i < C.T * 16; i += 256) pw[i] = pw_g[(i >> 4) * OZ_PW + (i & 15)];
+    oz_load_tile(s, o0, k0, tile);
+    __syncthreads();
+    const int oo = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+    const int64_t o = o0 + oo;
+    const int eo = (o < s.no) ? e[o] : 0;
+    double m[16];
+    int sh[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const double v = rint(ldexp(tile[oo][kq + u], eo));
+        oz_split(v, m[u], sh[u]);
+        if (writeback) tile[oo][kq + u] = ldexp(v, -eo);
+    }
+    const int64_t off = d.panel ? oz_panel_off(d.panel, d.npad, o, k0 + kq) : o * d.ld + k0 + kq;
+    for (int t = 0; t < C.T; ++t) {
+        const int p = C.p[t];
+        const int half = p >> 1;  // residues above half map to r - p
+        v4i packed;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            unsigned word = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                int r = oz_residue(m[4 * w + b], sh[4 * w + b], C, t, pw + t * 16);
+                r = (r > half || (r == half && (p & 1) == 0)) ? r - p : r;
+                word |= (unsigned)(r & 255) << (8 * b);
+            }
+            packed[w] = (int)word;
+        }
+        *reinterpret_cast<v4i *>(d.base + t * d.plane + off) = packed;
+    }
+    if (writeback) {
+        __syncthreads();
+        const bool kfast = (s.sk == 1);
+        for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
+            const int a = idx & 63, b = idx >> 6;
+            const int oq = kfast ? b : a, kk = kfast ? a : b;
+            const int64_t og = o0 + oq, kg = k0 + kk;
+            if (og < s.no && kg < s.nk) s.P[og * s.so + kg * s.sk] = tile[oq][kk];
+        }
+    }
+}
+
+// ---- the modular products ---------------------------------------------------
+
+// Per modulus t and 256 x 256 output tile (tm, tn):
+//   c(i, j) = sum_k Ares_t(i, k) Xres_t(j, k) (int32, exact: |c| < K 2^14),
+// stored as c mod p_t in [0, p_t).  A persistent grid takes (modulus, tile)
+// items from *queue (zero at launch) in modulus-major order, longest row
+// panels first, all column tiles of a row panel together (they share the A
+// panel in L2).  Eight waves, each a 64 x 128 block of the tile as 4 x 8
+// v_mfma_i32_16x16x64_i8 blocks; operands staged through double-buffered
+// LDS with rows padded to 80 bytes (the 16 rows a 16-lane group reads at one
+// k offset fall in distinct banks).  OUT_T: the output plane is (j, i) with
+// j contiguous (side R) instead of (i, j) with i contiguous.
+struct OzGemmArgs {
+    const int8_t *A;
+    int64_t aplane;
+    const int8_t *X;
+    int64_t xplane;
+    uint8_t *Cr;
+    int64_t cplane, ldc;
+    int64_t npad, mpad;
+    int tiles_m, tiles_n, T, lower;
+    int *queue;
+};
+
+template <bool OUT_T>
+__global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
+    constexpr int BM = 256, BN = 256, BK = 64, RS = BK + 16;
+    __shared__ __attribute__((aligned(16))) int8_t sA[2 * BM * RS];
+    __shared__ __attribute__((aligned(16))) int8_t sX[2 * BN * RS];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 3, wn = wave >> 2;
+    const int per_mod = a.tiles_m * a.tiles_n, total = per_mod * a.T;
+    const int fr = lane & 15, fk = (lane >> 4) * 16;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int w = s_item;
+        __syncthreads();
+        if (w >= total) break;
+        const int t = w / per_mod, rest = w - t * per_mod;
+        const int tm = a.lower ? a.tiles_m - 1 - rest / a.tiles_n : rest / a.tiles_n;
+        const int tn = rest % a.tiles_n;
+        int64_t L, off, kbeg;
+        if (a.lower) {
+            L = (int64_t)(tm + 1) * BM;
+            off = (int64_t)BM * BM * ((int64_t)tm * (tm + 1) / 2);
+            kbeg = 0;
+        } else {
+            L = a.npad - (int64_t)tm * BM;
+            off = (int64_t)BM * ((int64_t)tm * a.npad - (int64_t)BM * ((int64_t)tm * (tm - 1) / 2));
+            kbeg = (int64_t)tm * BM;
+        }
+        const int8_t *Ap = a.A + t * a.aplane + off;
+        const int8_t *Xp = a.X + t * a.xplane + (int64_t)tn * BN * a.npad + kbeg;
+        const int nkt = (int)(L / BK);
+
+        v4i acc[4][8];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = v4i{0, 0, 0, 0};
+
+        // thread tid moves chunks tid and tid + 512 of each 256 x 64 operand
+        // tile: row c >> 2, k offset (c & 3) * 16
+        v4i ra[2], rx[2];
+        auto gload = [&](int64_t k0) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int c = tid + 512 * q, row = c >> 2, kc = (c & 3) * 16;
+                ra[q] = *reinterpret_cast<const v4i *>(Ap + row * L + k0 + kc);
+                rx[q] = *reinterpret_cast<const v4i *>(Xp + row * a.npad + k0 + kc);
+            }
+        };
+        auto sstore = [&](int buf) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int c = tid + 512 * q, row = c >> 2, kc = (c & 3) * 16;
+                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sA + buf * BM * RS + row * RS + kc) = ra[q];
+                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sX + buf * BN * RS + row * RS + kc) = rx[q];
+            }
+        };
+        gload(0);
+        sstore(0);
+        __syncthreads();
+        for (int kt = 0; kt < nkt; ++kt) {
+            const int cur = kt & 1;
+            const bool more = kt + 1 < nkt;
+            if (more) gload((int64_t)(kt + 1) * BK);
+            const NLA_LDS int8_t *la = (const NLA_LDS int8_t *)sA + cur * BM * RS + (wm * 64 + fr) * RS + fk;
+            const NLA_LDS int8_t *lx = (const NLA_LDS int8_t *)sX + cur * BN * RS + (wn * 128 + fr) * RS + fk;
+            v4i af[4], xf[8];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) af[rb] = *reinterpret_cast<const NLA_LDS v4i *>(la + rb * 16 * RS);
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) xf[cb] = *reinterpret_cast<const NLA_LDS v4i *>(lx + cb * 16 * RS);
+            __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 8; ++cb)
+                    acc[rb][cb] = OUT_T ? __builtin_amdgcn_mfma_i32_16x16x64_i8(xf[cb], af[rb], acc[rb][cb], 0, 0, 0)
+                                        : __builtin_amdgcn_mfma_i32_16x16x64_i8(af[rb], xf[cb], acc[rb][cb], 0, 0, 0);
+            __builtin_amdgcn_s_setprio(0);
+            if (more) sstore(cur ^ 1);
+            __syncthreads();
+        }
+
+        // residues of the tile: lane holds 4 consecutive rows (columns for
+        // OUT_T) of one column (row) of each 16 x 16 block
+        const double pd = (double)C.p[t], ipd = C.invpd[t];
+        uint8_t *Cp = a.Cr + t * a.cplane;
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                unsigned word = 0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    word |= (unsigned)oz_mod53((double)acc[rb][cb][r], pd, ipd) << (8 * r);
+                const int64_t i0 = (int64_t)tm * BM + wm * 64 + rb * 16;
+                const int64_t j0 = (int64_t)tn * BN + wn * 128 + cb * 16;
+                if (!OUT_T)
+                    *reinterpret_cast<unsigned *>(Cp + (j0 + fr) * a.ldc + i0 + (lane >> 4) * 4) = word;
+                else
+                    *reinterpret_cast<unsigned *>(Cp + (i0 + fr) * a.ldc + j0 + (lane >> 4) * 4) = word;
+            }
+    }
+}
+
+// ---- reconstruction ---------------------------------------------------------
+
+// R(r, c) = S(r, c) - (A X) at (r, c) from the product residues: R_int =
+// S_int - c_t modulo p_t for every t, then x = M * centred-frac(sum_t u_t /
+// p_t) with u_t = (R_int mod p_t) (M/p_t)^-1 mod p_t.  1/p_t = c1 + c2 + c3
+// on fixed grids (2^-45, 2^-86, 2^-127) so that every partial sum u_t c is
+// exact in double; x has relative error ~2^-52 plus 2^-115 M.  er / ec: the
+// scale exponents of R's rows / columns.  |S_int| above C.smax, or a
+// non-finite S, sets *flag (the modulus product would not cover R_int).
+struct OzCrtArgs {
+    int64_t rows, cols;
+    const double *S;
+    int64_t lds;
+    double *R;
+    int64_t ldr;
+    const uint8_t *Cr;
+    int64_t cplane, ldc;
+    const int *er, *ec;
+    const uint8_t *pw;
+    int *flag;
+};
+
+__global__ __launch_bounds__(256) void oz_crt_kernel(OzCrtArgs a, OzConst C) {
+    __shared__ uint8_t pw[OZ_TMAX * OZ_PW];
+    for (int i = threadIdx.x; i < C.T * OZ_PW; i += 256) pw[i] = a.pw[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= a.rows) return;
+    const int er = a.er[r];
+    for (int64_t c = blockIdx.y; c < a.cols; c += gridDim.y) {
+        const int sc = er + a.ec[c];
+        const double v = rint(ldexp(a.S[r + c * a.lds], sc));
+        double m = 0.0;
+        int sh = 0;
+        bool bad = !(fabs(v) <= C.smax);
+        if (!bad) {
+            oz_split(v, m, sh);
+            bad = sh >= OZ_PW;
+        }
+        if (bad) {
+            *a.flag = 1;
+            a.R[r + c * a.ldr] = 0.0;
+            continue;
+        }
+        double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+        const uint8_t *cp = a.Cr + c * a.ldc + r;
+        for (int t = 0; t < C.T; ++t) {
+            const int p = C.p[t];
+            const int st = oz_residue(m, sh, C, t, pw + t * OZ_PW);
+            const int ct = cp[t * a.cplane];
+            const int d = st - ct + p;  // in (0, 2p)
+            const double u = (double)oz_mod24(d * C.y[t], p, C.invpf[t]);
+            f1 = fma(u, C.c1[t], f1);
+            f2 = fma(u, C.c2[t], f2);
+            f3 = fma(u, C.c3[t], f3);
+        }
+        const double q = rint(f1 + f2);
+        const double frac = ((f1 - q) + f2) + f3;
+        a.R[r + c * a.ldr] = ldexp(frac * C.M, -sc);
+    }
+}
+
+// ---- host side --------------------------------------------------------------
+
+static int64_t oz_round256(int64_t x) { return (x + 255) / 256 * 256; }
+
+static int64_t modinv(int64_t a, int64_t p) {
+    int64_t t = 0, nt = 1, r = p, nr = ((a % p) + p) % p;
+    while (nr != 0) {
+        const int64_t q = r / nr;
+        int64_t tmp = t - q * nt;
+        t = nt;
+        nt = tmp;
+        tmp = r - q * nr;
+        r = nr;
+        nr = tmp;
+    }
+    return (t % p + p) % p;
+}
+
+OzPlan oz_plan(int64_t n, int64_t m) {
+    OzPlan P{};
+    if (n <= 0 || m <= 0 || n > OZ_N_MAX) return P;
+    // bits of the modulus product needed: bA + bX + ceil(log2 n) + 3
+    // (|sum A_int X_int| <= n 2^(bA+bX) <= M/8 and |S_int| <= M/8 keep |R_int|
+    // <= M/4); take the fewest moduli that leave bA, bX >= OZ_BITS_MIN
+    int lg = 0;
+    while (((int64_t)1 << lg) < n) ++lg;
+    double logm = 0.0;
+    int T = 0;
+    for (; T < OZ_TMAX; ++T) {
+        logm += std::log2((double)OZ_MODULI[T]);
+        if ((int)std::floor(logm) - 3 - lg >= 2 * OZ_BITS_MIN) {
+            ++T;
+            break;
+        }
+    }
+    if (T > OZ_TMAX) T = OZ_TMAX;
+    int budget = (int)std::floor(logm) - 3 - lg;
+    if (budget > 2 * 62) budget = 2 * 62;
+    P.T = T;
+    P.bA = (budget + 1) / 2;
+    P.bX = budget - P.bA;
+    P.npad = oz_round256(n);
+    P.mpad = oz_round256(m);
+    const int64_t nt = P.npad / 256;
+    P.aplane = (int64_t)65536 * (nt * (nt + 1) / 2);
+    P.xplane = P.mpad * P.npad;
+    P.cplane = P.npad * P.mpad;
+    OzConst &C = P.C;
+    C.T = T;
+    double M = 1.0;
+    for (int t = 0; t < T; ++t) M *= (double)OZ_MODULI[t];
+    C.M = M;
+    C.smax = std::ldexp(1.0, (int)std::floor(logm) - 3);
+    for (int t = 0; t < T; ++t) {
+        const int64_t p = OZ_MODULI[t];
+        C.p[t] = (int)p;
+        C.invpf[t] = 1.0f / (float)p;
+        C.invpd[t] = 1.0 / (double)p;
+        int64_t mt = 1;  // (M / p_t) mod p_t
+        for (int s = 0; s < T; ++s)
+            if (s != t) mt = (mt * (OZ_MODULI[s] % p)) % p;
+        C.y[t] = (int)modinv(mt, p);
+        // 1/p = q1 2^-45 + q2 2^-86 + q3 2^-127 + O(2^-128), each q by
+        // rounding the exact remainder (integers, no floating rounding)
+        const int64_t q1 = ((int64_t)1 << 45) / p + ((((int64_t)1 << 45) % p) * 2 >= p ? 1 : 0);
+        const int64_t n1 = ((int64_t)1 << 45) - q1 * p;  // |n1| <= p/2
+        const int64_t a2 = n1 * ((int64_t)1 << 41);
+        int64_t q2 = a2 / p;
+        int64_t r2 = a2 - q2 * p;
+        if (2 * r2 >= p) { ++q2; r2 -= p; }
+        if (2 * r2 < -p) { --q2; r2 += p; }
+        const int64_t a3 = r2 * ((int64_t)1 << 41);
+        int64_t q3 = a3 / p;
+        int64_t r3 = a3 - q3 * p;
+        if (2 * r3 >= p) ++q3;
+        if (2 * r3 < -p) --q3;
+        C.c1[t] = std::ldexp((double)q1, -45);
+        C.c2[t] = std::ldexp((double)q2, -86);
+        C.c3[t] = std::ldexp((double)q3, -127);
+    }
+    return P;
+}
+
+OzLayout oz_layout(const OzPlan &P, int64_t base) {
+    OzLayout L{};
+    auto al = [](int64_t x) { return (x + 255) & ~(int64_t)255; };
+    int64_t o = al(base);
+    L.a = o;     o = al(o + P.T * P.aplane);
+    L.x = o;     o = al(o + P.T * P.xplane);
+    L.c = o;     o = al(o + P.T * P.cplane);
+    L.ea = o;    o = al(o + P.npad * 4);
+    L.fx = o;    o = al(o + P.mpad * 4);
+    L.mxa = o;   o = al(o + P.npad * 8);
+    L.mxx = o;   o = al(o + P.mpad * 8);
+    L.pw = o;    o = al(o + OZ_TMAX * OZ_PW);
+    L.flag = o;  o = al(o + 4);
+    L.queue = o; o = al(o + 4);
+    L.total = o;
+    return L;
+}
+
+static hipError_t oz_fill(void *p, int64_t words, hipStream_t s) {
+    return launch_fill_u32((unsigned *)p, words, 0u, s);
+}
+
+static hipError_t oz_scales(const OzSrc &src, unsigned long long *mx, int bits, int *e, int *flag, hipStream_t s) {
+    hipError_t err = oz_fill(mx, 2 * src.no, s);
+    if (err != hipSuccess) return err;
+    dim3 g((unsigned)((src.nk + 63) / 64), (unsigned)((src.no + 63) / 64));
+    hipLaunchKernelGGL(oz_absmax_kernel, g, dim3(256), 0, s, src, mx);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((src.no + 255) / 256)), dim3(256), 0, s, src.no, mx, bits,
+                       e, flag);
+    return hipGetLastError();
+}
+
+hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double *A, int64_t ars, int64_t acs,
+                      bool lower, int64_t n, hipStream_t s) {
+    hipError_t err = oz_fill(ws + L.flag, 1, s);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(oz_pow2_kernel, dim3(P.T), dim3(OZ_PW), 0, s, P.C, (uint8_t *)(ws + L.pw));
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    OzSrc src{const_cast<double *>(A), ars, acs, n, n, lower ? 1 : 2};
+    int *ea = (int *)(ws + L.ea);
+    err = oz_scales(src, (unsigned long long *)(ws + L.mxa), P.bA, ea, (int *)(ws + L.flag), s);
+    if (err != hipSuccess) return err;
+    OzDst d{(int8_t *)(ws + L.a), P.aplane, 0, P.npad, lower ? 1 : 2};
+    dim3 g((unsigned)(P.npad / 64), (unsigned)(P.npad / 64));
+    hipLaunchKernelGGL(oz_residue_kernel, g, dim3(256), 0, s, src, (const int *)ea, d, 0, P.C,
+                       (const uint8_t *)(ws + L.pw));
+    return hipGetLastError();
+}
+
+template <bool OUT_T>
+static int oz_gemm_occupancy() {
+    static const int occ = [] {
+        int o = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void *)oz_gemm_kernel<OUT_T>, 512, 0) !=
+            hipSuccess)
+            o = 1;
+        return o > 0 ? o : 1;
+    }();
+    return occ;
+}
+
+hipError_t oz_residual(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, double *X,
+                       int64_t xso, int64_t xsk, int64_t n, int64_t m, const double *S, int64_t lds, double *R,
+                       int64_t ldr, hipStream_t s) {
+    int *flag = (int *)(ws + L.flag);
+    // X: column scales (a system's largest entry), residues, rounded iterate
+    OzSrc xs{X, xso, xsk, m, n, 0};
+    int *fx = (int *)(ws + L.fx);
+    hipError_t err = oz_scales(xs, (unsigned long long *)(ws + L.mxx), P.bX, fx, flag, s);
+    if (err != hipSuccess) return err;
+    OzDst d{(int8_t *)(ws + L.x), P.xplane, P.npad, P.npad, 0};
+    hipLaunchKernelGGL(oz_residue_kernel, dim3((unsigned)(P.npad / 64), (unsigned)(P.mpad / 64)), dim3(256), 0, s,
+                       xs, (const int *)fx, d, 1, P.C, (const uint8_t *)(ws + L.pw));
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    // modular products
+    err = oz_fill(ws + L.queue, 1, s);
+    if (err != hipSuccess) return err;
+    OzGemmArgs g{};
+    g.A = (const int8_t *)(ws + L.a);
+    g.aplane = P.aplane;
+    g.X = (const int8_t *)(ws + L.x);
+    g.xplane = P.xplane;
+    g.Cr = (uint8_t *)(ws + L.c);
+    g.cplane = P.cplane;
+    g.ldc = out_t ? P.mpad : P.npad;
+    g.npad = P.npad;
+    g.mpad = P.mpad;
+    g.tiles_m = (int)(P.npad / 256);
+    g.tiles_n = (int)(P.mpad / 256);
+    g.T = P.T;
+    g.lower = lower ? 1 : 0;
+    g.queue = (int *)(ws + L.queue);
+    int dev = 0, cus = 0;
+    err = hipGetDevice(&dev);
+    if (err == hipSuccess) err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (err != hipSuccess) return err;
+    const int items = g.tiles_m * g.tiles_n * g.T;
+    const int occ = out_t ? oz_gemm_occupancy<true>() : oz_gemm_occupancy<false>();
+    const int grid = cus * occ < items ? cus * occ : items;
+    if (out_t)
+        hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3(grid), dim3(512), 0, s, g, P.C);
+    else
+        hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3(grid), dim3(512), 0, s, g, P.C);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    // reconstruction: R is rows x cols column-major (side L: n x m with rows
+    // scaled by A's exponents; side R: m x n with rows scaled by X's)
+    OzCrtArgs c{};
+    c.rows = out_t ? m : n;
+    c.cols = out_t ? n : m;
+    c.S = S;
+    c.lds = lds;
+    c.R = R;
+    c.ldr = ldr;
+    c.Cr = (const uint8_t *)(ws + L.c);
+    c.cplane = P.cplane;
+    c.ldc = out_t ? P.mpad : P.npad;
+    c.er = out_t ? fx : (const int *)(ws + L.ea);
+    c.ec = out_t ? (const int *)(ws + L.ea) : fx;
+    c.pw = (const uint8_t *)(ws + L.pw);
+    c.flag = flag;
+    const unsigned gy = (unsigned)(c.cols < 65535 ? c.cols : 65535);
+    hipLaunchKernelGGL(oz_crt_kernel, dim3((unsigned)((c.rows + 255) / 256), gy), dim3(256), 0, s, c, P.C);
+    return hipGetLastError();
+}
+
+}  // namespace nla

@@ -32,7 +32,7 @@ class MixedInfo(ctypes.Structure):
     """nla_mixed_info of include/nla.h."""
     _fields_ = [("iterations", ctypes.c_int32), ("precision", ctypes.c_int32), ("backward_error", ctypes.c_double),
                 ("history", ctypes.c_double * (NLA_MIXED_MAX_ITER + 1)),
-                ("backward_error_frobenius", ctypes.c_double)]
+                ("backward_error_frobenius", ctypes.c_double), ("residual", ctypes.c_int32)]
 
 
 # name -> (restype, argtypes); must list every function of include/nla.h
@@ -65,6 +65,9 @@ SIGNATURES = {
     "nla_mixed_workspace_bytes": (_i64, [_c_char, _i64, _i64]),
     "nla_rectrxm_mixed": (_i32, [_c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _dbl, _i32,
                                  _vp, _i64, _vp, ctypes.POINTER(MixedInfo)]),
+    "nla_residual_workspace_bytes": (_i64, [_c_char, _i32, _i64, _i64]),
+    "nla_residual": (_i32, [_c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp,
+                            _i64, _vp, _i64, _vp]),
     "nla_set_config": (_i32, [_i32, _i32]),
     "nla_prof_enable": (_i32, [_i32]),
     "nla_prof_summary": (_i32, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl),
@@ -89,6 +92,9 @@ NLA_CFG_PANEL_STRIP_F64 = 5
 NLA_CFG_BATCH_CLASSES = 6
 NLA_CFG_PANEL_WAVES = 7
 NLA_CFG_LU_SPLIT_ROWS = 8
+NLA_CFG_RESIDUAL = 9
+NLA_RES_FP64 = 0
+NLA_RES_INT8 = 1
 
 
 def set_config(key, value):

@@ -127,6 +127,46 @@ def cond1_estimate(solve, solve_t, anorm1, x0, iters=5):
     return anorm1 * est
 
 
+def cond2_estimate(mv, mv_t, solve, solve_t, x0, iters=40):
+    """kappa_2(A) = sigma_max / sigma_min (the 2-norm condition number, the
+    meaning of Julia's and numpy's cond(A)) by power iteration: sigma_max^2
+    is the largest eigenvalue of A^T A (x <- A^T A x), sigma_min^-2 the
+    largest of A^-1 A^-T (x <- A^-1 A^-T x).  mv / mv_t apply A / A^T,
+    solve / solve_t apply A^-1 / A^-T to float64 column vectors (numpy arrays
+    or torch tensors of shape (n, 1)); x0 is the (nonzero) starting vector.
+    Both Rayleigh-quotient estimates grow monotonically towards the true
+    values from below."""
+    def top(apply):
+        x = x0 / float((x0 * x0).sum()) ** 0.5
+        lam = 0.0
+        for _ in range(iters):
+            y = apply(x)
+            lam = float((x * y).sum())
+            x = y / float((y * y).sum()) ** 0.5
+        return lam
+    smax2 = top(lambda x: mv_t(mv(x)))
+    sinv2 = top(lambda x: solve(solve_t(x)))
+    return (smax2 * sinv2) ** 0.5
+
+
+def device_cond2(A, uplo="L", iters=40, seed=1):
+    """kappa_2 of a triangular device matrix (zero outside its triangle):
+    products by torch, solves by the library's own TRSM."""
+    import torch
+    from . import unified_rectrxm_
+    n = A.shape[0]
+
+    def solve(x, t="N"):
+        y = x.clone()
+        unified_rectrxm_("L", uplo, t, 1.0, "S", A, y)
+        return y
+
+    g = torch.Generator(device=A.device)
+    g.manual_seed(seed)
+    x0 = torch.rand((n, 1), dtype=torch.float64, device=A.device, generator=g) + 0.5
+    return cond2_estimate(lambda x: A @ x, lambda x: A.t() @ x, solve, lambda x: solve(x, "T"), x0, iters)
+
+
 def device_cond1(A, uplo="L"):
     """kappa_1 of a triangular device matrix (zero outside its triangle),
     with the solves done by the library's own TRSM."""
@@ -166,15 +206,16 @@ def device_precision_tri(n, m, c, seed, device="cuda"):
 
 
 def calibrate_precision_tri(n, m, target, seed, device="cuda", rounds=3, tol=0.15):
-    """Choose the diagonal scale c so that the measured kappa_1(A) of
-    device_precision_tri is within `tol` (relative, in log10 terms: about
-    +-40 % for tol 0.15) of `target`.  Returns (A, B, c, measured kappa_1)."""
+    """Choose the diagonal scale c so that the measured kappa_2(A) (the
+    2-norm condition number, the meaning of cond(A)) of device_precision_tri
+    is within `tol` (relative, in log10 terms: about +-40 % for tol 0.15) of
+    `target`.  Returns (A, B, c, measured kappa_2)."""
     c = float(target)
     A = B = None
     kappa = None
     for _ in range(rounds):
         A, B = device_precision_tri(n, m, c, seed, device)
-        kappa = device_cond1(A, "L")
+        kappa = device_cond2(A, "L")
         if abs(math.log10(kappa) - math.log10(target)) <= tol:
             break
         c *= target / kappa

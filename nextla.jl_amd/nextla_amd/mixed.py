@@ -19,6 +19,8 @@ from . import _lib, _canonical, _require_device, _stream
 PRECISIONS = {"bf16": _lib.NLA_LP_BF16, "fp16": _lib.NLA_LP_FP16, "fp32": _lib.NLA_LP_FP32,
               "auto": _lib.NLA_LP_AUTO}
 _NAMES = {v: k for k, v in PRECISIONS.items()}
+RESIDUALS = {"fp64": _lib.NLA_RES_FP64, "int8": _lib.NLA_RES_INT8}
+RESIDUALS_NAMES = {v: k for k, v in RESIDUALS.items()}
 
 
 def workspace_bytes(side, n, m):
@@ -58,6 +60,7 @@ def rectrxm_mixed_(side, uplo, transpose, alpha, A, B, precision="fp16", tol=1e-
                                        ldb, float(tol), int(max_iter), base + off, need, _stream(B),
                                        ctypes.byref(info))
     out = {"iterations": info.iterations, "precision": _NAMES.get(info.precision, info.precision),
+           "residual": RESIDUALS_NAMES.get(info.residual, info.residual),
            "backward_error": info.backward_error, "backward_error_frobenius": info.backward_error_frobenius,
            "history": [info.history[i] for i in range(min(info.iterations, _lib.NLA_MIXED_MAX_ITER) + 1)]}
     if st == _lib.NLA_ERR_NOT_CONVERGED and not raise_on_stall:
@@ -66,3 +69,43 @@ def rectrxm_mixed_(side, uplo, transpose, alpha, A, B, precision="fp16", tol=1e-
     _lib.check(st, "rectrxm_mixed!")
     out["converged"] = True
     return B, out
+
+
+def residual_workspace_bytes(side, method, n, m):
+    """Device workspace nla_residual needs (method "fp64" or "int8")."""
+    return int(_lib.load().nla_residual_workspace_bytes(_lib.ch(side.upper()), RESIDUALS[method], int(n), int(m)))
+
+
+def residual(side, uplo, transpose, alpha, A, X, B, R, method="int8", workspace=None):
+    """R = alpha*B - op(A)*X (side 'L') or alpha*B - X*op(A) (side 'R') to
+    FP64 accuracy (nla_residual of include/nla.h).  method "int8": int8
+    modular products, which first replace X by its value rounded to >= 55
+    bits of each system's largest entry; "fp64": the FP64 MFMA product.  A,
+    X, B, R: float64 tensors on one ROCm device, X, B and R column-major of one shape (A in any layout _canonical accepts).
+    Returns R."""
+    if method not in RESIDUALS:
+        raise ValueError(f"method must be one of {sorted(RESIDUALS)}, got {method!r}")
+    for t in (A, X, B, R):
+        if t.dtype != torch.float64:
+            raise ValueError("the residual takes float64 tensors")
+    _require_device(A, X, B, R)
+    side_u = side.upper()
+    side_c, uplo_c, tr, n, m, lda, ldb = _canonical(side_u, uplo.upper(), transpose.upper(), A, B)
+    for name, t in (("B", B), ("X", X), ("R", R)):
+        if t.shape != B.shape or t.stride(0) != 1:
+            raise ValueError(f"{name} must be column-major with B's shape")
+    need = residual_workspace_bytes(side_c, method, n, m)
+    if need < 0:
+        raise ValueError(f"the {method} residual does not take n = {n}")
+    if workspace is None:
+        workspace = torch.empty(need + 256, dtype=torch.uint8, device=B.device)
+    base = workspace.data_ptr()
+    off = (-base) % 256
+    if workspace.numel() - off < need:
+        raise ValueError(f"workspace holds {workspace.numel() - off} aligned bytes, {need} needed")
+    st = _lib.load().nla_residual(_lib.ch(side_c), _lib.ch(uplo_c), _lib.ch("T" if tr else "N"), RESIDUALS[method],
+                                  float(alpha), n, m, A.data_ptr(), lda, X.data_ptr(), max(1, X.stride(1)),
+                                  B.data_ptr(), ldb, R.data_ptr(), max(1, R.stride(1)), base + off, need,
+                                  _stream(B))
+    _lib.check(st, "residual")
+    return R

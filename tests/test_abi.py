@@ -244,3 +244,45 @@ def test_lu_lauum_larfb_validation(nla):
     assert larfb(13, 9) == -14       # ldc < m
     assert larfb(15, 7) == -16       # ldwork < n
     assert larfb(7, 0) == 0          # k = 0: nothing to apply
+
+
+def test_residual_validation(nla):
+    """nla_residual: argument checks in order, workspace sizes, method and
+    order limits (no device work)."""
+    from nextla_amd import _lib
+    lib = _lib.load()
+    assert lib.nla_residual_workspace_bytes(b"L", _lib.NLA_RES_FP64, 100, 8) == 256
+    w8 = lib.nla_residual_workspace_bytes(b"L", _lib.NLA_RES_INT8, 300, 21)
+    # 16 moduli at n = 300 (padded to 512): A planes 3 * 65536 each, X and
+    # product planes 256 * 512 each
+    assert w8 >= 16 * (3 * 65536 + 2 * 256 * 512)
+    assert lib.nla_residual_workspace_bytes(b"R", _lib.NLA_RES_INT8, 65537, 8) == -1
+    assert lib.nla_residual_workspace_bytes(b"Q", _lib.NLA_RES_INT8, 10, 8) == -1
+    assert lib.nla_residual_workspace_bytes(b"L", 7, 10, 8) == -1
+    good = [b"L", b"L", b"N", _lib.NLA_RES_INT8, 1.0, 300, 21, 4096, 300, 8192, 300, 12288, 300, 16384, 300,
+            1 << 20, w8, None]
+
+    def call(pos, val):
+        args = list(good)
+        args[pos] = val
+        return lib.nla_residual(*args)
+
+    for pos, val in ((0, b"X"), (1, b"Q"), (2, b"Z"), (3, 5), (4, float("inf")), (5, -1), (6, -1), (7, None),
+                     (8, 299), (9, None), (10, 299), (11, None), (12, 299), (13, None), (14, 299), (15, 1000),
+                     (16, w8 - 1)):
+        assert call(pos, val) == -(pos + 1), (pos, val)
+    assert call(5, 0) == 0 and call(6, 0) == 0  # empty: nothing to do
+    # the right side checks leading dimensions against m rows
+    args = list(good)
+    args[0] = b"R"
+    args[8] = args[10] = args[12] = args[14] = 21
+    args[16] = lib.nla_residual_workspace_bytes(b"R", _lib.NLA_RES_INT8, 300, 21)
+    args[5], args[6] = 300, 21
+    args[8] = 20
+    assert lib.nla_residual(*args) == -9
+    assert _lib.set_config(_lib.NLA_CFG_RESIDUAL, 3) == -1
+    old = _lib.set_config(_lib.NLA_CFG_RESIDUAL, _lib.NLA_RES_FP64)
+    assert old == _lib.NLA_RES_INT8
+    small = lib.nla_mixed_workspace_bytes(b"L", 4096, 128)
+    _lib.set_config(_lib.NLA_CFG_RESIDUAL, old)
+    assert lib.nla_mixed_workspace_bytes(b"L", 4096, 128) > small + 16 * 4096 * 4096 // 2

@@ -9,7 +9,7 @@ the FP64 oracle:
     OpenBLAS dtrsm / dtrmm in FP64 (the reference tests' oracle,
     test/unified_rectrxm.jl:36-40) on the same A;
   * the FP64-accurate refined solve at n = 16384 with 4096 right-hand sides
-    and measured cond_1(A) ~ 1e2, 1e4, 1e6, bf16 and fp16 updates: the
+    and measured cond_2(A) ~ 1e2, 1e4, 1e6, bf16 and fp16 updates: the
     backward error recomputed on the device in FP64, both normwise
     (Frobenius) and per column (max_j ||r_j|| / (||A||_2 ||x_j|| + ||s_j||),
     with ||A||_2 bounded below by A's largest column norm), and the solution
@@ -143,7 +143,7 @@ _C2 = {}
 
 
 def config2(cond):
-    """A (n = 16384, calibrated to cond_1 ~ cond) and B (4096 RHS), cached."""
+    """A (n = 16384, calibrated to cond_2 ~ cond) and B (4096 RHS), cached."""
     from nextla_amd import inputs
     if cond not in _C2:
         _C2.clear()

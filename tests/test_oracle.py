@@ -237,6 +237,42 @@ def test_cond1_estimator_against_exact():
         assert 0.9 * c <= np.linalg.cond(A, 2) <= 1.2 * c
 
 
+def test_cond2_estimator_against_exact():
+    """The power-iteration 2-norm estimator (used to calibrate the refinement
+    workload) against numpy's kappa_2 from the singular values, on the
+    workload's matrices and on a matrix with clustered small singular values;
+    a transposed solve in place of A^-T must be rejected."""
+    import scipy.linalg as sl
+    from nextla_amd import inputs as I
+    rng = I._rng(2)
+    for c in (1e2, 1e4, 1e6):
+        A = I.precision_tri(400, c, rng)
+        exact = np.linalg.cond(A, 2)
+        x0 = rng.random((400, 1)) + 0.5
+        est = I.cond2_estimate(lambda x: A @ x, lambda x: A.T @ x,
+                               lambda x: sl.solve_triangular(A, x, lower=True),
+                               lambda x: sl.solve_triangular(A, x, lower=True, trans="T"), x0)
+        assert 0.8 * exact <= est <= exact * (1 + 1e-9), (c, est, exact)
+    # a strongly non-normal triangle (unit diagonal, N(0,1) below it), where
+    # A^-1 A^-1 and A^-1 A^-T differ: the estimate holds, and a solve with
+    # A in place of A^T (a planted mistake) is rejected
+    A = np.tril(rng.standard_normal((60, 60)), -1) + np.eye(60)
+    exact = np.linalg.cond(A, 2)
+    x0 = rng.random((60, 1)) + 0.5
+    lo = lambda x: sl.solve_triangular(A, x, lower=True)
+    est = I.cond2_estimate(lambda x: A @ x, lambda x: A.T @ x, lo,
+                           lambda x: sl.solve_triangular(A, x, lower=True, trans="T"), x0, iters=200)
+    assert 0.8 * exact <= est <= exact * (1 + 1e-9), (est, exact)
+    wrong = I.cond2_estimate(lambda x: A @ x, lambda x: A.T @ x, lo, lo, x0, iters=200)
+    assert not (0.8 * exact <= wrong <= exact * (1 + 1e-9)), (wrong, exact)
+    # a diagonal matrix: kappa_2 is max/min of the diagonal exactly
+    d = np.array([3.0, 1e-3, 2.0, 5e-4, 7.0])
+    D = np.diag(d)
+    est = I.cond2_estimate(lambda x: D @ x, lambda x: D @ x, lambda x: x / d[:, None],
+                           lambda x: x / d[:, None], np.ones((5, 1)), iters=200)
+    assert abs(est / (7.0 / 5e-4) - 1) < 1e-6, est
+
+
 def test_blas_unit_diagonal_is_not_read():
     """The unit-diagonal oracle: BLAS with diag 'U' on a matrix whose
     diagonal holds NaN equals BLAS with diag 'N' on the same triangle with
@@ -325,3 +361,61 @@ def test_larfb_oracle_is_the_qr_q():
         D = rng.standard_normal((7, m))
         assert np.allclose(O.larfb("R", "N", V, T, D), D @ Qf, atol=1e-12)
         assert np.allclose(O.larfb("R", "C", V, T, D), D @ Qf.T, atol=1e-12)
+
+
+def test_residual_oracle_pinned():
+    """The exact residual oracle: with bit counts that round nothing, the
+    fixed-point model equals the exact residual bit for bit; the exact
+    residual lies within FP64's rounding bound of numpy's residual; the
+    model's rounding error stays inside its bound and is not zero (a model
+    that skips the rounding is caught); X is rounded to bX bits of each
+    system's largest entry."""
+    from oracle import residual_oracle as RO
+    rng = np.random.default_rng(11)
+    for side, uplo, trans in (("L", "L", "N"), ("L", "U", "T"), ("R", "U", "N"), ("R", "L", "T")):
+        n, m = 23, 4
+        A = np.tril(rng.uniform(-1, 1, (n, n)) / np.sqrt(n)) + np.diag(2 + rng.random(n))
+        if uplo == "U":
+            A = A.T.copy()
+        shp = (n, m) if side == "L" else (m, n)
+        X, B = rng.standard_normal(shp), rng.standard_normal(shp)
+        ex = RO.exact_residual(side, uplo, trans, 1.5, A, X, B)
+        big, Xb = RO.fixed_point_model(side, uplo, trans, 1.5, A, X, B, 300, 300)
+        assert np.array_equal(big, ex) and np.array_equal(Xb, X)
+        Op = O.op_matrix(uplo, trans, A)
+        R64 = 1.5 * B - (Op @ X if side == "L" else X @ Op)
+        mag = np.abs(Op) @ np.abs(X) if side == "L" else np.abs(X) @ np.abs(Op)
+        assert np.all(np.abs(R64 - ex) <= (n + 2) * 2.0 ** -53 * (mag + 1.5 * np.abs(B)))
+        small, Xs = RO.fixed_point_model(side, uplo, trans, 1.5, A, X, B, 20, 20)
+        d = np.abs(small - ex)
+        assert d.max() > 0
+        # rounding A to 20 bits per row and X to 20 bits per system
+        Ah = Op if side == "L" else Op.T
+        Xh = X if side == "L" else X.T
+        bound = (2.0 ** -20 * np.abs(Ah).max(1)[:, None] * np.abs(Xh).sum(0)[None, :]
+                 + np.abs(Ah) @ (np.ones_like(Xh) * 2.0 ** -20 * np.abs(Xh).max(0)[None, :]))
+        assert np.all((d if side == "L" else d.T) <= 1.01 * bound)
+        # X rounded to 20 bits of its system's largest entry
+        q = np.abs(Xh).max(0)
+        ulp = 2.0 ** (np.floor(np.log2(q)) + 1 - 20)
+        Xsh = Xs if side == "L" else Xs.T
+        assert np.all(np.abs(Xsh - Xh) <= 0.5 * ulp[None, :])
+        assert np.all(np.round(Xsh / ulp[None, :]) * ulp[None, :] == Xsh)
+
+
+def test_residual_plan_bits():
+    """Moduli count and bit split: the product of the first T moduli covers
+    bA + bX + ceil(log2 n) + 3 bits, bA and bX are at least 55, and T - 1
+    moduli would not leave 55 bits each (the fewest moduli)."""
+    import math
+    from oracle import residual_oracle as RO
+    for n in (2, 100, 300, 2048, 4097, 16384, 65536):
+        T, bA, bX = RO.plan_bits(n)
+        lg = math.ceil(math.log2(n))
+        logm = sum(math.log2(p) for p in RO.MODULI[:T])
+        assert bA + bX + lg + 3 <= math.floor(logm)
+        assert bA >= 55 and bX >= 55 and abs(bA - bX) <= 1
+        assert math.floor(sum(math.log2(p) for p in RO.MODULI[:T - 1])) - 3 - lg < 110
+    for i in range(len(RO.MODULI)):
+        for j in range(i):
+            assert math.gcd(RO.MODULI[i], RO.MODULI[j]) == 1
