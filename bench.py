@@ -1,47 +1,44 @@
 """Benchmark of the recursive triangular solve on one MI355X.
 
-Workload (the drop-in case): FP64 left-side lower non-transposed TRSM through
-unified_rectrxm!, A 8192 x 8192 lower triangular (off-diagonals U(-1,1)/sqrt(n),
-diagonal 2+U(0,1)), B 8192 x 8192 N(0,1), alpha = 1.5, inputs generated in HBM.
-A step restores B from a resident copy and solves in place; `value` is
-n^2*m flops over the mean solve time (HIP events on the solve's stream).
+Headline workload (the metric's comparison): FP64-accurate left-lower TRSM,
+n = 16384 with 4096 right-hand sides N(0,1), A lower with off-diagonals
+U(-1,1)/sqrt(n) and a log-spaced diagonal calibrated to cond_2(A) ~ 1e4,
+solved to a per-system backward error <= 1e-14 by low-precision correction
+solves (bf16 / fp16 / fp32 updates, the library's automatic choice) and
+refinement with FP64-accurate residuals (int8 modular products).  A step
+restores B from a resident copy (outside the timed region) and solves in
+place; `value` is n^2*m over the mean solve time (HIP events on the solve's
+stream).  The plain FP64 solve of the same system is timed the same way and
+the ratio is reported beside it.
 
-Also reported on the same line:
-  roofline      the dominant kernel (the MFMA GEMM update): algorithmic flops
-                over its own time, from per-launch HIP events, vs FP64 peak;
-                `traffic` is its HBM bytes per launch from the committed
-                rocprofv3 FETCH_SIZE / WRITE_SIZE profile of this workload;
+Also on the same line:
+  roofline      the dominant kernel of the refined solve (the int8 modular
+                products of its residuals): int8 operations over its own time
+                from per-launch HIP events, against the 5.0 POP/s int8 peak;
+                `traffic` its HBM bytes per launch from the committed
+                rocprofv3 FETCH_SIZE / WRITE_SIZE profile, when present;
   cpu_baseline  the oracle (C restatement of the reference algorithm, OpenMP
-                over the host's cores) on a bounded sample: the same A with
-                a subset of the right-hand sides;
-  extra         FP32 and TRMM rates, the whole-operation roofline fraction, the
-                backward error and p99 of the timed solve, and two more legs:
-                refined_mixed  FP64-accurate left-lower TRSM, n = 16384 with
-                               4096 RHS, cond_1(A) ~ 1e4: fp16 / bf16 updates
-                               with FP64 refinement to backward error <= 1e-14,
-                               against the plain FP64 solve of the same system;
-                batched        10,000 lower TRSMs (n log-uniform on [16, 1024],
-                               RHS log-uniform on [1, 256]) in one
-                               graph-captured persistent launch, against a
-                               loop of plain calls;
-                served         a stream of 600 TRSM / TRMM requests (n on
-                               [32, 4096], RHS on [1, 512], FP32 / FP64
-                               accuracy, cond(A) <= 1e4, 20 % on resident
-                               factors, 64 GB budget) through the
-                               continuous-batching server: p50 / p99 latency
-                               at Poisson rates, saturated solves/s against
-                               one-at-a-time plain calls, and the FP64 TRSMs
-                               served by refinement instead;
-                lu             getrf2! (recursive LU with partial pivoting,
-                               built on the unit-lower TRSM and GEMM update)
-                               at n = 4096 and 8192 in FP64, against LAPACK
-                               dgetrf on the host's cores;
-                lauum          lauum! (U*U^T, L^T*L in place, built on the
-                               TRMM and the GEMM update) at n = 4096 and
-                               8192 in FP64, against LAPACK dlauum;
-                larfb          larfb! (block reflector of tile QR, its
-                               triangular multiplies on the TRMM) at
-                               m = n = 8192, k = 64 and 256 in FP64.
+                over the host's cores) on a bounded sample: the same A with a
+                subset of the right-hand sides (plain FP64 solve);
+  extra         refined_mixed: the three condition numbers (1e2, 1e4, 1e6),
+                  bf16 / fp16 / automatic updates, iterations, backward errors
+                  (per system and whole-matrix, recomputed on the device),
+                  error against the plain FP64 solution, and the phase sum
+                  of roofline times over the measured time;
+                drop_in: the plain GPU drop-in at n = 8192 (B 8192 x 8192,
+                  alpha = 1.5), all 8 side/uplo/trans x TRSM/TRMM x
+                  FP32/FP64, TFLOP/s and the fraction of the MFMA peak, the
+                  GEMM update's own rate;
+                cpu_case: the 512 x 64 FP64 TRSM and TRMM on the GPU beside
+                  the oracle and OpenBLAS on the host;
+                batched: 10,000 lower TRSMs (n log-uniform on [16, 1024], RHS
+                  log-uniform on [1, 256]) in one graph-captured persistent
+                  launch against a loop of plain calls, results compared;
+                served: a stream of 600 TRSM / TRMM requests through the
+                  continuous-batching server: p50 / p99 latency at Poisson
+                  rates and saturated solves/s against one-at-a-time calls;
+                lu, lauum, larfb: getrf2!, lauum! and larfb! built on the
+                  TRSM / TRMM and GEMM update, against LAPACK on the host.
 """
 import argparse
 import json
@@ -55,7 +52,8 @@ for _p in (ROOT, os.path.join(ROOT, "nextla.jl_amd")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-PEAK = {"f64": 78.6, "f32": 157.3}  # dense TFLOP/s, MI355X (vector = matrix for FP64)
+# dense peaks, MI355X: TFLOP/s (vector = matrix for FP64); int8 in TOP/s
+PEAK = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0, "fp16": 2500.0, "int8": 5000.0}
 HBM_TBS = 8.0
 METRIC = "TRSM TFLOP/s + %roofline, refined-mixed vs plain FP64 at 1e-14 bwd err; p99 lat"
 
@@ -98,15 +96,13 @@ def backward_error(torch, side, uplo, trans, alpha, A, B0, X):
     return (torch.linalg.norm(R) / den).item()
 
 
-def traffic_from_profile(n, m):
-    """HBM bytes per launch of the GEMM update measured by rocprofv3 for this
-    workload (profiles/, n = m = 8192 only), or None."""
+def traffic_from_profile(name):
+    """HBM bytes per launch of a kernel measured by rocprofv3 (the committed
+    profiles/<name> summary, FETCH_SIZE + WRITE_SIZE copy-calibrated), or
+    None when that summary is absent."""
     import re
-    if (n, m) != (8192, 8192):
-        return None
-    path = os.path.join(ROOT, "profiles", "gemm_kernel_f64_trsm_n8192_pmc_traffic.md")
     try:
-        txt = open(path).read()
+        txt = open(os.path.join(ROOT, "profiles", name)).read()
     except OSError:
         return None
     hit = re.search(r"traffic_bytes_per_launch:\s*(\d+)", txt)
@@ -127,57 +123,196 @@ def event_ms(torch, fn, reps):
     return sum(ts) / len(ts), ts
 
 
-def mixed_leg(nla, torch):
-    """FP64-accurate TRSM by fp16 / bf16 correction solves (and the
-    automatic per-step choice among bf16 / fp16 / fp32) with FP64
-    refinement vs the plain FP64 solve on the same system."""
-    from nextla_amd import inputs
-    from nextla_amd.mixed import rectrxm_mixed_, workspace_bytes
-    n, m = 16384, 4096
-    A, B0, c, kappa = inputs.calibrate_precision_tri(n, m, 1e4, seed=5)
-    P = B0.clone()
-    ws = torch.empty(workspace_bytes("L", n, m) + 256, dtype=torch.uint8, device=A.device)
-
-    def plain():
-        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, P)
-
-    for _ in range(2):
-        P.copy_(B0)
-        plain()
+def timed_steps(torch, restore, fn, steps, warmup):
+    """HIP-event times (ms) of `steps` calls of fn on the current stream after
+    `warmup` untimed ones; restore() runs before each call, outside the
+    events."""
+    s = torch.cuda.current_stream()
+    out = None
+    for _ in range(warmup):
+        restore()
+        out = fn()
     torch.cuda.synchronize()
-    ts = []
-    for _ in range(3):
-        P.copy_(B0)
-        t, _ = event_ms(torch, plain, 1)
-        ts.append(t)
-    tp = sum(ts) / len(ts)
-    out = {"workload": "left-lower TRSM n=16384, 4096 RHS N(0,1); A off-diagonal U(-1,1)/sqrt(n), "
-                       "log-spaced diagonal calibrated to cond_1 ~ 1e4; target backward error 1e-14",
-           "cond1_measured": kappa, "plain_fp64_ms": round(tp, 3),
-           "plain_fp64_tflops": round(float(n) * n * m / tp / 1e9, 3)}
-    X = B0.clone()
-    for prec in ("fp16", "bf16", "auto"):
-        X.copy_(B0)
-        rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws, raise_on_stall=False)
+    times = []
+    for _ in range(steps):
+        restore()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        out = fn()
+        e1.record(s)
         torch.cuda.synchronize()
-        ts, info = [], None
-        for _ in range(2):
-            X.copy_(B0)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            _, info = rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws, raise_on_stall=False)
-            torch.cuda.synchronize()
-            ts.append((time.perf_counter() - t0) * 1e3)
+        times.append(e0.elapsed_time(e1))
+    return times, out
+
+
+def percentile(ts, q):
+    t = sorted(ts)
+    return t[min(len(t) - 1, max(0, int(math.ceil(q * len(t))) - 1))]
+
+
+def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16"), n=16384, m=4096, seed=5,
+                 keep=False):
+    """FP64-accurate TRSM by refinement against the plain FP64 solve of the
+    same calibrated system.  Per precision: mean / p99 time, speedup,
+    effective TFLOP/s, iterations, the per-system and whole-matrix backward
+    errors recomputed on the device, and the error against the plain FP64
+    solution; for the first precision also the per-phase kernel times, the
+    phase sum of roofline times over the measured time, and the int8 product
+    kernel's own rate.  keep: also return (A, B0) for the CPU baseline."""
+    from nextla_amd import _lib, inputs
+    from nextla_amd.mixed import rectrxm_mixed_, workspace_bytes
+    A, B0, c, kappa = inputs.calibrate_precision_tri(n, m, cond, seed=seed)
+    flops = float(n) * n * m
+    P = B0.clone()
+    tplain, _ = timed_steps(torch, lambda: P.copy_(B0), lambda: nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, P),
+                            steps, warmup)
+    tp = sum(tplain) / len(tplain)
+    out = {"workload": f"left-lower TRSM n={n}, {m} RHS N(0,1); A off-diagonal U(-1,1)/sqrt(n), log-spaced "
+                       f"diagonal calibrated to cond_2 ~ {cond:g}; target per-system backward error 1e-14",
+           "cond2_measured": kappa, "plain_fp64_ms": round(tp, 3), "plain_fp64_p99_ms": round(percentile(tplain, 0.99), 3),
+           "plain_fp64_tflops": round(flops / tp / 1e9, 3)}
+    ws = torch.empty(workspace_bytes("L", n, m) + 256, dtype=torch.uint8, device=A.device)
+    X = B0.clone()
+    T = torch.tril(A)
+    lb = torch.linalg.norm(T, dim=0).max()
+    for i, prec in enumerate(precs):
+        call = lambda: rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws,
+                                      raise_on_stall=False)[1]
+        ts, info = timed_steps(torch, lambda: X.copy_(B0), call, steps if i == 0 else max(2, steps // 3),
+                               warmup if i == 0 else 1)
         tm = sum(ts) / len(ts)
+        R = B0 - T @ X
+        bwd_c = (torch.linalg.norm(R, dim=0) / (lb * torch.linalg.norm(X, dim=0) + torch.linalg.norm(B0, dim=0))).max().item()
+        bwd_f = (torch.linalg.norm(R) / (torch.linalg.norm(T) * torch.linalg.norm(X) + torch.linalg.norm(B0))).item()
+        del R
         fwd = (torch.linalg.norm(X - P) / torch.linalg.norm(P)).item()
-        out[prec] = {"ms": round(tm, 3), "speedup_vs_plain_fp64": round(tp / tm, 4),
-                     "effective_tflops": round(float(n) * n * m / tm / 1e9, 3), "iterations": info["iterations"],
-                     "final_update_precision": info["precision"], "backward_error": info["backward_error"],
-                     "converged": info["converged"], "rel_diff_vs_plain_fp64": fwd}
-        log(f"bench: refined {prec}: {tm:.2f} ms vs plain {tp:.2f} ms ({tp / tm:.3f}x), "
-            f"{info['iterations']} steps, bwd {info['backward_error']:.2e}")
-    del A, B0, P, X, ws
+        r = {"ms": round(tm, 3), "p99_ms": round(percentile(ts, 0.99), 3), "speedup_vs_plain_fp64": round(tp / tm, 4),
+             "effective_tflops": round(flops / tm / 1e9, 3), "iterations": info["iterations"],
+             "final_update_precision": info["precision"], "residual_method": info["residual"],
+             "converged": info["converged"], "backward_error_per_system": bwd_c, "backward_error_frobenius": bwd_f,
+             "library_backward_error": info["backward_error"], "rel_diff_vs_plain_fp64": fwd, "times_ms": ts}
+        if i == 0:
+            # per-phase kernel times of one solve (library HIP events)
+            _lib.prof_enable(True)
+            X.copy_(B0)
+            call()
+            torch.cuda.synchronize()
+            ph = {}
+            for name, kind in (("fp32_updates", _lib.NLA_KIND_GEMM), ("panels_and_inverses", _lib.NLA_KIND_LEAF),
+                               ("lp_updates", _lib.NLA_KIND_LP_GEMM), ("residual_other", _lib.NLA_KIND_RESIDUAL),
+                               ("residual_int8_products", _lib.NLA_KIND_RESIDUAL_GEMM)):
+                k_n, k_ms, k_fl, k_by = _lib.prof_summary(kind)
+                ph[name] = {"launches": k_n, "ms": round(k_ms, 3), "flops": k_fl, "bytes": k_by}
+            _lib.prof_enable(False)
+            # roofline time of each phase at its unit's peak (or HBM)
+            lp_peak = PEAK["bf16"]
+            t_roof = (ph["fp32_updates"]["flops"] / (PEAK["f32"] * 1e12)
+                      + ph["panels_and_inverses"]["flops"] / (PEAK["f32"] * 1e12)
+                      + ph["lp_updates"]["flops"] / (lp_peak * 1e12)
+                      + max(ph["residual_other"]["bytes"] / (HBM_TBS * 1e12),
+                            0.0 if r["residual_method"] == "int8" else ph["residual_other"]["flops"] / (PEAK["f64"] * 1e12))
+                      + ph["residual_int8_products"]["flops"] / (PEAK["int8"] * 1e12))
+            r["phases"] = ph
+            r["phase_sum_roofline_frac"] = round(t_roof * 1e3 / tm, 4)
+            r["kernel_time_share"] = round(sum(v["ms"] for v in ph.values()) / tm, 4)
+            g = ph["residual_int8_products"]
+            if g["launches"]:
+                r["int8_products"] = {"launches": g["launches"], "avg_launch_ms": round(g["ms"] / g["launches"], 4),
+                                      "tops": round(g["flops"] / g["ms"] / 1e9, 1),
+                                      "frac_of_int8_peak": round(g["flops"] / g["ms"] / 1e9 / PEAK["int8"], 4),
+                                      "ops_per_launch": g["flops"] / g["launches"],
+                                      "bytes_per_launch": g["bytes"] / g["launches"]}
+        out[prec] = r
+        log(f"bench: cond {kappa:.2e} refined {prec}: {tm:.2f} ms vs plain {tp:.2f} ms ({tp / tm:.3f}x), "
+            f"{info['iterations']} steps ({info['residual']} residual), bwd per system {bwd_c:.2e}")
+    del X, P, ws, T
+    if keep:
+        return out, A, B0
+    del A, B0
     torch.cuda.empty_cache()
+    return out
+
+
+def drop_in_leg(nla, torch, n=8192, m=8192, steps=4, warmup=1):
+    """The plain drop-in at n = 8192 for every side/uplo/trans x TRSM/TRMM x
+    FP32/FP64: TFLOP/s and the fraction of the dtype's MFMA peak; for the
+    FP64 left-lower TRSM also the backward error and the GEMM update's own
+    rate (per-launch HIP events inside the library)."""
+    from nextla_amd import _lib
+    flops = float(n) * n * m
+    out = {"workload": f"A {n}x{n} triangular, off-diagonal U(-1,1)/sqrt(n), diagonal 2+U(0,1); B {n}x{m} (or "
+                       f"{m}x{n}) N(0,1); alpha 1.5", "rates": {}}
+    for dt, dn in ((torch.float64, "f64"), (torch.float32, "f32")):
+        for func in "SM":
+            for side in "LR":
+                for uplo in "LU":
+                    for trans in "NT":
+                        _, _, _, ts, _ = solve_rate(nla, torch, n, m, side, uplo, trans, func, 1.5, dt, steps, warmup)
+                        mm = sum(ts) / len(ts)
+                        tf = flops / (mm * 1e-3) / 1e12
+                        out["rates"][f"{dn}_{'trsm' if func == 'S' else 'trmm'}_{side}{uplo}{trans}"] = {
+                            "ms": round(mm, 3), "tflops": round(tf, 2), "frac_of_peak": round(tf / PEAK[dn], 4)}
+                torch.cuda.empty_cache()
+            log(f"bench: drop-in {dn} {'TRSM' if func == 'S' else 'TRMM'}: " + ", ".join(
+                f"{k[-3:]} {v['tflops']}" for k, v in out["rates"].items() if k.startswith(f"{dn}_{'trsm' if func == 'S' else 'trmm'}")))
+    # FP64 LL-N TRSM: accuracy and the GEMM update's own rate
+    A, B0, X, times, _ = solve_rate(nla, torch, n, m, "L", "L", "N", "S", 1.5, torch.float64, steps, warmup)
+    out["f64_trsm_LLN_backward_error"] = backward_error(torch, "L", "L", "N", 1.5, A, B0, X)
+    _lib.prof_enable(True)
+    for _ in range(3):
+        X.copy_(B0)
+        nla.unified_rectrxm_("L", "L", "N", 1.5, "S", A, X)
+    torch.cuda.synchronize()
+    g_n, g_ms, g_fl, _ = _lib.prof_summary(_lib.NLA_KIND_GEMM)
+    l_n, l_ms, l_fl, _ = _lib.prof_summary(_lib.NLA_KIND_LEAF)
+    _lib.prof_enable(False)
+    g_tf = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    out["f64_trsm_LLN_gemm_update"] = {
+        "tflops": round(g_tf, 2), "frac_of_peak": round(g_tf / PEAK["f64"], 4), "launches_per_solve": g_n // 3,
+        "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
+        "traffic": traffic_from_profile("gemm_kernel_f64_trsm_n8192_pmc_traffic.md"),
+        "panels_ms_per_solve": round(l_ms / 3, 3), "panels_tflops": round(l_fl / (l_ms * 1e-3) / 1e12, 2) if l_ms else None}
+    q_bytes = (n * (n + 1) / 2 + 2.0 * n * m) * 8
+    mean = sum(times) / len(times)
+    out["f64_trsm_LLN_op_roofline_frac"] = round(max(flops / (PEAK["f64"] * 1e12), q_bytes / (HBM_TBS * 1e12))
+                                                 / (mean * 1e-3), 4)
+    del A, B0, X
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_case_leg(nla, torch, reps=20):
+    """The 512 x 512 FP64 case (lower A: off-diagonals U(-1,1)/512, diagonal
+    1+U(0,1), seed 0; B 512 x 64 N(0,1)): TRSM and TRMM on the GPU (HIP
+    events, B restored outside them), the oracle restatement on one host core
+    and OpenBLAS (the reference tests' oracle) on the same inputs, and the
+    GPU result's error against OpenBLAS."""
+    import numpy as np
+    from oracle import oracle as O
+    from nextla_amd import inputs
+    A = inputs.workload0_tri(512, seed=0)
+    B = inputs.normal_rhs(512, 64, inputs._rng(0))
+    Ad = torch.from_numpy(np.ascontiguousarray(A.T)).cuda().t()
+    B0 = torch.from_numpy(np.ascontiguousarray(B.T)).cuda().t()
+    out = {"workload": "FP64 L-L-N, A 512x512 (off-diagonal U(-1,1)/512, diagonal 1+U(0,1), seed 0), B 512x64 N(0,1)"}
+    for func in "SM":
+        X = B0.clone()
+        ts, _ = timed_steps(torch, lambda: X.copy_(B0), lambda: nla.unified_rectrxm_("L", "L", "N", 1.0, func, Ad, X),
+                            reps, 3)
+        gpu_us = 1e3 * sorted(ts)[len(ts) // 2]
+        ref = O.blas("L", "L", "N", 1.0, func, A, B)
+        err = O.rel_error(X.cpu().numpy(), ref)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            O.rectrxm("L", "L", "N", 1.0, func, A, B, nthreads=1)
+        port_us = (time.perf_counter() - t0) / 5 * 1e6
+        t0 = time.perf_counter()
+        for _ in range(20):
+            O.blas("L", "L", "N", 1.0, func, A, B)
+        blas_us = (time.perf_counter() - t0) / 20 * 1e6
+        out["trsm" if func == "S" else "trmm"] = {
+            "gpu_median_us": round(gpu_us, 1), "oracle_port_1core_us": round(port_us, 1),
+            "openblas_us": round(blas_us, 1), "rel_error_vs_openblas": err}
     return out
 
 
@@ -220,6 +355,7 @@ def batch_leg(nla, torch, dtype, count=10000, seed=0):
         t, _ = event_ms(torch, lambda: plan.execute(use_graph=True), 1)
         ts.append(t)
     tb = sum(ts) / len(ts)
+    Xb = [b.clone() for b in Bs]  # the batch's results (last timed execution)
     restore()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -228,11 +364,26 @@ def batch_leg(nla, torch, dtype, count=10000, seed=0):
     torch.cuda.synchronize()
     tl = (time.perf_counter() - t0) * 1e3
     plan.close()
-    del As, Bs, B0s
+    # every system: the batch's solution against the plain call's, and its
+    # normwise backward error (both in units of n*eps of the dtype)
+    eps = float(torch.finfo(dtype).eps)
+    worst_d = worst_b = 0.0
+    for A, B0, X, Y in zip(As, B0s, Xb, Bs):
+        nn = A.shape[0]
+        T = torch.tril(A.double())
+        Xd, Yd, Bd = X.double(), Y.double(), B0.double()
+        d = (torch.linalg.norm(Xd - Yd) / torch.linalg.norm(Yd)).item()
+        b = (torch.linalg.norm(Bd - T @ Xd) / (torch.linalg.norm(T) * torch.linalg.norm(Xd)
+                                               + torch.linalg.norm(Bd))).item()
+        worst_d = max(worst_d, d / (nn * eps))
+        worst_b = max(worst_b, b / (nn * eps))
+    del As, Bs, B0s, Xb
     torch.cuda.empty_cache()
-    return {"systems": count, "ms": round(tb, 3), "tflops": round(flops / tb / 1e9, 3),
-            "roofline_frac": round(t_roof * 1e3 / tb, 4), "loop_of_plain_calls_ms": round(tl, 1),
-            "speedup_vs_loop": round(tl / tb, 1)}
+    return {"systems": count, "ms": round(tb, 3), "p99_ms": round(percentile(ts, 0.99), 3),
+            "tflops": round(flops / tb / 1e9, 3), "roofline_frac": round(t_roof * 1e3 / tb, 4),
+            "loop_of_plain_calls_ms": round(tl, 1), "speedup_vs_loop": round(tl / tb, 1),
+            "worst_rel_diff_vs_plain_in_n_eps": round(worst_d, 4),
+            "worst_backward_error_in_n_eps": round(worst_b, 4)}
 
 
 def lu_leg(nla, torch, sizes=(4096, 8192), seed=1):
@@ -418,7 +569,7 @@ def serve_leg(nla, torch, count=600, seed=0):
     return out
 
 
-def cpu_baseline(A_host, n, seconds_target=12.0):
+def cpu_baseline(A_host, n, m_total, alpha, seconds_target=12.0):
     """The oracle on host cores over a bounded number of RHS columns: a short
     calibration run at 128 columns, then a sample sized for ~seconds_target."""
     import numpy as np
@@ -429,22 +580,23 @@ def cpu_baseline(A_host, n, seconds_target=12.0):
     m_try = 128
     B = rng.standard_normal((n, m_try))
     t0 = time.perf_counter()
-    O.rectrxm("L", "L", "N", 1.5, "S", A_host, B, nthreads=threads)
+    O.rectrxm("L", "L", "N", alpha, "S", A_host, B, nthreads=threads)
     dt = time.perf_counter() - t0
-    m_s = int(max(128, min(8192, m_try * seconds_target / max(dt, 1e-3))))
+    m_s = int(max(128, min(m_total, m_try * seconds_target / max(dt, 1e-3))))
     m_s = (m_s // 32) * 32
     B = rng.standard_normal((n, m_s))
     t0 = time.perf_counter()
-    X = O.rectrxm("L", "L", "N", 1.5, "S", A_host, B, nthreads=threads)
+    X = O.rectrxm("L", "L", "N", alpha, "S", A_host, B, nthreads=threads)
     dt = time.perf_counter() - t0
     out = {"value": n * n * m_s / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
            "sample": f"oracle restatement of unified_rectrxm! (thresholds 256/16, 32x32 GEMM tiles, OpenMP over "
-                     f"work-groups), FP64 LL-N TRSM on the same A (n={n}) with {m_s} of the 8192 RHS columns, "
+                     f"work-groups), plain FP64 LL-N TRSM on the same A (n={n}) with {m_s} of the {m_total} RHS "
+                     f"columns, "
                      f"{dt:.1f} s"}
     # the reference tests' oracle library on the same sample, for context
     try:
         t0 = time.perf_counter()
-        Y = O.blas("L", "L", "N", 1.5, "S", A_host, B)
+        Y = O.blas("L", "L", "N", alpha, "S", A_host, B)
         dtb = time.perf_counter() - t0
         out["openblas_tflops"] = n * n * m_s / dtb / 1e12
         out["openblas_vs_port_rel_diff"] = O.rel_error(X, Y)
@@ -458,11 +610,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--m", type=int, default=8192)
+    ap.add_argument("--cond", type=float, default=1e4, help="cond_2(A) of the headline refined solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-extra", action="store_true", help="skip FP32/TRMM rates and the mixed and batched legs")
-    ap.add_argument("--leg", choices=("refined_mixed", "batched_f64", "batched_f32", "served", "lu", "lauum", "larfb"),
+    ap.add_argument("--no-extra", action="store_true", help="skip every leg but the headline")
+    ap.add_argument("--leg", choices=("refined_mixed", "drop_in", "cpu_case", "batched_f64", "batched_f32", "served",
+                                      "lu", "lauum", "larfb"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -472,12 +624,15 @@ def main():
 
     import torch
     import nextla_amd as nla
-    from nextla_amd import _lib
 
     nla.load()
     if args.leg:
         if args.leg == "refined_mixed":
-            res = mixed_leg(nla, torch)
+            res = refined_case(nla, torch, args.cond, max(3, args.steps // 3), 1)
+        elif args.leg == "drop_in":
+            res = drop_in_leg(nla, torch)
+        elif args.leg == "cpu_case":
+            res = cpu_case_leg(nla, torch)
         elif args.leg.startswith("batched"):
             res = batch_leg(nla, torch, torch.float64 if args.leg.endswith("f64") else torch.float32)
         elif args.leg == "lu":
@@ -490,75 +645,65 @@ def main():
             res = serve_leg(nla, torch)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
         return
-    n, m = args.n, args.m
-    flops = float(n) * n * m
-    log(f"bench: FP64 LL-N TRSM n={n} m={m} steps={args.steps} warmup={args.warmup}")
-    A, B0, X, times, wall = solve_rate(nla, torch, n, m, "L", "L", "N", "S", 1.5, torch.float64, args.steps,
-                                       args.warmup)
-    mean_ms = sum(times) / len(times)
-    value = flops / (mean_ms * 1e-3) / 1e12
-    p99 = times[min(len(times) - 1, int(math.ceil(0.99 * len(times))) - 1)]
-    bwd = backward_error(torch, "L", "L", "N", 1.5, A, B0, X)
-    log(f"bench: {value:.2f} TFLOP/s, mean {mean_ms:.3f} ms, bwd err {bwd:.2e}")
 
-    # per-kernel timing pass for the dominant kernel (GEMM update)
-    _lib.prof_enable(True)
-    B = B0.clone()
-    for _ in range(3):
-        B.copy_(B0)
-        nla.unified_rectrxm_("L", "L", "N", 1.5, "S", A, B)
-    torch.cuda.synchronize()
-    g_n, g_ms, g_fl, g_by = _lib.prof_summary(_lib.NLA_KIND_GEMM)
-    l_n, l_ms, l_fl, l_by = _lib.prof_summary(_lib.NLA_KIND_LEAF)
-    _lib.prof_enable(False)
-    g_tf = g_fl / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
-    roofline = {"bound": "mfma", "achieved": round(g_tf, 3), "peak": PEAK["f64"], "unit": "TFLOP/s",
-                "frac": round(g_tf / PEAK["f64"], 4), "traffic": traffic_from_profile(n, m),
-                "kernel": "gemm_kernel<double> (MFMA f64 16x16x4 GEMM update)",
-                "traffic_kernel": "gemm_kernel<double,256,128,...> (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
-                "launches_per_solve": g_n // 3, "avg_launch_ms": round(g_ms / max(g_n, 1), 4),
-                "gemm_time_share": round(g_ms / max(g_ms + l_ms, 1e-9), 4)}
-    leaf = {"launches_per_solve": l_n // 3, "avg_launch_ms": round(l_ms / max(l_n, 1), 4),
-            "gflops": round(l_fl / (l_ms * 1e-3) / 1e9, 1) if l_ms > 0 else None}
-    q_bytes = (n * (n + 1) / 2 + 2.0 * n * m) * 8
-    t_roof = max(flops / (PEAK["f64"] * 1e12), q_bytes / (HBM_TBS * 1e12))
-    extra = {"op_roofline_frac": round(t_roof / (mean_ms * 1e-3), 4), "median_ms": times[len(times) // 2],
-             "p99_ms": p99, "wall_ms_per_step_incl_restore": round(wall, 3), "backward_error": bwd,
-             "leaf": leaf}
-    del X, B
+    n, m = 16384, 4096
+    flops = float(n) * n * m
+    log(f"bench: refined FP64-accurate LL-N TRSM n={n} m={m} cond {args.cond:g} steps={args.steps} "
+        f"warmup={args.warmup}")
+    head, A, B0 = refined_case(nla, torch, args.cond, args.steps, args.warmup, keep=True)
+    auto = head["auto"]
+    mean_ms = auto["ms"]
+    value = flops / (mean_ms * 1e-3) / 1e12
+    g = auto.get("int8_products")
+    if g:
+        roofline = {"bound": "mfma", "achieved": g["tops"], "peak": PEAK["int8"], "unit": "TOP/s",
+                    "frac": g["frac_of_int8_peak"],
+                    "traffic": traffic_from_profile("oz_gemm_kernel_refined_n16384_pmc_traffic.md"),
+                    "kernel": "oz_gemm_kernel (int8 MFMA 16x16x64 modular products of the FP64-accurate residual)",
+                    "ops_per_launch": g["ops_per_launch"], "algorithmic_bytes_per_launch": g["bytes_per_launch"],
+                    "avg_launch_ms": g["avg_launch_ms"], "launches_per_solve": g["launches"]}
+    else:
+        roofline = {"bound": "mfma", "achieved": None, "peak": PEAK["int8"], "unit": "TOP/s", "frac": None,
+                    "traffic": None, "kernel": "int8 residual not used"}
+    extra = {"refined_mixed": {f"cond_{args.cond:g}": head}}
+    cpu = None
+    if not args.no_cpu:
+        A_host = A.cpu().numpy()
+        log("bench: CPU baseline (oracle) ...")
+        cpu = cpu_baseline(A_host, n, m, 1.0)
+        log(f"bench: CPU {cpu['value']:.4f} TFLOP/s on {cpu['cores']} cores")
+        del A_host
+    del A, B0
+    torch.cuda.empty_cache()
     if not args.no_extra:
-        for label, dt, func in (("f32_trsm", torch.float32, "S"), ("f64_trmm", torch.float64, "M"),
-                                ("f32_trmm", torch.float32, "M")):
-            _, _, _, t2, _ = solve_rate(nla, torch, n, m, "L", "L", "N", func, 1.5, dt, max(3, args.steps // 2), 2)
-            mm = sum(t2) / len(t2)
-            extra[label + "_tflops"] = round(flops / (mm * 1e-3) / 1e12, 3)
-            log(f"bench: {label} {extra[label + '_tflops']} TFLOP/s")
-        torch.cuda.empty_cache()
-        extra["refined_mixed"] = mixed_leg(nla, torch)
+        for cond in (1e2, 1e6):
+            if cond != args.cond:
+                extra["refined_mixed"][f"cond_{cond:g}"] = refined_case(nla, torch, cond, 3, 1)
+        extra["drop_in"] = drop_in_leg(nla, torch)
+        extra["cpu_case"] = cpu_case_leg(nla, torch)
         extra["batched"] = {}
         for label, dt in (("f64", torch.float64), ("f32", torch.float32)):
             extra["batched"][label] = batch_leg(nla, torch, dt)
             b = extra["batched"][label]
             log(f"bench: batched {label}: {b['ms']} ms ({b['tflops']} TFLOP/s, roofline {b['roofline_frac']}), "
-                f"loop of plain calls {b['loop_of_plain_calls_ms']} ms")
+                f"loop of plain calls {b['loop_of_plain_calls_ms']} ms, worst diff {b['worst_rel_diff_vs_plain_in_n_eps']} n eps")
         extra["served"] = serve_leg(nla, torch)
         extra["lu"] = lu_leg(nla, torch)
         extra["lauum"] = lauum_leg(nla, torch)
         extra["larfb"] = larfb_leg(nla, torch)
     torch.cuda.synchronize()
 
-    cpu = None
-    if not args.no_cpu:
-        A_host = A.cpu().numpy()
-        log("bench: CPU baseline (oracle) ...")
-        cpu = cpu_baseline(A_host, n)
-        log(f"bench: CPU {cpu['value']:.4f} TFLOP/s on {cpu['cores']} cores")
-
     line = {"metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(mean_ms, 4), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated in HBM)",
-            "config": {"workload": "plain FP64 TRSM drop-in, left lower no-transpose, alpha 1.5", "n": n, "m": m,
-                       "side": "L", "uplo": "L", "trans": "N", "func": "S", "parallelism": "single GPU"},
+            "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded, generated in HBM)",
+            "config": {"workload": "FP64-accurate refined-mixed left-lower TRSM (automatic bf16/fp16/fp32 updates, "
+                                   "int8-product FP64 residuals, per-system backward error <= 1e-14)",
+                       "n": n, "m": m, "cond2": head["cond2_measured"], "side": "L", "uplo": "L", "trans": "N",
+                       "compute_types": "bf16/fp16/fp32 correction solves, int8 residual products, f64 "
+                                        "interface and accuracy", "parallelism": "single GPU"},
+            "speedup_vs_plain_fp64": auto["speedup_vs_plain_fp64"], "plain_fp64_ms": head["plain_fp64_ms"],
+            "p99_ms": auto["p99_ms"], "backward_error_per_system": auto["backward_error_per_system"],
             "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
     print(json.dumps(line), flush=True)
 

@@ -407,6 +407,13 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      NLA_RES_FP64 for every size.  Read when the
  *                      workspace size is queried and when the solve starts
  *                      (the int8 residual needs the larger workspace).
+ *   NLA_CFG_LU_SPIN_LIMIT: polls of a multi-work-group LU leaf's bounded
+ *                      grid-barrier wait before it gives up (0 = default,
+ *                      about a second).  A leaf whose wait runs out (its
+ *                      work-groups not all resident, e.g. beside another
+ *                      persistent grid) is redone from a copy saved before
+ *                      it by the one-work-group leaf, so nla_getrf2's result
+ *                      does not depend on it; a small value forces that path.
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {
@@ -419,7 +426,8 @@ enum nla_config_key {
     NLA_CFG_BATCH_CLASSES = 6,
     NLA_CFG_PANEL_WAVES = 7,
     NLA_CFG_LU_SPLIT_ROWS = 8,
-    NLA_CFG_RESIDUAL = 9
+    NLA_CFG_RESIDUAL = 9,
+    NLA_CFG_LU_SPIN_LIMIT = 10
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);
@@ -433,9 +441,18 @@ int32_t nla_set_config(int32_t key, int32_t value);
  *   launch count, summed kernel time (ms), algorithmic flops and bytes of one
  *   kernel kind: NLA_KIND_GEMM (fp32/fp64 GEMM updates), NLA_KIND_LEAF
  *   (diagonal-block leaves and panels), NLA_KIND_LP_GEMM (bf16/fp16 updates of
- *   the mixed-precision solve) or NLA_KIND_RESIDUAL (its FP64 residuals).
+ *   the mixed-precision solve), NLA_KIND_RESIDUAL (its residuals: the FP64
+ *   product, or the int8 method's conversions and reconstruction) or
+ *   NLA_KIND_RESIDUAL_GEMM (the int8 method's modular products; flops are
+ *   its int8 operations, T·n^2·m for T moduli).
  */
-enum nla_kernel_kind { NLA_KIND_GEMM = 0, NLA_KIND_LEAF = 1, NLA_KIND_LP_GEMM = 2, NLA_KIND_RESIDUAL = 3 };
+enum nla_kernel_kind {
+    NLA_KIND_GEMM = 0,
+    NLA_KIND_LEAF = 1,
+    NLA_KIND_LP_GEMM = 2,
+    NLA_KIND_RESIDUAL = 3,
+    NLA_KIND_RESIDUAL_GEMM = 4
+};
 
 int32_t nla_prof_enable(int32_t on);
 

@@ -138,6 +138,8 @@ constexpr int64_t OZ_MIN_N = 2048, OZ_MIN_M = 64;
 // rows, several above); 0 = the global-memory leaf for every size.
 static std::atomic<int> g_lu_split_rows{1};
 int lu_split_rows() { return g_lu_split_rows.load(); }
+static std::atomic<int> g_lu_spin_limit{0};
+int lu_spin_limit() { return g_lu_spin_limit.load(); }
 
 // Leaf method: 1 = inverted diagonal blocks applied on MFMA (default),
 // 0 = register/LDS substitution kernel.
@@ -612,6 +614,23 @@ static double pow2_above(double x) {
     return std::ldexp(1.0, e);
 }
 
+// The int8 residual's phases, each recorded by the kernel timer: the
+// conversions and the reconstruction as NLA_KIND_RESIDUAL, the modular
+// products as NLA_KIND_RESIDUAL_GEMM (T n^2 m int8 operations).
+static hipError_t oz_timed(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, double *X,
+                           int64_t xso, int64_t xsk, int64_t n, int64_t m, const double *S, int64_t lds, double *R,
+                           int64_t ldr, hipStream_t s) {
+    const double nm = (double)n * m;
+    hipError_t e = timed(NLA_KIND_RESIDUAL, 0.0, nm * (16.0 + P.T), s,
+                         [&] { return oz_convert_x(P, L, ws, X, xso, xsk, n, m, s); });
+    if (e != hipSuccess) return e;
+    e = timed(NLA_KIND_RESIDUAL_GEMM, (double)P.T * n * n * m, (double)P.T * ((double)n * n / 2 + 2.0 * nm), s,
+              [&] { return oz_products(P, L, ws, lower, out_t, s); });
+    if (e != hipSuccess) return e;
+    return timed(NLA_KIND_RESIDUAL, nm, nm * (16.0 + P.T), s,
+                 [&] { return oz_reconstruct(P, L, ws, out_t, n, m, S, lds, R, ldr, s); });
+}
+
 static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precision, double alpha, int64_t n,
                                   int64_t m, const double *A, int64_t lda, double *B, int64_t ldb, double tol,
                                   int max_iter, char *ws, const MixedLayout &L, hipStream_t s,
@@ -778,7 +797,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     // row (side R); X is rounded in place to bX bits per system
     const int64_t xso = (side == 'L') ? ldb : 1, xsk = (side == 'L') ? 1 : ldb;
     auto int8_residual = [&]() -> hipError_t {
-        return oz_residual(L.op, L.ol, ws, ahat_lower, side == 'R', B, xso, xsk, n, m, S, L.ldr, R, L.ldr, s);
+        return oz_timed(L.op, L.ol, ws, ahat_lower, side == 'R', B, xso, xsk, n, m, S, L.ldr, R, L.ldr, s);
     };
     // per-system test: max_j ||r_j|| / (||A||_lb ||x_j|| + ||s_j||), with the
     // int8 residual's range flag read in the same synchronization
@@ -797,7 +816,10 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     double prev = INFINITY, bwd = INFINITY, bwd_f = INFINITY;
     int it = 0;
     for (;; ++it) {
-        NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, [&] { return use_oz ? int8_residual() : fp64_residual(); }));
+        if (use_oz)
+            NLA_CK(int8_residual());
+        else
+            NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, fp64_residual));
         NLA_CK(stop_norms(use_oz));
         if (use_oz && hflag) {
             // S or X outside the moduli's range (or not finite): this and
@@ -1096,12 +1118,8 @@ int32_t nla_residual(char side, char uplo, char trans, int32_t method, double al
     const bool ahat_is_a = (side == 'L') != tr;
     const bool ahat_lower = ahat_is_a ? (uplo == 'L') : (uplo == 'U');
     const int64_t xso = (side == 'L') ? ldx : 1, xsk = (side == 'L') ? 1 : ldx;
-    e = timed(NLA_KIND_RESIDUAL, (double)n * n * m, ((double)n * (n + 1) / 2 + 3.0 * n * m) * 8, s, [&] {
-        hipError_t e2 = oz_prepare(P, L, ws, A, ahat_is_a ? 1 : lda, ahat_is_a ? lda : 1, ahat_lower, n, s);
-        if (e2 == hipSuccess)
-            e2 = oz_residual(P, L, ws, ahat_lower, side == 'R', X, xso, xsk, n, m, R, ldr, R, ldr, s);
-        return e2;
-    });
+    e = oz_prepare(P, L, ws, A, ahat_is_a ? 1 : lda, ahat_is_a ? lda : 1, ahat_lower, n, s);
+    if (e == hipSuccess) e = oz_timed(P, L, ws, ahat_lower, side == 'R', X, xso, xsk, n, m, R, ldr, R, ldr, s);
     if (e != hipSuccess) return hip_status(e, "residual");
     int flag = 0;
     e = hipMemcpyAsync(&flag, ws + L.flag, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -1143,6 +1161,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
         if (value != 0 && value != 4 && value != 8) return -1;
         return g_panel_waves.exchange(value);
     }
+    if (key == NLA_CFG_LU_SPIN_LIMIT) {
+        if (value < 0) return -1;
+        return g_lu_spin_limit.exchange(value);
+    }
     if (key == NLA_CFG_RESIDUAL) {
         if (value != NLA_RES_FP64 && value != NLA_RES_INT8) return -1;
         return g_residual.exchange(value);
@@ -1167,7 +1189,7 @@ int32_t nla_prof_enable(int32_t on) {
 }
 
 int32_t nla_prof_summary(int32_t kind, int64_t *launches, double *ms, double *flops, double *bytes) {
-    if (kind < NLA_KIND_GEMM || kind > NLA_KIND_RESIDUAL) return bad_arg(1, "kind");
+    if (kind < NLA_KIND_GEMM || kind > NLA_KIND_RESIDUAL_GEMM) return bad_arg(1, "kind");
     std::lock_guard<std::mutex> g(g_prof_mu);
     int64_t n = 0;
     double t = 0, f = 0, b = 0;

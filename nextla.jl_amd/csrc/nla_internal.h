@@ -155,6 +155,10 @@ constexpr int BATCH_CLASSES[4] = {128, 256, 512, 1024};
 // largest is always used); set with nla_set_config(NLA_CFG_BATCH_CLASSES).
 int batch_class_mask();
 int lu_split_rows();
+// Polls of a split LU leaf's bounded grid-barrier wait before it gives up
+// and leaves the block to the recovery launch (0 = the default, about a
+// second); set with nla_set_config(NLA_CFG_LU_SPIN_LIMIT).
+int lu_spin_limit();
 
 // Kernel entry of the persistent batched solver for one size class (for
 // occupancy queries); nullptr for an unknown class.
@@ -318,13 +322,19 @@ OzLayout oz_layout(const OzPlan &P, int64_t base);
 // lower or upper).  Clears the flag word; a non-finite A sets it.
 hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double *A, int64_t ars, int64_t acs,
                       bool lower, int64_t n, hipStream_t s);
-// R = S - Ahat * Xhat (out_t = false: R n x m, Xhat(k,j) = X[j*xso + k*xsk])
-// or its transpose (out_t = true: R m x n, side R).  X is replaced by its
-// value rounded to bX bits per system.  Sets the flag word when S or X is
-// not finite or S exceeds the moduli's range (R is then not valid).
-hipError_t oz_residual(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, double *X,
-                       int64_t xso, int64_t xsk, int64_t n, int64_t m, const double *S, int64_t lds, double *R,
-                       int64_t ldr, hipStream_t s);
+// The three phases of R = S - Ahat * Xhat (out_t = false: R n x m; out_t =
+// true: R m x n, its transpose, for side R), Xhat(k,j) = X[j*xso + k*xsk]:
+//   oz_convert_x: X's system scales and residue planes; X is replaced by its
+//     value rounded to bX bits per system (a non-finite X sets the flag);
+//   oz_products: the T modular products (int8 MFMA);
+//   oz_reconstruct: R from S and the product residues; an S that is not
+//     finite or exceeds the moduli's range sets the flag word (R is then
+//     not valid).
+hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X, int64_t xso, int64_t xsk,
+                        int64_t n, int64_t m, hipStream_t s);
+hipError_t oz_products(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, hipStream_t s);
+hipError_t oz_reconstruct(const OzPlan &P, const OzLayout &L, char *ws, bool out_t, int64_t n, int64_t m,
+                          const double *S, int64_t lds, double *R, int64_t ldr, hipStream_t s);
 
 // ---- batched refinement ----------------------------------------------------
 

@@ -66,9 +66,8 @@ __device__ inline void wave_argmax(T &bv, T &bs, int64_t &bi) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, T *__restrict__ A, int64_t lda,
-                                                              int64_t *__restrict__ ipiv, int64_t *__restrict__ info,
-                                                              int64_t r0) {
+__device__ void lu_leaf_body(int64_t m, int n, T *__restrict__ A, int64_t lda, int64_t *__restrict__ ipiv,
+                             int64_t *__restrict__ info, int64_t r0) {
     __shared__ T s_val[2][LEAF_WAVES], s_sgn[2][LEAF_WAVES];
     __shared__ int64_t s_row[2][LEAF_WAVES];
     __shared__ T s_u[LEAF_COLS];  // row j of the leaf after the interchange
@@ -184,6 +183,13 @@ __global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, 
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, T *__restrict__ A, int64_t lda,
+                                                              int64_t *__restrict__ ipiv, int64_t *__restrict__ info,
+                                                              int64_t r0) {
+    lu_leaf_body<T>(m, n, A, lda, ipiv, info, r0);
+}
+
 // The same leaf for tall blocks, spread over G work-groups of SPLIT_ROWS
 // rows each, the block held in registers from the first load to the final
 // store.  Per column: each work-group finds its candidate (first index of
@@ -196,9 +202,14 @@ __global__ __launch_bounds__(PIV_THREADS) void lu_leaf_kernel(int64_t m, int n, 
 // 64 groups of one CU each are launched, which an idle part of the chip
 // holds at once; a cooperative launch would guarantee it, but costs about
 // 150 us once other streams have been used.  Each wait is bounded, so a
-// group that is not resident cannot hang the others: a wait that runs out
-// sets info to -1.  The last group to finish resets the counters for the
-// next launch on the stream.
+// group that is not resident (another persistent grid holds the CUs) cannot
+// hang the others.  Every group saves its rows before the first barrier (and
+// group 0 the prior info), and a wait that runs out raises a flag; the
+// one-work-group recovery launch that follows every split leaf returns at
+// once unless the flag is up, and otherwise restores the saved block and
+// factors it with the global-memory leaf, so the factorization stays exact.
+// The last group to finish resets the counters for the next launch on the
+// stream.
 // With ONE (a single work-group) the group's own winner is the pivot and
 // nothing is published: the whole leaf stays in registers with two
 // work-group barriers per column and no memory round trip.
@@ -207,6 +218,11 @@ constexpr int ONE_THREADS = 512, ONE_RPT = 8, ONE_ROWS = ONE_THREADS * ONE_RPT;
 constexpr int SPLIT_MAX_WG = 64, SPLIT_SLOT = 16;  // 64-bit words per published candidate
 constexpr int SYNC_SLOTS = 64, SYNC_DIAG = SYNC_SLOTS + 2 * SPLIT_MAX_WG * SPLIT_SLOT;
 constexpr int SYNC_WORDS = SYNC_DIAG + 2 * SPLIT_SLOT;
+// words 2, 3 of the buffer: the timeout flag and the info value saved before
+// the leaf; after SYNC_WORDS: the saved block, column c at c * SAVE_LD
+constexpr int SYNC_FLAG = 2, SYNC_INFO = 3;
+constexpr int64_t SAVE_LD = (int64_t)SPLIT_ROWS * SPLIT_MAX_WG;
+constexpr int64_t SYNC_BYTES = SYNC_WORDS * 8 + SAVE_LD * LEAF_COLS * 8;
 constexpr int64_t SPLIT_SPIN_LIMIT = int64_t(1) << 20;  // about a second of polling
 
 __device__ inline void st_agent(uint64_t *p, uint64_t v) {
@@ -219,7 +235,8 @@ __device__ inline double bitsd(uint64_t v) { return __longlong_as_double((long l
 template <typename T, int NT, int RPT, bool ONE>
 __global__ __launch_bounds__(NT) void lu_leaf_reg_kernel(int64_t m, int n, T *__restrict__ A, int64_t lda,
                                                          int64_t *__restrict__ ipiv, int64_t *__restrict__ info,
-                                                         int64_t r0, uint64_t *__restrict__ sync) {
+                                                         int64_t r0, uint64_t *__restrict__ sync,
+                                                         int64_t spin_limit) {
     constexpr int W = NT / 64, ROWS = NT * RPT;
     __shared__ T s_val[W], s_sgn[W];
     __shared__ int64_t s_row[W];
@@ -241,6 +258,22 @@ __global__ __launch_bounds__(NT) void lu_leaf_reg_kernel(int64_t m, int n, T *__
         for (int c = 0; c < LEAF_COLS; ++c) a[r][c] = (i < m && c < n) ? A[i + (int64_t)c * lda] : T(0);
     }
     if (t == 0) s_fail = 0;
+    if constexpr (!ONE) {
+        // the block as it was, for the recovery launch
+        T *save = reinterpret_cast<T *>(sync + SYNC_WORDS);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int64_t i = base + (int64_t)r * NT;
+            if (i < m) {
+#pragma unroll
+                for (int c = 0; c < LEAF_COLS; ++c)
+                    if (c < n) save[i + (int64_t)c * SAVE_LD] = a[r][c];
+            }
+        }
+        // only group 0 writes info, and the same thread arrives first at
+        // every barrier after this store
+        if (g == 0 && t == 0) st_agent(sync + SYNC_INFO, (uint64_t)*info);
+    }
     bool failed = false;  // lane 0 of wave 0: a wait ran out; later ones are skipped
 #pragma unroll
     for (int j = 0; j < LEAF_COLS; ++j) {
@@ -298,7 +331,7 @@ __global__ __launch_bounds__(NT) void lu_leaf_reg_kernel(int64_t m, int n, T *__
                     const uint64_t target = (uint64_t)G * (uint64_t)(j + 1);
                     int64_t spins = 0;
                     while (!failed && ld_agent(arrive) < target) {
-                        if (++spins > SPLIT_SPIN_LIMIT) {
+                        if (++spins > spin_limit) {
                             failed = true;
                             s_fail = 1;
                             break;
@@ -397,7 +430,7 @@ __global__ __launch_bounds__(NT) void lu_leaf_reg_kernel(int64_t m, int n, T *__
     if constexpr (ONE) return;
     __syncthreads();
     if (t == 0) {
-        if (s_fail) *info = -1;
+        if (s_fail) st_agent(sync + SYNC_FLAG, 1);
         // the last group out resets the counters (every group has passed
         // every barrier by then)
         const uint64_t prev = __hip_atomic_fetch_add(leave, (uint64_t)1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -406,6 +439,27 @@ __global__ __launch_bounds__(NT) void lu_leaf_reg_kernel(int64_t m, int n, T *__
             st_agent(leave, 0);
         }
     }
+}
+
+// After every split leaf: nothing unless one of its waits ran out; then the
+// block saved before the leaf is put back, info restored, and the block
+// factored by the one-work-group global-memory leaf (pivots rewritten).
+template <typename T>
+__global__ __launch_bounds__(PIV_THREADS) void lu_leaf_recover_kernel(int64_t m, int n, T *__restrict__ A, int64_t lda,
+                                                                      int64_t *__restrict__ ipiv,
+                                                                      int64_t *__restrict__ info, int64_t r0,
+                                                                      uint64_t *__restrict__ sync) {
+    if (__hip_atomic_load(sync + SYNC_FLAG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    const T *save = reinterpret_cast<const T *>(sync + SYNC_WORDS);
+    for (int64_t idx = threadIdx.x; idx < m * n; idx += PIV_THREADS) {
+        const int64_t i = idx % m, c = idx / m;
+        A[i + c * lda] = save[i + c * SAVE_LD];
+    }
+    if (threadIdx.x == 0) *info = (int64_t)sync[SYNC_INFO];
+    __syncthreads();
+    lu_leaf_body<T>(m, n, A, lda, ipiv, info, r0);
+    __syncthreads();
+    if (threadIdx.x == 0) st_agent(sync + SYNC_FLAG, 0);
 }
 
 __global__ void sync_zero_kernel(uint64_t *p, int words) {
@@ -550,7 +604,7 @@ uint64_t *split_sync(hipStream_t s, int32_t *st) {
     for (auto &c : cache)
         if (c.dev == dev && c.stream == s) return c.ptr;
     uint64_t *p = nullptr;
-    e = hipMalloc((void **)&p, SYNC_WORDS * sizeof(uint64_t));
+    e = hipMalloc((void **)&p, SYNC_BYTES);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(sync_zero_kernel, dim3(1), dim3(256), 0, s, p, SYNC_WORDS);
         e = hipGetLastError();
@@ -571,6 +625,7 @@ struct LuRec {
     hipStream_t s;
     int32_t st = NLA_OK;
     int64_t split_rows = lu_split_rows();
+    int64_t spin_limit = lu_spin_limit() > 0 ? lu_spin_limit() : SPLIT_SPIN_LIMIT;
 
     // an m x n leaf (n <= LEAF_COLS) over ceil(m / SPLIT_ROWS) work-groups
     void split_leaf(int64_t m, int n, T *A, int64_t *ipiv, int64_t r0) {
@@ -578,8 +633,11 @@ struct LuRec {
         if (!sync) return;
         const unsigned g = (unsigned)((m + SPLIT_ROWS - 1) / SPLIT_ROWS);
         hipLaunchKernelGGL((lu_leaf_reg_kernel<T, SPLIT_THREADS, SPLIT_RPT, false>), dim3(g), dim3(SPLIT_THREADS), 0, s,
-                           m, n, A, lda, ipiv, info, r0, sync);
-        ok(hipGetLastError(), "lu split leaf");
+                           m, n, A, lda, ipiv, info, r0, sync, spin_limit);
+        if (!ok(hipGetLastError(), "lu split leaf")) return;
+        hipLaunchKernelGGL(lu_leaf_recover_kernel<T>, dim3(1), dim3(PIV_THREADS), 0, s, m, n, A, lda, ipiv, info, r0,
+                           sync);
+        ok(hipGetLastError(), "lu split leaf recovery");
     }
 
     bool ok(hipError_t e, const char *where) {
@@ -612,7 +670,7 @@ struct LuRec {
         if (n <= LEAF_COLS && split_rows > 0 && m >= split_rows && m <= (int64_t)SPLIT_ROWS * SPLIT_MAX_WG) {
             if (m <= ONE_ROWS) {
                 hipLaunchKernelGGL((lu_leaf_reg_kernel<T, ONE_THREADS, ONE_RPT, true>), dim3(1), dim3(ONE_THREADS), 0, s,
-                                   m, (int)n, A, lda, ipiv, info, r0, (uint64_t *)nullptr);
+                                   m, (int)n, A, lda, ipiv, info, r0, (uint64_t *)nullptr, (int64_t)0);
                 ok(hipGetLastError(), "lu register leaf");
             } else {
                 split_leaf(m, (int)n, A, ipiv, r0);

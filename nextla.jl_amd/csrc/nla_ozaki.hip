@@ -139,7 +139,9 @@ __global__ __launch_bounds__(256) void oz_absmax_kernel(OzSrc s, unsigned long l
 
 // e[o] = bits - 1 - floor(log2 mx[o]) (so that max |src(o,.)| 2^e[o] lies in
 // [2^(bits-1), 2^bits)), 0 for an all-zero row; a non-finite maximum sets
-// *flag.
+// *flag and e[o] = OZ_BAD_EXP (the row is then neither converted nor
+// rewritten).
+constexpr int OZ_BAD_EXP = -(1 << 20);
 __global__ void oz_exponent_kernel(int64_t no, const unsigned long long *mx, int bits, int *e, int *flag) {
     const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= no) return;
@@ -147,6 +149,7 @@ __global__ void oz_exponent_kernel(int64_t no, const unsigned long long *mx, int
     int r = 0;
     if (!(v <= 1.7976931348623157e308)) {
         *flag = 1;
+        r = OZ_BAD_EXP;
     } else if (v > 0.0) {
         r = bits - 1 - ilogb(v);
     }
@@ -195,14 +198,16 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, 
     __syncthreads();
     const int oo = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
     const int64_t o = o0 + oo;
-    const int eo = (o < s.no) ? e[o] : 0;
+    const int eraw = (o < s.no) ? e[o] : 0;
+    const bool bad = (eraw == OZ_BAD_EXP);  // a non-finite row: residues 0, values kept
+    const int eo = bad ? 0 : eraw;
     double m[16];
     int sh[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-        const double v = rint(ldexp(tile[oo][kq + u], eo));
+        const double v = bad ? 0.0 : rint(ldexp(tile[oo][kq + u], eo));
         oz_split(v, m[u], sh[u]);
-        if (writeback) tile[oo][kq + u] = ldexp(v, -eo);
+        if (writeback && !bad) tile[oo][kq + u] = ldexp(v, -eo);
     }
     const int64_t off = d.panel ? oz_panel_off(d.panel, d.npad, o, k0 + kq) : o * d.ld + k0 + kq;
     for (int t = 0; t < C.T; ++t) {
@@ -573,22 +578,21 @@ static int oz_gemm_occupancy() {
     return occ;
 }
 
-hipError_t oz_residual(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, double *X,
-                       int64_t xso, int64_t xsk, int64_t n, int64_t m, const double *S, int64_t lds, double *R,
-                       int64_t ldr, hipStream_t s) {
-    int *flag = (int *)(ws + L.flag);
-    // X: column scales (a system's largest entry), residues, rounded iterate
+hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X, int64_t xso, int64_t xsk,
+                        int64_t n, int64_t m, hipStream_t s) {
+    // column scales (a system's largest entry), residues, rounded iterate
     OzSrc xs{X, xso, xsk, m, n, 0};
     int *fx = (int *)(ws + L.fx);
-    hipError_t err = oz_scales(xs, (unsigned long long *)(ws + L.mxx), P.bX, fx, flag, s);
+    hipError_t err = oz_scales(xs, (unsigned long long *)(ws + L.mxx), P.bX, fx, (int *)(ws + L.flag), s);
     if (err != hipSuccess) return err;
     OzDst d{(int8_t *)(ws + L.x), P.xplane, P.npad, P.npad, 0};
     hipLaunchKernelGGL(oz_residue_kernel, dim3((unsigned)(P.npad / 64), (unsigned)(P.mpad / 64)), dim3(256), 0, s,
                        xs, (const int *)fx, d, 1, P.C, (const uint8_t *)(ws + L.pw));
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    // modular products
-    err = oz_fill(ws + L.queue, 1, s);
+    return hipGetLastError();
+}
+
+hipError_t oz_products(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, hipStream_t s) {
+    hipError_t err = oz_fill(ws + L.queue, 1, s);
     if (err != hipSuccess) return err;
     OzGemmArgs g{};
     g.A = (const int8_t *)(ws + L.a);
@@ -616,10 +620,14 @@ hipError_t oz_residual(const OzPlan &P, const OzLayout &L, char *ws, bool lower,
         hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3(grid), dim3(512), 0, s, g, P.C);
     else
         hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3(grid), dim3(512), 0, s, g, P.C);
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    // reconstruction: R is rows x cols column-major (side L: n x m with rows
-    // scaled by A's exponents; side R: m x n with rows scaled by X's)
+    return hipGetLastError();
+}
+
+hipError_t oz_reconstruct(const OzPlan &P, const OzLayout &L, char *ws, bool out_t, int64_t n, int64_t m,
+                          const double *S, int64_t lds, double *R, int64_t ldr, hipStream_t s) {
+    // R is rows x cols column-major (side L: n x m with rows scaled by A's
+    // exponents; side R: m x n with rows scaled by X's)
+    const int *fx = (const int *)(ws + L.fx), *ea = (const int *)(ws + L.ea);
     OzCrtArgs c{};
     c.rows = out_t ? m : n;
     c.cols = out_t ? n : m;
@@ -630,10 +638,10 @@ hipError_t oz_residual(const OzPlan &P, const OzLayout &L, char *ws, bool lower,
     c.Cr = (const uint8_t *)(ws + L.c);
     c.cplane = P.cplane;
     c.ldc = out_t ? P.mpad : P.npad;
-    c.er = out_t ? fx : (const int *)(ws + L.ea);
-    c.ec = out_t ? (const int *)(ws + L.ea) : fx;
+    c.er = out_t ? fx : ea;
+    c.ec = out_t ? ea : fx;
     c.pw = (const uint8_t *)(ws + L.pw);
-    c.flag = flag;
+    c.flag = (int *)(ws + L.flag);
     const unsigned gy = (unsigned)(c.cols < 65535 ? c.cols : 65535);
     hipLaunchKernelGGL(oz_crt_kernel, dim3((unsigned)((c.rows + 255) / 256), gy), dim3(256), 0, s, c, P.C);
     return hipGetLastError();

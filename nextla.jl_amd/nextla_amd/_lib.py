@@ -78,6 +78,7 @@ NLA_KIND_GEMM = 0
 NLA_KIND_LEAF = 1
 NLA_KIND_LP_GEMM = 2
 NLA_KIND_RESIDUAL = 3
+NLA_KIND_RESIDUAL_GEMM = 4
 NLA_LP_BF16 = 0
 NLA_LP_FP16 = 1
 NLA_LP_FP32 = 2
@@ -93,6 +94,7 @@ NLA_CFG_BATCH_CLASSES = 6
 NLA_CFG_PANEL_WAVES = 7
 NLA_CFG_LU_SPLIT_ROWS = 8
 NLA_CFG_RESIDUAL = 9
+NLA_CFG_LU_SPIN_LIMIT = 10
 NLA_RES_FP64 = 0
 NLA_RES_INT8 = 1
 

@@ -47,7 +47,7 @@ from .batch import BATCH_N_MAX, BatchPlan, MixedBatchPlan
 
 
 class Request:
-    __slots__ = ("rid", "side", "uplo", "trans", "func", "alpha", "A", "B", "arrival", "done", "a_key")
+    __slots__ = ("rid", "side", "uplo", "trans", "func", "alpha", "A", "B", "arrival", "done", "a_key", "ready")
 
     def __init__(self, rid, side, uplo, trans, func, alpha, A, B, arrival=None, a_key=None):
         self.rid = rid
@@ -57,6 +57,16 @@ class Request:
         self.arrival = arrival
         self.done = None
         self.a_key = a_key  # identifies a device-resident A shared between requests
+        self.ready = None  # event on the submitting stream after A and B were written
+
+    def wait_ready(self, stream):
+        """Order `stream` after the submitter's writes of A and B, and tell the
+        caching allocator that `stream` uses them (so memory the submitter
+        frees is not reused while the server's work still reads it)."""
+        if self.ready is not None:
+            stream.wait_event(self.ready)
+        self.A.record_stream(stream)
+        self.B.record_stream(stream)
 
     def signature(self):
         return (self.A.dtype, self.side, self.uplo, self.trans, self.func)
@@ -91,6 +101,8 @@ class _MixedWorkers:
         backward error of each."""
         from .mixed import rectrxm_mixed_
         stream = self._stream()
+        for r in reqs:
+            r.wait_ready(stream)
         small = [r for r in reqs if r.A.shape[0] <= BATCH_N_MAX]
         big = [r for r in reqs if r.A.shape[0] > BATCH_N_MAX]
         errors = {}
@@ -179,8 +191,13 @@ class Server:
 
     # -- request flow -------------------------------------------------------
     def submit(self, req):
+        """Queue a request.  Its A and B may still be being written by work
+        queued on the caller's current stream: the request's batch waits for
+        that work (an event recorded here) before reading them."""
         if req.arrival is None:
             req.arrival = time.perf_counter()
+        req.ready = torch.cuda.Event()
+        req.ready.record(torch.cuda.current_stream(self.device))
         self.queue.append(req)
 
     def _is_mixed(self, r):
@@ -264,6 +281,8 @@ class Server:
         f.a_keys = {r.a_key for r in take if r.a_key is not None}
         small = [r for r in take if r.A.shape[0] <= self.direct_rows]
         big = [r for r in take if r.A.shape[0] > self.direct_rows]
+        for r in take:
+            r.wait_ready(stream)
         with torch.cuda.stream(stream):
             f.plan = None
             if small:

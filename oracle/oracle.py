@@ -234,7 +234,7 @@ def _getrf2_rec(A, ipiv):
 
 def _unit_lower_solve(L, B):
     import scipy.linalg
-    return scipy.linalg.solve_triangular(L, B, lower=True, unit_diagonal=True)
+    return scipy.linalg.solve_triangular(L, B, lower=True, unit_diagonal=True, check_finite=False)
 
 
 def getrf2(A):

@@ -200,6 +200,10 @@ def test_lu_split_rows_config(nla):
     finally:
         _lib.set_config(_lib.NLA_CFG_LU_SPLIT_ROWS, old)
     assert _lib.set_config(99, 1) == -1
+    # the split leaf's spin limit: 0 (default) restores the built-in limit
+    assert _lib.set_config(_lib.NLA_CFG_LU_SPIN_LIMIT, -5) == -1
+    assert _lib.set_config(_lib.NLA_CFG_LU_SPIN_LIMIT, 7) == 0
+    assert _lib.set_config(_lib.NLA_CFG_LU_SPIN_LIMIT, 0) == 7
 
 
 def test_lu_lauum_larfb_validation(nla):

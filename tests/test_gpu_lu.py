@@ -157,18 +157,77 @@ def test_split_leaf_equals_single_group_leaf(nla, split_rows, dtype, m, n):
     assert torch.max(torch.abs(L1 - L2)).item() <= tol * torch.max(torch.abs(L1)).item()
 
 
-def test_split_leaf_singular_and_padded(nla, split_rows):
-    """A zero column and lda > m through the split kernel: info, pivots and
-    factors as the oracle's, the rows past m untouched."""
-    split_rows(1)
+@pytest.mark.parametrize("rows,split", [(3000, 1), (5000, 1), (5000, 0)],
+                         ids=["one-group-register", "split", "global-memory"])
+def test_leaf_singular_and_padded(nla, split_rows, rows, split):
+    """A zero column and lda > m through each leaf kernel (m = 5000 with
+    split rows 1 takes the multi-work-group kernel for its leaves, 3000 the
+    one-work-group register kernel, split rows 0 the global-memory leaf):
+    info, pivots and factors as the oracle's, the rows past m untouched."""
+    split_rows(split)
     rng = np.random.default_rng(11)
-    A = rng.standard_normal((3000, 12))
+    A = rng.standard_normal((rows, 12))
     A[:, 2] = 0.0
-    big = torch.full((12, 3100), 7.0, dtype=torch.float64, device=DEV).t()
-    big[:3000].copy_(torch.from_numpy(A).to(DEV))
-    LU, ipiv, info = nla.getrf2_(big[:3000])
+    big = torch.full((12, rows + 100), 7.0, dtype=torch.float64, device=DEV).t()
+    big[:rows].copy_(torch.from_numpy(A).to(DEV))
+    LU, ipiv, info = nla.getrf2_(big[:rows])
     ref, ref_piv, ref_info = O.getrf2(A)
     assert info == ref_info == 3
     assert np.array_equal(host(ipiv), ref_piv)
     assert np.max(np.abs(host(LU) - ref)) <= 1e-12 * np.max(np.abs(ref))
-    assert torch.all(big[3000:] == 7.0)
+    assert torch.all(big[rows:] == 7.0)
+
+
+@pytest.mark.parametrize("rows,split", [(5000, 1), (5000, 0), (2000, 1)])
+def test_leaf_all_nan_column(nla, split_rows, rows, split):
+    """An all-NaN column: no finite pivot, so its row stays (pivot = the row
+    itself, as the oracle's first-maximum rule gives for all-NaN), info 0,
+    and the NaNs spread exactly where the oracle's do."""
+    split_rows(split)
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((rows, 10))
+    A[:, 4] = np.nan
+    LU, ipiv, info = nla.getrf2_(dev(A))
+    ref, ref_piv, ref_info = O.getrf2(A)
+    got = host(LU)
+    assert info == ref_info == 0
+    assert np.array_equal(host(ipiv), ref_piv)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    assert np.max(np.abs(got[ok] - ref[ok])) <= 1e-12 * np.max(np.abs(ref[ok]))
+
+
+@pytest.fixture
+def spin_limit():
+    from nextla_amd import _lib
+    olds = []
+
+    def set_(v):
+        olds.append(_lib.set_config(_lib.NLA_CFG_LU_SPIN_LIMIT, v))
+    yield set_
+    if olds:
+        _lib.set_config(_lib.NLA_CFG_LU_SPIN_LIMIT, olds[0])
+
+
+@pytest.mark.parametrize("m,n", [(20000, 8), (9000, 24)])
+def test_split_leaf_timeout_is_recovered(nla, split_rows, spin_limit, m, n):
+    """A grid-barrier wait of the multi-work-group leaf that runs out (forced
+    from the host with a one-poll limit, so work-groups that are merely a
+    little behind count as missing): the leaf is redone from its saved copy
+    by the one-work-group leaf, and the factorization equals the normal one
+    (pivots equal, factors to rounding), singular info included."""
+    g = torch.Generator(device=DEV)
+    g.manual_seed(m + n)
+    A0 = torch.empty((n, m), dtype=torch.float64, device=DEV).normal_(generator=g).t()
+    A0[:, 3] = 0.0
+    split_rows(1)
+    L1, p1, i1 = nla.getrf2_(A0.clone())
+    spin_limit(1)
+    for _ in range(2):
+        L2, p2, i2 = nla.getrf2_(A0.clone())
+        assert i1 == i2 == 4
+        assert torch.equal(p1, p2)
+        assert torch.max(torch.abs(L1 - L2)).item() <= 1e-12 * torch.max(torch.abs(L1)).item()
+    spin_limit(0)
+    L3, p3, i3 = nla.getrf2_(A0.clone())  # and the normal path after a recovery
+    assert i3 == 4 and torch.equal(p1, p3) and torch.equal(L1, L3)

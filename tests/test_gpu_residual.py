@@ -153,13 +153,19 @@ def test_int8_residual_falls_back_outside_range(nla):
     R8, _ = run("L", "L", "N", 1.0, A, X, Bbig, "int8")
     R64, _ = run("L", "L", "N", 1.0, A, X, Bbig, "fp64")
     assert np.array_equal(R8, R64)
+    # a NaN in system 2: that system is left as it was, the others rounded
+    # as the model rounds them, and R is the FP64 product's for that X
     Xn = X.copy()
     Xn[5, 2] = np.nan
-    R8, _ = run("L", "L", "N", 1.0, A, Xn, B, "int8")
-    R64, _ = run("L", "L", "N", 1.0, A, Xn, B, "fp64")
-    assert np.array_equal(np.isnan(R8), np.isnan(R64))
-    ok = ~np.isnan(R64)
-    assert np.array_equal(R8[ok], R64[ok])
+    R8, X8 = run("L", "L", "N", 1.0, A, Xn, B, "int8")
+    T, bA, bX = RO.plan_bits(n)
+    _, Xm = RO.fixed_point_model("L", "L", "N", 1.0, A, np.nan_to_num(Xn), B, bA, bX)
+    assert np.array_equal(X8[:, 2], Xn[:, 2], equal_nan=True)
+    others = [j for j in range(m) if j != 2]
+    assert np.array_equal(X8[:, others], Xm[:, others])
+    R64, _ = run("L", "L", "N", 1.0, A, X8, B, "fp64")
+    assert np.array_equal(R8, R64, equal_nan=True)
+    assert np.all(np.isnan(R8[:, 2]))
 
 
 def test_int8_residual_alpha_zero_and_zero_system(nla):
