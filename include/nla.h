@@ -414,6 +414,9 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      persistent grid) is redone from a copy saved before
  *                      it by the one-work-group leaf, so nla_getrf2's result
  *                      does not depend on it; a small value forces that path.
+ *   NLA_CFG_OZ_VARIANT: global prefetch depth of the int8 residual's
+ *                      modular products (0, default: two k steps; 1: one);
+ *                      results are identical, only the speed differs.
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {
@@ -427,7 +430,8 @@ enum nla_config_key {
     NLA_CFG_PANEL_WAVES = 7,
     NLA_CFG_LU_SPLIT_ROWS = 8,
     NLA_CFG_RESIDUAL = 9,
-    NLA_CFG_LU_SPIN_LIMIT = 10
+    NLA_CFG_LU_SPIN_LIMIT = 10,
+    NLA_CFG_OZ_VARIANT = 11
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);

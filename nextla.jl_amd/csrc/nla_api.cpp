@@ -131,6 +131,8 @@ int batch_class_mask() { return g_batch_classes.load(); }
 // Residual of the refined solve: 1 = int8 modular products (Ozaki scheme II)
 // where the shape allows, 0 = FP64 MFMA GEMM.
 static std::atomic<int> g_residual{1};
+static std::atomic<int> g_oz_variant{0};
+int oz_variant() { return g_oz_variant.load(); }
 // Smallest shape given the int8 residual: below it the FP64 product is a
 // small part of the solve and the residue conversions cost as much.
 constexpr int64_t OZ_MIN_N = 2048, OZ_MIN_M = 64;
@@ -1164,6 +1166,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_LU_SPIN_LIMIT) {
         if (value < 0) return -1;
         return g_lu_spin_limit.exchange(value);
+    }
+    if (key == NLA_CFG_OZ_VARIANT) {
+        if (value < 0 || value > 1) return -1;
+        return g_oz_variant.exchange(value);
     }
     if (key == NLA_CFG_RESIDUAL) {
         if (value != NLA_RES_FP64 && value != NLA_RES_INT8) return -1;

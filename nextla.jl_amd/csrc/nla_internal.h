@@ -302,6 +302,10 @@ struct OzConst {
     double smax;  // largest |S_int| the moduli cover (M / 8)
 };
 
+// Global prefetch depth of the modular-product kernel: 0 = two k steps
+// (default), 1 = one.
+int oz_variant();
+
 // Moduli count and bit split for an order-n residual, and the residue planes'
 // sizes (256-padded orders npad, mpad).  T = 0: n outside (0, OZ_N_MAX].
 struct OzPlan {

@@ -247,10 +247,24 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, 
 // items from *queue (zero at launch) in modulus-major order, longest row
 // panels first, all column tiles of a row panel together (they share the A
 // panel in L2).  Eight waves, each a 64 x 128 block of the tile as 4 x 8
-// v_mfma_i32_16x16x64_i8 blocks; operands staged through double-buffered
-// LDS with rows padded to 80 bytes (the 16 rows a 16-lane group reads at one
-// k offset fall in distinct banks).  OUT_T: the output plane is (j, i) with
-// j contiguous (side R) instead of (i, j) with i contiguous.
+// v_mfma_i32_16x16x64_i8 blocks, from operands staged through double-buffered
+// LDS 64 k-bytes at a time.
+//
+// LDS rows are 96 bytes (64 of data): gfx950 serves a 16-byte-per-lane LDS
+// read in four phases of 16 lanes (lanes 0-7 with 16-23, 8-15 with 24-31,
+// ...) over 64 banks of 4 bytes (a conflict-free pattern measured at 256
+// bytes per clock, a 2-way one at half that); a fragment read gives lane l
+// row l % 16, chunk l / 16, and with 96-byte rows the 16 lanes of a phase
+// fall in 16 distinct 16-byte bank groups (80-byte rows conflict 2-way).
+//
+// PD: steps of global prefetch held in registers.  The step count is a
+// multiple of 4 and the loop body has no branches (loads past the last step
+// re-read it and are never stored), so each step waits only for the loads
+// issued PD - 1 steps before it; with conditional loads the compiler waited
+// for every outstanding load at the top of each step.
+//
+// OUT_T: the output plane is (j, i) with j contiguous (side R) instead of
+// (i, j) with i contiguous.
 struct OzGemmArgs {
     const int8_t *A;
     int64_t aplane;
@@ -263,16 +277,19 @@ struct OzGemmArgs {
     int *queue;
 };
 
-template <bool OUT_T>
+template <bool OUT_T, int PD>
 __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
-    constexpr int BM = 256, BN = 256, BK = 64, RS = BK + 16;
-    __shared__ __attribute__((aligned(16))) int8_t sA[2 * BM * RS];
-    __shared__ __attribute__((aligned(16))) int8_t sX[2 * BN * RS];
+    constexpr int BM = 256, BN = 256, BK = 64, RS = 96, TILE = BM * RS;
+    __shared__ __attribute__((aligned(16))) int8_t sA[2 * TILE];
+    __shared__ __attribute__((aligned(16))) int8_t sX[2 * TILE];
     __shared__ int s_item;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 3, wn = wave >> 2;
     const int per_mod = a.tiles_m * a.tiles_n, total = per_mod * a.T;
-    const int fr = lane & 15, fk = (lane >> 4) * 16;
+    const int fr = lane & 15, fc = lane >> 4;
+    // this thread's chunks c = tid + 512 q of each 256 x 64 operand tile:
+    // row c / 4, 16-byte column c % 4 (a row's four chunks in adjacent lanes)
+    const int srow0 = tid >> 2, sch = tid & 3;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -292,8 +309,9 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
             off = (int64_t)BM * ((int64_t)tm * a.npad - (int64_t)BM * ((int64_t)tm * (tm - 1) / 2));
             kbeg = (int64_t)tm * BM;
         }
-        const int8_t *Ap = a.A + t * a.aplane + off;
-        const int8_t *Xp = a.X + t * a.xplane + (int64_t)tn * BN * a.npad + kbeg;
+        const int8_t *Ap = a.A + t * a.aplane + off + (int64_t)srow0 * L + sch * 16;
+        const int8_t *Xp = a.X + t * a.xplane + ((int64_t)tn * BN + srow0) * a.npad + kbeg + sch * 16;
+        const int64_t a_q = (int64_t)128 * L, x_q = (int64_t)128 * a.npad;  // rows + 128 for q = 1
         const int nkt = (int)(L / BK);
 
         v4i acc[4][8];
@@ -302,49 +320,51 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
 #pragma unroll
             for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = v4i{0, 0, 0, 0};
 
-        // thread tid moves chunks tid and tid + 512 of each 256 x 64 operand
-        // tile: row c >> 2, k offset (c & 3) * 16
-        v4i ra[2], rx[2];
-        auto gload = [&](int64_t k0) {
+        v4i ra[PD][2], rx[PD][2];
+        auto gload = [&](v4i(&xa)[2], v4i(&xx)[2], int64_t k0) {
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const int c = tid + 512 * q, row = c >> 2, kc = (c & 3) * 16;
-                ra[q] = *reinterpret_cast<const v4i *>(Ap + row * L + k0 + kc);
-                rx[q] = *reinterpret_cast<const v4i *>(Xp + row * a.npad + k0 + kc);
+                xa[q] = *reinterpret_cast<const v4i *>(Ap + q * a_q + k0);
+                xx[q] = *reinterpret_cast<const v4i *>(Xp + q * x_q + k0);
             }
         };
-        auto sstore = [&](int buf) {
+        auto sstore = [&](const v4i(&xa)[2], const v4i(&xx)[2], int buf) {
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const int c = tid + 512 * q, row = c >> 2, kc = (c & 3) * 16;
-                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sA + buf * BM * RS + row * RS + kc) = ra[q];
-                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sX + buf * BN * RS + row * RS + kc) = rx[q];
+                const int o = buf * TILE + (q * 128 + srow0) * RS + sch * 16;
+                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sA + o) = xa[q];
+                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sX + o) = xx[q];
             }
         };
-        gload(0);
-        sstore(0);
+#pragma unroll
+        for (int p = 0; p < PD; ++p) gload(ra[p], rx[p], (int64_t)(p < nkt ? p : nkt - 1) * BK);
+        sstore(ra[0], rx[0], 0);
         __syncthreads();
-        for (int kt = 0; kt < nkt; ++kt) {
-            const int cur = kt & 1;
-            const bool more = kt + 1 < nkt;
-            if (more) gload((int64_t)(kt + 1) * BK);
-            const NLA_LDS int8_t *la = (const NLA_LDS int8_t *)sA + cur * BM * RS + (wm * 64 + fr) * RS + fk;
-            const NLA_LDS int8_t *lx = (const NLA_LDS int8_t *)sX + cur * BN * RS + (wn * 128 + fr) * RS + fk;
-            v4i af[4], xf[8];
+        for (int kb = 0; kb < nkt; kb += PD) {
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) af[rb] = *reinterpret_cast<const NLA_LDS v4i *>(la + rb * 16 * RS);
+            for (int p = 0; p < PD; ++p) {
+                const int kt = kb + p;
+                const int cur = kt & 1;
+                const int kn = kt + PD < nkt ? kt + PD : nkt - 1;
+                gload(ra[p], rx[p], (int64_t)kn * BK);
+                const NLA_LDS int8_t *la = (const NLA_LDS int8_t *)sA + cur * TILE + (wm * 64 + fr) * RS + fc * 16;
+                const NLA_LDS int8_t *lx = (const NLA_LDS int8_t *)sX + cur * TILE + (wn * 128 + fr) * RS + fc * 16;
+                v4i af[4], xf[8];
 #pragma unroll
-            for (int cb = 0; cb < 8; ++cb) xf[cb] = *reinterpret_cast<const NLA_LDS v4i *>(lx + cb * 16 * RS);
-            __builtin_amdgcn_s_setprio(1);
+                for (int rb = 0; rb < 4; ++rb) af[rb] = *reinterpret_cast<const NLA_LDS v4i *>(la + rb * 16 * RS);
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+                for (int cb = 0; cb < 8; ++cb) xf[cb] = *reinterpret_cast<const NLA_LDS v4i *>(lx + cb * 16 * RS);
+                __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-                for (int cb = 0; cb < 8; ++cb)
-                    acc[rb][cb] = OUT_T ? __builtin_amdgcn_mfma_i32_16x16x64_i8(xf[cb], af[rb], acc[rb][cb], 0, 0, 0)
-                                        : __builtin_amdgcn_mfma_i32_16x16x64_i8(af[rb], xf[cb], acc[rb][cb], 0, 0, 0);
-            __builtin_amdgcn_s_setprio(0);
-            if (more) sstore(cur ^ 1);
-            __syncthreads();
+                for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 8; ++cb)
+                        acc[rb][cb] = OUT_T ? __builtin_amdgcn_mfma_i32_16x16x64_i8(xf[cb], af[rb], acc[rb][cb], 0, 0, 0)
+                                            : __builtin_amdgcn_mfma_i32_16x16x64_i8(af[rb], xf[cb], acc[rb][cb], 0, 0, 0);
+                __builtin_amdgcn_s_setprio(0);
+                sstore(ra[(p + 1) % PD], rx[(p + 1) % PD], cur ^ 1);
+                __syncthreads();
+            }
         }
 
         // residues of the tile: lane holds 4 consecutive rows (columns for
@@ -362,9 +382,9 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
                 const int64_t i0 = (int64_t)tm * BM + wm * 64 + rb * 16;
                 const int64_t j0 = (int64_t)tn * BN + wn * 128 + cb * 16;
                 if (!OUT_T)
-                    *reinterpret_cast<unsigned *>(Cp + (j0 + fr) * a.ldc + i0 + (lane >> 4) * 4) = word;
+                    *reinterpret_cast<unsigned *>(Cp + (j0 + fr) * a.ldc + i0 + fc * 4) = word;
                 else
-                    *reinterpret_cast<unsigned *>(Cp + (i0 + fr) * a.ldc + j0 + (lane >> 4) * 4) = word;
+                    *reinterpret_cast<unsigned *>(Cp + (i0 + fr) * a.ldc + j0 + fc * 4) = word;
             }
     }
 }
@@ -566,16 +586,26 @@ hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double
     return hipGetLastError();
 }
 
+// Variant of the modular-product kernel: (BK, PD) = (64, 1), (64, 2),
+// (128, 1); set with nla_set_config(NLA_CFG_OZ_VARIANT).
+template <typename K>
+static hipError_t oz_launch_persistent(K kern, const OzGemmArgs &g, const OzConst &C, int items, hipStream_t s,
+                                       int threads = 512) {  // grid: CUs x occupancy, at most the item count
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void *)kern, threads, 0) != hipSuccess || o < 1) o = 1;
+    int dev = 0, cus = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err == hipSuccess) err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (err != hipSuccess) return err;
+    const int grid = cus * o < items ? cus * o : items;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, s, g, C);
+    return hipGetLastError();
+}
+
 template <bool OUT_T>
-static int oz_gemm_occupancy() {
-    static const int occ = [] {
-        int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, (const void *)oz_gemm_kernel<OUT_T>, 512, 0) !=
-            hipSuccess)
-            o = 1;
-        return o > 0 ? o : 1;
-    }();
-    return occ;
+static hipError_t oz_gemm_variant(const OzGemmArgs &g, const OzConst &C, int items, hipStream_t s) {
+    if (oz_variant() == 1) return oz_launch_persistent(oz_gemm_kernel<OUT_T, 1>, g, C, items, s);
+    return oz_launch_persistent(oz_gemm_kernel<OUT_T, 2>, g, C, items, s);
 }
 
 hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X, int64_t xso, int64_t xsk,
@@ -609,18 +639,8 @@ hipError_t oz_products(const OzPlan &P, const OzLayout &L, char *ws, bool lower,
     g.T = P.T;
     g.lower = lower ? 1 : 0;
     g.queue = (int *)(ws + L.queue);
-    int dev = 0, cus = 0;
-    err = hipGetDevice(&dev);
-    if (err == hipSuccess) err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (err != hipSuccess) return err;
     const int items = g.tiles_m * g.tiles_n * g.T;
-    const int occ = out_t ? oz_gemm_occupancy<true>() : oz_gemm_occupancy<false>();
-    const int grid = cus * occ < items ? cus * occ : items;
-    if (out_t)
-        hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3(grid), dim3(512), 0, s, g, P.C);
-    else
-        hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3(grid), dim3(512), 0, s, g, P.C);
-    return hipGetLastError();
+    return out_t ? oz_gemm_variant<true>(g, P.C, items, s) : oz_gemm_variant<false>(g, P.C, items, s);
 }
 
 hipError_t oz_reconstruct(const OzPlan &P, const OzLayout &L, char *ws, bool out_t, int64_t n, int64_t m,

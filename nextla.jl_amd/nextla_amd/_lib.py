@@ -95,6 +95,7 @@ NLA_CFG_PANEL_WAVES = 7
 NLA_CFG_LU_SPLIT_ROWS = 8
 NLA_CFG_RESIDUAL = 9
 NLA_CFG_LU_SPIN_LIMIT = 10
+NLA_CFG_OZ_VARIANT = 11
 NLA_RES_FP64 = 0
 NLA_RES_INT8 = 1
 
