@@ -66,6 +66,24 @@ int32_t nla_rectrxm(char side, char uplo, char trans, char func, int32_t dtype, 
                     int64_t m, const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
 
 /*
+ * nla_rectrxm without allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * result and argument checks as nla_rectrxm.  The TRSM's scratch (the
+ * inverses of A's diagonal 64-blocks and a flag per block, at most
+ * NLA_NOALLOC_SCRATCH_BYTES: enough for FP64 n up to 32768, FP32 up to 65536) lives in a
+ * device array of the library that each device holds until the process
+ * ends; a larger n returns NLA_ERR_WORKSPACE before any device work (B
+ * untouched).  The call makes no hipMalloc / hipFree.  Calls that overlap on
+ * one device (two streams or host threads at once) share that array and
+ * race; serialize them.
+ */
+#define NLA_NOALLOC_SCRATCH_BYTES ((int64_t)16 * 1024 * 1024 + 64 * 1024)
+
+int32_t nla_rectrxm_noalloc(char side, char uplo, char trans, char func, int32_t dtype, double alpha, int64_t n,
+                            int64_t m, const void *A, int64_t lda, void *B, int64_t ldb, void *stream);
+
+/*
  * GEMM_ADD!(X, Y, C): C += X·Y, X is N x R, Y is R x M, C is N x M.
  * Replaces src/matmul.jl:69-73 (kernel matmul!, src/matmul.jl:5-66).
  */

@@ -40,6 +40,8 @@ SIGNATURES = {
     "nla_abi_version": (_i32, []),
     "nla_last_error": (ctypes.c_char_p, []),
     "nla_rectrxm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "nla_rectrxm_noalloc": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64,
+                                   _vp]),
     "nla_trsm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_trmm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_gemm_add": (_i32, [_i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
@@ -84,6 +86,8 @@ NLA_LP_FP16 = 1
 NLA_LP_FP32 = 2
 NLA_LP_AUTO = 3
 NLA_ERR_NOT_CONVERGED = 5
+NLA_ERR_WORKSPACE = 4
+NLA_NOALLOC_SCRATCH_BYTES = 16 * 1024 * 1024 + 64 * 1024
 NLA_CFG_LEAF_MODE = 0
 NLA_CFG_PANEL = 1
 NLA_CFG_PANEL_ROWS_F32 = 2
