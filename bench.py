@@ -569,6 +569,38 @@ def serve_leg(nla, torch, count=600, seed=0):
     return out
 
 
+def profile_leg(nla, torch, what, reps=3):
+    """Short runs for rocprofv3: the int8 residual of the refined solve's
+    workload (n = 16384, m = 4096; what = "residual_int8"), or the plain LL-N
+    TRSM at n = m = 8192 (what = "trsm_f64" / "trsm_f32"); each followed by
+    one 512 MiB device copy, whose FETCH_SIZE / WRITE_SIZE calibrate the
+    counters."""
+    from nextla_amd import inputs
+    out = {"what": what, "reps": reps}
+    if what == "residual_int8":
+        from nextla_amd.mixed import residual, residual_workspace_bytes
+        n, m = 16384, 4096
+        A, B = inputs.device_precision_tri(n, m, 1e4, seed=5)
+        X = B.clone()
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, X)
+        R = torch.empty_like(B)
+        ws = torch.empty(residual_workspace_bytes("L", "int8", n, m) + 256, dtype=torch.uint8, device=A.device)
+        ts, _ = timed_steps(torch, lambda: None,
+                            lambda: residual("L", "L", "N", 1.0, A, X, B, R, method="int8", workspace=ws), reps, 1)
+        out["ms"] = ts
+        del A, B, X, R, ws
+    else:
+        dt = torch.float64 if what == "trsm_f64" else torch.float32
+        _, _, _, ts, _ = solve_rate(nla, torch, 8192, 8192, "L", "L", "N", "S", 1.5, dt, reps, 1)
+        out["ms"] = ts
+    a = torch.empty(64 << 20, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    out["copy_bytes"] = a.numel() * 8
+    return out
+
+
 def cpu_baseline(A_host, n, m_total, alpha, seconds_target=12.0):
     """The oracle on host cores over a bounded number of RHS columns: a short
     calibration run at 128 columns, then a sample sized for ~seconds_target."""
@@ -614,7 +646,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip every leg but the headline")
     ap.add_argument("--leg", choices=("refined_mixed", "drop_in", "cpu_case", "batched_f64", "batched_f32", "served",
-                                      "lu", "lauum", "larfb"),
+                                      "lu", "lauum", "larfb", "residual_int8", "trsm_f64", "trsm_f32"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -641,6 +673,8 @@ def main():
             res = lauum_leg(nla, torch)
         elif args.leg == "larfb":
             res = larfb_leg(nla, torch)
+        elif args.leg in ("residual_int8", "trsm_f64", "trsm_f32"):
+            res = profile_leg(nla, torch, args.leg)
         else:
             res = serve_leg(nla, torch)
         print(json.dumps({"leg": args.leg, "result": res}), flush=True)
