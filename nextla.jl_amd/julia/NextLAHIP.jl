@@ -22,7 +22,9 @@ TRSM by low-precision correction solves and FP64 refinement) and
 NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`, and
 `getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM),
 `lauum!` (U·U^T / L^T·L on the TRMM) and `larfb!` (tile QR's block reflector,
-its triangular multiplies on the TRMM).
+its triangular multiplies on the TRMM), `residual!` (the FP64-accurate
+triangular residual on the int8 MFMA) and `unified_rectrxm_noalloc!` (the
+drop-in without device allocation).
 
 Differences from the reference, by design: arguments are validated before any
 device work (`ArgumentError`, B untouched), lower-case characters are
@@ -235,14 +237,16 @@ function batched_rectrxm!(side::Char, uplo::Char, transpose::Char, func::Char, A
 end
 
 """
-    rectrxm_mixed!(side, uplo, transpose, alpha, A, B; precision=:fp16, tol=1e-14, max_iter=20)
+    rectrxm_mixed!(side, uplo, transpose, alpha, A, B; precision=:auto, tol=1e-14, max_iter=20)
 
-FP64-accurate TRSM by bf16 / fp16 / fp32 correction solves and FP64 iterative
-refinement (nla_rectrxm_mixed).  Returns (B, steps, backward_error).
+FP64-accurate TRSM by bf16 / fp16 / fp32 correction solves (`:auto` picks each
+correction's precision by predicted cost) and iterative refinement with
+FP64-accurate residuals (int8 modular products or FP64; nla_rectrxm_mixed).
+Returns (B, steps, per-system backward error).
 """
 function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, A::DeviceMat{Float64},
-                        B::DeviceMat{Float64}; precision::Symbol=:fp16, tol::Real=1e-14, max_iter::Integer=20)
-    prec = Dict(:bf16 => Int32(0), :fp16 => Int32(1), :fp32 => Int32(2))[precision]
+                        B::DeviceMat{Float64}; precision::Symbol=:auto, tol::Real=1e-14, max_iter::Integer=20)
+    prec = Dict(:bf16 => Int32(0), :fp16 => Int32(1), :fp32 => Int32(2), :auto => Int32(3))[precision]
     n = size(A, 1)
     m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
     pa, lda = ptr_ld(A)
@@ -250,8 +254,8 @@ function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, 
     bytes = ccall((:nla_mixed_workspace_bytes, libnlahip), Int64, (Cchar, Int64, Int64), Cchar(side), n, m)
     buf, ws = workspace(bytes)
     # nla_mixed_info: int32 iterations, int32 precision, double backward_error, double history[31],
-    # double backward_error_frobenius
-    info = zeros(UInt8, 4 + 4 + 8 + 8 * 31 + 8)
+    # double backward_error_frobenius, int32 residual (+4 bytes of padding to the struct's 8-byte alignment)
+    info = zeros(UInt8, 4 + 4 + 8 + 8 * 31 + 8 + 8)
     st = GC.@preserve A B buf info ccall((:nla_rectrxm_mixed, libnlahip), Int32,
         (Cchar, Cchar, Cchar, Int32, Float64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Float64, Int32,
          Ptr{Cvoid}, Int64, Ptr{Cvoid}, Ptr{UInt8}),
@@ -261,6 +265,59 @@ function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, 
     steps = reinterpret(Int32, info[1:4])[1]
     bwd = reinterpret(Float64, info[9:16])[1]
     return B, steps, bwd
+end
+
+"""
+    residual!(side, uplo, transpose, alpha, A, X, B, R; method=:int8)
+
+R = alpha*B - op(A)*X (side 'L') or alpha*B - X*op(A) (side 'R') to FP64
+accuracy (nla_residual).  `:int8`: Ozaki scheme II on the int8 MFMA; X is
+overwritten by its value rounded to >= 55 bits of each system's largest entry,
+and R is the residual of that X.  `:fp64`: the FP64 MFMA product.  Returns R.
+"""
+function residual!(side::Char, uplo::Char, transpose::Char, alpha::Number, A::DeviceMat{Float64},
+                   X::DeviceMat{Float64}, B::DeviceMat{Float64}, R::DeviceMat{Float64}; method::Symbol=:int8)
+    meth = Dict(:fp64 => Int32(0), :int8 => Int32(1))[method]
+    n = size(A, 1)
+    m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
+    pa, lda = ptr_ld(A)
+    px, ldx = ptr_ld(X)
+    pb, ldb = ptr_ld(B)
+    pr, ldr = ptr_ld(R)
+    bytes = ccall((:nla_residual_workspace_bytes, libnlahip), Int64, (Cchar, Int32, Int64, Int64),
+                  Cchar(side), meth, n, m)
+    bytes >= 0 || throw(ArgumentError("residual!: invalid shape"))
+    buf, ws = workspace(bytes)
+    st = GC.@preserve A X B R buf ccall((:nla_residual, libnlahip), Int32,
+        (Cchar, Cchar, Cchar, Int32, Float64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64,
+         Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+        Cchar(side), Cchar(uplo), Cchar(transpose), meth, Float64(alpha), Int64(n), Int64(m), pa, lda, px, ldx,
+        pb, ldb, pr, ldr, ws, bytes, hip_stream())
+    check(st, "residual!")
+    return R
+end
+
+"""
+    unified_rectrxm_noalloc!(side, uplo, transpose, alpha, func, A, B)
+
+`unified_rectrxm!` through nla_rectrxm_noalloc: no device allocation (the
+TRSM's scratch is a static device array of the library; FP64 n <= 32768,
+FP32 n <= 65536), for callers that cannot allocate.  Calls that overlap on one
+device race on that array.
+"""
+function unified_rectrxm_noalloc!(side::Char, uplo::Char, transpose::Char, alpha::Number, func::Char,
+                                  A::DeviceMat{T}, B::DeviceMat{T}) where {T <: Union{Float32, Float64}}
+    n = size(A, 1)
+    size(A, 2) == n || throw(DimensionMismatch("A must be square"))
+    m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
+    pa, lda = ptr_ld(A)
+    pb, ldb = ptr_ld(B)
+    st = GC.@preserve A B ccall((:nla_rectrxm_noalloc, libnlahip), Int32,
+        (Cchar, Cchar, Cchar, Cchar, Int32, Float64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+        Cchar(side), Cchar(uplo), Cchar(transpose), Cchar(func), dtype_code(T), Float64(alpha), Int64(n), Int64(m),
+        pa, lda, pb, ldb, hip_stream())
+    check(st, "unified_rectrxm_noalloc!")
+    return B
 end
 
 """
