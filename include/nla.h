@@ -432,9 +432,13 @@ int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
  *                      persistent grid) is redone from a copy saved before
  *                      it by the one-work-group leaf, so nla_getrf2's result
  *                      does not depend on it; a small value forces that path.
- *   NLA_CFG_OZ_VARIANT: global prefetch depth of the int8 residual's
- *                      modular products (0, default: two k steps; 1: one);
- *                      results are identical, only the speed differs.
+ *   NLA_CFG_OZ_VARIANT: schedule of the int8 residual's modular products
+ *                      (0, default: two k steps of global prefetch, per-XCD
+ *                      counters over super-tiles of 4 x 8 tiles; 1: one step
+ *                      of prefetch, tiles in row order from one counter; 2:
+ *                      two steps, row order; 3: two steps, 8 x 4
+ *                      super-tiles); results are identical, only the speed
+ *                      differs.
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {

@@ -1168,7 +1168,7 @@ int32_t nla_set_config(int32_t key, int32_t value) {
         return g_lu_spin_limit.exchange(value);
     }
     if (key == NLA_CFG_OZ_VARIANT) {
-        if (value < 0 || value > 1) return -1;
+        if (value < 0 || value > 3) return -1;
         return g_oz_variant.exchange(value);
     }
     if (key == NLA_CFG_RESIDUAL) {

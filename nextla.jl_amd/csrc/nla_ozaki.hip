@@ -265,6 +265,15 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, 
 //
 // OUT_T: the output plane is (j, i) with j contiguous (side R) instead of
 // (i, j) with i contiguous.
+//
+// Tile order.  sr == 0: one counter; tiles modulus by modulus, the longest
+// rows of the triangle first, all column tiles of a row together.  sr > 0:
+// one counter per XCD (work-group g runs on XCD g % 8) and tiles in
+// super-tiles of sr row tiles x sc column tiles of one modulus; an XCD's 32
+// CUs take the tiles of one super-tile together, so its L2 fetches sr A
+// panels and sc X panels for sr * sc tiles.  Super-tiles, longest first, go
+// to the XCDs in rounds of 8, in reverse order on odd rounds so that every
+// XCD gets rows of every length.
 struct OzGemmArgs {
     const int8_t *A;
     int64_t aplane;
@@ -274,7 +283,8 @@ struct OzGemmArgs {
     int64_t cplane, ldc;
     int64_t npad, mpad;
     int tiles_m, tiles_n, T, lower;
-    int *queue;
+    int sr, sc;  // super-tile shape (sr == 0: one counter, row order)
+    int *queue;  // 8 counters
 };
 
 template <bool OUT_T, int PD>
@@ -287,18 +297,36 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
     const int wm = wave & 3, wn = wave >> 2;
     const int per_mod = a.tiles_m * a.tiles_n, total = per_mod * a.T;
     const int fr = lane & 15, fc = lane >> 4;
+    const int xcd = blockIdx.x & 7;
+    const int nbands = a.sr > 0 ? (a.tiles_m + a.sr - 1) / a.sr : 0;
+    const int ngroups = a.sr > 0 ? (a.tiles_n + a.sc - 1) / a.sc : 0;
+    const int sper = a.sr * a.sc, ssup = nbands * ngroups * a.T;
     // this thread's chunks c = tid + 512 q of each 256 x 64 operand tile:
     // row c / 4, 16-byte column c % 4 (a row's four chunks in adjacent lanes)
     const int srow0 = tid >> 2, sch = tid & 3;
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(a.queue, 1);
+        if (tid == 0) s_item = atomicAdd(a.queue + (a.sr > 0 ? xcd : 0), 1);
         __syncthreads();
         const int w = s_item;
         __syncthreads();
-        if (w >= total) break;
-        const int t = w / per_mod, rest = w - t * per_mod;
-        const int tm = a.lower ? a.tiles_m - 1 - rest / a.tiles_n : rest / a.tiles_n;
-        const int tn = rest % a.tiles_n;
+        int t, tr, tn;  // modulus, row tile counted from the longest, column tile
+        if (a.sr == 0) {
+            if (w >= total) break;
+            t = w / per_mod;
+            const int rest = w - t * per_mod;
+            tr = rest / a.tiles_n;
+            tn = rest % a.tiles_n;
+        } else {
+            const int k = w / sper, within = w - k * sper;
+            const int sup = 8 * k + ((k & 1) ? 7 - xcd : xcd);
+            if (sup >= ssup) break;
+            t = sup / (nbands * ngroups);
+            const int rest = sup - t * nbands * ngroups;
+            tr = (rest / ngroups) * a.sr + within / a.sc;
+            tn = (rest % ngroups) * a.sc + within % a.sc;
+            if (tr >= a.tiles_m || tn >= a.tiles_n) continue;  // past the edge of a partial super-tile
+        }
+        const int tm = a.lower ? a.tiles_m - 1 - tr : tr;
         int64_t L, off, kbeg;
         if (a.lower) {
             L = (int64_t)(tm + 1) * BM;
@@ -547,7 +575,7 @@ OzLayout oz_layout(const OzPlan &P, int64_t base) {
     L.mxx = o;   o = al(o + P.mpad * 8);
     L.pw = o;    o = al(o + OZ_TMAX * OZ_PW);
     L.flag = o;  o = al(o + 4);
-    L.queue = o; o = al(o + 4);
+    L.queue = o; o = al(o + 8 * 4);
     L.total = o;
     return L;
 }
@@ -586,8 +614,10 @@ hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double
     return hipGetLastError();
 }
 
-// Variant of the modular-product kernel: (BK, PD) = (64, 1), (64, 2),
-// (128, 1); set with nla_set_config(NLA_CFG_OZ_VARIANT).
+// Schedules of the modular-product kernel (nla_set_config(NLA_CFG_OZ_VARIANT)):
+// 0 two-step prefetch, 4 x 8 super-tiles per XCD; 1 one-step prefetch, row
+// order; 2 two-step prefetch, row order; 3 two-step prefetch, 8 x 4
+// super-tiles.
 template <typename K>
 static hipError_t oz_launch_persistent(K kern, const OzGemmArgs &g, const OzConst &C, int items, hipStream_t s,
                                        int threads = 512) {  // grid: CUs x occupancy, at most the item count
@@ -598,13 +628,27 @@ static hipError_t oz_launch_persistent(K kern, const OzGemmArgs &g, const OzCons
     if (err == hipSuccess) err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (err != hipSuccess) return err;
     const int grid = cus * o < items ? cus * o : items;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, s, g, C);
+    OzGemmArgs h = g;
+    if (grid < 8) h.sr = h.sc = 0;  // per-XCD counters need a work-group on each of the 8
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, s, h, C);
     return hipGetLastError();
 }
 
 template <bool OUT_T>
-static hipError_t oz_gemm_variant(const OzGemmArgs &g, const OzConst &C, int items, hipStream_t s) {
-    if (oz_variant() == 1) return oz_launch_persistent(oz_gemm_kernel<OUT_T, 1>, g, C, items, s);
+static hipError_t oz_gemm_variant(OzGemmArgs g, const OzConst &C, int items, hipStream_t s) {
+    // 0: per-XCD counters over 4 x 8 super-tiles (9.3 ms per 17-modulus
+    // launch at n = 16384, m = 4096, against 9.7 ms in row order)
+    const int v = oz_variant();
+    g.sr = 0;
+    g.sc = 0;
+    if (v == 0) {
+        g.sr = 4;
+        g.sc = 8;
+    } else if (v == 3) {
+        g.sr = 8;
+        g.sc = 4;
+    }
+    if (v == 1) return oz_launch_persistent(oz_gemm_kernel<OUT_T, 1>, g, C, items, s);
     return oz_launch_persistent(oz_gemm_kernel<OUT_T, 2>, g, C, items, s);
 }
 
@@ -622,7 +666,7 @@ hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X,
 }
 
 hipError_t oz_products(const OzPlan &P, const OzLayout &L, char *ws, bool lower, bool out_t, hipStream_t s) {
-    hipError_t err = oz_fill(ws + L.queue, 1, s);
+    hipError_t err = oz_fill(ws + L.queue, 8, s);
     if (err != hipSuccess) return err;
     OzGemmArgs g{};
     g.A = (const int8_t *)(ws + L.a);

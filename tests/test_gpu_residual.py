@@ -209,3 +209,32 @@ def test_refined_solve_int8_and_fp64_residuals(nla, side, uplo, trans):
         rel = (torch.linalg.norm(X - P) / torch.linalg.norm(P)).item()
         assert rel <= 1e-13, (info["residual"], rel)
     assert set(out) == {"int8", "fp64"}
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("side,uplo,trans,n,m", [("L", "L", "N", 2304, 600), ("R", "U", "T", 2500, 300),
+                                                 ("L", "U", "N", 3000, 2100)])
+def test_product_schedules_identical(nla, variant, side, uplo, trans, n, m):
+    """Every schedule of the modular products (prefetch depth, one counter in
+    row order or per-XCD counters over super-tiles, partial super-tiles at
+    the edges) gives the same residual and the same rounded X, bit for bit."""
+    from nextla_amd import _lib
+    from nextla_amd.mixed import residual
+    A = gpu_tri(n, uplo, 21)
+    shp = (n, m) if side == "L" else (m, n)
+    g = torch.Generator(device=DEV)
+    g.manual_seed(22)
+    B = torch.empty((shp[1], shp[0]), dtype=torch.float64, device=DEV).t().normal_(generator=g)
+    X = B.clone().mul_(0.5).add_(torch.empty_like(B).normal_(generator=g), alpha=1e-6)
+    out = []
+    for v in (0, variant):
+        old = _lib.set_config(_lib.NLA_CFG_OZ_VARIANT, v)
+        try:
+            Xv, Rv = X.clone(), torch.empty_like(B)
+            residual(side, uplo, trans, 1.0, A, Xv, B, Rv, method="int8")
+            torch.cuda.synchronize()
+        finally:
+            _lib.set_config(_lib.NLA_CFG_OZ_VARIANT, old)
+        out.append((Xv, Rv))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
