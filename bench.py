@@ -503,6 +503,58 @@ def larfb_leg(nla, torch, shapes=((8192, 8192, 64), (8192, 8192, 256)), seed=1):
     return out
 
 
+def parfb_leg(nla, torch, shapes=((8192, 8192, 256, 0), (8192, 8192, 256, 256)), seed=2, check_cols=16):
+    """parfb! (tile QR's triangular-pentagonal block reflector, the step of
+    tsmqr! (l = 0) and ttmqr! (l = k)): [A1; A2] := H^T [A1; A2] from the left,
+    forward columnwise, FP64; 4*m2*n2*k flops over the device time.  Its
+    triangular multiplies are this library's TRMM, its products the GEMM
+    update.  The result is checked on `check_cols` sampled columns (columns
+    transform independently) against the closed form c - W T^T W^T c with
+    W = [I; V pentagonal], formed with torch in FP64."""
+    import numpy as np
+    out = {"workload": "[A1; A2] := H^T [A1; A2], A1 k x n, A2 m x n, V m x k pentagonal (l rows triangular), "
+                       "T k x k upper, FP64; V entries N(0,1)/sqrt(m), T N(0,1)/sqrt(k), A N(0,1)"}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    for m, n, k, l in shapes:
+        V = torch.empty((k, m), dtype=torch.float64, device="cuda").t().normal_(generator=g).mul_(m ** -0.5)
+        T = torch.empty((k, k), dtype=torch.float64, device="cuda").t().normal_(generator=g).mul_(k ** -0.5)
+        A10 = torch.empty((n, k), dtype=torch.float64, device="cuda").t().normal_(generator=g)
+        A20 = torch.empty((n, m), dtype=torch.float64, device="cuda").t().normal_(generator=g)
+        A1, A2 = A10.clone(), A20.clone()
+        work = torch.empty((n, k), dtype=torch.float64, device="cuda").t()
+
+        def call():
+            nla.parfb_("L", "T", "F", "C", A1, A2, V, T, work=work, k=k, l=l)
+        call()
+        torch.cuda.synchronize()
+        cols = torch.from_numpy(np.random.default_rng(seed).choice(n, size=check_cols, replace=False)).cuda()
+        Vp = V.clone()
+        if l > 0:  # V's last l rows are upper trapezoidal
+            Vp[m - l:, :l] = torch.triu(Vp[m - l:, :l])
+        Wm = torch.cat([torch.eye(k, dtype=torch.float64, device="cuda"), Vp], 0)
+        c = torch.cat([A10[:, cols], A20[:, cols]], 0)
+        e = c - Wm @ (torch.triu(T).t() @ (Wm.t() @ c))
+        d1 = ((A1[:, cols] - e[:k]).norm() / e[:k].norm()).item()
+        d2 = ((A2[:, cols] - e[k:]).norm() / e[k:].norm()).item()
+        ts = []
+        for _ in range(5):
+            A1.copy_(A10)
+            A2.copy_(A20)
+            t, _ = event_ms(torch, call, 1)
+            ts.append(t)
+        ms = min(ts)
+        fl = 4.0 * m * n * k
+        out[f"m{m}_n{n}_k{k}_l{l}"] = {"ms": round(ms, 3), "tflops": round(fl / ms / 1e9, 3),
+                                       "roofline_frac": round(fl / ms / 1e9 / PEAK["f64"], 4),
+                                       "rel_error_sampled_columns": float(max(d1, d2))}
+        log(f"bench: parfb m={m} n={n} k={k} l={l}: {ms:.3f} ms ({fl / ms / 1e9:.1f} TFLOP/s), "
+            f"sampled-column error {max(d1, d2):.2e}")
+        del V, T, A10, A20, A1, A2, work
+    torch.cuda.empty_cache()
+    return out
+
+
 def serve_leg(nla, torch, count=600, seed=0):
     """The serving workload through nextla_amd.serve (see the module)."""
     import numpy as np
@@ -646,7 +698,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip every leg but the headline")
     ap.add_argument("--leg", choices=("refined_mixed", "drop_in", "cpu_case", "batched_f64", "batched_f32", "served",
-                                      "lu", "lauum", "larfb", "residual_int8", "trsm_f64", "trsm_f32"),
+                                      "lu", "lauum", "larfb", "parfb", "residual_int8", "trsm_f64", "trsm_f32"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -673,6 +725,8 @@ def main():
             res = lauum_leg(nla, torch)
         elif args.leg == "larfb":
             res = larfb_leg(nla, torch)
+        elif args.leg == "parfb":
+            res = parfb_leg(nla, torch)
         elif args.leg in ("residual_int8", "trsm_f64", "trsm_f32"):
             res = profile_leg(nla, torch, args.leg)
         else:
@@ -725,6 +779,7 @@ def main():
         extra["lu"] = lu_leg(nla, torch)
         extra["lauum"] = lauum_leg(nla, torch)
         extra["larfb"] = larfb_leg(nla, torch)
+        extra["parfb"] = parfb_leg(nla, torch)
     torch.cuda.synchronize()
 
     line = {"metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,

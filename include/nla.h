@@ -252,6 +252,52 @@ int32_t nla_larfb(char side, char trans, char direct, char storev, int32_t dtype
                   int64_t ldwork, void *stream);
 
 /*
+ * Triangular-pentagonal block reflectors of tile QR (tsmqr!/ttmqr!):
+ * H = I - W·T·W^T with W = [I; V] (columnwise, forward; [V; I] backward) or
+ * its rowwise transpose, V pentagonal: a rectangular block V1 and an l-row
+ * (columnwise) or l-column (rowwise) block V2 whose l x l part is triangular
+ * (upper for columnwise forward and rowwise backward, lower otherwise); real
+ * types only.
+ *
+ * nla_pamm replaces pamm!(op, side, storev, direct, m, n, k, l, A1, A2, V, W),
+ * src/pamm.jl:50-466:
+ *   op 'W': W = A1 + op(V)·A2 (side 'L': W, A1 k x n, A2 m x n, V m x k
+ *           columnwise or k x m rowwise) or W = A1 + A2·op(V) (side 'R':
+ *           W, A1 m x k, A2 m x n, V n x k or k x n);
+ *   op 'A': A2 -= op(V)·W or A2 -= W·op(V); W's l triangle rows (columns)
+ *           are overwritten.  A1 is not read (may be NULL).
+ * Every product with V2's triangle is this library's TRMM on that l x l
+ * block in place; the rest are GEMM updates.  l <= k and l <= m (side 'L')
+ * or n ('R'); m, n or k zero returns at once.
+ * Argument positions: 1 op, 2 side, 3 storev, 4 direct, 5 dtype, 6 m, 7 n,
+ * 8 k, 9 l, 10 A1, 11 lda1, 12 A2, 13 lda2, 14 V, 15 ldv, 16 W, 17 ldw,
+ * 18 stream.
+ */
+int32_t nla_pamm(char op, char side, char storev, char direct, int32_t dtype, int64_t m, int64_t n, int64_t k,
+                 int64_t l, const void *A1, int64_t lda1, void *A2, int64_t lda2, const void *V, int64_t ldv, void *W,
+                 int64_t ldw, void *stream);
+
+/*
+ * nla_parfb replaces parfb!(side, trans, direct, storev, m1, n1, m2, n2, k,
+ * l, A1, A2, V, T, work), src/parfb.jl:46-202 (LAPACK's tprfb): applies H
+ * (trans 'N') or H^T ('T'/'C') to C = [A1; A2] (side 'L', direct 'F'; [A2; A1]
+ * for 'B'; A1's first k of its m1 >= k rows, A2 m2 x n2 with n1 = n2) or
+ * C = [A1 A2] (side 'R'; A1's first k of its n1 >= k columns, m1 = m2), in
+ * place.  T is k x k, upper
+ * (forward) or lower (backward), only that triangle read.  work: k x n2
+ * (side 'L') or m2 x k ('R') device block.  Steps: nla_pamm 'W', the TRMM
+ * by op(T), A1 -= work, nla_pamm 'A'.  An empty A1 or A2 returns at once
+ * (as LAPACK; the reference would then apply T to a work block it never
+ * wrote).
+ * Argument positions: 1 side, 2 trans, 3 direct, 4 storev, 5 dtype, 6 m1,
+ * 7 n1, 8 m2, 9 n2, 10 k, 11 l, 12 A1, 13 lda1, 14 A2, 15 lda2, 16 V, 17 ldv,
+ * 18 T, 19 ldt, 20 work, 21 ldwork, 22 stream.
+ */
+int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype, int64_t m1, int64_t n1, int64_t m2,
+                  int64_t n2, int64_t k, int64_t l, void *A1, int64_t lda1, void *A2, int64_t lda2, const void *V,
+                  int64_t ldv, const void *T, int64_t ldt, void *work, int64_t ldwork, void *stream);
+
+/*
  * Mixed-precision TRSM with FP64 iterative refinement (no counterpart in the
  * reference; the FP64 solution it converges to is that of unified_rectrxm!'s
  * TRSM, src/rectrxm.jl:43-76).  Solves

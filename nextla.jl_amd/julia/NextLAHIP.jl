@@ -21,8 +21,9 @@ TRSM by low-precision correction solves and FP64 refinement) and
 `batched_rectrxm_mixed!` (the same for many systems at once), and methods of
 NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`, and
 `getrf2!` (the recursive LU whose panel solve is the unit-lower TRSM),
-`lauum!` (U·U^T / L^T·L on the TRMM) and `larfb!` (tile QR's block reflector,
-its triangular multiplies on the TRMM), `residual!` (the FP64-accurate
+`lauum!` (U·U^T / L^T·L on the TRMM), `larfb!` (tile QR's block reflector,
+its triangular multiplies on the TRMM), `pamm!` / `parfb!` (tile QR's
+triangular-pentagonal reflectors, LAPACK's xTPRFB), `residual!` (the FP64-accurate
 triangular residual on the int8 MFMA) and `unified_rectrxm_noalloc!` (the
 drop-in without device allocation).
 
@@ -170,6 +171,36 @@ function NextLA.larfb!(side::Char, trans::Char, direct::Char, storev::Char, V::D
     check(st, "larfb!")
     AMDGPU.synchronize()  # work is released when this returns
     return C
+end
+
+# pamm!(op, side, storev, direct, m, n, k, l, A1, A2, V, W) (src/pamm.jl:50): the pentagonal
+# products of tile QR; V's l x l triangle through the device TRMM.  Returns W ('W') or A2 ('A').
+function NextLA.pamm!(op::Char, side::Char, storev::Char, direct::Char, m::Integer, n::Integer, k::Integer,
+                      l::Integer, A1::DeviceMat{T}, A2::DeviceMat{T}, V::DeviceMat{T},
+                      W::DeviceMat{T}) where {T <: Union{Float32, Float64}}
+    p1, l1 = ptr_ld(A1); p2, l2 = ptr_ld(A2); pv, ldv = ptr_ld(V); pw, ldw = ptr_ld(W)
+    st = GC.@preserve A1 A2 V W ccall((:nla_pamm, libnlahip), Int32,
+        (Cchar, Cchar, Cchar, Cchar, Int32, Int64, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64,
+         Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+        Cchar(op), Cchar(side), Cchar(storev), Cchar(direct), dtype_code(T), Int64(m), Int64(n), Int64(k), Int64(l),
+        p1, l1, p2, l2, pv, ldv, pw, ldw, hip_stream())
+    check(st, "pamm!")
+    return op in ('W', 'w') ? W : A2
+end
+
+# parfb!(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T, work) (src/parfb.jl:46):
+# LAPACK's xTPRFB on the device.
+function NextLA.parfb!(side::Char, trans::Char, direct::Char, storev::Char, m1::Integer, n1::Integer,
+                       m2::Integer, n2::Integer, k::Integer, l::Integer, A1::DeviceMat{T}, A2::DeviceMat{T},
+                       V::DeviceMat{T}, Tm::DeviceMat{T}, work::DeviceMat{T}) where {T <: Union{Float32, Float64}}
+    p1, l1 = ptr_ld(A1); p2, l2 = ptr_ld(A2); pv, ldv = ptr_ld(V); pt, ldt = ptr_ld(Tm); pw, ldw = ptr_ld(work)
+    st = GC.@preserve A1 A2 V Tm work ccall((:nla_parfb, libnlahip), Int32,
+        (Cchar, Cchar, Cchar, Cchar, Int32, Int64, Int64, Int64, Int64, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid},
+         Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+        Cchar(side), Cchar(trans), Cchar(direct), Cchar(storev), dtype_code(T), Int64(m1), Int64(n1), Int64(m2),
+        Int64(n2), Int64(k), Int64(l), p1, l1, p2, l2, pv, ldv, pt, ldt, pw, ldw, hip_stream())
+    check(st, "parfb!")
+    return A1, A2
 end
 
 # ---- forms without a reference counterpart ---------------------------------

@@ -35,7 +35,7 @@ __all__ = [
     "unified_rectrxm_", "GEMM_ADD_", "GEMM_SUB_",
     "LeftLowerTRSM_", "LeftUpperTRSM_", "RightLowerTRSM_", "RightUpperTRSM_",
     "LeftLowerTRMM_", "LeftUpperTRMM_", "RightLowerTRMM_", "RightUpperTRMM_",
-    "trsm", "trmm", "getrf2_", "lauum_", "larfb_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
+    "trsm", "trmm", "getrf2_", "lauum_", "larfb_", "pamm_", "parfb_", "NLAError", "load", "NLA_F32", "NLA_F64", "colmajor",
 ]
 
 _DT = {torch.float32: NLA_F32, torch.float64: NLA_F64}
@@ -310,3 +310,73 @@ def larfb_(side, trans, direct, storev, V, T, C, work=None):
                           max(ldw, wr), _stream(C))
     _lib.check(st, "larfb!")
     return C
+
+
+def _cm_ld(t, name):
+    """Leading dimension of a column-major (stride (1, ld)) matrix."""
+    tr, ld = _layout(t, name)
+    if tr and t.shape[0] > 1 and t.shape[1] > 1:
+        raise ValueError(f"{name} must be column-major (stride (1, ld)); use colmajor({name})")
+    return ld
+
+
+def _chars(*pairs):
+    for k_, (v, ok) in enumerate(pairs, 1):
+        if not isinstance(v, str) or len(v) != 1 or v not in ok:
+            raise ValueError(f"argument {k_} has invalid value {v!r}")
+
+
+def pamm_(op, side, storev, direct, m, n, k, l, A1, A2, V, W):
+    """pamm!(op, side, storev, direct, m, n, k, l, A1, A2, V, W) of
+    src/pamm.jl:50: op 'W': W = A1 + op(V)*A2 (side 'L') or A1 + A2*op(V)
+    ('R'); op 'A': A2 -= op(V)*W or A2 -= W*op(V), with V's pentagonal
+    structure (the l x l triangle through the device TRMM).  Column-major
+    float32/float64 tensors on one ROCm device; A1 is not read for op 'A'.
+    Returns W ('W') or A2 ('A')."""
+    _chars((op, "WAwa"), (side, "LRlr"), (storev, "CRcr"), (direct, "FBfb"))
+    for v, nm in ((m, "m"), (n, "n"), (k, "k"), (l, "l")):
+        if not isinstance(v, int) or v < 0:
+            raise ValueError(f"{nm} must be non-negative, got {v!r}")
+    op, side, storev, direct = op.upper(), side.upper(), storev.upper(), direct.upper()
+    ts = [A2, V, W] + ([A1] if op == "W" else [])
+    if A2.dtype not in _DT or any(t.dtype != A2.dtype for t in ts):
+        raise ValueError("A1, A2, V and W must share one element type, float32 or float64")
+    _require_device(*ts)
+    lda1 = _cm_ld(A1, "A1") if op == "W" else 1
+    st = load().nla_pamm(_lib.ch(op), _lib.ch(side), _lib.ch(storev), _lib.ch(direct), _DT[A2.dtype], m, n, k, l,
+                         A1.data_ptr() if op == "W" else None, lda1, A2.data_ptr(), _cm_ld(A2, "A2"), V.data_ptr(),
+                         _cm_ld(V, "V"), W.data_ptr(), _cm_ld(W, "W"), _stream(A2))
+    _lib.check(st, "pamm!")
+    return W if op == "W" else A2
+
+
+def parfb_(side, trans, direct, storev, A1, A2, V, T, work=None, k=None, l=None):
+    """parfb!(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T,
+    work) of src/parfb.jl:46 (LAPACK's tprfb): A1 and A2 := H*C, H^T*C, C*H
+    or C*H^T for C = [A1; A2] / [A2; A1] (side 'L', forward / backward) or
+    [A1 A2] / [A2 A1] (side 'R'), H = I - W*T*W^T with W the identity and the
+    pentagonal V.  k defaults to T's order; l, the order of V's triangle,
+    defaults to the reference's short form (src/parfb.jl:229: the number of
+    columns of V), which is only meaningful there when it does not exceed
+    k.  Column-major float32/float64 tensors on one ROCm device; work (k x n2
+    for side 'L', m2 x k for 'R') is allocated when omitted.  Returns (A1, A2)."""
+    _chars((side, "LRlr"), (trans, "NTCntc"), (direct, "FBfb"), (storev, "CRcr"))
+    side, trans, direct, storev = side.upper(), trans.upper(), direct.upper(), storev.upper()
+    ts = [A1, A2, V, T]
+    if A1.dtype not in _DT or any(t.dtype != A1.dtype for t in ts):
+        raise ValueError("A1, A2, V and T must share one element type, float32 or float64")
+    _require_device(*ts)
+    m1, n1 = A1.shape
+    m2, n2 = A2.shape
+    k = T.shape[0] if k is None else k
+    l = V.shape[1] if l is None else l
+    wr, wc = (k, n2) if side == "L" else (m2, k)
+    if work is None:
+        work = torch.empty((max(wc, 1), max(wr, 1)), dtype=A1.dtype, device=A1.device).t()
+    _require_device(A1, work)
+    st = load().nla_parfb(_lib.ch(side), _lib.ch(trans), _lib.ch(direct), _lib.ch(storev), _DT[A1.dtype], m1, n1,
+                          m2, n2, k, l, A1.data_ptr(), _cm_ld(A1, "A1"), A2.data_ptr(), _cm_ld(A2, "A2"),
+                          V.data_ptr(), _cm_ld(V, "V"), T.data_ptr(), _cm_ld(T, "T"), work.data_ptr(),
+                          max(_cm_ld(work, "work"), wr), _stream(A1))
+    _lib.check(st, "parfb!")
+    return A1, A2

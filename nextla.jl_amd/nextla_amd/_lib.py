@@ -63,6 +63,10 @@ SIGNATURES = {
     "nla_lauum_workspace_bytes": (_i64, [_i32, _i64]),
     "nla_larfb": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
                          _vp, _i64, _vp]),
+    "nla_pamm": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _i64, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp,
+                        _i64, _vp, _i64, _vp]),
+    "nla_parfb": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _vp,
+                         _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_lauum": (_i32, [_c_char, _i32, _i64, _vp, _i64, _vp, _i64, _vp]),
     "nla_mixed_workspace_bytes": (_i64, [_c_char, _i64, _i64]),
     "nla_rectrxm_mixed": (_i32, [_c_char, _c_char, _c_char, _i32, _dbl, _i64, _i64, _vp, _i64, _vp, _i64, _dbl, _i32,

@@ -136,6 +136,7 @@ def test_mixed_info_buffer_size():
 
 @pytest.mark.parametrize("needle", ["nla_rectrxm", "nla_rectrxm_mixed", "nla_residual", "nla_rectrxm_noalloc",
                                     "nla_getrf2", "nla_lauum", "nla_larfb", "nla_batch_plan_create",
-                                    "nla_trsm", "nla_trmm", "nla_gemm_add", "nla_gemm_sub"])
+                                    "nla_trsm", "nla_trmm", "nla_gemm_add", "nla_gemm_sub", "nla_pamm",
+                                    "nla_parfb"])
 def test_entry_points_bound(needle):
     assert needle in {c[0] for c in julia_ccalls()}

@@ -419,3 +419,121 @@ def test_residual_plan_bits():
     for i in range(len(RO.MODULI)):
         for j in range(i):
             assert math.gcd(RO.MODULI[i], RO.MODULI[j]) == 1
+
+
+# ---- tile-QR pentagonal block reflectors (pamm!/parfb!) ----------------------
+
+def _tpqrt_factors(rng, m, k, l):
+    """V (m x k pentagonal) and T (k x k upper) of LAPACK's dtpqrt on
+    [R; B] with B's last l rows upper trapezoidal."""
+    from scipy.linalg import lapack
+    from oracle import tprfb_oracle as TO
+    Rm = np.triu(rng.standard_normal((k, k))) + 3 * np.eye(k)
+    B = TO.pentagonal("L", "F", "C", k, l, rng.standard_normal((m, k)))
+    _, V, T, info = lapack.dtpqrt(l, k, Rm, B)
+    assert info == 0
+    return V, np.triu(T)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("m,k,l", [(9, 4, 0), (9, 4, 3), (7, 5, 5), (20, 8, 4)])
+def test_tprfb_closed_form_against_lapack_tpmqrt(side, trans, m, k, l):
+    """Forward columnwise reflectors of a dtpqrt factorization applied by
+    LAPACK's dtpmqrt (one block, so one xTPRFB call) equal the closed form;
+    H is orthogonal (H^T H C = C)."""
+    from scipy.linalg import lapack
+    from oracle import tprfb_oracle as TO
+    rng = np.random.default_rng(m * 100 + k * 10 + l)
+    V, T = _tpqrt_factors(rng, m, k, l)
+    nc = 6
+    A = rng.standard_normal((k, nc) if side == "L" else (nc, k))
+    B = rng.standard_normal((m, nc) if side == "L" else (nc, m))
+    a_ref, b_ref, info = lapack.dtpmqrt(l, V, T, A, B, side=side, trans=trans)
+    assert info == 0
+    a_cf, b_cf = TO.tprfb_dense(side, trans, "F", "C", l, V, T, A, B)
+    assert np.allclose(a_cf, a_ref, rtol=1e-12, atol=1e-12) and np.allclose(b_cf, b_ref, rtol=1e-12, atol=1e-12)
+    other = "T" if trans == "N" else "N"
+    a2, b2 = TO.tprfb_dense(side, other, "F", "C", l, V, T, a_cf, b_cf)
+    assert np.allclose(a2, A, atol=1e-12) and np.allclose(b2, B, atol=1e-12)
+    # a dropped transpose is caught: H is not symmetric
+    a_n, _ = TO.tprfb_dense(side, other, "F", "C", l, V, T, A, B)
+    assert not np.allclose(a_n, a_ref, atol=1e-6)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("storev", ["C", "R"])
+def test_tprfb_backward_and_rowwise_identities(side, trans, storev):
+    """The backward form on reversed rows of V / reversed reflectors equals
+    the forward one (C = [B; A] read bottom-up is [A; B] in reverse); the
+    rowwise form with V^T equals the columnwise one."""
+    from oracle import tprfb_oracle as TO
+    rng = np.random.default_rng(7)
+    m, k, l, nc = 11, 5, 3, 4
+    p = m
+    Vc = rng.standard_normal((p, k))
+    T = rng.standard_normal((k, k))
+    A = rng.standard_normal((k, nc) if side == "L" else (nc, k))
+    B = rng.standard_normal((m, nc) if side == "L" else (nc, m))
+    V = Vc if storev == "C" else Vc.T.copy()
+    af, bf = TO.tprfb_dense(side, trans, "F", storev, l, V, T, A, B)
+    # rowwise == columnwise on the transpose
+    if storev == "R":
+        ac, bc = TO.tprfb_dense(side, trans, "F", "C", l, Vc, T, A, B)
+        assert np.allclose(af, ac, atol=1e-12) and np.allclose(bf, bc, atol=1e-12)
+    # backward: reverse the rows of V (p) and the reflectors (k), T -> J T J
+    Vb = Vc[::-1, ::-1].copy() if storev == "C" else Vc[::-1, ::-1].T.copy()
+    Tb = T[::-1, ::-1].copy()
+    if side == "L":
+        ab, bb = TO.tprfb_dense(side, trans, "B", storev, l, Vb, Tb, A[::-1], B[::-1])
+        assert np.allclose(ab[::-1], af, atol=1e-12) and np.allclose(bb[::-1], bf, atol=1e-12)
+    else:
+        ab, bb = TO.tprfb_dense(side, trans, "B", storev, l, Vb, Tb, A[:, ::-1], B[:, ::-1])
+        assert np.allclose(ab[:, ::-1], af, atol=1e-12) and np.allclose(bb[:, ::-1], bf, atol=1e-12)
+
+
+PARFB_COMBOS = [(s, t, d, v) for s in "LR" for t in "NT" for d in "FB" for v in "CR"]
+
+
+@pytest.mark.parametrize("side,trans,direct,storev", PARFB_COMBOS)
+@pytest.mark.parametrize("m2,n2,k,l", [(20, 24, 8, 4), (9, 7, 5, 0), (6, 9, 6, 6), (30, 17, 12, 7)])
+def test_parfb_restatement_against_closed_form(side, trans, direct, storev, m2, n2, k, l):
+    """The restatement of parfb! / pamm! equals the closed form on random V
+    and T (the reference's test against LAPACK's tprfb, test/parfb.jl:12-46,
+    every side/trans/direct/storev), including l = 0 and l = k; the W of
+    pamm 'W' is op(W-block)^T [A; B] as the closed form's first factor."""
+    from oracle import tprfb_oracle as TO
+    import zlib
+    rng = np.random.default_rng(zlib.crc32(f"{side}{trans}{direct}{storev}{m2},{n2},{k},{l}".encode()))
+    if side == "R" and l > n2:
+        pytest.skip("triangle larger than V")
+    if side == "L" and l > m2:
+        pytest.skip("triangle larger than V")
+    m1, n1 = (k, n2) if side == "L" else (m2, k)
+    A1 = rng.standard_normal((m1, n1))
+    A2 = rng.standard_normal((m2, n2))
+    p = m2 if side == "L" else n2
+    V = rng.standard_normal((p, k) if storev == "C" else (k, p))
+    T = rng.standard_normal((k, k))
+    work = rng.standard_normal((k, n2) if side == "L" else (m2, k))
+    a1, a2 = TO.parfb(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, T, work)
+    c1, c2 = TO.tprfb_dense(side, trans, direct, storev, l, V, T, A1, A2)
+    assert np.allclose(a1, c1, rtol=1e-11, atol=1e-11) and np.allclose(a2, c2, rtol=1e-11, atol=1e-11)
+    # a planted mistake in the restatement's triangle (T's other triangle used) is caught
+    Tbad = np.tril(T) if direct == "F" else np.triu(T)
+    b1, _ = TO.parfb(side, trans, direct, storev, m1, n1, m2, n2, k, l, A1, A2, V, Tbad, work)
+    assert not np.allclose(b1, c1, rtol=1e-6, atol=1e-6)
+
+
+def test_pamm_w_is_the_reflector_product():
+    """pamm 'W' on the left, columnwise, forward: W = A1 + Vp^T A2 with Vp
+    the pentagonal V (closed form)."""
+    from oracle import tprfb_oracle as TO
+    rng = np.random.default_rng(3)
+    m, n, k, l = 12, 7, 5, 3
+    A1, A2 = rng.standard_normal((k, n)), rng.standard_normal((m, n))
+    V = rng.standard_normal((m, k))
+    W, _ = TO.pamm("W", "L", "C", "F", m, n, k, l, A1, A2, V, np.zeros((k, n)))
+    Vp = TO.pentagonal("L", "F", "C", k, l, V)
+    assert np.allclose(W, A1 + Vp.T @ A2, atol=1e-12)
