@@ -150,7 +150,7 @@ def percentile(ts, q):
     return t[min(len(t) - 1, max(0, int(math.ceil(q * len(t))) - 1))]
 
 
-def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16"), n=16384, m=4096, seed=5,
+def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16", "fp64"), n=16384, m=4096, seed=5,
                  keep=False):
     """FP64-accurate TRSM by refinement against the plain FP64 solve of the
     same calibrated system.  Per precision: mean / p99 time, speedup,

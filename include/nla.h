@@ -323,7 +323,12 @@ int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype
  *     precision for the following steps (bf16 -> fp16 -> fp32);
  *   - NLA_LP_AUTO: X0 from fp32 updates, then each correction in the
  *     precision (bf16, fp16 or fp32) with the least predicted cost to reach
- *     tol, from the measured error and the contraction of the last step.
+ *     tol, from the measured error and the contraction of the last step;
+ *   - NLA_LP_FP64: X0 is the plain FP64 solve (nla_rectrxm, its scratch in
+ *     the workspace), certified by the same residual and stopping test;
+ *     corrections, if ever needed, in fp32.  No low-precision update: on
+ *     MI355X this is the fastest certified route at the sizes measured (a
+ *     solve and one residual), offered beside the refinement routes.
  * X overwrites B.  The call synchronizes `stream` once per step (the
  * stopping test reads two norms on the host).  NLA_ERR_NOT_CONVERGED: after
  * max_iter (<= NLA_MIXED_MAX_ITER) correction steps the error is still above
@@ -335,7 +340,7 @@ int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype
  * 7 m, 8 A, 9 lda, 10 B, 11 ldb, 12 tol, 13 max_iter, 14 workspace,
  * 15 workspace_bytes, 16 stream, 17 info.
  */
-enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2, NLA_LP_AUTO = 3 };
+enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2, NLA_LP_AUTO = 3, NLA_LP_FP64 = 4 };
 
 #define NLA_MIXED_MAX_ITER 30
 

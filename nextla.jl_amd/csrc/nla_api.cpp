@@ -565,7 +565,7 @@ struct MixedLayout {
     int64_t ld32, ldr;   // leading dimension of the fp32 A copy / of S, R, D
     int64_t amax_slots;
     int64_t nsys;  // systems: columns of B (side L) or rows (side R)
-    int64_t a32, w, wf, s, r, d, amax, part, norms, queue, spart, xpart, s2, total;
+    int64_t a32, w, wf, s, r, d, amax, part, norms, queue, spart, xpart, s2, t64, t64_bytes, total;
     // int8 residual: its plan and workspace (after the fields above)
     bool oz;
     OzPlan op;
@@ -598,6 +598,9 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     L.spart = o; o = al256(o + np * 8);
     L.xpart = o; o = al256(o + np * 8);
     L.s2 = o;    o = al256(o + (m > 0 ? m : 1) * 8);
+    // the plain FP64 solve's scratch (NLA_LP_FP64)
+    L.t64_bytes = rectrxm_workspace_bytes('S', NLA_F64, n);
+    L.t64 = o;   o = al256(o + (L.t64_bytes > 0 ? L.t64_bytes : 1));
     L.total = o;
     L.oz = g_residual.load() == 1 && n >= OZ_MIN_N && n <= OZ_N_MAX && m >= OZ_MIN_M;
     if (L.oz) {
@@ -759,12 +762,20 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     };
 
     // X0: the low-precision solve of S, scaled by a power of two so the fp32
-    // buffer holds O(1) values
+    // buffer holds O(1) values; NLA_LP_FP64: the plain FP64 solve of alpha*B
+    // in place, corrections (if any) in fp32
     int prec = autop ? NLA_LP_FP32 : precision;
     double sig = pow2_above(maxS);
-    NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, S, L.ldr, D, L.ldr, s));
-    NLA_CK(lp_solve(prec));
-    NLA_CK(launch_up(L.rows, L.cols, sig, D, L.ldr, 0.0, B, ldb, s));
+    if (precision == NLA_LP_FP64) {
+        const int32_t st = rectrxm_in_workspace(side, uplo, trans, 'S', NLA_F64, alpha, n, m, A, lda, B, ldb, s,
+                                                ws + L.t64, L.t64_bytes);
+        if (st != NLA_OK) return st;
+        prec = NLA_LP_FP32;
+    } else {
+        NLA_CK(launch_down(L.rows, L.cols, 1.0 / sig, S, L.ldr, D, L.ldr, s));
+        NLA_CK(lp_solve(prec));
+        NLA_CK(launch_up(L.rows, L.cols, sig, D, L.ldr, 0.0, B, ldb, s));
+    }
 
     // R = S - op(A) X (side L) or S - X op(A) (side R): the FP64 MFMA GEMM
     // with the triangle of op(A) masked and the k-steps beyond it skipped
@@ -855,7 +866,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
 #undef NLA_CK
     if (info) {
         info->iterations = it;
-        info->precision = prec;
+        info->precision = (precision == NLA_LP_FP64 && it == 0) ? NLA_LP_FP64 : prec;
         info->backward_error = bwd;
         info->backward_error_frobenius = bwd_f;
         info->residual = use_oz ? NLA_RES_INT8 : NLA_RES_FP64;
@@ -1022,7 +1033,8 @@ int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, d
     if (side != 'L' && side != 'R') return bad_arg(1, "side");
     if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
     if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
-    if (precision != NLA_LP_BF16 && precision != NLA_LP_FP16 && precision != NLA_LP_FP32 && precision != NLA_LP_AUTO)
+    if (precision != NLA_LP_BF16 && precision != NLA_LP_FP16 && precision != NLA_LP_FP32 && precision != NLA_LP_AUTO &&
+        precision != NLA_LP_FP64)
         return bad_arg(4, "precision");
     if (!std::isfinite(alpha)) return bad_arg(5, "alpha");
     if (n < 0) return bad_arg(6, "n");

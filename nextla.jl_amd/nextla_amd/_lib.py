@@ -89,6 +89,7 @@ NLA_LP_BF16 = 0
 NLA_LP_FP16 = 1
 NLA_LP_FP32 = 2
 NLA_LP_AUTO = 3
+NLA_LP_FP64 = 4
 NLA_ERR_NOT_CONVERGED = 5
 NLA_ERR_WORKSPACE = 4
 NLA_NOALLOC_SCRATCH_BYTES = 16 * 1024 * 1024 + 64 * 1024

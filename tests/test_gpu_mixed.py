@@ -147,3 +147,34 @@ def test_full_size_self_certified(nla, prec):
     nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, P)
     rel = (torch.linalg.norm(B - P) / torch.linalg.norm(P)).item()
     assert rel <= 1e-10, rel
+
+
+@pytest.mark.parametrize("side,uplo,trans", [("L", "L", "N"), ("R", "U", "T"), ("L", "U", "T"), ("R", "L", "N")])
+@pytest.mark.parametrize("residual", ["int8", "fp64"])
+def test_fp64_route_certifies_the_plain_solve(nla, side, uplo, trans, residual):
+    """precision 'fp64': the plain FP64 solve certified by one residual.  With
+    the FP64 residual X is the plain solve bit for bit; with the int8 one it
+    is that solve rounded to >= 55 bits per system.  No correction is needed
+    (iterations 0, precision 'fp64')."""
+    from nextla_amd import _lib
+    from nextla_amd.mixed import rectrxm_mixed_
+    n, m, alpha = 2304, 96, -0.75  # n >= 2048, m >= 64: the int8 residual applies
+    A, B = case(n, m, side, uplo, 1e3, seed=13)
+    Ad = dev(A)
+    P = dev(B)
+    nla.unified_rectrxm_(side, uplo, trans, alpha, "S", Ad, P)
+    old = _lib.set_config(_lib.NLA_CFG_RESIDUAL, _lib.NLA_RES_INT8 if residual == "int8" else _lib.NLA_RES_FP64)
+    try:
+        Bd = dev(B)
+        _, info = rectrxm_mixed_(side, uplo, trans, alpha, Ad, Bd, precision="fp64", tol=1e-14)
+    finally:
+        _lib.set_config(_lib.NLA_CFG_RESIDUAL, old)
+    assert info["converged"] and info["iterations"] == 0 and info["precision"] == "fp64", info
+    assert info["residual"] == residual
+    X, Xp = host(Bd), host(P)
+    if residual == "fp64":
+        assert np.array_equal(X, Xp)
+    else:
+        sysmax = np.abs(Xp).max(0 if side == "L" else 1, keepdims=True)
+        assert np.all(np.abs(X - Xp) <= 2.0 ** -55 * sysmax)
+    assert O.backward_error(side, uplo, trans, alpha, A, B, X) <= 1e-14
