@@ -439,6 +439,82 @@ int32_t nla_batch_mixed_plan_execute(nla_batch_mixed_plan plan, double tol, int3
 int32_t nla_batch_mixed_plan_destroy(nla_batch_mixed_plan plan);
 
 /*
+ * Continuous-batching request server (no counterpart in the reference, which
+ * serves one unified_rectrxm! call at a time, test/unified_rectrxm.jl:33).
+ * A request is one unified_rectrxm!(side, uplo, trans, alpha, func, A, B)
+ * call whose column-major A and B already live in HBM.  The server admits
+ * requests against an HBM budget (B; A unless it is a resident factor, which
+ * is charged once; the batched workspace a request needs), groups queued
+ * requests of one signature (dtype, side, uplo, trans, func) into one
+ * batched plan (nla_batch_*) on one of max_inflight streams, and reports
+ * completions from nla_serve_step.  With fp64_trsm = NLA_SERVE_REFINED,
+ * FP64 TRSM requests are served by batched refinement (nla_batch_mixed_*,
+ * per-system backward error <= tol) on `workers` worker threads, each with
+ * its own stream.  A request's result is the same alone and in any batch.
+ *
+ *   nla_serve_submit records an event on `stream` (the submitter's): the
+ *     request runs after the work queued there so far (writing A or B).  A
+ *     and B must stay allocated until the request's completion is reported.
+ *     a_key > 0 names a resident factor: its first request registers A under
+ *     the key (charged to the budget once); later requests may pass A = NULL.
+ *     Resident factors no request refers to are evicted, least recently used
+ *     first, when the budget needs room; nla_serve_evicted lists them.
+ *   nla_serve_step retires finished batches, launches what the budget and
+ *     free streams allow (never blocks on the device) and hands out up to
+ *     `capacity` completion records (a status other than NLA_OK fails that
+ *     request only; NLA_ERR_WORKSPACE: it can never fit the budget).
+ *   nla_serve_drain steps until nothing is queued or running; the
+ *     completions stay for nla_serve_step to hand out.
+ *   nla_serve_destroy drains, stops the workers, releases the streams.
+ * One host thread at a time may call a server's functions.
+ */
+enum nla_serve_fp64 { NLA_SERVE_PLAIN = 0, NLA_SERVE_REFINED = 1 };
+
+typedef struct nla_serve_request {
+    char side, uplo, trans, func; /* as nla_rectrxm */
+    int32_t dtype;
+    double alpha;
+    int64_t n, m;
+    const void *A; /* may be NULL for a registered resident factor */
+    int64_t lda;
+    void *B;
+    int64_t ldb;
+    int64_t a_key; /* > 0: resident factor key; 0: none */
+} nla_serve_request;
+
+typedef struct nla_serve_done {
+    int64_t ticket;
+    int32_t status;        /* NLA_OK or the request's error */
+    int32_t iterations;    /* refined requests: correction steps */
+    double backward_error; /* refined requests: per-system backward error */
+} nla_serve_done;
+
+typedef struct nla_serve_stats_t {
+    int64_t budget_bytes, used_bytes, queued, running, completed_pending, resident;
+} nla_serve_stats_t;
+
+typedef struct nla_server_s *nla_server;
+
+int32_t nla_serve_create(int64_t budget_bytes, int32_t max_inflight, int32_t max_batch, int32_t fp64_trsm,
+                         int32_t workers, int32_t use_graph, nla_server *out);
+
+/* refined serving: target, step limit, and the update precision of requests
+ * above the batched size limit (defaults 1e-14, 10, NLA_LP_FP16) */
+int32_t nla_serve_set_refinement(nla_server server, double tol, int32_t max_iter, int32_t precision);
+
+int32_t nla_serve_submit(nla_server server, const nla_serve_request *request, void *stream, int64_t *ticket);
+
+int32_t nla_serve_step(nla_server server, nla_serve_done *done, int64_t capacity, int64_t *ndone);
+
+int32_t nla_serve_drain(nla_server server);
+
+int32_t nla_serve_stats(nla_server server, nla_serve_stats_t *stats);
+
+int32_t nla_serve_evicted(nla_server server, int64_t *keys, int64_t capacity, int64_t *count);
+
+int32_t nla_serve_destroy(nla_server server);
+
+/*
  * Process-wide choice of the methods used below the recursion (no
  * counterpart in the reference, whose thresholds are constants,
  * src/rectrxm.jl:52,63).  Defaults come from the environment

@@ -24,8 +24,9 @@ NextLA's `trsm` / `trmm` wrappers that honour `transa` and `diag`, and
 `lauum!` (U·U^T / L^T·L on the TRMM), `larfb!` (tile QR's block reflector,
 its triangular multiplies on the TRMM), `pamm!` / `parfb!` (tile QR's
 triangular-pentagonal reflectors, LAPACK's xTPRFB), `residual!` (the FP64-accurate
-triangular residual on the int8 MFMA) and `unified_rectrxm_noalloc!` (the
-drop-in without device allocation).
+triangular residual on the int8 MFMA), `unified_rectrxm_noalloc!` (the
+drop-in without device allocation) and `Server` / `submit!` / `step!` /
+`drain!` (the continuous-batching request server).
 
 Differences from the reference, by design: arguments are validated before any
 device work (`ArgumentError`, B untouched), lower-case characters are
@@ -298,6 +299,74 @@ function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, 
     bwd = reinterpret(Float64, info[9:16])[1]
     return B, steps, bwd
 end
+
+# ---- request server (nla_serve_*) ----------------------------------------------------------
+
+struct NlaServeRequest
+    side::Cchar; uplo::Cchar; trans::Cchar; func::Cchar
+    dtype::Int32
+    alpha::Float64
+    n::Int64; m::Int64
+    A::Ptr{Cvoid}; lda::Int64
+    B::Ptr{Cvoid}; ldb::Int64
+    a_key::Int64
+end
+
+struct NlaServeDone
+    ticket::Int64
+    status::Int32
+    iterations::Int32
+    backward_error::Float64
+end
+
+"""
+    Server(; budget_bytes=64 << 30, max_inflight=4, max_batch=512, refined=false, workers=4)
+
+The continuous-batching server core of libnlahip (nla_serve_*): `submit!` one
+`unified_rectrxm!` call at a time (A and B already on the device), `step!` to
+retire and launch batches and collect `(ticket, status, iterations,
+backward_error)` completions, `drain!` to finish everything.  With
+`refined=true`, FP64 TRSMs are served by batched refinement to backward error
+1e-14.  The caller keeps A and B alive until their ticket completes.
+"""
+mutable struct Server
+    handle::Ptr{Cvoid}
+    function Server(; budget_bytes::Integer=64 << 30, max_inflight::Integer=4, max_batch::Integer=512,
+                    refined::Bool=false, workers::Integer=4, use_graph::Bool=false)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:nla_serve_create, libnlahip), Int32, (Int64, Int32, Int32, Int32, Int32, Int32, Ptr{Ptr{Cvoid}}),
+                    Int64(budget_bytes), Int32(max_inflight), Int32(max_batch), Int32(refined ? 1 : 0),
+                    Int32(refined ? workers : 0), Int32(use_graph ? 1 : 0), h), "nla_serve_create")
+        srv = new(h[])
+        finalizer(s -> s.handle == C_NULL || ccall((:nla_serve_destroy, libnlahip), Int32, (Ptr{Cvoid},), s.handle),
+                  srv)
+        return srv
+    end
+end
+
+function submit!(srv::Server, side::Char, uplo::Char, transpose::Char, alpha::Number, func::Char,
+                 A::DeviceMat{T}, B::DeviceMat{T}; a_key::Integer=0) where {T <: Union{Float32, Float64}}
+    n = size(A, 1)
+    m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
+    pa, lda = ptr_ld(A)
+    pb, ldb = ptr_ld(B)
+    req = Ref(NlaServeRequest(Cchar(side), Cchar(uplo), Cchar(transpose), Cchar(func), dtype_code(T),
+                              Float64(alpha), Int64(n), Int64(m), pa, lda, pb, ldb, Int64(a_key)))
+    ticket = Ref{Int64}(0)
+    check(ccall((:nla_serve_submit, libnlahip), Int32, (Ptr{Cvoid}, Ptr{NlaServeRequest}, Ptr{Cvoid}, Ptr{Int64}),
+                srv.handle, req, hip_stream(), ticket), "nla_serve_submit")
+    return ticket[]
+end
+
+function step!(srv::Server; capacity::Integer=1024)
+    done = Vector{NlaServeDone}(undef, capacity)
+    nd = Ref{Int64}(0)
+    check(ccall((:nla_serve_step, libnlahip), Int32, (Ptr{Cvoid}, Ptr{NlaServeDone}, Int64, Ptr{Int64}),
+                srv.handle, done, Int64(capacity), nd), "nla_serve_step")
+    return done[1:nd[]]
+end
+
+drain!(srv::Server) = check(ccall((:nla_serve_drain, libnlahip), Int32, (Ptr{Cvoid},), srv.handle), "nla_serve_drain")
 
 """
     residual!(side, uplo, transpose, alpha, A, X, B, R; method=:int8)

@@ -28,6 +28,30 @@ class BatchItem(ctypes.Structure):
 NLA_MIXED_MAX_ITER = 30
 
 
+class ServeRequest(ctypes.Structure):
+    """nla_serve_request of include/nla.h."""
+    _fields_ = [("side", ctypes.c_char), ("uplo", ctypes.c_char), ("trans", ctypes.c_char), ("func", ctypes.c_char),
+                ("dtype", ctypes.c_int32), ("alpha", ctypes.c_double), ("n", ctypes.c_int64), ("m", ctypes.c_int64),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64), ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64),
+                ("a_key", ctypes.c_int64)]
+
+
+class ServeDone(ctypes.Structure):
+    """nla_serve_done of include/nla.h."""
+    _fields_ = [("ticket", ctypes.c_int64), ("status", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("backward_error", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    """nla_serve_stats_t of include/nla.h."""
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("used_bytes", ctypes.c_int64), ("queued", ctypes.c_int64),
+                ("running", ctypes.c_int64), ("completed_pending", ctypes.c_int64), ("resident", ctypes.c_int64)]
+
+
+NLA_SERVE_PLAIN = 0
+NLA_SERVE_REFINED = 1
+
+
 class MixedInfo(ctypes.Structure):
     """nla_mixed_info of include/nla.h."""
     _fields_ = [("iterations", ctypes.c_int32), ("precision", ctypes.c_int32), ("backward_error", ctypes.c_double),
@@ -59,6 +83,14 @@ SIGNATURES = {
     "nla_batch_mixed_plan_execute": (_i32, [_vp, _dbl, _i32, _vp, _i32, ctypes.POINTER(_dbl),
                                             ctypes.POINTER(_i32)]),
     "nla_batch_mixed_plan_destroy": (_i32, [_vp]),
+    "nla_serve_create": (_i32, [_i64, _i32, _i32, _i32, _i32, _i32, _vp]),
+    "nla_serve_set_refinement": (_i32, [_vp, _dbl, _i32, _i32]),
+    "nla_serve_submit": (_i32, [_vp, _vp, _vp, _vp]),
+    "nla_serve_step": (_i32, [_vp, _vp, _i64, _vp]),
+    "nla_serve_drain": (_i32, [_vp]),
+    "nla_serve_stats": (_i32, [_vp, _vp]),
+    "nla_serve_evicted": (_i32, [_vp, _vp, _i64, _vp]),
+    "nla_serve_destroy": (_i32, [_vp]),
     "nla_getrf2": (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _vp]),
     "nla_lauum_workspace_bytes": (_i64, [_i32, _i64]),
     "nla_larfb": (_i32, [_c_char, _c_char, _c_char, _c_char, _i32, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,

@@ -1,8 +1,10 @@
 """Continuous-batching request server for triangular solves / multiplies on
-one GPU (no counterpart in the reference, which serves one call at a time).
+one GPU (no counterpart in the reference, which serves one call at a time):
+a thin client of the server core behind the C ABI (nla_serve_* of
+include/nla.h, csrc/nla_serve.cpp).
 
 A request is one unified_rectrxm!(side, uplo, trans, alpha, func, A, B) call
-(src/rectrxm.jl:43) whose A and B already live in HBM.  The server
+(src/rectrxm.jl:43) whose A and B already live in HBM.  The core
 
   * admits requests against an HBM budget (the bytes of A, B and the
     batched workspace they need; A counted once while it is resident),
@@ -12,22 +14,22 @@ A request is one unified_rectrxm!(side, uplo, trans, alpha, func, A, B) call
     grouped GEMM per stage), so a batch costs a fixed number of launches
     whatever its sizes,
   * keeps several batches in flight on separate streams, so requests that
-    arrive while others run are admitted as slots and memory free up and
+    arrive while others run are admitted as streams and memory free up and
     their work overlaps on the device,
-  * records each request's latency from arrival to completion (the
-    completion event of its batch).
+  * orders each request after the work queued on the submitter's stream
+    (an event recorded at submit).
+
+This client keeps each request's tensors alive until its completion is
+reported, and stamps Request.done when it collects the completion (its
+latency runs from arrival to that moment).
 
 FP64 requests are FP64-accuracy requests.  Their TRSMs are served either by
 the plain FP64 path (fp64_trsm="plain", batched like the rest) or by fp32
-correction solves with FP64 refinement to backward error <= 1e-14
-(fp64_trsm="mixed"): same-signature requests are batched into one
-MixedBatchPlan (nla_batch_mixed_*: batched fp32 solves, batched FP64 TRMM
-residuals, each system frozen once it reaches the target).  The refinement
-reads its stopping test on the host once per step, so each refined batch
-runs on a worker thread with its own stream while the loop keeps admitting
-and launching.  The plain path is the faster of the two on MI355X (FP64 MFMA
-runs at the FP64 vector rate, so each FP64 residual costs about as much as a
-whole plain solve); "mixed" is there to serve and measure the refined form.
+correction solves with FP64 refinement to backward error <= mixed_tol
+(fp64_trsm="mixed"): same-signature requests are batched into one refined
+batch (nla_batch_mixed_*), run by the core's worker threads, each with its
+own stream, because the refinement reads its stopping test on the host once
+per step.
 
 A system's result does not depend on what else is in its batch: the batched
 kernel solves each (system, strip) task with the same code path whatever the
@@ -35,19 +37,18 @@ batch holds, so a request gives bit-identical output alone and in the mix
 (a refined request likewise: its steps, error and result do not depend on
 the rest of its batch).
 """
-import collections
-import concurrent.futures
-import threading
+import ctypes
 import time
 
 import torch
 
-from . import unified_rectrxm_
-from .batch import BATCH_N_MAX, BatchPlan, MixedBatchPlan
+from . import _lib, unified_rectrxm_
+from .batch import BATCH_N_MAX  # noqa: F401  (the core's batched size limit)
 
 
 class Request:
-    __slots__ = ("rid", "side", "uplo", "trans", "func", "alpha", "A", "B", "arrival", "done", "a_key", "ready")
+    __slots__ = ("rid", "side", "uplo", "trans", "func", "alpha", "A", "B", "arrival", "done", "a_key",
+                 "status", "backward_error", "iterations")
 
     def __init__(self, rid, side, uplo, trans, func, alpha, A, B, arrival=None, a_key=None):
         self.rid = rid
@@ -57,16 +58,9 @@ class Request:
         self.arrival = arrival
         self.done = None
         self.a_key = a_key  # identifies a device-resident A shared between requests
-        self.ready = None  # event on the submitting stream after A and B were written
-
-    def wait_ready(self, stream):
-        """Order `stream` after the submitter's writes of A and B, and tell the
-        caching allocator that `stream` uses them (so memory the submitter
-        frees is not reused while the server's work still reads it)."""
-        if self.ready is not None:
-            stream.wait_event(self.ready)
-        self.A.record_stream(stream)
-        self.B.record_stream(stream)
+        self.status = None
+        self.backward_error = None  # refined requests: the per-system backward error
+        self.iterations = None
 
     def signature(self):
         return (self.A.dtype, self.side, self.uplo, self.trans, self.func)
@@ -75,60 +69,21 @@ class Request:
         return self.B.numel() * self.B.element_size()
 
 
-class _InFlight:
-    __slots__ = ("reqs", "plan", "event", "stream", "nbytes", "a_keys", "future")
+def _colmajor_ld(t, name, rid):
+    r, c = t.shape
+    if r > 1 and t.stride(0) != 1:
+        raise ValueError(f"request {rid}: {name} must be column-major (stride (1, ld))")
+    ld = t.stride(1) if c > 1 else max(r, 1)
+    if c > 1 and ld < r:
+        raise ValueError(f"request {rid}: {name} must be column-major (stride (1, ld))")
+    return max(ld, 1)
 
 
-class _MixedWorkers:
-    """Worker threads for refined FP64 batches, each with its own stream."""
-
-    def __init__(self, device, workers, precision, tol, max_iter=10):
-        self.device = device
-        self.precision = precision
-        self.tol = tol
-        self.max_iter = max_iter
-        self.local = threading.local()
-        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
-
-    def _stream(self):
-        st = getattr(self.local, "stream", None)
-        if st is None:
-            st = self.local.stream = torch.cuda.Stream(device=self.device)
-        return st
-
-    def run(self, reqs):
-        """Refine a batch of same-signature FP64 TRSM requests; returns the
-        backward error of each."""
-        from .mixed import rectrxm_mixed_
-        stream = self._stream()
-        for r in reqs:
-            r.wait_ready(stream)
-        small = [r for r in reqs if r.A.shape[0] <= BATCH_N_MAX]
-        big = [r for r in reqs if r.A.shape[0] > BATCH_N_MAX]
-        errors = {}
-        with torch.cuda.stream(stream):
-            if small:
-                r0 = small[0]
-                plan = MixedBatchPlan(r0.side, r0.uplo, r0.trans, [r.A for r in small], [r.B for r in small],
-                                      [r.alpha for r in small], stream=stream)
-                bwd, _, _ = plan.execute(tol=self.tol, max_iter=self.max_iter, stream=stream)
-                plan.close()
-                errors.update({r.rid: b for r, b in zip(small, bwd)})
-            for r in big:  # beyond the batched planner's size limit: one refined call each
-                _, info = rectrxm_mixed_(r.side, r.uplo, r.trans, r.alpha, r.A, r.B, precision=self.precision,
-                                         tol=self.tol)
-                errors[r.rid] = info["backward_error"]
-        now = time.perf_counter()
-        for r in reqs:
-            r.done = now
-        return errors
-
-    def close(self):
-        self.pool.shutdown(wait=True)
+_DT = {torch.float32: 0, torch.float64: 1}
 
 
 class Server:
-    """Continuous-batching server.
+    """Continuous-batching server (the nla_serve_* core).
 
     budget_bytes: HBM admission budget for in-flight requests.
     max_inflight: batches in flight at once (one stream each).
@@ -138,208 +93,154 @@ class Server:
         fp32 correction solves with FP64 refinement to mixed_tol on
         mixed_workers worker threads; mixed_precision is the update precision
         of the single-system refinement used above the batched size limit).
-    direct_rows: requests with more rows go through the recursion one call
-        each instead of into the batched plan (default: none; measured slower
-        in aggregate, see the serving notes).
     """
 
     def __init__(self, device="cuda", budget_bytes=64 << 30, max_inflight=4, max_batch=512, use_graph=False,
-                 fp64_trsm="plain", mixed_precision="fp16", mixed_tol=1e-14, mixed_workers=4, direct_rows=None):
+                 fp64_trsm="plain", mixed_precision="fp16", mixed_tol=1e-14, mixed_workers=4):
+        from .mixed import PRECISIONS
         if fp64_trsm not in ("plain", "mixed"):
             raise ValueError(f"fp64_trsm must be 'plain' or 'mixed', got {fp64_trsm!r}")
         self.device = torch.device(device)
-        self.budget = int(budget_bytes)
-        self.max_batch = max_batch
-        self.use_graph = use_graph
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max_inflight)]
-        self.free_streams = list(self.streams)
-        self.queue = collections.deque()
-        self.inflight = []
-        self.used = 0
-        self.resident = collections.OrderedDict()  # a_key -> (A tensor, refcount)
-        self.completed = []
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self.fp64_trsm = fp64_trsm
-        self.mixed = (_MixedWorkers(self.device, mixed_workers, mixed_precision, mixed_tol)
-                      if fp64_trsm == "mixed" else None)
-        self.mixed_slots = mixed_workers
-        # requests with more rows than this go through the recursion one call
-        # each (same stream, no wait) instead of into the batched plan
-        self.direct_rows = BATCH_N_MAX if direct_rows is None else min(int(direct_rows), BATCH_N_MAX)
+        self._lib = _lib.load()
+        self._srv = ctypes.c_void_p()
+        mode = _lib.NLA_SERVE_REFINED if fp64_trsm == "mixed" else _lib.NLA_SERVE_PLAIN
+        with torch.cuda.device(self.device):
+            st = self._lib.nla_serve_create(int(budget_bytes), int(max_inflight), int(max_batch), mode,
+                                            int(mixed_workers) if fp64_trsm == "mixed" else 0, 1 if use_graph else 0,
+                                            ctypes.byref(self._srv))
+        _lib.check(st, "nla_serve_create")
+        if fp64_trsm == "mixed":
+            _lib.check(self._lib.nla_serve_set_refinement(self._srv, float(mixed_tol), 10, PRECISIONS[mixed_precision]),
+                       "nla_serve_set_refinement")
+        self.tickets = {}  # ticket -> Request, until its completion is collected
+        self.resident = {}  # key -> A (kept alive while the core holds it)
+        self._key_ids = {}
+        self._id_keys = {}
+        self.completed = []
+        self._done = (_lib.ServeDone * 1024)()
+        self._evict_buf = (ctypes.c_int64 * 256)()
 
     # -- resident factors ----------------------------------------------------
+    def _key_id(self, key):
+        if key not in self._key_ids:
+            kid = len(self._key_ids) + 1
+            self._key_ids[key] = kid
+            self._id_keys[kid] = key
+        return self._key_ids[key]
+
     def resident_a(self, key, A=None):
         """Return the device-resident A stored under key (storing A first if
-        given).  Resident factors are charged to the budget once."""
+        given).  The core charges a resident factor to the budget once, from
+        the first request that uses it."""
         if key in self.resident:
-            self.resident.move_to_end(key)
             return self.resident[key]
         if A is None:
             raise KeyError(key)
         self.resident[key] = A
-        self.used += A.numel() * A.element_size()
+        self._key_id(key)
         return A
 
-    def _evict(self, need):
-        for key in list(self.resident):
-            if self.used + need <= self.budget:
+    def _sync_evicted(self):
+        n = ctypes.c_int64()
+        while True:
+            _lib.check(self._lib.nla_serve_evicted(self._srv, self._evict_buf, 256, ctypes.byref(n)),
+                       "nla_serve_evicted")
+            for i in range(n.value):
+                self.resident.pop(self._id_keys.get(self._evict_buf[i]), None)
+            if n.value < 256:
                 break
-            if any(r.a_key == key for r in self.queue) or any(
-                    key in f.a_keys for f in self.inflight):
-                continue
-            A = self.resident.pop(key)
-            self.used -= A.numel() * A.element_size()
 
     # -- request flow -------------------------------------------------------
     def submit(self, req):
         """Queue a request.  Its A and B may still be being written by work
-        queued on the caller's current stream: the request's batch waits for
-        that work (an event recorded here) before reading them."""
+        queued on the caller's current stream: the core records an event there
+        and runs the request after it."""
         if req.arrival is None:
             req.arrival = time.perf_counter()
-        req.ready = torch.cuda.Event()
-        req.ready.record(torch.cuda.current_stream(self.device))
-        self.queue.append(req)
-
-    def _is_mixed(self, r):
-        return self.mixed is not None and r.func == "S" and r.A.dtype == torch.float64
-
-    def _req_bytes(self, r):
-        b = r.nbytes()
-        if r.a_key is None:
-            b += r.A.numel() * r.A.element_size()
-        n = r.A.shape[0]
-        if self._is_mixed(r):
-            m = r.B.shape[1] if r.side == "L" else r.B.shape[0]
-            b += 4 * n * n + 20 * n * m + 256 * n + 4096  # refinement arrays, fp32 inverses, descriptors
-        elif r.func == "S":
-            b += ((n + 63) // 64) * 64 * 64 * r.A.element_size()  # diagonal inverses
-        return b
-
-    def _launch_mixed(self):
-        """Hand the queued refined requests that share the first one's
-        signature (within the budget) to a free worker as one batch."""
-        busy = sum(1 for f in self.inflight if f.future is not None)
-        if busy >= self.mixed_slots:
-            return False
-        head = next((r for r in self.queue if self._is_mixed(r)), None)
-        if head is None:
-            return False
-        sig = head.signature()
-        take, keep, nbytes = [], collections.deque(), 0
-        while self.queue and len(take) < self.max_batch:
-            r = self.queue.popleft()
-            rb = self._req_bytes(r)
-            if self._is_mixed(r) and r.signature() == sig and self.used + nbytes + rb <= self.budget:
-                take.append(r)
-                nbytes += rb
-            else:
-                keep.append(r)
-        self.queue.extendleft(reversed(keep))
-        if not take:
-            need = self._req_bytes(head)
-            self._evict(need)
-            if not self.inflight and self.used + need > self.budget:
-                self.queue.remove(head)
-                raise RuntimeError(f"request {head.rid} needs {need} bytes, more than the HBM budget of "
-                                   f"{self.budget}")
-            return False
-        f = _InFlight()
-        f.reqs, f.stream, f.nbytes, f.plan, f.event = take, None, nbytes, None, None
-        f.a_keys = {r.a_key for r in take if r.a_key is not None}
-        f.future = self.mixed.pool.submit(self.mixed.run, take)
-        self.used += nbytes
-        self.inflight.append(f)
-        return True
-
-    def _launch(self):
-        if self.mixed is not None and self._launch_mixed():
-            return True
-        head = next((r for r in self.queue if not self._is_mixed(r)), None)
-        if head is None or not self.free_streams:
-            return False
-        sig = head.signature()
-        take, keep, nbytes = [], collections.deque(), 0
-        while self.queue and len(take) < self.max_batch:
-            r = self.queue.popleft()
-            rb = self._req_bytes(r)
-            if not self._is_mixed(r) and r.signature() == sig and self.used + nbytes + rb <= self.budget:
-                take.append(r)
-                nbytes += rb
-            else:
-                keep.append(r)
-        self.queue.extendleft(reversed(keep))
-        if not take:
-            self._evict(self._req_bytes(head))
-            if not self.inflight and self.used + self._req_bytes(head) > self.budget:
-                self.queue.remove(head)
-                raise RuntimeError(f"request {head.rid} needs {self._req_bytes(head)} bytes, "
-                                   f"more than the HBM budget of {self.budget}")
-            return False
-        stream = self.free_streams.pop()
-        f = _InFlight()
-        f.reqs, f.stream, f.nbytes, f.future = take, stream, nbytes, None
-        f.a_keys = {r.a_key for r in take if r.a_key is not None}
-        small = [r for r in take if r.A.shape[0] <= self.direct_rows]
-        big = [r for r in take if r.A.shape[0] > self.direct_rows]
-        for r in take:
-            r.wait_ready(stream)
-        with torch.cuda.stream(stream):
-            f.plan = None
-            if small:
-                dt, side, uplo, trans, func = sig
-                f.plan = BatchPlan(side, uplo, trans, func, [r.A for r in small], [r.B for r in small],
-                                   [r.alpha for r in small], stream=stream)
-                f.plan.execute(use_graph=self.use_graph, stream=stream)
-            for r in big:
-                unified_rectrxm_(r.side, r.uplo, r.trans, r.alpha, r.func, r.A, r.B)
-            f.event = torch.cuda.Event()
-            f.event.record(stream)
-        self.used += nbytes
-        self.inflight.append(f)
-        return True
-
-    def _reap(self):
-        done = [f for f in self.inflight if (f.future.done() if f.future is not None else f.event.query())]
-        now = time.perf_counter()
-        for f in done:
-            self.inflight.remove(f)
-            if f.future is not None:
-                f.future.result()  # re-raises a worker's error here
-                self.completed.extend(f.reqs)  # done was stamped by the worker
-            else:
-                for r in f.reqs:
-                    r.done = now
-                    self.completed.append(r)
-                self.free_streams.append(f.stream)
-            if f.plan is not None:
-                f.plan.close()
-            self.used -= f.nbytes
-        return len(done)
+        A, B = req.A, req.B
+        if A.dtype not in _DT or B.dtype != A.dtype:
+            raise ValueError(f"request {req.rid}: A and B must share float32 or float64")
+        if not (A.is_cuda and B.is_cuda and A.device == self.device and B.device == self.device):
+            raise ValueError(f"request {req.rid}: A and B must live on {self.device}")
+        n = A.shape[0]
+        side = req.side.upper()
+        r = _lib.ServeRequest()
+        r.side, r.uplo, r.trans, r.func = (_lib.ch(c.upper()) for c in (req.side, req.uplo, req.trans, req.func))
+        r.dtype = _DT[A.dtype]
+        r.alpha = float(req.alpha)
+        r.n = n
+        r.m = B.shape[1] if side == "L" else B.shape[0]
+        r.A, r.lda = A.data_ptr(), _colmajor_ld(A, "A", req.rid)
+        r.B, r.ldb = B.data_ptr(), _colmajor_ld(B, "B", req.rid)
+        r.a_key = self._key_id(req.a_key) if req.a_key is not None else 0
+        if req.a_key is not None and req.a_key not in self.resident:
+            self.resident[req.a_key] = A
+        ticket = ctypes.c_int64()
+        st = self._lib.nla_serve_submit(self._srv, ctypes.byref(r), torch.cuda.current_stream(self.device).cuda_stream,
+                                        ctypes.byref(ticket))
+        _lib.check(st, "nla_serve_submit")
+        self.tickets[ticket.value] = req
 
     def step(self):
-        """One turn of the loop: retire finished batches, launch new ones."""
-        n = self._reap()
-        launched = False
-        while self._launch():
-            launched = True
-        if not n and not launched and self.mixed is not None:
-            # Nothing to do but wait: block on the refinement workers for a
-            # moment instead of polling, so their Python side gets the GIL.
-            pending = [f.future for f in self.inflight if f.future is not None]
-            if pending:
-                concurrent.futures.wait(pending, timeout=1e-4, return_when=concurrent.futures.FIRST_COMPLETED)
-        return n
+        """One turn of the loop: retire finished batches, launch new ones,
+        collect completions (stamping Request.done).  Returns how many
+        requests completed."""
+        nd = ctypes.c_int64()
+        st = self._lib.nla_serve_step(self._srv, self._done, len(self._done), ctypes.byref(nd))
+        now = time.perf_counter()
+        failed = []
+        for i in range(nd.value):
+            d = self._done[i]
+            req = self.tickets.pop(d.ticket)
+            req.done = now
+            req.status = d.status
+            if self.fp64_trsm == "mixed" and req.func.upper() == "S" and req.A.dtype == torch.float64:
+                req.backward_error, req.iterations = d.backward_error, d.iterations
+            self.completed.append(req)
+            if d.status != 0:
+                failed.append(req)
+        _lib.check(st, "nla_serve_step")
+        self._sync_evicted()
+        if failed:
+            r = failed[0]
+            raise RuntimeError(f"request {r.rid} failed with status {r.status}"
+                               + (" (it needs more than the HBM budget)" if r.status == _lib.NLA_ERR_WORKSPACE else ""))
+        return nd.value
 
     def drain(self):
-        while self.queue or self.inflight:
+        while self.tickets:
             self.step()
 
     def idle(self):
-        return not self.queue and not self.inflight
+        return not self.tickets
+
+    def _stats(self):
+        s = _lib.ServeStats()
+        _lib.check(self._lib.nla_serve_stats(self._srv, ctypes.byref(s)), "nla_serve_stats")
+        return s
+
+    @property
+    def used(self):
+        return self._stats().used_bytes
+
+    @property
+    def budget(self):
+        return self._stats().budget_bytes
 
     def close(self):
-        if self.mixed is not None:
-            self.mixed.close()
+        if getattr(self, "_srv", None) is not None and self._srv.value:
+            st = self._lib.nla_serve_destroy(self._srv)
+            self._srv = ctypes.c_void_p()
+            _lib.check(st, "nla_serve_destroy")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def workload_requests(count, seed=0, device="cuda", reuse=0.2, pool=8, n_range=(32, 4096), m_range=(1, 512),

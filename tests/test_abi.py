@@ -291,3 +291,18 @@ def test_residual_validation(nla):
     small = lib.nla_mixed_workspace_bytes(b"L", 4096, 128)
     _lib.set_config(_lib.NLA_CFG_RESIDUAL, old)
     assert lib.nla_mixed_workspace_bytes(b"L", 4096, 128) > small + 16 * 4096 * 4096 // 2
+
+
+def test_serve_create_validation(nla):
+    """nla_serve_create rejects bad arguments before touching the device."""
+    from nextla_amd import _lib
+    lib = _lib.load()
+    srv = ctypes.c_void_p()
+    assert lib.nla_serve_create(0, 1, 1, 0, 0, 0, ctypes.byref(srv)) == -1
+    assert lib.nla_serve_create(1 << 20, 0, 1, 0, 0, 0, ctypes.byref(srv)) == -2
+    assert lib.nla_serve_create(1 << 20, 1, 0, 0, 0, 0, ctypes.byref(srv)) == -3
+    assert lib.nla_serve_create(1 << 20, 1, 1, 2, 0, 0, ctypes.byref(srv)) == -4
+    assert lib.nla_serve_create(1 << 20, 1, 1, _lib.NLA_SERVE_REFINED, 0, 0, ctypes.byref(srv)) == -5
+    assert lib.nla_serve_create(1 << 20, 1, 1, 0, 0, 0, None) == -7
+    assert lib.nla_serve_destroy(None) == 0
+    assert lib.nla_serve_step(None, None, 0, None) == -1

@@ -140,3 +140,105 @@ def test_refined_fp64_requests(nla):
     s2.close()
     torch.cuda.synchronize()
     assert torch.equal(solo.B, reqs[k].B)
+
+
+COMBOS8 = [(s, u, t) for s in "LR" for u in "LU" for t in "NT"]
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_every_signature_poisson_stream(nla, dt):
+    """Every side/uplo/trans x TRSM/TRMM, n up to 4096, m up to 512, fed at
+    Poisson arrivals (2,000 requests/s): each result agrees with its single
+    unified_rectrxm! call."""
+    from nextla_amd import serve
+    gen = torch.Generator(device=DEV)
+    gen.manual_seed(21)
+    rng = np.random.default_rng(21)
+    reqs, expect = [], []
+    for k in range(48):
+        side, uplo, trans = COMBOS8[k % 8]
+        func = "S" if (k // 8) % 2 == 0 else "M"
+        n = int(np.exp(rng.uniform(np.log(32), np.log(4096))))
+        m = int(np.exp(rng.uniform(np.log(1), np.log(512))))
+        A = tri(n, dt, gen)
+        B = colmajor_rand(n, m, dt, gen) if side == "L" else colmajor_rand(m, n, dt, gen)
+        x = B.clone()
+        nla.unified_rectrxm_(side, uplo, trans, 0.5, func, A, x)
+        expect.append(x)
+        reqs.append(serve.Request(k, side, uplo, trans, func, 0.5, A, B))
+    torch.cuda.synchronize()
+    arrivals = np.cumsum(rng.exponential(1.0 / 2000.0, size=len(reqs)))
+    srv = serve.Server(budget_bytes=16 << 30, max_inflight=4, max_batch=32)
+    lat, wall = serve.run_stream(srv, reqs, arrivals)
+    srv.close()
+    torch.cuda.synchronize()
+    eps = {torch.float64: 2.2e-16, torch.float32: 1.2e-7}[dt]
+    for r, x in zip(reqs, expect):
+        n = r.A.shape[0]
+        rel = (torch.linalg.norm(r.B.double() - x.double()) / torch.linalg.norm(x.double())).item()
+        assert rel <= 100 * n * eps, (r.rid, r.side, r.uplo, r.trans, r.func, n, rel)
+        assert r.status == 0
+    assert all(v >= 0 for v in lat)
+
+
+def test_c_abi_server_core(nla):
+    """nla_serve_* called directly: tickets, completion records, a resident
+    factor submitted with A = NULL after its first use, a request larger than
+    the budget failed with NLA_ERR_WORKSPACE (the others unaffected), stats."""
+    import ctypes
+    from nextla_amd import _lib
+    lib = _lib.load()
+    gen = torch.Generator(device=DEV)
+    gen.manual_seed(4)
+    A = tri(200, torch.float64, gen)
+    Bs = [colmajor_rand(200, 5, torch.float64, gen) for _ in range(3)]
+    B0 = [b.clone() for b in Bs]
+    budget = 8 << 20
+    srv = ctypes.c_void_p()
+    assert lib.nla_serve_create(budget, 2, 8, _lib.NLA_SERVE_PLAIN, 0, 0, ctypes.byref(srv)) == 0
+    s = torch.cuda.current_stream().cuda_stream
+    tickets = []
+    for k, B in enumerate(Bs):
+        r = _lib.ServeRequest(b"L", b"L", b"N", b"S", 1, 1.0, 200, 5, A.data_ptr() if k == 0 else None, 200,
+                              B.data_ptr(), 200, 7)
+        t = ctypes.c_int64()
+        assert lib.nla_serve_submit(srv, ctypes.byref(r), s, ctypes.byref(t)) == 0
+        tickets.append(t.value)
+    big = torch.zeros((2048, 2048), dtype=torch.float64, device=DEV)  # 32 MB of A: above the budget
+    bigB = torch.zeros((2048, 1), dtype=torch.float64, device=DEV)
+    r = _lib.ServeRequest(b"L", b"L", b"N", b"M", 1, 1.0, 2048, 1, big.data_ptr(), 2048, bigB.data_ptr(), 2048, 0)
+    t = ctypes.c_int64()
+    assert lib.nla_serve_submit(srv, ctypes.byref(r), s, ctypes.byref(t)) == 0
+    big_ticket = t.value
+    assert lib.nla_serve_drain(srv) == 0
+    st = _lib.ServeStats()
+    assert lib.nla_serve_stats(srv, ctypes.byref(st)) == 0
+    assert st.queued == 0 and st.running == 0 and st.completed_pending == 4 and st.resident == 1
+    assert st.used_bytes == 200 * 200 * 8 and st.budget_bytes == budget  # only the resident factor stays
+    done = (_lib.ServeDone * 8)()
+    nd = ctypes.c_int64()
+    assert lib.nla_serve_step(srv, done, 8, ctypes.byref(nd)) == 0 and nd.value == 4
+    status = {done[i].ticket: done[i].status for i in range(nd.value)}
+    assert status[big_ticket] == _lib.NLA_ERR_WORKSPACE
+    assert all(status[t] == 0 for t in tickets)
+    keys = (ctypes.c_int64 * 4)()
+    cnt = ctypes.c_int64()
+    assert lib.nla_serve_evicted(srv, keys, 4, ctypes.byref(cnt)) == 0 and cnt.value == 0
+    # a request that fits only once the unused resident factor is evicted
+    A2 = tri(980, torch.float64, gen)
+    B2 = colmajor_rand(980, 1, torch.float64, gen)
+    B20 = B2.clone()
+    r = _lib.ServeRequest(b"L", b"L", b"N", b"S", 1, 1.0, 980, 1, A2.data_ptr(), 980, B2.data_ptr(), 980, 0)
+    assert lib.nla_serve_submit(srv, ctypes.byref(r), s, ctypes.byref(t)) == 0
+    assert lib.nla_serve_drain(srv) == 0
+    assert lib.nla_serve_step(srv, done, 8, ctypes.byref(nd)) == 0 and nd.value == 1 and done[0].status == 0
+    assert lib.nla_serve_evicted(srv, keys, 4, ctypes.byref(cnt)) == 0 and cnt.value == 1 and keys[0] == 7
+    assert lib.nla_serve_stats(srv, ctypes.byref(st)) == 0 and st.resident == 0 and st.used_bytes == 0
+    assert lib.nla_serve_destroy(srv) == 0
+    x = B20.clone()
+    nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A2, x)
+    assert (torch.linalg.norm(B2 - x) / torch.linalg.norm(x)).item() < 1e-13
+    for B, b0 in zip(Bs, B0):
+        x = b0.clone()
+        nla.unified_rectrxm_("L", "L", "N", 1.0, "S", A, x)
+        assert (torch.linalg.norm(B - x) / torch.linalg.norm(x)).item() < 1e-13

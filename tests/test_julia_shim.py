@@ -68,7 +68,7 @@ def c_prototypes():
 
 def c_kind(decl):
     d = decl.strip()
-    if "*" in d or d.split()[0] in ("nla_batch_plan", "nla_batch_mixed_plan"):
+    if "*" in d or d.split()[0] in ("nla_batch_plan", "nla_batch_mixed_plan", "nla_server"):
         return "ptr"
     t = d.split()[0] if not d.startswith("const") else d.split()[1]
     return {"char": "char", "int32_t": "i32", "int64_t": "i64", "double": "f64"}[t]
@@ -137,6 +137,7 @@ def test_mixed_info_buffer_size():
 @pytest.mark.parametrize("needle", ["nla_rectrxm", "nla_rectrxm_mixed", "nla_residual", "nla_rectrxm_noalloc",
                                     "nla_getrf2", "nla_lauum", "nla_larfb", "nla_batch_plan_create",
                                     "nla_trsm", "nla_trmm", "nla_gemm_add", "nla_gemm_sub", "nla_pamm",
-                                    "nla_parfb"])
+                                    "nla_parfb", "nla_serve_create", "nla_serve_submit", "nla_serve_step",
+                                    "nla_serve_drain", "nla_serve_destroy"])
 def test_entry_points_bound(needle):
     assert needle in {c[0] for c in julia_ccalls()}
