@@ -172,6 +172,32 @@ int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_gr
 int32_t nla_batch_plan_destroy(nla_batch_plan plan);
 
 /*
+ * The task graph of a batch plan, for host-side checking (no device work):
+ * one record per task, in launch order (launch 0 is the 64-block inversion
+ * for func 'S'), in canonical coordinates (rows: the order of op(A); side
+ * 'R' systems are described on their transpose, columns = the m systems'
+ * right-hand sides).
+ *   kind 0: invert the 64-block of item `system` at rows r0..r0+rows;
+ *   kind 1: panel strip: rows r0..r0+rows, columns c0..c0+cols of B are
+ *           solved (TRSM: X = alpha inv(M_rr) B) or multiplied
+ *           (TRMM: B = alpha tri(M_rr) B) with the diagonal block M_rr;
+ *   kind 2: GEMM tile: B(rows, cols) = beta B(rows, cols)
+ *           + alpha M(rows, k0..k0+kn) B(k0..k0+kn, cols).
+ * Tasks of one launch run concurrently in any order; launches run in order.
+ * Returns the number of records (writing at most `capacity`), or -k for an
+ * invalid argument k (the checks of nla_batch_plan_create).
+ */
+typedef struct nla_batch_task_desc {
+    int32_t launch, kind;
+    int64_t system;
+    int64_t r0, rows, c0, cols, k0, kn;
+    double alpha, beta;
+} nla_batch_task_desc;
+
+int64_t nla_batch_plan_describe(char side, char uplo, char trans, char func, int32_t dtype, const nla_batch_item *items,
+                                int64_t count, nla_batch_task_desc *out, int64_t capacity);
+
+/*
  * BLAS-style xTRSM / xTRMM with the diagonal argument: the same operation as
  * nla_rectrxm with func 'S' / 'M', plus diag:
  *   'N'  non-unit diagonal (as nla_rectrxm);
@@ -566,6 +592,10 @@ int32_t nla_serve_destroy(nla_server server);
  *                      two steps, row order; 3: two steps, 8 x 4
  *                      super-tiles); results are identical, only the speed
  *                      differs.
+ *   NLA_CFG_BATCH_ORDER: order of the tasks within every launch of new batch
+ *                      plans (0, default: longest first for load balance;
+ *                      k > 0: a pseudo-random order from seed k, for testing
+ *                      that results do not depend on it).
  * Returns the previous value, or -1 for an unknown key or invalid value.
  */
 enum nla_config_key {
@@ -580,7 +610,8 @@ enum nla_config_key {
     NLA_CFG_LU_SPLIT_ROWS = 8,
     NLA_CFG_RESIDUAL = 9,
     NLA_CFG_LU_SPIN_LIMIT = 10,
-    NLA_CFG_OZ_VARIANT = 11
+    NLA_CFG_OZ_VARIANT = 11,
+    NLA_CFG_BATCH_ORDER = 12
 };
 
 int32_t nla_set_config(int32_t key, int32_t value);

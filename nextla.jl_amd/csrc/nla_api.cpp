@@ -128,6 +128,8 @@ static std::atomic<int> g_panel_waves{0};
 static std::atomic<int> g_batch_classes{0x1F};
 
 int batch_class_mask() { return g_batch_classes.load(); }
+static std::atomic<int> g_batch_order{0};
+int batch_task_order() { return g_batch_order.load(); }
 // Residual of the refined solve: 1 = int8 modular products (Ozaki scheme II)
 // where the shape allows, 0 = FP64 MFMA GEMM.
 static std::atomic<int> g_residual{1};
@@ -1178,6 +1180,10 @@ int32_t nla_set_config(int32_t key, int32_t value) {
     if (key == NLA_CFG_LU_SPIN_LIMIT) {
         if (value < 0) return -1;
         return g_lu_spin_limit.exchange(value);
+    }
+    if (key == NLA_CFG_BATCH_ORDER) {
+        if (value < 0) return -1;
+        return g_batch_order.exchange(value);
     }
     if (key == NLA_CFG_OZ_VARIANT) {
         if (value < 0 || value > 3) return -1;

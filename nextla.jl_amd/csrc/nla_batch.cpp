@@ -114,29 +114,57 @@ namespace nla {
 
 static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// The host side of a plan: descriptors, tasks and the launch partition
+// (grid sizes come later, from the device's occupancy).  Besides what the
+// device reads, it records for every panel and GEMM the item and rows it
+// belongs to, in the canonical coordinates (rows: the order of op(A); for
+// side 'R' the transposed system), for nla_batch_plan_describe.
 template <typename T>
-static int32_t build(char side, char uplo, char trans, char func, const nla_batch_item *items, int64_t count,
-                     nla_batch_plan_s *P, hipStream_t stream) {
-    const Layout &L = P->L;
-    const bool trmm = (func == 'M');
-    std::vector<BatchSys<T>> sys((size_t)count), pan;
+struct HostPlan {
+    std::vector<BatchSys<T>> sys, pan;
     std::vector<int> pan_stage;
+    std::vector<int64_t> pan_item, pan_r0;
     std::vector<BatchTask> ptasks;
     std::vector<GroupGemm<T>> gem;
     std::vector<int> gem_stage;
-    std::vector<int2> blk;
+    std::vector<int64_t> gem_item, gem_r0, gem_rn, gem_k0, gem_kn;
+    std::vector<int2> gtasks, blk;
+    std::vector<Launch> launches;
+    bool tx = false, ty = false;
+};
+
+// Fisher-Yates with a 64-bit LCG: the same permutation for the same seed.
+template <typename V>
+static void shuffle_range(std::vector<V> &v, size_t start, size_t count, uint64_t &state) {
+    for (size_t k = count; k > 1; --k) {
+        state = state * 6364136223846793005ULL + 1442695040888963407ULL;
+        const size_t j = (size_t)((state >> 33) % k);
+        std::swap(v[start + k - 1], v[start + j]);
+    }
+}
+
+template <typename T>
+static int32_t plan_host(char side, char uplo, char trans, char func, const nla_batch_item *items, int64_t count,
+                         const Layout &L, uintptr_t ws, int order, HostPlan<T> &H) {
+    const bool trmm = (func == 'M');
+    std::vector<BatchSys<T>> &sys = H.sys, &pan = H.pan;
+    std::vector<int> &pan_stage = H.pan_stage, &gem_stage = H.gem_stage;
+    std::vector<BatchTask> &ptasks = H.ptasks;
+    std::vector<GroupGemm<T>> &gem = H.gem;
+    std::vector<int2> &blk = H.blk;
+    sys.assign((size_t)count, BatchSys<T>{});
     pan.reserve((size_t)L.npan);
     ptasks.reserve((size_t)L.nptasks);
     gem.reserve((size_t)L.ngem);
     blk.reserve((size_t)L.nblk);
-    T *inv = (T *)(P->ws + L.inv);
-    int *flg = (int *)(P->ws + L.flg);
+    T *inv = reinterpret_cast<T *>(ws + (uintptr_t)L.inv);
+    int *flg = reinterpret_cast<int *>(ws + (uintptr_t)L.flg);
     int64_t inv_off = 0;
     const bool tr = (trans != 'N');
     const char ue = tr ? (uplo == 'L' ? 'U' : 'L') : uplo;  // uplo of op(A)
     // grouped GEMM orientation (see the descriptors below)
-    P->tx = (side == 'L') ? tr : false;
-    P->ty = (side == 'L') ? false : tr;
+    H.tx = (side == 'L') ? tr : false;
+    H.ty = (side == 'L') ? false : tr;
     for (int64_t i = 0; i < count; ++i) {
         const nla_batch_item &it = items[i];
         BatchSys<T> &S = sys[(size_t)i];
@@ -179,6 +207,8 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             const int pi = (int)pan.size();
             pan.push_back(S);
             pan_stage.push_back(0);
+            H.pan_item.push_back(i);
+            H.pan_r0.push_back(0);
             for (int64_t st = 0; st < strips; ++st) ptasks.push_back(BatchTask{pi, (int)st});
             continue;
         }
@@ -201,6 +231,8 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             const int pi = (int)pan.size();
             pan.push_back(D);
             pan_stage.push_back(s);
+            H.pan_item.push_back(i);
+            H.pan_r0.push_back(r0);
             for (int64_t st = 0; st < strips; ++st) ptasks.push_back(BatchTask{pi, (int)st});
             // the GEMM of the stage: rows R of the output, columns K of M
             int64_t R0, RN, K0, KN;
@@ -236,13 +268,13 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             const T *Mblk = S.M + R0 * S.mrs + K0 * S.mcs;  // op(M)(r, k), r in R, k in K
             if (side == 'L') {  // B(R, :) = gb*B(R, :) + ga * M(R, K) * B(K, :)
                 g.M = RN; g.N = it.m; g.K = KN;
-                g.X = Mblk; g.ldx = P->tx ? S.mrs : S.mcs;
+                g.X = Mblk; g.ldx = H.tx ? S.mrs : S.mcs;
                 g.Y = S.B + K0; g.ldy = S.bcs;
                 g.C = S.B + R0; g.ldc = S.bcs;
             } else {  // transposed: Bt(:, R) = gb*Bt(:, R) + ga * Bt(:, K) * M(R, K)^T
                 g.M = it.m; g.N = RN; g.K = KN;
                 g.X = S.B + K0 * S.brs; g.ldx = S.brs;
-                g.Y = Mblk; g.ldy = P->ty ? S.mcs : S.mrs;
+                g.Y = Mblk; g.ldy = H.ty ? S.mcs : S.mrs;
                 g.C = S.B + R0 * S.brs; g.ldc = S.brs;
             }
             g.alpha = ga;
@@ -250,6 +282,11 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             g.tiles_m = (int)((g.M + GROUP_TILE - 1) / GROUP_TILE);
             gem.push_back(g);
             gem_stage.push_back(s);
+            H.gem_item.push_back(i);
+            H.gem_r0.push_back(R0);
+            H.gem_rn.push_back(RN);
+            H.gem_k0.push_back(K0);
+            H.gem_kn.push_back(KN);
         }
     }
     // Panel tasks: by stage, then size class (largest first), then cost.
@@ -275,7 +312,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
         if (gem_stage[(size_t)a] != gem_stage[(size_t)b]) return gem_stage[(size_t)a] < gem_stage[(size_t)b];
         return gem[(size_t)a].K > gem[(size_t)b].K;
     });
-    std::vector<int2> gtasks;
+    std::vector<int2> &gtasks = H.gtasks;
     gtasks.reserve((size_t)L.ngtasks);
     for (int gi : gorder) {
         const GroupGemm<T> &g = gem[(size_t)gi];
@@ -286,17 +323,8 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
         (int64_t)gtasks.size() > L.ngtasks || (int64_t)blk.size() > L.nblk)
         return report_hip(hipErrorInvalidValue, "batch plan sizes");
 
-    // launches, in execution order, with persistent grids from occupancy
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return report_hip(e, "batch occupancy");
-    auto grid = [&](const void *k, int count_) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-        return std::max(1, std::min(cus * per_cu, count_));
-    };
-    P->launches.clear();
+    // launches, in execution order (grids sized later)
+    H.launches.clear();
     size_t pt = 0, gt = 0;
     for (int s = 0; s < (int)L.nstages; ++s) {
         for (int c = 3; c >= 0; --c) {
@@ -304,10 +332,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             while (pt < ptasks.size() && pan_stage[(size_t)ptasks[pt].sys] == s &&
                    cls_of(pan[(size_t)ptasks[pt].sys].n) == c)
                 ++pt;
-            if (pt > start)
-                P->launches.push_back(Launch{s, c, (int)start, (int)(pt - start),
-                                             grid(batch_kernel_ptr<T>(trmm, BATCH_CLASSES[c]), (int)(pt - start)),
-                                             false});
+            if (pt > start) H.launches.push_back(Launch{s, c, (int)start, (int)(pt - start), 0, false});
         }
         const size_t start = gt;
         bool vec = true;
@@ -317,10 +342,49 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
             vec = vec && al16(g.X) && al16(g.Y) && g.ldx % V == 0 && g.ldy % V == 0;
             ++gt;
         }
-        if (gt > start)
-            P->launches.push_back(Launch{s, 4, (int)start, (int)(gt - start),
-                                         grid(grouped_gemm_ptr<T>(P->tx, P->ty, vec), (int)(gt - start)), vec});
+        if (gt > start) H.launches.push_back(Launch{s, 4, (int)start, (int)(gt - start), 0, vec});
     }
+    // a perturbed task order within every launch (testing: a task's result
+    // must not depend on which work-group runs it or when)
+    if (order > 0) {
+        uint64_t state = 0x9E3779B97F4A7C15ULL * (uint64_t)order;
+        for (const Launch &l : H.launches) {
+            if (l.cls < 4) shuffle_range(ptasks, (size_t)l.start, (size_t)l.count, state);
+            else shuffle_range(gtasks, (size_t)l.start, (size_t)l.count, state);
+        }
+        shuffle_range(blk, 0, blk.size(), state);
+    }
+    return NLA_OK;
+}
+
+template <typename T>
+static int32_t build(char side, char uplo, char trans, char func, const nla_batch_item *items, int64_t count,
+                     nla_batch_plan_s *P, hipStream_t stream) {
+    const Layout &L = P->L;
+    const bool trmm = (func == 'M');
+    HostPlan<T> H;
+    int32_t st = plan_host<T>(side, uplo, trans, func, items, count, L, (uintptr_t)P->ws, batch_task_order(), H);
+    if (st != NLA_OK) return st;
+    P->tx = H.tx;
+    P->ty = H.ty;
+    const std::vector<BatchSys<T>> &sys = H.sys, &pan = H.pan;
+    const std::vector<BatchTask> &ptasks = H.ptasks;
+    const std::vector<GroupGemm<T>> &gem = H.gem;
+    const std::vector<int2> &gtasks = H.gtasks, &blk = H.blk;
+    // persistent grids from occupancy
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return report_hip(e, "batch occupancy");
+    auto grid = [&](const void *k, int count_) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+        return std::max(1, std::min(cus * per_cu, count_));
+    };
+    P->launches = H.launches;
+    for (Launch &l : P->launches)
+        l.groups = l.cls < 4 ? grid(batch_kernel_ptr<T>(trmm, BATCH_CLASSES[l.cls]), l.count)
+                             : grid(grouped_gemm_ptr<T>(P->tx, P->ty, l.vec), l.count);
     // one staged image of the descriptor region, copied in stream order
     P->host.assign((size_t)(L.inv - L.sys), 0);
     char *h = P->host.data() - L.sys;
@@ -409,6 +473,83 @@ int32_t nla_batch_plan_create(char side, char uplo, char trans, char func, int32
     }
     *plan = P;
     return NLA_OK;
+}
+
+int64_t nla_batch_plan_describe(char side, char uplo, char trans, char func, int32_t dtype, const nla_batch_item *items,
+                                int64_t count, nla_batch_task_desc *out, int64_t capacity) {
+    side = upc(side); uplo = upc(uplo); trans = upc(trans); func = upc(func);
+    if (side != 'L' && side != 'R') return report_bad_arg(1, "side");
+    if (uplo != 'L' && uplo != 'U') return report_bad_arg(2, "uplo");
+    if (trans != 'N' && trans != 'T' && trans != 'C') return report_bad_arg(3, "trans");
+    if (func != 'S' && func != 'M') return report_bad_arg(4, "func");
+    if (dtype != NLA_F32 && dtype != NLA_F64) return report_bad_arg(5, "dtype");
+    if (count < 0 || (count > 0 && items == nullptr)) return report_bad_arg(6, "items");
+    if (count > (int64_t)1 << 30) return report_bad_arg(7, "count");
+    for (int64_t i = 0; i < count; ++i)
+        if (items[i].n < 0 || items[i].n > BATCH_N_MAX || items[i].m < 0)
+            return report_bad_arg(6, "items (n must be in [0, 65536])");
+    if (capacity < 0 || (capacity > 0 && out == nullptr)) return report_bad_arg(8, "out");
+    const Layout L = layout_of(func, dtype, items, count, batch_class_mask());
+    auto describe = [&](auto tag) -> int64_t {
+        typedef decltype(tag) T;
+        HostPlan<T> H;
+        const int32_t st = plan_host<T>(side, uplo, trans, func, items, count, L, 0, batch_task_order(), H);
+        if (st != NLA_OK) return st > 0 ? -(int64_t)st : (int64_t)st;
+        int64_t k = 0;
+        auto put = [&](const nla_batch_task_desc &d) {
+            if (k < capacity) out[k] = d;
+            ++k;
+        };
+        const int first = func == 'S' ? 1 : 0;
+        for (const int2 &b : H.blk) {
+            const int64_t n = items[b.x].n, r0 = (int64_t)b.y * 64;
+            put(nla_batch_task_desc{0, 0, b.x, r0, std::min<int64_t>(64, n - r0), 0, 0, 0, 0, 0.0, 0.0});
+        }
+        for (size_t li = 0; li < H.launches.size(); ++li) {
+            const Launch &l = H.launches[li];
+            for (int t = l.start; t < l.start + l.count; ++t) {
+                nla_batch_task_desc d{};
+                d.launch = (int32_t)li + first;
+                if (l.cls < 4) {
+                    const BatchTask &bt = H.ptasks[(size_t)t];
+                    const BatchSys<T> &P = H.pan[(size_t)bt.sys];
+                    d.kind = 1;
+                    d.system = H.pan_item[(size_t)bt.sys];
+                    d.r0 = H.pan_r0[(size_t)bt.sys];
+                    d.rows = P.n;
+                    d.c0 = (int64_t)bt.strip * 16;
+                    d.cols = std::min<int64_t>(16, P.m - d.c0);
+                    d.alpha = (double)P.scale;
+                } else {
+                    const int2 &gt = H.gtasks[(size_t)t];
+                    const GroupGemm<T> &g = H.gem[(size_t)gt.x];
+                    const int64_t tm = gt.y % g.tiles_m, tn = gt.y / g.tiles_m;
+                    const int64_t a0 = tm * GROUP_TILE, b0 = tn * GROUP_TILE;
+                    const int64_t an = std::min<int64_t>(GROUP_TILE, g.M - a0), bn = std::min<int64_t>(GROUP_TILE, g.N - b0);
+                    d.kind = 2;
+                    d.system = H.gem_item[(size_t)gt.x];
+                    if (side == 'L') {  // tile rows along R, columns along the RHS
+                        d.r0 = H.gem_r0[(size_t)gt.x] + a0;
+                        d.rows = an;
+                        d.c0 = b0;
+                        d.cols = bn;
+                    } else {  // transposed: tile rows along the RHS, columns along R
+                        d.c0 = a0;
+                        d.cols = an;
+                        d.r0 = H.gem_r0[(size_t)gt.x] + b0;
+                        d.rows = bn;
+                    }
+                    d.k0 = H.gem_k0[(size_t)gt.x];
+                    d.kn = H.gem_kn[(size_t)gt.x];
+                    d.alpha = (double)g.alpha;
+                    d.beta = (double)g.beta;
+                }
+                put(d);
+            }
+        }
+        return k;
+    };
+    return dtype == NLA_F64 ? describe(double{}) : describe(float{});
 }
 
 int32_t nla_batch_plan_execute(nla_batch_plan plan, void *stream, int32_t use_graph) {

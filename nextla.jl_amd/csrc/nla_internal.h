@@ -154,6 +154,9 @@ constexpr int BATCH_CLASSES[4] = {128, 256, 512, 1024};
 // Classes a new batch plan may use: bit c enables BATCH_CLASSES[c] (the
 // largest is always used); set with nla_set_config(NLA_CFG_BATCH_CLASSES).
 int batch_class_mask();
+// Task order of new batch plans: 0 longest first; k > 0 a pseudo-random order
+// within every launch from seed k (nla_set_config(NLA_CFG_BATCH_ORDER)).
+int batch_task_order();
 int lu_split_rows();
 // Polls of a split LU leaf's bounded grid-barrier wait before it gives up
 // and leaves the block to the recovery launch (0 = the default, about a

@@ -28,6 +28,13 @@ class BatchItem(ctypes.Structure):
 NLA_MIXED_MAX_ITER = 30
 
 
+class BatchTaskDesc(ctypes.Structure):
+    """nla_batch_task_desc of include/nla.h."""
+    _fields_ = [("launch", ctypes.c_int32), ("kind", ctypes.c_int32), ("system", ctypes.c_int64),
+                ("r0", ctypes.c_int64), ("rows", ctypes.c_int64), ("c0", ctypes.c_int64), ("cols", ctypes.c_int64),
+                ("k0", ctypes.c_int64), ("kn", ctypes.c_int64), ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+
 class ServeRequest(ctypes.Structure):
     """nla_serve_request of include/nla.h."""
     _fields_ = [("side", ctypes.c_char), ("uplo", ctypes.c_char), ("trans", ctypes.c_char), ("func", ctypes.c_char),
@@ -83,6 +90,7 @@ SIGNATURES = {
     "nla_batch_mixed_plan_execute": (_i32, [_vp, _dbl, _i32, _vp, _i32, ctypes.POINTER(_dbl),
                                             ctypes.POINTER(_i32)]),
     "nla_batch_mixed_plan_destroy": (_i32, [_vp]),
+    "nla_batch_plan_describe": (_i64, [_c_char, _c_char, _c_char, _c_char, _i32, _vp, _i64, _vp, _i64]),
     "nla_serve_create": (_i32, [_i64, _i32, _i32, _i32, _i32, _i32, _vp]),
     "nla_serve_set_refinement": (_i32, [_vp, _dbl, _i32, _i32]),
     "nla_serve_submit": (_i32, [_vp, _vp, _vp, _vp]),
@@ -137,6 +145,7 @@ NLA_CFG_LU_SPLIT_ROWS = 8
 NLA_CFG_RESIDUAL = 9
 NLA_CFG_LU_SPIN_LIMIT = 10
 NLA_CFG_OZ_VARIANT = 11
+NLA_CFG_BATCH_ORDER = 12
 NLA_RES_FP64 = 0
 NLA_RES_INT8 = 1
 

@@ -222,3 +222,59 @@ def test_staged_above_512_option(nla, dt):
                 assert torch.equal(alone[0], Bs[2]), func
     finally:
         _lib.set_config(_lib.NLA_CFG_BATCH_CLASSES, old)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("side,uplo,trans,func", [("L", "L", "N", "S"), ("R", "U", "T", "M"), ("L", "U", "T", "S"),
+                                                  ("R", "L", "N", "S")])
+def test_perturbed_task_order_bitwise(nla, dt, side, uplo, trans, func):
+    """Plans whose tasks run in pseudo-random orders within each launch
+    (NLA_CFG_BATCH_ORDER; the host replay in tests/test_batch_dag.py checks
+    the same plans for races) give the same bits as the default order: no
+    task depends on which work-group runs it or when."""
+    from nextla_amd import _lib, batch
+    shapes = [(16, 3), (128, 20), (300, 40), (513, 40), (700, 70), (1300, 33), (64, 100)]
+    out = []
+    for order in (0, 3, 11):
+        rng = np.random.default_rng(17)
+        As, Bs = [], []
+        for n, m in shapes:
+            A, B = make_system(rng, n, m, side, uplo, dt)
+            As.append(dev(A))
+            Bs.append(dev(B))
+        old = _lib.set_config(_lib.NLA_CFG_BATCH_ORDER, order)
+        try:
+            batch.batched_rectrxm_(side, uplo, trans, func, As, Bs, alphas=-0.5)
+        finally:
+            _lib.set_config(_lib.NLA_CFG_BATCH_ORDER, old)
+        torch.cuda.synchronize()
+        out.append([b.cpu().numpy() for b in Bs])
+    for got in out[1:]:
+        for r, g in zip(out[0], got):
+            assert np.array_equal(r, g)
+
+
+def test_launch_blocking_bitwise(nla, tmp_path):
+    """The same work with every launch serialized (HIP_LAUNCH_BLOCKING=1, in
+    a child process: the variable is read when HIP starts) gives the same
+    bits: the plain recursion, a batch, the refined solve and the int8
+    residual do not depend on launches overlapping."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    child = os.path.join(here, "launch_blocking_child.py")
+    env = dict(os.environ, HIP_LAUNCH_BLOCKING="1")
+    blocked = tmp_path / "blocked.npz"
+    r = subprocess.run([sys.executable, child, str(blocked)], env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    here_out = tmp_path / "async.npz"
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("launch_blocking_child", child)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    mod.run(str(here_out))
+    a, b = np.load(blocked), np.load(here_out)
+    assert sorted(a.files) == sorted(b.files) and len(a.files) >= 4
+    for k in a.files:
+        assert np.array_equal(a[k], b[k]), k
