@@ -48,10 +48,11 @@ __device__ __forceinline__ int oz_mod53(double v, double p, double invp) {
     return r;
 }
 
-// x mod p in [0, p) for 0 <= x < 2^24 (exact in float); invp = 1/p.
+// x mod p in [0, p) for 0 <= x < 2^24 (exact in float); invp = 1/p.  The
+// quotient is below 2^17 and p below 2^9: a 24-bit multiply (full rate).
 __device__ __forceinline__ int oz_mod24(int x, int p, float invp) {
     const int q = (int)((float)x * invp);
-    int r = x - q * p;
+    int r = x - (int)__umul24((unsigned)q, (unsigned)p);
     r += (r < 0) ? p : 0;
     r -= (r >= p) ? p : 0;
     return r;
@@ -71,7 +72,7 @@ __device__ __forceinline__ void oz_split(double v, double &m, int &sh) {
 __device__ __forceinline__ int oz_residue(double m, int sh, const OzConst &C, int t, const uint8_t *pw) {
     const int r = oz_mod53(m, (double)C.p[t], C.invpd[t]);
     if (sh == 0) return r;
-    return oz_mod24(r * (int)pw[sh], C.p[t], C.invpf[t]);
+    return oz_mod24((int)__umul24((unsigned)r, (unsigned)pw[sh]), C.p[t], C.invpf[t]);
 }
 
 // ---- tables -----------------------------------------------------------------
@@ -439,43 +440,64 @@ struct OzCrtArgs {
     int *flag;
 };
 
+// Four consecutive rows per thread: one 32-bit load gives their residues
+// of one plane, and every plane's load is issued before the first is used.
+// The planes are padded to multiples of 256 rows, so the 4-row word is
+// always inside the plane; rows past `rows` are skipped.
 __global__ __launch_bounds__(256) void oz_crt_kernel(OzCrtArgs a, OzConst C) {
     __shared__ uint8_t pw[OZ_TMAX * OZ_PW];
-    for (int i = threadIdx.x; i < C.T * OZ_PW; i += 256) pw[i] = a.pw[i];
+    for (int i = threadIdx.x; i < C.T * OZ_PW / 4; i += 256)
+        reinterpret_cast<uint32_t *>(pw)[i] = reinterpret_cast<const uint32_t *>(a.pw)[i];
     __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (r >= a.rows) return;
-    const int er = a.er[r];
+    const int64_t r0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    if (r0 >= a.rows) return;
+    int er[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) er[j] = (r0 + j < a.rows) ? a.er[r0 + j] : 0;
     for (int64_t c = blockIdx.y; c < a.cols; c += gridDim.y) {
-        const int sc = er + a.ec[c];
-        const double v = rint(ldexp(a.S[r + c * a.lds], sc));
-        double m = 0.0;
-        int sh = 0;
-        bool bad = !(fabs(v) <= C.smax);
-        if (!bad) {
-            oz_split(v, m, sh);
-            bad = sh >= OZ_PW;
+        const uint8_t *cp = a.Cr + c * a.ldc + r0;
+        uint32_t cw[OZ_TMAX];
+#pragma unroll
+        for (int t = 0; t < OZ_TMAX; ++t)
+            cw[t] = (t < C.T) ? *reinterpret_cast<const uint32_t *>(cp + t * a.cplane) : 0u;
+        const int ecc = a.ec[c];
+        double sv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sv[j] = (r0 + j < a.rows) ? a.S[r0 + j + c * a.lds] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (r0 + j >= a.rows) break;
+            const int sc = er[j] + ecc;
+            const double v = rint(ldexp(sv[j], sc));
+            double m = 0.0;
+            int sh = 0;
+            bool bad = !(fabs(v) <= C.smax);
+            if (!bad) {
+                oz_split(v, m, sh);
+                bad = sh >= OZ_PW;
+            }
+            if (bad) {
+                *a.flag = 1;
+                a.R[r0 + j + c * a.ldr] = 0.0;
+                continue;
+            }
+            double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+#pragma unroll
+            for (int t = 0; t < OZ_TMAX; ++t) {
+                if (t >= C.T) break;
+                const int p = C.p[t];
+                const int st = oz_residue(m, sh, C, t, pw + t * OZ_PW);
+                const int ct = (int)((cw[t] >> (8 * j)) & 255u);
+                const int d = st - ct + p;  // in (0, 2p)
+                const double u = (double)oz_mod24((int)__umul24((unsigned)d, (unsigned)C.y[t]), p, C.invpf[t]);
+                f1 = fma(u, C.c1[t], f1);
+                f2 = fma(u, C.c2[t], f2);
+                f3 = fma(u, C.c3[t], f3);
+            }
+            const double q = rint(f1 + f2);
+            const double frac = ((f1 - q) + f2) + f3;
+            a.R[r0 + j + c * a.ldr] = ldexp(frac * C.M, -sc);
         }
-        if (bad) {
-            *a.flag = 1;
-            a.R[r + c * a.ldr] = 0.0;
-            continue;
-        }
-        double f1 = 0.0, f2 = 0.0, f3 = 0.0;
-        const uint8_t *cp = a.Cr + c * a.ldc + r;
-        for (int t = 0; t < C.T; ++t) {
-            const int p = C.p[t];
-            const int st = oz_residue(m, sh, C, t, pw + t * OZ_PW);
-            const int ct = cp[t * a.cplane];
-            const int d = st - ct + p;  // in (0, 2p)
-            const double u = (double)oz_mod24(d * C.y[t], p, C.invpf[t]);
-            f1 = fma(u, C.c1[t], f1);
-            f2 = fma(u, C.c2[t], f2);
-            f3 = fma(u, C.c3[t], f3);
-        }
-        const double q = rint(f1 + f2);
-        const double frac = ((f1 - q) + f2) + f3;
-        a.R[r + c * a.ldr] = ldexp(frac * C.M, -sc);
     }
 }
 
@@ -707,7 +729,7 @@ hipError_t oz_reconstruct(const OzPlan &P, const OzLayout &L, char *ws, bool out
     c.pw = (const uint8_t *)(ws + L.pw);
     c.flag = (int *)(ws + L.flag);
     const unsigned gy = (unsigned)(c.cols < 65535 ? c.cols : 65535);
-    hipLaunchKernelGGL(oz_crt_kernel, dim3((unsigned)((c.rows + 255) / 256), gy), dim3(256), 0, s, c, P.C);
+    hipLaunchKernelGGL(oz_crt_kernel, dim3((unsigned)((c.rows + 1023) / 1024), gy), dim3(256), 0, s, c, P.C);
     return hipGetLastError();
 }
 
