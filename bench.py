@@ -623,9 +623,11 @@ def serve_leg(nla, torch, count=600, seed=0):
 
 def profile_leg(nla, torch, what, reps=3):
     """Short runs for rocprofv3: the int8 residual of the refined solve's
-    workload (n = 16384, m = 4096; what = "residual_int8"), or the plain LL-N
-    TRSM at n = m = 8192 (what = "trsm_f64" / "trsm_f32"); each followed by
-    one 512 MiB device copy, whose FETCH_SIZE / WRITE_SIZE calibrate the
+    workload (n = 16384, m = 4096; what = "residual_int8"), the refined solve
+    with bf16 updates on that workload ("refined_bf16": the bf16 GEMM
+    updates of its correction solves), or the plain LL-N TRSM at
+    n = m = 8192 (what = "trsm_f64" / "trsm_f32"); each followed by one
+    512 MiB device copy, whose FETCH_SIZE / WRITE_SIZE calibrate the
     counters."""
     from nextla_amd import inputs
     out = {"what": what, "reps": reps}
@@ -641,6 +643,17 @@ def profile_leg(nla, torch, what, reps=3):
                             lambda: residual("L", "L", "N", 1.0, A, X, B, R, method="int8", workspace=ws), reps, 1)
         out["ms"] = ts
         del A, B, X, R, ws
+    elif what == "refined_bf16":
+        from nextla_amd.mixed import rectrxm_mixed_, workspace_bytes
+        n, m = 16384, 4096
+        A, B = inputs.device_precision_tri(n, m, 1e4, seed=5)
+        X = B.clone()
+        ws = torch.empty(workspace_bytes("L", n, m) + 256, dtype=torch.uint8, device=A.device)
+        ts, _ = timed_steps(torch, lambda: X.copy_(B),
+                            lambda: rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision="bf16", workspace=ws,
+                                                   raise_on_stall=False), reps, 1)
+        out["ms"] = ts
+        del A, B, X, ws
     else:
         dt = torch.float64 if what == "trsm_f64" else torch.float32
         _, _, _, ts, _ = solve_rate(nla, torch, 8192, 8192, "L", "L", "N", "S", 1.5, dt, reps, 1)
@@ -698,7 +711,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip every leg but the headline")
     ap.add_argument("--leg", choices=("refined_mixed", "drop_in", "cpu_case", "batched_f64", "batched_f32", "served",
-                                      "lu", "lauum", "larfb", "parfb", "residual_int8", "trsm_f64", "trsm_f32"),
+                                      "lu", "lauum", "larfb", "parfb", "residual_int8", "refined_bf16", "trsm_f64",
+                                      "trsm_f32"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -727,7 +741,7 @@ def main():
             res = larfb_leg(nla, torch)
         elif args.leg == "parfb":
             res = parfb_leg(nla, torch)
-        elif args.leg in ("residual_int8", "trsm_f64", "trsm_f32"):
+        elif args.leg in ("residual_int8", "refined_bf16", "trsm_f64", "trsm_f32"):
             res = profile_leg(nla, torch, args.leg)
         else:
             res = serve_leg(nla, torch)
