@@ -61,65 +61,67 @@ __device__ inline float block_scale(unsigned bits) {
     return ldexpf(1.0f, k);
 }
 
-// Stages a BO x BK block of an fp32 operand into LDS as rows of k (one row
-// per outer index, row stride BK + 8 elements) in the low-precision type.
-// Each thread moves chunks of 4 outer x 4 k elements: four 16-byte loads
-// along the operand's contiguous dimension, four 8-byte LDS stores along k.
+// Stages a 128 x 32 block of an fp32 operand into LDS as 128 rows of 32 k
+// (64 bytes) in the low-precision type.  Row r sits in row slot
+// r ^ ((r >> 2) & 3) (the low two bits flipped by the next two), and its
+// 16-byte chunk c at chunk c ^ (2 * (bit 2 ^ bit 4 of r)) (lp_lds_off):
+//   fragment reads (16 rows x one chunk per 16 lanes, 8 rows x 2 chunks per
+//     b128 phase) meet 16 distinct 16-byte bank groups;
+//   each 32-lane half of a tile store meets 32 distinct 8-byte bank pairs.
+// Each thread moves 4 rows x 4 k (one 8-byte quad per row):
+//   k-contiguous operand: quad t % 8 of rows t/8 + 32 v (v = 0..3), four
+//     16-byte loads along k (8 lanes cover 128 contiguous bytes of a row);
+//   outer-contiguous operand: rows 4 g .. 4 g + 3 (g = 8 (t / 64) + t % 8),
+//     quad (t / 8) % 8, four 16-byte loads along the rows (8 lanes cover
+//     128 contiguous bytes of a column).
+__device__ __forceinline__ int lp_lds_off(int row, int k) {  // element offset, k a multiple of 4
+    const int slot = row ^ ((row >> 2) & 3);
+    const int chunk = (k >> 3) ^ (((row >> 1) ^ (row >> 3)) & 2);
+    return slot * 32 + ((chunk << 3) | (k & 7));
+}
+
 template <int LP, int BO, int BK, int NT, bool KCONTIG>
 struct LpLoader {
     typedef typename LpType<LP>::elem_t E;
     typedef typename LpType<LP>::quad_t Q;
-    static constexpr int CHUNKS = (BO / 4) * (BK / 4);
-    static constexpr int CH = CHUNKS / NT;
-    static_assert(CHUNKS % NT == 0, "tile does not divide among threads");
-    static_assert(CH * 16 <= 32, "bounds bits do not fit");
-    static constexpr int LDK = BK + 8;
+    static_assert(BO == 128 && BK == 32 && NT == 256, "the LDS layout is laid out for 128 x 32 tiles, 256 threads");
+    static constexpr int LDK = BK;
     static constexpr int LDS_ELEMS = BO * LDK;
 
-    float r[CH][4][4];  // [chunk][outer offset][k offset]
+    float r[4][4];  // [row offset][k offset]
     uint32_t okbits;
 
-    __device__ static inline void coords(int t, int c, int &o, int &k) {
-        int q = t + c * NT;
-        if (KCONTIG) {  // neighbouring threads take neighbouring k
-            k = (q % (BK / 4)) * 4;
-            o = (q / (BK / 4)) * 4;
-        } else {  // neighbouring threads take neighbouring outer indices
-            o = (q % (BO / 4)) * 4;
-            k = (q / (BO / 4)) * 4;
-        }
-    }
+    __device__ static inline int g_of(int t) { return KCONTIG ? t / 8 : 8 * (t / 64) + (t % 8); }
+    __device__ static inline int q_of(int t) { return KCONTIG ? t % 8 : (t / 8) % 8; }
+    // tile row of the thread's v-th row (r[v])
+    __device__ static inline int row_of(int t, int v) { return KCONTIG ? g_of(t) + 32 * v : 4 * g_of(t) + v; }
 
     // Element (outer, k) at P[outer*so + k*sk]; (KCONTIG ? sk : so) == 1.
     template <bool CHECK>
     __device__ inline void load(const float *__restrict__ P, int64_t so, int64_t sk, int64_t o0, int64_t k0,
                                 int64_t no, int64_t nk, int t) {
         if (CHECK) okbits = 0;
+        const int k = 4 * q_of(t);
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            int o, k;
-            coords(t, c, o, k);
+        for (int v = 0; v < 4; ++v) {
+            // KCONTIG: row v, 4 contiguous k; else: k + v, 4 contiguous rows
+            const int64_t go = o0 + (KCONTIG ? row_of(t, v) : row_of(t, 0)), gk = k0 + k + (KCONTIG ? 0 : v);
+            if (!CHECK) {
+                f4_t x = *reinterpret_cast<const f4_t *>(P + go * so + gk * sk);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                // v walks the non-contiguous dimension; 4 contiguous elements per load
-                int64_t go = o0 + o + (KCONTIG ? v : 0), gk = k0 + k + (KCONTIG ? 0 : v);
-                if (!CHECK) {
-                    f4_t x = *reinterpret_cast<const f4_t *>(P + go * so + gk * sk);
+                for (int e = 0; e < 4; ++e) {
+                    if (KCONTIG) r[v][e] = x[e];
+                    else r[e][v] = x[e];
+                }
+            } else {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        if (KCONTIG) r[c][v][e] = x[e];
-                        else r[c][e][v] = x[e];
-                    }
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        int64_t eo = KCONTIG ? go : go + e, ek = KCONTIG ? gk + e : gk;
-                        bool ok = (eo < no) && (ek < nk);
-                        float x = *(ok ? P + eo * so + ek * sk : P);
-                        if (KCONTIG) r[c][v][e] = x;
-                        else r[c][e][v] = x;
-                        okbits |= (ok ? 1u : 0u) << (c * 16 + v * 4 + e);
-                    }
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t eo = KCONTIG ? go : go + e, ek = KCONTIG ? gk + e : gk;
+                    const bool ok = (eo < no) && (ek < nk);
+                    const float x = *(ok ? P + eo * so + ek * sk : P);
+                    if (KCONTIG) r[v][e] = x;
+                    else r[e][v] = x;
+                    okbits |= (ok ? 1u : 0u) << (v * 4 + e);
                 }
             }
         }
@@ -128,29 +130,23 @@ struct LpLoader {
     // Zero the out-of-bounds elements of a CHECK load (bit order of load()).
     __device__ inline void apply_bounds() {
 #pragma unroll
-        for (int c = 0; c < CH; ++c)
+        for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (!((okbits >> (c * 16 + v * 4 + e)) & 1u)) {
-                        if (KCONTIG) r[c][v][e] = 0.0f;
-                        else r[c][e][v] = 0.0f;
-                    }
+            for (int e = 0; e < 4; ++e)
+                if (!((okbits >> (v * 4 + e)) & 1u)) {
+                    if (KCONTIG) r[v][e] = 0.0f;
+                    else r[e][v] = 0.0f;
+                }
     }
 
     __device__ inline void store(NLA_LDS E *lds, int t, float sc) const {
+        const int k = 4 * q_of(t);
 #pragma unroll
-        for (int c = 0; c < CH; ++c) {
-            int o, k;
-            coords(t, c, o, k);
+        for (int v = 0; v < 4; ++v) {
+            Q q;
 #pragma unroll
-            for (int oo = 0; oo < 4; ++oo) {
-                Q q;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) q[kk] = (E)(r[c][oo][kk] * sc);
-                *reinterpret_cast<NLA_LDS Q *>(lds + (o + oo) * LDK + k) = q;
-            }
+            for (int kk = 0; kk < 4; ++kk) q[kk] = (E)(r[v][kk] * sc);
+            *reinterpret_cast<NLA_LDS Q *>(lds + lp_lds_off(row_of(t, v), k)) = q;
         }
     }
 };
@@ -219,10 +215,10 @@ __device__ __forceinline__ void lp_tile(int64_t M, int64_t N, int64_t K, float a
             OP a[CB], b[RB];
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb)
-                a[cb] = *reinterpret_cast<const NLA_LDS OP *>(sy_ + (wn * TWN + cb * 16 + lo) * LY::LDK + ks + lk);
+                a[cb] = *reinterpret_cast<const NLA_LDS OP *>(sy_ + lp_lds_off(wn * TWN + cb * 16 + lo, ks + lk));
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
-                b[rb] = *reinterpret_cast<const NLA_LDS OP *>(sx_ + (wm * TWM + rb * 16 + lo) * LX::LDK + ks + lk);
+                b[rb] = *reinterpret_cast<const NLA_LDS OP *>(sx_ + lp_lds_off(wm * TWM + rb * 16 + lo, ks + lk));
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
