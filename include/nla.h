@@ -586,11 +586,11 @@ int32_t nla_serve_destroy(nla_server server);
  *                      it by the one-work-group leaf, so nla_getrf2's result
  *                      does not depend on it; a small value forces that path.
  *   NLA_CFG_OZ_VARIANT: schedule of the int8 residual's modular products
- *                      (0, default: two k steps of global prefetch, per-XCD
- *                      counters over super-tiles of 4 x 8 tiles; 1: one step
- *                      of prefetch, tiles in row order from one counter; 2:
- *                      two steps, row order; 3: two steps, 8 x 4
- *                      super-tiles); results are identical, only the speed
+ *                      (0, default: per-XCD counters over super-tiles of
+ *                      4 x 8 tiles, a 4-stage LDS ring of operand copies;
+ *                      1: tiles in row order from one counter, 4 stages;
+ *                      2: row order, 3 stages; 3: 8 x 4 super-tiles, 4
+ *                      stages); results are identical, only the speed
  *                      differs.
  *   NLA_CFG_BATCH_ORDER: order of the tasks within every launch of new batch
  *                      plans (0, default: longest first for load balance;

@@ -305,8 +305,7 @@ struct OzConst {
     double smax;  // largest |S_int| the moduli cover (M / 8)
 };
 
-// Global prefetch depth of the modular-product kernel: 0 = two k steps
-// (default), 1 = one.
+// Schedule of the modular-product kernel (NLA_CFG_OZ_VARIANT, 0..3).
 int oz_variant();
 
 // Moduli count and bit split for an order-n residual, and the residue planes'

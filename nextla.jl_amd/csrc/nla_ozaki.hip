@@ -159,19 +159,40 @@ __global__ void oz_exponent_kernel(int64_t no, const unsigned long long *mx, int
 
 // ---- residues ---------------------------------------------------------------
 
-// Destination of the residue planes: plane t at base + t*plane; element
-// (o, k) at o*ld + k (panel = 0) or in the row-panel layout of a lower
-// (panel = 1) / upper (panel = 2) triangle of order npad.
+// Destination of the residue planes: plane t at base + t*plane.  Both
+// operands of the modular products are stored as the product kernel stages
+// them: in groups of 256 rows (a row panel of the triangle of op(A), or 256
+// systems of X), each group a run of 16 KB blocks, one per 64-wide k step
+// (the triangle's k range for A: k < (tm+1)*256 lower, k >= tm*256 upper;
+// all of npad for X).  Row r of a block is 64 bytes at r*64, its 16-byte
+// chunk c at chunk c ^ 2*bit2(r): the block is copied to LDS as it is, and
+// the fragment reads of a wave (lane l: row l % 16, chunk l / 16) then meet
+// 16 distinct 16-byte bank groups per phase.  panel: 0 = X (groups
+// 256 * npad bytes apart), 1 = lower triangle, 2 = upper triangle.
 struct OzDst {
     int8_t *base;
-    int64_t plane, ld, npad;
+    int64_t plane, npad;
     int panel;
 };
 
-__device__ __forceinline__ int64_t oz_panel_off(int panel, int64_t npad, int64_t o, int64_t k) {
-    const int64_t tm = o >> 8, il = o & 255;
-    if (panel == 1) return (int64_t)65536 * (tm * (tm + 1) / 2) + il * ((tm + 1) * 256) + k;
-    return (int64_t)256 * (tm * npad - 256 * (tm * (tm - 1) / 2)) + il * (npad - tm * 256) + (k - tm * 256);
+// Byte offset of the 16-byte chunk of row o holding k .. k + 15 (k a
+// multiple of 16) in a plane of layout `panel`.
+__device__ __forceinline__ int64_t oz_tiled_off(int panel, int64_t npad, int64_t o, int64_t k) {
+    const int64_t g = o >> 8;
+    const int r = (int)(o & 255);
+    int64_t base, kk;
+    if (panel == 0) {
+        base = g * 256 * npad;
+        kk = k;
+    } else if (panel == 1) {
+        base = (int64_t)65536 * (g * (g + 1) / 2);
+        kk = k;
+    } else {
+        base = (int64_t)256 * (g * npad - 256 * (g * (g - 1) / 2));
+        kk = k - g * 256;
+    }
+    const int c = (int)((kk >> 4) & 3);
+    return base + (kk >> 6) * 16384 + r * 64 + ((c ^ ((r >> 1) & 2)) << 4);
 }
 
 // Whether the 64 x 64 tile (o0, k0) lies in the stored region of a panel
@@ -210,7 +231,7 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, 
         oz_split(v, m[u], sh[u]);
         if (writeback && !bad) tile[oo][kq + u] = ldexp(v, -eo);
     }
-    const int64_t off = d.panel ? oz_panel_off(d.panel, d.npad, o, k0 + kq) : o * d.ld + k0 + kq;
+    const int64_t off = oz_tiled_off(d.panel, d.npad, o, k0 + kq);
     for (int t = 0; t < C.T; ++t) {
         const int p = C.p[t];
         const int half = p >> 1;  // residues above half map to r - p
@@ -245,24 +266,36 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, 
 // Per modulus t and 256 x 256 output tile (tm, tn):
 //   c(i, j) = sum_k Ares_t(i, k) Xres_t(j, k) (int32, exact: |c| < K 2^14),
 // stored as c mod p_t in [0, p_t).  A persistent grid takes (modulus, tile)
-// items from *queue (zero at launch) in modulus-major order, longest row
-// panels first, all column tiles of a row panel together (they share the A
-// panel in L2).  Eight waves, each a 64 x 128 block of the tile as 4 x 8
-// v_mfma_i32_16x16x64_i8 blocks, from operands staged through double-buffered
-// LDS 64 k-bytes at a time.
+// items from the queue counters (zero at launch).  Eight waves, each a
+// 64 x 128 block of the tile as 4 x 8 v_mfma_i32_16x16x64_i8 blocks.
 //
-// LDS rows are 96 bytes (64 of data): gfx950 serves a 16-byte-per-lane LDS
-// read in four phases of 16 lanes (lanes 0-7 with 16-23, 8-15 with 24-31,
-// ...) over 64 banks of 4 bytes (a conflict-free pattern measured at 256
-// bytes per clock, a 2-way one at half that); a fragment read gives lane l
-// row l % 16, chunk l / 16, and with 96-byte rows the 16 lanes of a phase
-// fall in 16 distinct 16-byte bank groups (80-byte rows conflict 2-way).
+// Operands reach LDS without passing through registers: every 64-wide k
+// step of the A panel and of the X panel is a 16 KB block of the residue
+// planes (oz_tiled_off), and each wave copies its 2 KB share of both with
+// four global_load_lds_dwordx4 (64 lanes x 16 bytes into 1 KB of LDS, lane
+// l at byte 16 l).  S stages of 32 KB form a ring: the copies of step
+// kt + S - 1 are issued when step kt starts, so S - 1 steps (96 KB with
+// S = 4) are in flight per CU while one is multiplied.  At the top of step
+// kt each wave waits until its own copies of step kt have landed (the later
+// steps' 4 (S - 2) copies may still be outstanding: vmcnt counts in issue
+// order) and then meets the others at a barrier, after which every copy of
+// step kt is in LDS and every wave has finished reading the stage that the
+// new copies overwrite.  The barrier and the fragment reads are written
+// with their waits in inline assembly: the compiler cannot tell which
+// stage an LDS-bound copy writes, and would otherwise wait for all of them
+// before every read.  The copies past the last step repeat it into stages
+// nobody reads; they land before the next tile (the queue's barrier waits
+// for every outstanding memory operation).
 //
-// PD: steps of global prefetch held in registers.  The step count is a
-// multiple of 4 and the loop body has no branches (loads past the last step
-// re-read it and are never stored), so each step waits only for the loads
-// issued PD - 1 steps before it; with conditional loads the compiler waited
-// for every outstanding load at the top of each step.
+// The planes' layout also keeps each copy instruction on 1 KB of
+// contiguous memory.  Against the earlier kernel (rows of the plane L bytes
+// apart, staged through registers into 96-byte padded LDS rows, two steps
+// in flight) the products of the n = 16384, m = 4096 residual went from
+// 9.0 to 8.2 ms.  What bounds them now, measured with variants of this
+// kernel: the fragment reads (96 KB per CU per step; LDS delivers about
+// 128 bytes per clock) and the MFMAs serialize at the per-step barrier
+// (6.4 ms with no global traffic at all), and the operand copies alone
+// take 5.2 ms.
 //
 // OUT_T: the output plane is (j, i) with j contiguous (side R) instead of
 // (i, j) with i contiguous.
@@ -288,11 +321,19 @@ struct OzGemmArgs {
     int *queue;  // 8 counters
 };
 
-template <bool OUT_T, int PD>
+template <int S>
+__device__ __forceinline__ void oz_stage_barrier() {
+    // this wave's copies of the current step have landed (4 (S - 2) later
+    // ones may be outstanding), then every wave's have
+    static_assert(S == 3 || S == 4, "ring depth");
+    if constexpr (S == 4) asm volatile("s_waitcnt vmcnt(8)\n\ts_barrier" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(4)\n\ts_barrier" ::: "memory");
+}
+
+template <bool OUT_T, int S>
 __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
-    constexpr int BM = 256, BN = 256, BK = 64, RS = 96, TILE = BM * RS;
-    __shared__ __attribute__((aligned(16))) int8_t sA[2 * TILE];
-    __shared__ __attribute__((aligned(16))) int8_t sX[2 * TILE];
+    constexpr int BM = 256, BN = 256, BK = 64, TILE = BM * BK;
+    __shared__ __attribute__((aligned(1024))) int8_t sbuf[S * 2 * TILE];  // stage: A block, X block
     __shared__ int s_item;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 3, wn = wave >> 2;
@@ -302,9 +343,10 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
     const int nbands = a.sr > 0 ? (a.tiles_m + a.sr - 1) / a.sr : 0;
     const int ngroups = a.sr > 0 ? (a.tiles_n + a.sc - 1) / a.sc : 0;
     const int sper = a.sr * a.sc, ssup = nbands * ngroups * a.T;
-    // this thread's chunks c = tid + 512 q of each 256 x 64 operand tile:
-    // row c / 4, 16-byte column c % 4 (a row's four chunks in adjacent lanes)
-    const int srow0 = tid >> 2, sch = tid & 3;
+    // fragment of lane l: row 16 b + l % 16, chunk l / 16 (stored at chunk
+    // (l / 16) ^ 2*bit2(row); bit 2 of the row is bit 2 of l % 16)
+    const int frag_off = fr * BK + ((fc ^ ((fr >> 1) & 2)) << 4);
+    NLA_LDS int8_t *lds = (NLA_LDS int8_t *)sbuf;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(a.queue + (a.sr > 0 ? xcd : 0), 1);
         __syncthreads();
@@ -338,10 +380,19 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
             off = (int64_t)BM * ((int64_t)tm * a.npad - (int64_t)BM * ((int64_t)tm * (tm - 1) / 2));
             kbeg = (int64_t)tm * BM;
         }
-        const int8_t *Ap = a.A + t * a.aplane + off + (int64_t)srow0 * L + sch * 16;
-        const int8_t *Xp = a.X + t * a.xplane + ((int64_t)tn * BN + srow0) * a.npad + kbeg + sch * 16;
-        const int64_t a_q = (int64_t)128 * L, x_q = (int64_t)128 * a.npad;  // rows + 128 for q = 1
-        const int nkt = (int)(L / BK);
+        // this wave's 2 KB of every 16 KB step block (lane l: 16 bytes at 16 l)
+        const int8_t *Ag = a.A + t * a.aplane + off + wave * 2048 + lane * 16;
+        const int8_t *Xg = a.X + t * a.xplane + (int64_t)tn * BN * a.npad + kbeg * 256 + wave * 2048 + lane * 16;
+        const int nkt = (int)(L / BK);  // >= 4: L is a positive multiple of 256
+        auto copy = [&](int step, int stage) {
+            const int64_t k0 = (int64_t)step * TILE;
+            NLA_LDS int8_t *dA = lds + stage * 2 * TILE + wave * 2048;
+            NLA_LDS int8_t *dX = dA + TILE;
+            __builtin_amdgcn_global_load_lds((const void *)(Ag + k0), (NLA_LDS void *)dA, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(Ag + k0 + 1024), (NLA_LDS void *)(dA + 1024), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(Xg + k0), (NLA_LDS void *)dX, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(Xg + k0 + 1024), (NLA_LDS void *)(dX + 1024), 16, 0, 0);
+        };
 
         v4i acc[4][8];
 #pragma unroll
@@ -349,51 +400,40 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
 #pragma unroll
             for (int cb = 0; cb < 8; ++cb) acc[rb][cb] = v4i{0, 0, 0, 0};
 
-        v4i ra[PD][2], rx[PD][2];
-        auto gload = [&](v4i(&xa)[2], v4i(&xx)[2], int64_t k0) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                xa[q] = *reinterpret_cast<const v4i *>(Ap + q * a_q + k0);
-                xx[q] = *reinterpret_cast<const v4i *>(Xp + q * x_q + k0);
-            }
-        };
-        auto sstore = [&](const v4i(&xa)[2], const v4i(&xx)[2], int buf) {
+        for (int p = 0; p < S - 1; ++p) copy(p < nkt ? p : nkt - 1, p);
+        for (int kt = 0; kt < nkt; ++kt) {
+            oz_stage_barrier<S>();
+            const int nxt = kt + S - 1;
+            copy(nxt < nkt ? nxt : nkt - 1, nxt % S);
+            const NLA_LDS int8_t *la = lds + (kt % S) * 2 * TILE + wm * 64 * BK + frag_off;
+            const NLA_LDS int8_t *lx = lds + (kt % S) * 2 * TILE + TILE + wn * 128 * BK + frag_off;
+            v4i af[4], xf[8];
+            asm volatile(
+                "ds_read_b128 %0, %12\n\t"
+                "ds_read_b128 %1, %12 offset:1024\n\t"
+                "ds_read_b128 %2, %12 offset:2048\n\t"
+                "ds_read_b128 %3, %12 offset:3072\n\t"
+                "ds_read_b128 %4, %13\n\t"
+                "ds_read_b128 %5, %13 offset:1024\n\t"
+                "ds_read_b128 %6, %13 offset:2048\n\t"
+                "ds_read_b128 %7, %13 offset:3072\n\t"
+                "ds_read_b128 %8, %13 offset:4096\n\t"
+                "ds_read_b128 %9, %13 offset:5120\n\t"
+                "ds_read_b128 %10, %13 offset:6144\n\t"
+                "ds_read_b128 %11, %13 offset:7168\n\t"
+                "s_waitcnt lgkmcnt(0)"
+                : "=&v"(af[0]), "=&v"(af[1]), "=&v"(af[2]), "=&v"(af[3]), "=&v"(xf[0]), "=&v"(xf[1]),
+                  "=&v"(xf[2]), "=&v"(xf[3]), "=&v"(xf[4]), "=&v"(xf[5]), "=&v"(xf[6]), "=&v"(xf[7])
+                : "v"((unsigned)(uintptr_t)la), "v"((unsigned)(uintptr_t)lx));
+            __builtin_amdgcn_s_setprio(1);
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const int o = buf * TILE + (q * 128 + srow0) * RS + sch * 16;
-                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sA + o) = xa[q];
-                *reinterpret_cast<NLA_LDS v4i *>((NLA_LDS int8_t *)sX + o) = xx[q];
-            }
-        };
+            for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-        for (int p = 0; p < PD; ++p) gload(ra[p], rx[p], (int64_t)(p < nkt ? p : nkt - 1) * BK);
-        sstore(ra[0], rx[0], 0);
-        __syncthreads();
-        for (int kb = 0; kb < nkt; kb += PD) {
-#pragma unroll
-            for (int p = 0; p < PD; ++p) {
-                const int kt = kb + p;
-                const int cur = kt & 1;
-                const int kn = kt + PD < nkt ? kt + PD : nkt - 1;
-                gload(ra[p], rx[p], (int64_t)kn * BK);
-                const NLA_LDS int8_t *la = (const NLA_LDS int8_t *)sA + cur * TILE + (wm * 64 + fr) * RS + fc * 16;
-                const NLA_LDS int8_t *lx = (const NLA_LDS int8_t *)sX + cur * TILE + (wn * 128 + fr) * RS + fc * 16;
-                v4i af[4], xf[8];
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) af[rb] = *reinterpret_cast<const NLA_LDS v4i *>(la + rb * 16 * RS);
-#pragma unroll
-                for (int cb = 0; cb < 8; ++cb) xf[cb] = *reinterpret_cast<const NLA_LDS v4i *>(lx + cb * 16 * RS);
-                __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int cb = 0; cb < 8; ++cb)
-                        acc[rb][cb] = OUT_T ? __builtin_amdgcn_mfma_i32_16x16x64_i8(xf[cb], af[rb], acc[rb][cb], 0, 0, 0)
-                                            : __builtin_amdgcn_mfma_i32_16x16x64_i8(af[rb], xf[cb], acc[rb][cb], 0, 0, 0);
-                __builtin_amdgcn_s_setprio(0);
-                sstore(ra[(p + 1) % PD], rx[(p + 1) % PD], cur ^ 1);
-                __syncthreads();
-            }
+                for (int cb = 0; cb < 8; ++cb)
+                    acc[rb][cb] = OUT_T ? __builtin_amdgcn_mfma_i32_16x16x64_i8(xf[cb], af[rb], acc[rb][cb], 0, 0, 0)
+                                        : __builtin_amdgcn_mfma_i32_16x16x64_i8(af[rb], xf[cb], acc[rb][cb], 0, 0, 0);
+            __builtin_amdgcn_s_setprio(0);
         }
 
         // residues of the tile: lane holds 4 consecutive rows (columns for
@@ -629,7 +669,7 @@ hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double
     int *ea = (int *)(ws + L.ea);
     err = oz_scales(src, (unsigned long long *)(ws + L.mxa), P.bA, ea, (int *)(ws + L.flag), s);
     if (err != hipSuccess) return err;
-    OzDst d{(int8_t *)(ws + L.a), P.aplane, 0, P.npad, lower ? 1 : 2};
+    OzDst d{(int8_t *)(ws + L.a), P.aplane, P.npad, lower ? 1 : 2};
     dim3 g((unsigned)(P.npad / 64), (unsigned)(P.npad / 64));
     hipLaunchKernelGGL(oz_residue_kernel, g, dim3(256), 0, s, src, (const int *)ea, d, 0, P.C,
                        (const uint8_t *)(ws + L.pw));
@@ -637,9 +677,8 @@ hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double
 }
 
 // Schedules of the modular-product kernel (nla_set_config(NLA_CFG_OZ_VARIANT)):
-// 0 two-step prefetch, 4 x 8 super-tiles per XCD; 1 one-step prefetch, row
-// order; 2 two-step prefetch, row order; 3 two-step prefetch, 8 x 4
-// super-tiles.
+// 0 4 x 8 super-tiles per XCD, 4-stage ring; 1 row order, 4 stages; 2 row
+// order, 3 stages; 3 8 x 4 super-tiles, 4 stages.
 template <typename K>
 static hipError_t oz_launch_persistent(K kern, const OzGemmArgs &g, const OzConst &C, int items, hipStream_t s,
                                        int threads = 512) {  // grid: CUs x occupancy, at most the item count
@@ -658,8 +697,6 @@ static hipError_t oz_launch_persistent(K kern, const OzGemmArgs &g, const OzCons
 
 template <bool OUT_T>
 static hipError_t oz_gemm_variant(OzGemmArgs g, const OzConst &C, int items, hipStream_t s) {
-    // 0: per-XCD counters over 4 x 8 super-tiles (9.3 ms per 17-modulus
-    // launch at n = 16384, m = 4096, against 9.7 ms in row order)
     const int v = oz_variant();
     g.sr = 0;
     g.sc = 0;
@@ -670,8 +707,8 @@ static hipError_t oz_gemm_variant(OzGemmArgs g, const OzConst &C, int items, hip
         g.sr = 8;
         g.sc = 4;
     }
-    if (v == 1) return oz_launch_persistent(oz_gemm_kernel<OUT_T, 1>, g, C, items, s);
-    return oz_launch_persistent(oz_gemm_kernel<OUT_T, 2>, g, C, items, s);
+    if (v == 2) return oz_launch_persistent(oz_gemm_kernel<OUT_T, 3>, g, C, items, s);
+    return oz_launch_persistent(oz_gemm_kernel<OUT_T, 4>, g, C, items, s);
 }
 
 hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X, int64_t xso, int64_t xsk,
@@ -681,7 +718,7 @@ hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X,
     int *fx = (int *)(ws + L.fx);
     hipError_t err = oz_scales(xs, (unsigned long long *)(ws + L.mxx), P.bX, fx, (int *)(ws + L.flag), s);
     if (err != hipSuccess) return err;
-    OzDst d{(int8_t *)(ws + L.x), P.xplane, P.npad, P.npad, 0};
+    OzDst d{(int8_t *)(ws + L.x), P.xplane, P.npad, 0};
     hipLaunchKernelGGL(oz_residue_kernel, dim3((unsigned)(P.npad / 64), (unsigned)(P.mpad / 64)), dim3(256), 0, s,
                        xs, (const int *)fx, d, 1, P.C, (const uint8_t *)(ws + L.pw));
     return hipGetLastError();
