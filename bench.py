@@ -150,7 +150,7 @@ def percentile(ts, q):
     return t[min(len(t) - 1, max(0, int(math.ceil(q * len(t))) - 1))]
 
 
-def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16", "fp64"), n=16384, m=4096, seed=5,
+def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16", "fp16x2", "fp64"), n=16384, m=4096, seed=5,
                  keep=False):
     """FP64-accurate TRSM by refinement against the plain FP64 solve of the
     same calibrated system.  Per precision: mean / p99 time, speedup,
@@ -199,6 +199,7 @@ def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16",
             torch.cuda.synchronize()
             ph = {}
             for name, kind in (("fp32_updates", _lib.NLA_KIND_GEMM), ("panels_and_inverses", _lib.NLA_KIND_LEAF),
+                               ("fp16_pair_updates", _lib.NLA_KIND_LP_PAIR_GEMM),
                                ("lp_updates", _lib.NLA_KIND_LP_GEMM), ("residual_other", _lib.NLA_KIND_RESIDUAL),
                                ("residual_int8_products", _lib.NLA_KIND_RESIDUAL_GEMM)):
                 k_n, k_ms, k_fl, k_by = _lib.prof_summary(kind)
@@ -208,6 +209,7 @@ def refined_case(nla, torch, cond, steps, warmup, precs=("auto", "bf16", "fp16",
             lp_peak = PEAK["bf16"]
             t_roof = (ph["fp32_updates"]["flops"] / (PEAK["f32"] * 1e12)
                       + ph["panels_and_inverses"]["flops"] / (PEAK["f32"] * 1e12)
+                      + 3.0 * ph["fp16_pair_updates"]["flops"] / (PEAK["fp16"] * 1e12)  # three fp16 products
                       + ph["lp_updates"]["flops"] / (lp_peak * 1e12)
                       + max(ph["residual_other"]["bytes"] / (HBM_TBS * 1e12),
                             0.0 if r["residual_method"] == "int8" else ph["residual_other"]["flops"] / (PEAK["f64"] * 1e12))
@@ -625,7 +627,8 @@ def profile_leg(nla, torch, what, reps=3):
     """Short runs for rocprofv3: the int8 residual of the refined solve's
     workload (n = 16384, m = 4096; what = "residual_int8"), the refined solve
     with bf16 updates on that workload ("refined_bf16": the bf16 GEMM
-    updates of its correction solves), or the plain LL-N TRSM at
+    updates of its correction solves) or in its automatic precisions
+    ("refined_auto": the headline's own solve), or the plain LL-N TRSM at
     n = m = 8192 (what = "trsm_f64" / "trsm_f32"); each followed by one
     512 MiB device copy, whose FETCH_SIZE / WRITE_SIZE calibrate the
     counters."""
@@ -643,14 +646,15 @@ def profile_leg(nla, torch, what, reps=3):
                             lambda: residual("L", "L", "N", 1.0, A, X, B, R, method="int8", workspace=ws), reps, 1)
         out["ms"] = ts
         del A, B, X, R, ws
-    elif what == "refined_bf16":
+    elif what in ("refined_bf16", "refined_auto"):
         from nextla_amd.mixed import rectrxm_mixed_, workspace_bytes
         n, m = 16384, 4096
         A, B = inputs.device_precision_tri(n, m, 1e4, seed=5)
         X = B.clone()
         ws = torch.empty(workspace_bytes("L", n, m) + 256, dtype=torch.uint8, device=A.device)
+        prec = "bf16" if what == "refined_bf16" else "auto"
         ts, _ = timed_steps(torch, lambda: X.copy_(B),
-                            lambda: rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision="bf16", workspace=ws,
+                            lambda: rectrxm_mixed_("L", "L", "N", 1.0, A, X, precision=prec, workspace=ws,
                                                    raise_on_stall=False), reps, 1)
         out["ms"] = ts
         del A, B, X, ws
@@ -711,7 +715,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip every leg but the headline")
     ap.add_argument("--leg", choices=("refined_mixed", "drop_in", "cpu_case", "batched_f64", "batched_f32", "served",
-                                      "lu", "lauum", "larfb", "parfb", "residual_int8", "refined_bf16", "trsm_f64",
+                                      "lu", "lauum", "larfb", "parfb", "residual_int8", "refined_bf16", "refined_auto", "trsm_f64",
                                       "trsm_f32"),
                     help="run only this extra leg and print its JSON (for profiling one subsystem)")
     args = ap.parse_args()
@@ -741,7 +745,7 @@ def main():
             res = larfb_leg(nla, torch)
         elif args.leg == "parfb":
             res = parfb_leg(nla, torch)
-        elif args.leg in ("residual_int8", "refined_bf16", "trsm_f64", "trsm_f32"):
+        elif args.leg in ("residual_int8", "refined_bf16", "refined_auto", "trsm_f64", "trsm_f32"):
             res = profile_leg(nla, torch, args.leg)
         else:
             res = serve_leg(nla, torch)
@@ -800,10 +804,10 @@ def main():
             "warmup": args.warmup, "ms_per_step": round(mean_ms, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded, generated in HBM)",
-            "config": {"workload": "FP64-accurate refined-mixed left-lower TRSM (automatic bf16/fp16/fp32 updates, "
+            "config": {"workload": "FP64-accurate refined-mixed left-lower TRSM (automatic fp16-pair/bf16/fp16/fp32 updates, "
                                    "int8-product FP64 residuals, per-system backward error <= 1e-14)",
                        "n": n, "m": m, "cond2": head["cond2_measured"], "side": "L", "uplo": "L", "trans": "N",
-                       "compute_types": "bf16/fp16/fp32 correction solves, int8 residual products, f64 "
+                       "compute_types": "fp16-pair/bf16/fp16/fp32 correction solves, int8 residual products, f64 "
                                         "interface and accuracy", "parallelism": "single GPU"},
             "speedup_vs_plain_fp64": auto["speedup_vs_plain_fp64"], "plain_fp64_ms": head["plain_fp64_ms"],
             "p99_ms": auto["p99_ms"], "backward_error_per_system": auto["backward_error_per_system"],

@@ -332,7 +332,10 @@ int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype
  *     with fp32 diagonal blocks and GEMM updates in `precision`:
  *     NLA_LP_BF16 / NLA_LP_FP16 round both operands after a power-of-two
  *     scaling per operand block and accumulate in fp32 on MFMA;
- *     NLA_LP_FP32 is the plain fp32 update;
+ *     NLA_LP_FP32 is the plain fp32 update; NLA_LP_FP16X2 splits each
+ *     scaled operand into fp16 pairs hi + lo (about 22 significant bits)
+ *     and forms hi*hi + hi*lo + lo*hi with three fp16 MFMA products, an
+ *     update close to fp32's accuracy at the fp16 kernel's cost;
  *   - every step forms R = alpha·B - op(A)·X (or alpha·B - X·op(A)) to FP64
  *     accuracy (nla_residual: int8 modular products for the shapes
  *     NLA_CFG_RESIDUAL allows, which round the iterate X in B to >= 55 bits
@@ -346,10 +349,12 @@ int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype
  *     correction solve of R.  The whole-matrix form ||R||_F / (||A||_F
  *     ||X||_F + ||alpha·B||_F), never larger, is reported beside it;
  *   - a step that less than halves the backward error raises the update
- *     precision for the following steps (bf16 -> fp16 -> fp32);
- *   - NLA_LP_AUTO: X0 from fp32 updates, then each correction in the
- *     precision (bf16, fp16 or fp32) with the least predicted cost to reach
- *     tol, from the measured error and the contraction of the last step;
+ *     precision for the following steps (bf16 -> fp16 -> fp16 pairs ->
+ *     fp32);
+ *   - NLA_LP_AUTO: X0 from fp16-pair updates, then each correction in the
+ *     precision (bf16, fp16, fp16 pairs or fp32) with the least predicted
+ *     cost to reach tol, from the measured error and the contraction of the
+ *     last step;
  *   - NLA_LP_FP64: X0 is the plain FP64 solve (nla_rectrxm, its scratch in
  *     the workspace), certified by the same residual and stopping test;
  *     corrections, if ever needed, in fp32.  No low-precision update: on
@@ -366,7 +371,14 @@ int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype
  * 7 m, 8 A, 9 lda, 10 B, 11 ldb, 12 tol, 13 max_iter, 14 workspace,
  * 15 workspace_bytes, 16 stream, 17 info.
  */
-enum nla_precision { NLA_LP_BF16 = 0, NLA_LP_FP16 = 1, NLA_LP_FP32 = 2, NLA_LP_AUTO = 3, NLA_LP_FP64 = 4 };
+enum nla_precision {
+    NLA_LP_BF16 = 0,
+    NLA_LP_FP16 = 1,
+    NLA_LP_FP32 = 2,
+    NLA_LP_AUTO = 3,
+    NLA_LP_FP64 = 4,
+    NLA_LP_FP16X2 = 5
+};
 
 #define NLA_MIXED_MAX_ITER 30
 
@@ -626,16 +638,19 @@ int32_t nla_set_config(int32_t key, int32_t value);
  *   kernel kind: NLA_KIND_GEMM (fp32/fp64 GEMM updates), NLA_KIND_LEAF
  *   (diagonal-block leaves and panels), NLA_KIND_LP_GEMM (bf16/fp16 updates of
  *   the mixed-precision solve), NLA_KIND_RESIDUAL (its residuals: the FP64
- *   product, or the int8 method's conversions and reconstruction) or
+ *   product, or the int8 method's conversions and reconstruction),
  *   NLA_KIND_RESIDUAL_GEMM (the int8 method's modular products; flops are
- *   its int8 operations, T·n^2·m for T moduli).
+ *   its int8 operations, T·n^2·m for T moduli) or NLA_KIND_LP_PAIR_GEMM (the
+ *   fp16-pair updates; flops are the update's 2·M·N·K, which the kernel
+ *   forms with three fp16 products).
  */
 enum nla_kernel_kind {
     NLA_KIND_GEMM = 0,
     NLA_KIND_LEAF = 1,
     NLA_KIND_LP_GEMM = 2,
     NLA_KIND_RESIDUAL = 3,
-    NLA_KIND_RESIDUAL_GEMM = 4
+    NLA_KIND_RESIDUAL_GEMM = 4,
+    NLA_KIND_LP_PAIR_GEMM = 5
 };
 
 int32_t nla_prof_enable(int32_t on);

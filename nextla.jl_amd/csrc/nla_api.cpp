@@ -195,7 +195,8 @@ struct Recursion {
     bool unit = false;  // unit diagonal: only the panels and the inverses read the diagonal
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
-    // -1 plain fp32.  fp16 scales each operand block by a power of two from
+    // 2 fp16 pairs (hi + lo, three fp16 products), -1 plain fp32.  fp16 and
+    // fp16 pairs scale each operand block by a power of two from
     // its absolute maximum: update u uses slot amax_a[u] for its block of A
     // and amax_b[u] for its block of B (bf16 has fp32's exponent range and is
     // not scaled).  The k-th update of every solve on one A uses the same
@@ -325,7 +326,7 @@ struct Recursion {
         g.X = X.p; g.Y = Y.p; g.C = C.p; g.ldc = C.cs;
         g.alpha = alpha; g.beta = beta;
         g.lp = lp;
-        if (lp == 1) {
+        if (lp == 1 || lp == 2) {
             // operand roles: left side X = block of A, Y = block of B; right side swapped
             const bool a_is_x = (side == 'L');
             g.amax_x = a_is_x ? sa : sb;
@@ -337,7 +338,8 @@ struct Recursion {
         }
         const double fl = 2.0 * g.M * g.N * g.K;
         const double by = ((double)g.M * g.K + (double)g.K * g.N + 2.0 * g.M * g.N) * sizeof(float);
-        err = timed(NLA_KIND_LP_GEMM, fl, by, stream, [&] { return launch_lp_gemm(g, stream); });
+        err = timed(lp == 2 ? NLA_KIND_LP_PAIR_GEMM : NLA_KIND_LP_GEMM, fl, by, stream,
+                    [&] { return launch_lp_gemm(g, stream); });
     }
 
     // C = beta*C + alpha*X*Y on views (GEMM_ADD!/GEMM_SUB!, src/matmul.jl:69-80).
@@ -711,12 +713,13 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         r.W = W;
         r.F = WF;
         r.panel_waves = g_panel_waves.load() ? g_panel_waves.load() : 8;
+        const bool scaled = (prec == NLA_LP_FP16 || prec == NLA_LP_FP16X2);
         if (prec != NLA_LP_FP32) {
-            r.lp = prec;
+            r.lp = (prec == NLA_LP_FP16X2) ? 2 : prec;
             r.amax_a = amax;
             r.amax_b = amax + half;
             r.amax_cap = half;
-            if (prec == NLA_LP_FP16) {
+            if (scaled) {
                 r.amax_a_ready = amax_a_ready;
                 hipError_t e2 = launch_fill_u32(amax_a_ready ? amax + half : amax,
                                                 amax_a_ready ? half : 2 * half, 0u, s);
@@ -725,39 +728,52 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         }
         View<float> Dv{D, 1, L.ldr, L.rows, L.cols};
         r.rec(A32v, Dv, n, 0, 1.0f, 1.0f);
-        if (prec == NLA_LP_FP16 && r.err == hipSuccess) amax_a_ready = true;
+        if (scaled && r.err == hipSuccess) amax_a_ready = true;
         return r.err;
     };
 
-    // NLA_LP_AUTO: X0 in fp32 (a bf16 / fp16 X0 starts so far from the
-    // target at small cond that the saved solve time is lost in an extra
-    // residual), then each correction in the precision whose
-    // predicted cost to reach tol is least.  Costs are in units of one FP64
-    // residual (the refinement's dominant kernel): a correction solve costs
-    // about 0.30 (bf16), 0.34 (fp16) or 0.60 (fp32) of one on MI355X at the
+    // NLA_LP_AUTO: X0 with fp16-pair updates (as accurate as fp32 updates
+    // on the measured workloads, in about 45 % of their time; a bf16 / fp16
+    // X0 starts so far from the target at small cond that the saved solve
+    // time is lost in an extra residual), then each correction in the
+    // precision whose predicted cost to reach tol is least.  Costs are in
+    // units of one int8 residual (the refinement's dominant phase, about
+    // 10.5 ms at n = 16384, m = 4096): a correction solve costs about 0.41
+    // (bf16, fp16), 0.55 (fp16 pairs) or 1.0 (fp32) of one on MI355X at the
     // sizes where refinement is used (n >= 4096).  A step in precision p
     // shrinks the backward error by a factor rho_p ~ 0.03 u_p for bf16 / fp16
-    // (u_p the unit roundoff; fitted over cond 1e2..1e6) and ~1e-7 for fp32;
-    // once a step has been observed, its factor rescaled by u_p / u_q
-    // replaces the prior (never below the fp32 floor).
+    // (u_p the unit roundoff; fitted over cond 1e2..1e6) and ~1e-7 for fp16
+    // pairs and fp32; once a step has been observed, its factor rescaled by
+    // u_p / u_q replaces the prior (never below the 1e-7 floor).
     const bool autop = (precision == NLA_LP_AUTO);
-    const double unit_round[3] = {0x1p-8, 0x1p-11, 0x1p-24};
-    const double solve_cost[3] = {0.30, 0.34, 0.60};
+    struct Cand {
+        int prec;
+        double u, cost, rho0;
+    };
+    const Cand cands[4] = {{NLA_LP_BF16, 0x1p-8, 0.41, 0.03 * 0x1p-8},
+                           {NLA_LP_FP16, 0x1p-11, 0.41, 0.03 * 0x1p-11},
+                           {NLA_LP_FP16X2, 0x1p-22, 0.55, 1e-7},
+                           {NLA_LP_FP32, 0x1p-24, 1.0, 1e-7}};
+    auto unit_of = [&](int p) {
+        for (const Cand &c : cands)
+            if (c.prec == p) return c.u;
+        return 0x1p-24;
+    };
     double rho_obs = -1.0;
-    int rho_prec = NLA_LP_FP32;
+    int rho_prec = NLA_LP_FP16X2;
     auto choose = [&](double err) {
-        int best = NLA_LP_FP32;
+        int best = NLA_LP_FP16X2;
         double best_cost = INFINITY;
-        for (int p = NLA_LP_BF16; p <= NLA_LP_FP32; ++p) {
-            double rho = p == NLA_LP_FP32 ? 1e-7 : 0.03 * unit_round[p];
-            if (rho_obs > 0.0) rho = std::max(rho_obs * unit_round[p] / unit_round[rho_prec], 1e-7);
+        for (const Cand &c : cands) {
+            double rho = c.rho0;
+            if (rho_obs > 0.0) rho = std::max(rho_obs * c.u / unit_of(rho_prec), 1e-7);
             if (!(rho < 1.0)) continue;
             const double need = std::log(tol / err) / std::log(rho);
             const double steps = std::max(1.0, std::ceil(need - 1e-9));
-            const double cost = steps * (solve_cost[p] + 1.0);
+            const double cost = steps * (c.cost + 1.0);
             if (cost < best_cost) {
                 best_cost = cost;
-                best = p;
+                best = c.prec;
             }
         }
         return best;
@@ -766,7 +782,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     // X0: the low-precision solve of S, scaled by a power of two so the fp32
     // buffer holds O(1) values; NLA_LP_FP64: the plain FP64 solve of alpha*B
     // in place, corrections (if any) in fp32
-    int prec = autop ? NLA_LP_FP32 : precision;
+    int prec = autop ? NLA_LP_FP16X2 : precision;
     double sig = pow2_above(maxS);
     if (precision == NLA_LP_FP64) {
         const int32_t st = rectrxm_in_workspace(side, uplo, trans, 'S', NLA_F64, alpha, n, m, A, lda, B, ldb, s,
@@ -857,7 +873,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
             prec = choose(bwd);
         } else if (it > 0 && bwd > 0.5 * prev && prec != NLA_LP_FP32) {
             // too little progress: raise the update precision for the next steps
-            prec = (prec == NLA_LP_BF16) ? NLA_LP_FP16 : NLA_LP_FP32;
+            prec = (prec == NLA_LP_BF16) ? NLA_LP_FP16 : (prec == NLA_LP_FP16) ? NLA_LP_FP16X2 : NLA_LP_FP32;
         }
         prev = bwd;
         sig = pow2_above(h[1]);
@@ -1036,7 +1052,7 @@ int32_t nla_rectrxm_mixed(char side, char uplo, char trans, int32_t precision, d
     if (uplo != 'L' && uplo != 'U') return bad_arg(2, "uplo");
     if (trans != 'N' && trans != 'T' && trans != 'C') return bad_arg(3, "trans");
     if (precision != NLA_LP_BF16 && precision != NLA_LP_FP16 && precision != NLA_LP_FP32 && precision != NLA_LP_AUTO &&
-        precision != NLA_LP_FP64)
+        precision != NLA_LP_FP64 && precision != NLA_LP_FP16X2)
         return bad_arg(4, "precision");
     if (!std::isfinite(alpha)) return bad_arg(5, "alpha");
     if (n < 0) return bad_arg(6, "n");
@@ -1213,7 +1229,7 @@ int32_t nla_prof_enable(int32_t on) {
 }
 
 int32_t nla_prof_summary(int32_t kind, int64_t *launches, double *ms, double *flops, double *bytes) {
-    if (kind < NLA_KIND_GEMM || kind > NLA_KIND_RESIDUAL_GEMM) return bad_arg(1, "kind");
+    if (kind < NLA_KIND_GEMM || kind > NLA_KIND_LP_PAIR_GEMM) return bad_arg(1, "kind");
     std::lock_guard<std::mutex> g(g_prof_mu);
     int64_t n = 0;
     double t = 0, f = 0, b = 0;

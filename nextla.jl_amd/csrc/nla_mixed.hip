@@ -47,6 +47,21 @@ struct LpType<1> {  // fp16: 11 significant bits, largest finite 65504
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
 };
+// fp16 pairs: each scaled fp32 value v as hi + lo with hi = fp16(v) and
+// lo = fp16(v - hi) (about 22 significant bits); a product takes the three
+// terms hi*hi + hi*lo + lo*hi (each fp16 x fp16 product is exact in fp32; the
+// dropped lo*lo term is below 2^-22 of it), three fp16 MFMAs instead of one.
+template <>
+struct LpType<2> : LpType<1> {};
+
+template <int LP>
+struct LpSplit {
+    static constexpr bool value = false;
+};
+template <>
+struct LpSplit<2> {
+    static constexpr bool value = true;
+};
 
 // Scale that puts a block with absolute maximum `amax` (bit pattern of a
 // non-negative float) just below 2^15: exact (a power of two), and the
@@ -85,8 +100,10 @@ struct LpLoader {
     typedef typename LpType<LP>::elem_t E;
     typedef typename LpType<LP>::quad_t Q;
     static_assert(BO == 128 && BK == 32 && NT == 256, "the LDS layout is laid out for 128 x 32 tiles, 256 threads");
+    static constexpr bool SPLIT = LpSplit<LP>::value;
     static constexpr int LDK = BK;
-    static constexpr int LDS_ELEMS = BO * LDK;
+    static constexpr int TILE_ELEMS = BO * LDK;                     // one tile (hi parts for a split type)
+    static constexpr int LDS_ELEMS = (SPLIT ? 2 : 1) * TILE_ELEMS;  // per stage; lo parts follow the hi tile
 
     float r[4][4];  // [row offset][k offset]
     uint32_t okbits;
@@ -143,10 +160,15 @@ struct LpLoader {
         const int k = 4 * q_of(t);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            Q q;
+            Q q, ql;
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) q[kk] = (E)(r[v][kk] * sc);
+            for (int kk = 0; kk < 4; ++kk) {
+                const float x = r[v][kk] * sc;
+                q[kk] = (E)x;
+                if (SPLIT) ql[kk] = (E)(x - (float)q[kk]);
+            }
             *reinterpret_cast<NLA_LDS Q *>(lds + lp_lds_off(row_of(t, v), k)) = q;
+            if (SPLIT) *reinterpret_cast<NLA_LDS Q *>(lds + TILE_ELEMS + lp_lds_off(row_of(t, v), k)) = ql;
         }
     }
 };
@@ -219,6 +241,25 @@ __device__ __forceinline__ void lp_tile(int64_t M, int64_t N, int64_t K, float a
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
                 b[rb] = *reinterpret_cast<const NLA_LDS OP *>(sx_ + lp_lds_off(wm * TWM + rb * 16 + lo, ks + lk));
+            if constexpr (LX::SPLIT) {
+                // the small terms first: hi*lo + lo*hi, then hi*hi
+                OP al[CB], bl[RB];
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+                    al[cb] = *reinterpret_cast<const NLA_LDS OP *>(sy_ + LY::TILE_ELEMS +
+                                                                   lp_lds_off(wn * TWN + cb * 16 + lo, ks + lk));
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+                    bl[rb] = *reinterpret_cast<const NLA_LDS OP *>(sx_ + LX::TILE_ELEMS +
+                                                                   lp_lds_off(wm * TWM + rb * 16 + lo, ks + lk));
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) {
+                        acc[cb][rb] = LT::mma(a[cb], bl[rb], acc[cb][rb]);
+                        acc[cb][rb] = LT::mma(al[cb], b[rb], acc[cb][rb]);
+                    }
+            }
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
@@ -338,6 +379,7 @@ hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s) {
     if (!vec) return hipErrorInvalidValue;
     if (a.lp == 0) return lp_launch_trans<0, 128, 128, 32, 2, 2>(a, s);
     if (a.lp == 1) return lp_launch_trans<1, 128, 128, 32, 2, 2>(a, s);
+    if (a.lp == 2) return lp_launch_trans<2, 128, 128, 32, 2, 2>(a, s);
     return hipErrorInvalidValue;
 }
 

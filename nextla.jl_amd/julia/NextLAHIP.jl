@@ -279,7 +279,7 @@ Returns (B, steps, per-system backward error).
 function rectrxm_mixed!(side::Char, uplo::Char, transpose::Char, alpha::Number, A::DeviceMat{Float64},
                         B::DeviceMat{Float64}; precision::Symbol=:auto, tol::Real=1e-14, max_iter::Integer=20)
     prec = Dict(:bf16 => Int32(0), :fp16 => Int32(1), :fp32 => Int32(2), :auto => Int32(3),
-                :fp64 => Int32(4))[precision]
+                :fp64 => Int32(4), :fp16x2 => Int32(5))[precision]
     n = size(A, 1)
     m = side in ('L', 'l') ? size(B, 2) : size(B, 1)
     pa, lda = ptr_ld(A)

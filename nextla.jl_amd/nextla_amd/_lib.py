@@ -125,11 +125,13 @@ NLA_KIND_LEAF = 1
 NLA_KIND_LP_GEMM = 2
 NLA_KIND_RESIDUAL = 3
 NLA_KIND_RESIDUAL_GEMM = 4
+NLA_KIND_LP_PAIR_GEMM = 5
 NLA_LP_BF16 = 0
 NLA_LP_FP16 = 1
 NLA_LP_FP32 = 2
 NLA_LP_AUTO = 3
 NLA_LP_FP64 = 4
+NLA_LP_FP16X2 = 5
 NLA_ERR_NOT_CONVERGED = 5
 NLA_ERR_WORKSPACE = 4
 NLA_NOALLOC_SCRATCH_BYTES = 16 * 1024 * 1024 + 64 * 1024

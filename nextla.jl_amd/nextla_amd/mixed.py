@@ -10,7 +10,10 @@ row for 'R'), max_j ||r_j|| / (||A||_lb ||x_j|| + ||alpha*b_j||) with ||A||_lb
 the largest column 2-norm of A (<= ||A||_2); the whole-matrix Frobenius form
 is reported beside it as backward_error_frobenius.
 
-precision "fp64" takes no low-precision step: X0 is the plain FP64 solve,
+precision "fp16x2" updates with fp16 pairs (each scaled fp32 operand as
+hi + lo, three fp16 MFMA products: about 22 significant bits at the fp16
+kernel's cost).  precision "fp64" takes no low-precision step: X0 is the
+plain FP64 solve,
 certified by the same residual and test (corrections, if ever needed, in
 fp32).
 """
@@ -21,7 +24,7 @@ import torch
 from . import _lib, _canonical, _require_device, _stream
 
 PRECISIONS = {"bf16": _lib.NLA_LP_BF16, "fp16": _lib.NLA_LP_FP16, "fp32": _lib.NLA_LP_FP32,
-              "auto": _lib.NLA_LP_AUTO, "fp64": _lib.NLA_LP_FP64}
+              "auto": _lib.NLA_LP_AUTO, "fp64": _lib.NLA_LP_FP64, "fp16x2": _lib.NLA_LP_FP16X2}
 _NAMES = {v: k for k, v in PRECISIONS.items()}
 RESIDUALS = {"fp64": _lib.NLA_RES_FP64, "int8": _lib.NLA_RES_INT8}
 RESIDUALS_NAMES = {v: k for k, v in RESIDUALS.items()}

@@ -130,7 +130,7 @@ def test_mixed_validation(nla):
 
     assert call(0, b"X") == -1
     assert call(3, 7) == -4
-    assert call(3, 5) == -4
+    assert call(3, 6) == -4
     assert call(3, -1) == -4
     assert call(4, float("nan")) == -5
     assert call(5, -1) == -6

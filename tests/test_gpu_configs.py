@@ -157,7 +157,7 @@ def config2(cond):
     return _C2[cond]
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp16", "auto"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp16x2", "auto"])
 @pytest.mark.parametrize("cond", [1e2, 1e4, 1e6])
 def test_refined_16384_4096(nla, cond, prec):
     from nextla_amd.mixed import rectrxm_mixed_

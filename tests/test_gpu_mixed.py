@@ -20,7 +20,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
 COMBOS = list(itertools.product("LR", "LU", "NT"))
-PREC_NAMES = ("bf16", "fp16", "fp32")
+PREC_NAMES = ("bf16", "fp16", "fp32", "fp16x2")
 
 
 def dev(a):
@@ -44,7 +44,7 @@ def case(n, m, side, uplo, c, seed):
 
 
 @pytest.mark.parametrize("side,uplo,trans", COMBOS)
-@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32", "auto"])
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp32", "fp16x2", "auto"])
 def test_refined_matches_fp64_oracle(nla, side, uplo, trans, prec):
     from nextla_amd.mixed import rectrxm_mixed_
     n, m, alpha = 1100, 37, 1.5  # n > 512: the recursion makes low-precision updates
@@ -62,21 +62,23 @@ def test_refined_matches_fp64_oracle(nla, side, uplo, trans, prec):
     # a well-conditioned system converges in the requested precision (auto:
     # in one of them, within as few steps as fp32 needs)
     assert info["precision"] == prec or (prec == "auto" and info["precision"] in PREC_NAMES), info
-    assert info["iterations"] <= {"bf16": 10, "fp16": 6, "fp32": 3, "auto": 3}[prec], info
+    assert info["iterations"] <= {"bf16": 10, "fp16": 6, "fp32": 3, "fp16x2": 3, "auto": 3}[prec], info
     h = info["history"]
     assert all(h[i + 1] < h[i] for i in range(len(h) - 1)), h
 
 
 def test_low_precision_solve_accuracy(nla):
     """max_iter = 0 returns the first correction solve.  Its backward error
-    follows the update precision (bf16 u = 2^-8, fp16 u = 2^-11, fp32
-    u = 2^-24): each step down costs accuracy, and none is far off (which
-    would mean a wrong product rather than a rounded one)."""
+    follows the update precision (bf16 u = 2^-8, fp16 u = 2^-11, fp16 pairs
+    about 2^-22, fp32 u = 2^-24): each step down costs accuracy, and none is
+    far off (which would mean a wrong product rather than a rounded one).
+    The fp16 pairs land near fp32, far from fp16: a pair product that lost
+    its hi*lo or lo*hi term would fall back to fp16's error."""
     from nextla_amd.mixed import rectrxm_mixed_
     n, m = 2048, 64
     A, B = case(n, m, "L", "L", 100.0, seed=11)
     bwd = {}
-    for prec in ("bf16", "fp16", "fp32"):
+    for prec in ("bf16", "fp16", "fp16x2", "fp32"):
         Bd = dev(B)
         _, info = rectrxm_mixed_("L", "L", "N", 1.0, dev(A), Bd, precision=prec, tol=1e-14, max_iter=0,
                                  raise_on_stall=False)
@@ -89,6 +91,8 @@ def test_low_precision_solve_accuracy(nla):
     assert bwd["bf16"] < 1e-3, bwd
     assert 3.0 < bwd["bf16"] / bwd["fp16"] < 40.0, bwd   # 3 bits apart
     assert bwd["fp16"] / bwd["fp32"] > 100.0, bwd          # 13 bits apart
+    assert bwd["fp16"] / bwd["fp16x2"] > 100.0, bwd        # about 11 bits apart
+    assert bwd["fp16x2"] / bwd["fp32"] < 16.0, bwd         # about 2 bits apart
 
 
 def test_not_converged_reports_last_iterate(nla):
