@@ -330,8 +330,9 @@ int32_t nla_parfb(char side, char trans, char direct, char storev, int32_t dtype
  * op(A)·X = alpha·B (side 'L') or X·op(A) = alpha·B (side 'R') for FP64 A, B:
  *   - correction solves run the recursion on an fp32 copy of A's triangle,
  *     with fp32 diagonal blocks and GEMM updates in `precision`:
- *     NLA_LP_BF16 / NLA_LP_FP16 round both operands after a power-of-two
- *     scaling per operand block and accumulate in fp32 on MFMA;
+ *     NLA_LP_BF16 / NLA_LP_FP16 round both operands (fp16: after a
+ *     power-of-two scaling of each work-group's operand panels) and
+ *     accumulate in fp32 on MFMA;
  *     NLA_LP_FP32 is the plain fp32 update; NLA_LP_FP16X2 splits each
  *     scaled operand into fp16 pairs hi + lo (about 22 significant bits)
  *     and forms hi*hi + hi*lo + lo*hi with three fp16 MFMA products, an

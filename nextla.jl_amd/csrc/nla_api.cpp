@@ -196,16 +196,20 @@ struct Recursion {
     hipError_t err = hipSuccess;
     // Low-precision GEMM updates (T = float only): lp = 0 bf16, 1 fp16,
     // 2 fp16 pairs (hi + lo, three fp16 products), -1 plain fp32.  fp16 and
-    // fp16 pairs scale each operand block by a power of two from
-    // its absolute maximum: update u uses slot amax_a[u] for its block of A
-    // and amax_b[u] for its block of B (bf16 has fp32's exponent range and is
-    // not scaled).  The k-th update of every solve on one A uses the same
-    // block of A, so with amax_a_ready its maximum from an earlier solve is
-    // reused; amax_b is cleared before each solve.
+    // fp16 pairs scale each work-group's operand panels by powers of two:
+    // the blocks of A from a_tab, the 64 x 64 tile maxima of the stored fp32
+    // triangle at a_base (leading dimension a_ld; the views are positive-
+    // stride sub-views of it); the solved blocks of B from x_slots, where
+    // every panel records the largest |X| it wrote at slot off / 64 (cleared
+    // before each solve).  upd_koff: the first row (of the effective system)
+    // of the solved block the next update reads.
     int lp = -1;
-    unsigned *amax_a = nullptr, *amax_b = nullptr;
-    bool amax_a_ready = false;
-    int64_t amax_used = 0, amax_cap = 0;
+    const unsigned *a_tab = nullptr;
+    int64_t a_tab_ld = 0;
+    const float *a_base = nullptr;
+    int64_t a_ld = 0;
+    unsigned *x_slots = nullptr;
+    int64_t upd_koff = 0;
 
     // Leaf as an MFMA product: TRSM applies inv(A_kk) from W, TRMM applies
     // A_kk itself with the other triangle masked to zero.
@@ -270,6 +274,9 @@ struct Recursion {
         a.strip = panel_strip;
         a.waves = n <= 512 ? panel_waves : 4;
         a.unit = unit;
+        if constexpr (sizeof(T) == 4) {
+            if (x_slots && func == 'S') a.xmax = x_slots + off / 64;
+        }
         const double fl = (double)n * (double)n * (double)a.m;
         const double by = ((double)n * (n + 1) / 2 + 2.0 * n * a.m) * sizeof(T);
         err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_panel<T>(a, stream); });
@@ -277,6 +284,10 @@ struct Recursion {
 
     void leaf(View<T> A, View<T> B, int64_t n, int64_t off, T scale) {
         if (err != hipSuccess) return;
+        if (x_slots) {  // only the panels record the maxima the scaled updates read
+            err = hipErrorInvalidValue;
+            return;
+        }
         if (mode == 1) {
             leaf_mfma(A, B, n, off, scale);
             return;
@@ -310,15 +321,9 @@ struct Recursion {
         err = timed(NLA_KIND_LEAF, fl, by, stream, [&] { return launch_leaf<T>(a, stream); });
     }
 
-    // The update with operands rounded to bf16/fp16 after power-of-two
-    // scaling per operand block (scales from the blocks' absolute maxima).
+    // The update with operands rounded to bf16 / fp16 / fp16 pairs after
+    // power-of-two scaling of each work-group's operand panels (fp16 kinds).
     void lp_gemm(View<float> C, View<float> X, View<float> Y, float alpha, float beta) {
-        if (amax_used + 1 > amax_cap) {
-            err = hipErrorInvalidValue;
-            return;
-        }
-        unsigned *sa = amax_a + amax_used, *sb = amax_b + amax_used;
-        ++amax_used;
         LpGemmArgs g;
         g.M = C.nr; g.N = C.nc; g.K = X.nc;
         g.tx = (X.rs != 1); g.ldx = g.tx ? X.rs : X.cs;
@@ -327,14 +332,29 @@ struct Recursion {
         g.alpha = alpha; g.beta = beta;
         g.lp = lp;
         if (lp == 1 || lp == 2) {
+            if (!a_tab || !x_slots) {
+                err = hipErrorInvalidValue;
+                return;
+            }
             // operand roles: left side X = block of A, Y = block of B; right side swapped
             const bool a_is_x = (side == 'L');
-            g.amax_x = a_is_x ? sa : sb;
-            g.amax_y = a_is_x ? sb : sa;
-            const View<float> &Av = a_is_x ? X : Y, &Bv = a_is_x ? Y : X;
-            if (!amax_a_ready) err = launch_absmax(Av.nr, Av.nc, Av.p, Av.rs, Av.cs, sa, stream);
-            if (err == hipSuccess) err = launch_absmax(Bv.nr, Bv.nc, Bv.p, Bv.rs, Bv.cs, sb, stream);
-            if (err != hipSuccess) return;
+            const View<float> &Av = a_is_x ? X : Y;
+            const int64_t e = (int64_t)(Av.p - a_base);
+            LpScale sa, sb;
+            sa.tab = a_tab;
+            sa.ld = a_tab_ld;
+            sa.r0 = e % a_ld;
+            sa.c0 = e / a_ld;
+            // operand element (o, k): X(o, k) = p[o*rs + k*cs] as X, Y(k, o) =
+            // p[k*rs + o*cs] as Y; with rs == 1 the first index runs down the
+            // stored rows
+            sa.trans = a_is_x ? (Av.rs != 1) : (Av.rs == 1);
+            sb.tab = x_slots;
+            sb.ld = 0;
+            sb.r0 = upd_koff / 64;
+            sb.c0 = (g.K + 63) / 64;
+            g.scx = a_is_x ? sa : sb;
+            g.scy = a_is_x ? sb : sa;
         }
         const double fl = 2.0 * g.M * g.N * g.K;
         const double by = ((double)g.M * g.K + (double)g.K * g.N + 2.0 * g.M * g.N) * sizeof(float);
@@ -400,11 +420,13 @@ struct Recursion {
         if (func == 'S') {
             if (forward) {
                 rec(A11, B1, mid, off, pre, one);
+                upd_koff = off;
                 if (side == 'L') gemm(B2, A21, B1, -one, pre);  // B2 -= A21*B1
                 else gemm(B2, B1, A12, -one, pre);              // B2 -= B1*A12
                 rec(A22, B2, rem, off + mid, one, one);
             } else {
                 rec(A22, B2, rem, off + mid, pre, one);
+                upd_koff = off + mid;
                 if (side == 'L') gemm(B1, A12, B2, -one, pre);  // B1 -= A12*B2
                 else gemm(B1, B2, A21, -one, pre);              // B1 -= B2*A21
                 rec(A11, B1, mid, off, one, one);
@@ -567,9 +589,9 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
 struct MixedLayout {
     int64_t rows, cols;  // shape of B
     int64_t ld32, ldr;   // leading dimension of the fp32 A copy / of S, R, D
-    int64_t amax_slots;
+    int64_t nt;  // 64-row blocks of A: the tile table is nt x nt, the panel slots nt
     int64_t nsys;  // systems: columns of B (side L) or rows (side R)
-    int64_t a32, w, wf, s, r, d, amax, part, norms, queue, spart, xpart, s2, t64, t64_bytes, total;
+    int64_t a32, w, wf, s, r, d, tab, slots, part, norms, queue, spart, xpart, s2, t64, t64_bytes, total;
     // int8 residual: its plan and workspace (after the fields above)
     bool oz;
     OzPlan op;
@@ -585,7 +607,7 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     L.ld32 = (n + 63) / 64 * 64;
     L.ldr = (L.rows + 63) / 64 * 64;
     const int64_t nblk = (n + 63) / 64;
-    L.amax_slots = 2 * (2 * nblk + 2);  // A and B halves: an update per internal node of the recursion
+    L.nt = nblk;
     int64_t o = 0;
     L.a32 = o;   o = al256(o + L.ld32 * n * 4);
     L.w = o;     o = al256(o + nblk * 64 * 64 * 4);
@@ -593,7 +615,8 @@ static MixedLayout mixed_layout(char side, int64_t n, int64_t m) {
     L.s = o;     o = al256(o + L.ldr * L.cols * 8);
     L.r = o;     o = al256(o + L.ldr * L.cols * 8);
     L.d = o;     o = al256(o + L.ldr * L.cols * 4);
-    L.amax = o;  o = al256(o + L.amax_slots * 4);
+    L.tab = o;   o = al256(o + nblk * nblk * 4);
+    L.slots = o; o = al256(o + nblk * 4);
     L.part = o;  o = al256(o + NORM_PARTS_MAX * 2 * 8);
     L.norms = o; o = al256(o + 16 * 8);
     L.queue = o; o = al256(o + 4);
@@ -648,7 +671,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     float *W = (float *)(ws + L.w);
     double *S = (double *)(ws + L.s), *R = (double *)(ws + L.r);
     float *D = (float *)(ws + L.d);
-    unsigned *amax = (unsigned *)(ws + L.amax);
+    unsigned *tab = (unsigned *)(ws + L.tab), *slots = (unsigned *)(ws + L.slots);
     double *part = (double *)(ws + L.part), *nrm = (double *)(ws + L.norms);
     const bool tr = (trans != 'N');
     const char ue = tr ? (uplo == 'L' ? 'U' : 'L') : uplo;  // uplo of op(A)
@@ -699,8 +722,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     (void)part;
 
     // D <- op(A)^-1 D (or D op(A)^-1) on the fp32 copy, updates in `prec`
-    bool amax_a_ready = false;  // A-block maxima of the fp16 updates computed
-    const int64_t half = L.amax_slots / 2;
+    bool tab_ready = false;  // tile maxima of the fp32 A (fp16 / fp16-pair updates)
     auto lp_solve = [&](int prec) -> hipError_t {
         Recursion<float> r;
         r.func = 'S';
@@ -714,21 +736,21 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         r.F = WF;
         r.panel_waves = g_panel_waves.load() ? g_panel_waves.load() : 8;
         const bool scaled = (prec == NLA_LP_FP16 || prec == NLA_LP_FP16X2);
-        if (prec != NLA_LP_FP32) {
-            r.lp = (prec == NLA_LP_FP16X2) ? 2 : prec;
-            r.amax_a = amax;
-            r.amax_b = amax + half;
-            r.amax_cap = half;
-            if (scaled) {
-                r.amax_a_ready = amax_a_ready;
-                hipError_t e2 = launch_fill_u32(amax_a_ready ? amax + half : amax,
-                                                amax_a_ready ? half : 2 * half, 0u, s);
-                if (e2 != hipSuccess) return e2;
-            }
+        if (prec != NLA_LP_FP32) r.lp = (prec == NLA_LP_FP16X2) ? 2 : prec;
+        if (scaled) {
+            hipError_t e2 = hipSuccess;
+            if (!tab_ready) e2 = launch_tile_absmax(n, A32, L.ld32, uplo == 'L', tab, L.nt, s);
+            if (e2 == hipSuccess) e2 = launch_fill_u32(slots, L.nt, 0u, s);
+            if (e2 != hipSuccess) return e2;
+            tab_ready = true;
+            r.a_tab = tab;
+            r.a_tab_ld = L.nt;
+            r.a_base = A32;
+            r.a_ld = L.ld32;
+            r.x_slots = slots;
         }
         View<float> Dv{D, 1, L.ldr, L.rows, L.cols};
         r.rec(A32v, Dv, n, 0, 1.0f, 1.0f);
-        if (scaled && r.err == hipSuccess) amax_a_ready = true;
         return r.err;
     };
 

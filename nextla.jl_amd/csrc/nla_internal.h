@@ -78,6 +78,9 @@ struct PanelArgs {
     int strip = PANEL_STRIP;  // RHS columns per work-group: 16 or 32
     bool unit = false;        // unit diagonal (taken as 1, not read)
     int waves = 4;            // waves per work-group: 4, or 8 splitting each phase's k range (<= 512 rows)
+    // TRSM, fp32: *xmax = max(*xmax, max |X|) over the panel's solution (a
+    // float bit pattern), the scale source of the updates that read it
+    unsigned *xmax = nullptr;
 };
 
 template <typename T>
@@ -220,12 +223,30 @@ int32_t report_bad_arg(int k, const char *what);
 
 // ---- mixed precision ------------------------------------------------------
 
+// Where the power-of-two scale of one low-precision GEMM operand comes from
+// (fp16 and fp16-pair updates; bf16 has fp32's range and is not scaled).
+// Each work-group scales its own operand panel (its BM rows of op(X), its BN
+// columns of op(Y), over the whole k range) from the largest magnitude the
+// source records for that panel:
+//   tab == nullptr: no scaling;
+//   ld > 0: a table of 64 x 64 tile maxima of a stored matrix, tile (i, j) at
+//     tab[i + j*ld]; operand element (o, k) is stored at (r0 + o, c0 + k),
+//     or at (r0 + k, c0 + o) with trans (r0, c0 multiples of 64);
+//   ld == 0: slots tab[r0 .. r0 + c0 - 1], one maximum for the whole operand.
+// Maxima are float bit patterns (non-negative floats order like them).
+struct LpScale {
+    const unsigned *tab = nullptr;
+    int64_t ld = 0;
+    int64_t r0 = 0, c0 = 0;
+    bool trans = false;
+};
+
 // Low-precision GEMM update C = beta*C + alpha*op(X)*op(Y) on fp32 storage:
-// op(X) and op(Y) are scaled by powers of two chosen from *amax_x / *amax_y
-// (absolute maxima of the two operand blocks as float bit patterns; nullptr
-// for no scaling), rounded to bf16 (lp = 0) or fp16 (lp = 1), multiplied on
-// MFMA with fp32 accumulation and unscaled on the way out.  X, Y need
-// 16-byte aligned columns (rows when transposed) and ldx, ldy % 4 == 0.
+// op(X) and op(Y) are scaled by powers of two chosen from their scale
+// sources, rounded to bf16 (lp = 0), fp16 (lp = 1) or fp16 pairs (lp = 2),
+// multiplied on MFMA with fp32 accumulation and unscaled on the way out.
+// X, Y need 16-byte aligned columns (rows when transposed) and
+// ldx, ldy % 4 == 0.
 struct LpGemmArgs {
     int64_t M, N, K;
     bool tx, ty;
@@ -237,13 +258,14 @@ struct LpGemmArgs {
     float *C;
     int64_t ldc;
     int lp;
-    const unsigned *amax_x = nullptr, *amax_y = nullptr;
+    LpScale scx, scy;
 };
 hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s);
-// amax[0] = max(amax[0], max |P(r,c)|) over P(r,c) = P[r*rs + c*cs]; the slot
-// must hold 0 (or an earlier maximum) before the launch.
-hipError_t launch_absmax(int64_t rows, int64_t cols, const float *P, int64_t rs, int64_t cs, unsigned *amax,
-                         hipStream_t s);
+// tab[i + j*nt] = max |A32(r, c)| over tile (i, j) (64 x 64) of the stored
+// triangle (lower: r >= c, upper: r <= c), as a float bit pattern; 0 for
+// tiles outside the triangle.  nt = ceil(n / 64).
+hipError_t launch_tile_absmax(int64_t n, const float *A32, int64_t ld32, bool lower, unsigned *tab, int64_t nt,
+                              hipStream_t s);
 hipError_t launch_fill_u32(unsigned *p, int64_t n, unsigned v, hipStream_t s);
 // Column-major elementwise steps of the refinement loop.
 hipError_t launch_tri_to_f32(int64_t n, const double *A, int64_t lda, bool lower, float *A32, int64_t ld32,

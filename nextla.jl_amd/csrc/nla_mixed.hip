@@ -3,11 +3,11 @@
 //
 // The refinement loop needs, besides the plain recursion:
 //   - a low-precision GEMM update for the correction solves: op(X), op(Y)
-//     stored as fp32, scaled by a power of two per operand block (from the
-//     block's absolute maximum, so fp16 neither overflows nor flushes the
-//     block's large entries), rounded to bf16 or fp16 on their way into LDS,
-//     multiplied on the 16x16x32 MFMA with fp32 accumulation, and unscaled in
-//     the epilogue;
+//     stored as fp32, scaled by a power of two per work-group operand panel
+//     (fp16 kinds: from the largest magnitude recorded for the panel, so
+//     fp16 neither overflows nor flushes its large entries), rounded to
+//     bf16, fp16 or fp16 pairs on their way into LDS, multiplied on the
+//     16x16x32 MFMA with fp32 accumulation, and unscaled in the epilogue;
 //   - conversions between the FP64 iterate / residual and the fp32 solve
 //     buffers, and the norms of the stopping test, as HBM-bound elementwise
 //     kernels with deterministic (fixed-order) reductions.
@@ -326,24 +326,66 @@ __device__ inline void lp_tile_coords(int G, int g, int tiles_m, int &tm, int &t
     tn = in / gm;
 }
 
+// Largest magnitude the scale source records for the operand panel with
+// outer indices [o0, o0 + BO) (clipped to no) over k in [0, K), as a float
+// bit pattern (0 when there is none); every thread of the work-group gets it.
+template <int NT>
+__device__ unsigned lp_panel_max(const LpScale &sc, int64_t o0, int BO, int64_t no, int64_t K, unsigned *red) {
+    const int t = threadIdx.x;
+    unsigned m = 0;
+    if (sc.ld == 0) {
+        for (int64_t i = t; i < sc.c0; i += NT) {
+            const unsigned v = sc.tab[sc.r0 + i];
+            m = v > m ? v : m;
+        }
+    } else {
+        const int64_t o1 = (o0 + BO < no ? o0 + BO : no);
+        int64_t rlo, rhi, clo, chi;  // stored element ranges [lo, hi)
+        if (!sc.trans) {
+            rlo = sc.r0 + o0; rhi = sc.r0 + o1; clo = sc.c0; chi = sc.c0 + K;
+        } else {
+            rlo = sc.r0; rhi = sc.r0 + K; clo = sc.c0 + o0; chi = sc.c0 + o1;
+        }
+        const int64_t tr0 = rlo >> 6, ntr = ((rhi + 63) >> 6) - tr0;
+        const int64_t tc0 = clo >> 6, ntc = ((chi + 63) >> 6) - tc0;
+        for (int64_t i = t; i < ntr * ntc; i += NT) {
+            const int64_t tr = tr0 + i % ntr, tc = tc0 + i / ntr;
+            const unsigned v = sc.tab[tr + tc * sc.ld];
+            m = v > m ? v : m;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned v = __shfl_xor(m, o);
+        m = v > m ? v : m;
+    }
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    m = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) m = red[w] > m ? red[w] : m;
+    __syncthreads();  // red is reused by the next call
+    return m;
+}
+
 template <int LP, int BM, int BN, int BK, int WM, int WN, bool TX, bool TY>
 __global__ __launch_bounds__(WM *WN * 64) void lp_gemm_kernel(int64_t M, int64_t N, int64_t K, float alpha,
                                                               const float *__restrict__ X, int64_t ldx,
                                                               const float *__restrict__ Y, int64_t ldy, float beta,
-                                                              float *C, int64_t ldc, int tiles_m,
-                                                              const unsigned *__restrict__ amax_x,
-                                                              const unsigned *__restrict__ amax_y) {
+                                                              float *C, int64_t ldc, int tiles_m, LpScale scx,
+                                                              LpScale scy) {
     typedef typename LpType<LP>::elem_t E;
     constexpr int NT = WM * WN * 64;
     typedef LpLoader<LP, BM, BK, NT, TX> LX;
     typedef LpLoader<LP, BN, BK, NT, !TY> LY;
     __shared__ __attribute__((aligned(16))) E lds_x[2 * LX::LDS_ELEMS];
     __shared__ __attribute__((aligned(16))) E lds_y[2 * LY::LDS_ELEMS];
+    __shared__ unsigned red[NT / 64];
     int tm, tn;
     lp_tile_coords(gridDim.x, blockIdx.x, tiles_m, tm, tn);
     const int64_t row0 = (int64_t)tm * BM, col0 = (int64_t)tn * BN;
-    const float sx = amax_x ? block_scale(*amax_x) : 1.0f;
-    const float sy = amax_y ? block_scale(*amax_y) : 1.0f;
+    // per-work-group scales: this tile's rows of op(X) and columns of op(Y)
+    const float sx = scx.tab ? block_scale(lp_panel_max<NT>(scx, row0, BM, M, K, red)) : 1.0f;
+    const float sy = scy.tab ? block_scale(lp_panel_max<NT>(scy, col0, BN, N, K, red)) : 1.0f;
     const bool interior = (row0 + BM <= M) && (col0 + BN <= N) && (K % BK == 0);
     if (interior)
         lp_tile<LP, BM, BN, BK, WM, WN, TX, TY, false>(M, N, K, alpha, X, ldx, Y, ldy, beta, C, ldc, row0, col0,
@@ -359,7 +401,7 @@ static hipError_t lp_launch_cfg(const LpGemmArgs &a, hipStream_t s) {
     int tiles_n = (int)((a.N + BN - 1) / BN);
     hipLaunchKernelGGL((lp_gemm_kernel<LP, BM, BN, BK, WM, WN, TX, TY>), dim3(tiles_m * tiles_n),
                        dim3(WM * WN * 64), 0, s, a.M, a.N, a.K, a.alpha, a.X, a.ldx, a.Y, a.ldy, a.beta, a.C,
-                       a.ldc, tiles_m, a.amax_x, a.amax_y);
+                       a.ldc, tiles_m, a.scx, a.scy);
     return hipGetLastError();
 }
 
@@ -401,34 +443,48 @@ static inline dim3 ew_grid(int64_t rows, int64_t cols, int64_t max_blocks = 1 <<
     return dim3((unsigned)bx, (unsigned)by);
 }
 
-// amax[0] = max(amax[0], max |P(r,c)|), P(r,c) = P[r*rs + c*cs]; the walk
-// runs along whichever of the two dimensions has unit stride.  Non-negative
-// floats order like their bit patterns, so the maximum is an integer max.
-__global__ void absmax_kernel(int64_t rows, int64_t cols, const float *__restrict__ P, int64_t rs, int64_t cs,
-                              unsigned *amax) {
-    __shared__ unsigned red[256];
+// tab[i + j*nt] = max |A32| over tile (i, j) of the stored triangle: one
+// work-group per tile, thread t on row t % 64 of 16 columns (the walk runs
+// down the unit-stride rows).  Non-negative floats order like their bit
+// patterns, so the maximum is an integer max.
+__global__ __launch_bounds__(256) void tile_absmax_kernel(int64_t n, const float *__restrict__ A32, int64_t ld32,
+                                                          int lower, unsigned *__restrict__ tab, int64_t nt) {
+    __shared__ unsigned red[4];
+    const int64_t ti = blockIdx.x, tj = blockIdx.y;
+    const int t = threadIdx.x;
     unsigned m = 0;
-    NLA_EW_BEGIN
-    unsigned b = __float_as_uint(fabsf(P[r * rs + c * cs]));
-    m = b > m ? b : m;
-    NLA_EW_END
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
-        __syncthreads();
+    if (lower ? ti >= tj : ti <= tj) {
+        const int64_t r = ti * 64 + (t & 63);
+        const int64_t c0 = tj * 64 + (t >> 6) * 16;
+        if (r < n) {
+#pragma unroll 4
+            for (int u = 0; u < 16; ++u) {
+                const int64_t c = c0 + u;
+                if (c < n && (lower ? r >= c : r <= c)) {
+                    const unsigned b = __float_as_uint(fabsf(A32[r + c * ld32]));
+                    m = b > m ? b : m;
+                }
+            }
+        }
     }
-    if (threadIdx.x == 0 && red[0] != 0) atomicMax(amax, red[0]);
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned v = __shfl_xor(m, o);
+        m = v > m ? v : m;
+    }
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        unsigned r = red[0];
+        for (int w = 1; w < 4; ++w) r = red[w] > r ? red[w] : r;
+        tab[ti + tj * nt] = r;
+    }
 }
 
-hipError_t launch_absmax(int64_t rows, int64_t cols, const float *P, int64_t rs, int64_t cs, unsigned *amax,
-                         hipStream_t s) {
-    if (rows <= 0 || cols <= 0) return hipSuccess;
-    if (rs != 1) {  // walk the transposed view so threads stay on the unit stride
-        int64_t t = rows; rows = cols; cols = t;
-        t = rs; rs = cs; cs = t;
-    }
-    hipLaunchKernelGGL(absmax_kernel, ew_grid(rows, cols, 2048), dim3(256), 0, s, rows, cols, P, rs, cs, amax);
+hipError_t launch_tile_absmax(int64_t n, const float *A32, int64_t ld32, bool lower, unsigned *tab, int64_t nt,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tile_absmax_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, s, n, A32, ld32,
+                       lower ? 1 : 0, tab, nt);
     return hipGetLastError();
 }
 

@@ -131,7 +131,8 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                                             int64_t mcs, bool lower, bool unit, T *__restrict__ Bp, int64_t brs,
                                             int64_t bcs,
                                             const T *__restrict__ Wp, int64_t wrs, int64_t wcs, T scale,
-                                            const int *__restrict__ Fp, NLA_LDS T *bs, NLA_LDS T *sc = nullptr) {
+                                            const int *__restrict__ Fp, NLA_LDS T *bs, NLA_LDS T *sc = nullptr,
+                                            unsigned *xmax = nullptr) {
     typedef PanelShape<T, WS> PS;
     typedef typename PS::MF MF;
     constexpr int W = PS::W;
@@ -414,12 +415,16 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         if (np_full + u < np) run(phr[u], fr[u], okr[u]);
 
     // ---- write the strip back ----
+    T vmax = T(0);  // largest |X| written (fp32 TRSM with xmax)
     if (rowc) {
         for (int r = t; r < nbig; r += NT) {
 #pragma unroll
             for (int c = 0; c < W; ++c) {
                 const T v = bs[c * LDB + r];
-                if (c0 + c < m) Bp[r * brs + (c0 + c) * bcs] = TRMM ? v * scale : v;
+                if (c0 + c < m) {
+                    Bp[r * brs + (c0 + c) * bcs] = TRMM ? v * scale : v;
+                    if (!TRMM) vmax = fabs(v) > vmax ? fabs(v) : vmax;
+                }
             }
         }
     } else {
@@ -428,7 +433,20 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
             for (int r = t / W; r < nbig; r += RSTEP) {
                 const T v = bs[c * LDB + r];
                 Bp[r * brs + (c0 + c) * bcs] = TRMM ? v * scale : v;
+                if (!TRMM) vmax = fabs(v) > vmax ? fabs(v) : vmax;
             }
+        }
+    }
+    if constexpr (sizeof(T) == 4 && !TRMM) {
+        if (xmax != nullptr) {
+            // a NaN compares false above and never raises the maximum; the
+            // refinement's own checks see it in the residual
+            unsigned b = __float_as_uint((float)vmax);
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned u = __shfl_xor(b, o);
+                b = u > b ? u : b;
+            }
+            if ((t & 63) == 0 && b != 0) atomicMax(xmax, b);
         }
     }
 }
@@ -438,14 +456,14 @@ __global__ __launch_bounds__(NW * 64) void panel_kernel(int nbig, int64_t m, con
                                                         int64_t mcs, int lower, int unit, T *__restrict__ Bp,
                                                         int64_t brs, int64_t bcs, const T *__restrict__ Wp,
                                                         int64_t wrs, int64_t wcs, T scale,
-                                                        const int *__restrict__ Fp) {
+                                                        const int *__restrict__ Fp, unsigned *xmax) {
     constexpr int LDB = NBIG + (sizeof(T) == 8 ? 2 : 4);
     __shared__ __attribute__((aligned(16))) T bs_raw[WS * LDB];
     // k-split partial sums (8 waves) and the refinement step's scratch
     __shared__ __attribute__((aligned(16))) T sc_raw[(NW == 8 || !TRMM) ? 64 * WS : 1];
     panel_strip<T, NBIG, WS, TRMM, NW>(nbig, m, (int64_t)blockIdx.x * WS, Mp, mrs, mcs, lower != 0, unit != 0, Bp,
                                        brs, bcs, Wp, wrs, wcs, scale, Fp, (NLA_LDS T *)bs_raw,
-                                       (NLA_LDS T *)sc_raw);
+                                       (NLA_LDS T *)sc_raw, xmax);
 }
 
 template <typename T, int NBIG, int W, int NW>
@@ -455,11 +473,11 @@ static hipError_t launch_panel_w(const PanelArgs<T> &a, hipStream_t s) {
     if (a.trmm)
         hipLaunchKernelGGL((panel_kernel<T, NBIG, W, true, NW>), dim3((unsigned)groups), dim3(NW * 64), 0, s,
                            a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale,
-                           (const int *)nullptr);
+                           (const int *)nullptr, (unsigned *)nullptr);
     else
         hipLaunchKernelGGL((panel_kernel<T, NBIG, W, false, NW>), dim3((unsigned)groups), dim3(NW * 64), 0, s,
                            a.nbig, a.m, a.M, a.mrs, a.mcs, lo, un, a.B, a.brs, a.bcs, a.W, a.wrs, a.wcs, a.scale,
-                           a.F);
+                           a.F, a.xmax);
     return hipGetLastError();
 }
 
