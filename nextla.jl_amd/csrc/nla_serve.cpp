@@ -523,7 +523,7 @@ extern "C" int32_t nla_serve_set_refinement(nla_server S, double tol, int32_t ma
     if (!S) return report_bad_arg(1, "server");
     if (!(tol > 0.0)) return report_bad_arg(2, "tol");
     if (max_iter < 0 || max_iter > NLA_MIXED_MAX_ITER) return report_bad_arg(3, "max_iter");
-    if (precision < NLA_LP_BF16 || precision > NLA_LP_FP64) return report_bad_arg(4, "precision");
+    if (precision < NLA_LP_BF16 || precision > NLA_LP_FP16X2) return report_bad_arg(4, "precision");
     S->tol = tol;
     S->max_iter = max_iter;
     S->mixed_precision = precision;
