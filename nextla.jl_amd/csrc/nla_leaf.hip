@@ -143,16 +143,21 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
 }
 
 // Inverse of one 64 x 64 triangular block with 256 threads, in three stages
-// (the blocked scheme of LAPACK's trtri):
+// (the blocked scheme of LAPACK's trtri), in place in LDS:
 //   1. the four 16 x 16 diagonal sub-blocks by substitution on the columns of
-//      the identity (64 lanes, one column each, 16-step chains);
-//   2. 32 x 32 inverses: inv([A 0; C B]) = [inv A, 0; -inv B * C * inv A, inv B];
+//      the identity (64 lanes, one column each, 16-step chains), written over
+//      the sub-blocks once every lane has read them;
+//   2. 32 x 32 inverses: inv([A 0; C B]) = [inv A, 0; -inv B * C * inv A, inv B],
+//      C * inv A through the scratch st, the result written over C;
 //   3. the same step from 32 to 64.
-// sl: the canonical lower block (column-major, 64 x 64); si: the inverse
-// being built; st: 32 x 32 scratch.  Upper blocks are inverted as J U J.
+// sl: the canonical lower block (column-major, 64 x 64), which becomes its
+// inverse; st: 32 x 32 scratch.  The parts above the diagonal stay zero, so
+// the products run over full, unrolled k ranges and the extra terms add
+// exact zeros.  Upper blocks are inverted as J U J.  40 KB of LDS in FP64
+// (four work-groups per CU; with a separate inverse it was 72 KB, two).
 template <typename T>
 __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
-                              bool unit, T *__restrict__ W, T *sl, T *si, T *st, int *__restrict__ F) {
+                              bool unit, T *__restrict__ W, T *sl, T *st, int *__restrict__ F) {
     constexpr int NB = 64, NT = 256;
     const int t = threadIdx.x;
     const int64_t off = blk * NB;
@@ -160,8 +165,8 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
     const T *base = lower ? A + off * (rs + cs) : A + (off + nb - 1) * (rs + cs);
     const int64_t ars = lower ? rs : -rs, acs = lower ? cs : -cs;
     const bool arow = (ars == 1 || ars == -1);
-    // stage L (identity padding past nb); zero the inverse.  All 16 loads of
-    // a thread are issued before the first LDS store.
+    // stage L (identity padding past nb).  All 16 loads of a thread are
+    // issued before the first LDS store.
     constexpr int PER = NB * NB / NT;
     T v[PER];
 #pragma unroll
@@ -180,57 +185,65 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
         const int i = arow ? e % NB : e / NB;
         const int j = arow ? e / NB : e % NB;
         sl[j * NB + i] = v[u];
-        si[e] = T(0);
     }
     __syncthreads();
+    // ||L||_1 (largest column sum) while L is still there: wave 0, lane = column
+    T nl = T(0);
+    if (F != nullptr && t < NB) {
+#pragma unroll 8
+        for (int i = 0; i < NB; ++i) nl += sl[t * NB + i] < T(0) ? -sl[t * NB + i] : sl[t * NB + i];
+        for (int o = 32; o > 0; o >>= 1) {
+            const T other = __shfl_xor(nl, o);
+            nl = other > nl ? other : nl;
+        }
+    }
     // 1. sub-block b, column c: y = inv(D_b) e_c
+    T x[16];
+    const int b1 = t >> 4, c1 = t & 15, o1 = 16 * b1;
     if (t < 64) {
-        const int b = t >> 4, c = t & 15, o = 16 * b;
-        T x[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = (i == c) ? T(1) : T(0);
+        for (int i = 0; i < 16; ++i) x[i] = (i == c1) ? T(1) : T(0);
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            x[j] = x[j] / sl[(o + j) * NB + o + j];
+            x[j] = x[j] / sl[(o1 + j) * NB + o1 + j];
 #pragma unroll
-            for (int r = j + 1; r < 16; ++r) x[r] -= sl[(o + j) * NB + o + r] * x[j];
+            for (int r = j + 1; r < 16; ++r) x[r] -= sl[(o1 + j) * NB + o1 + r] * x[j];
         }
+    }
+    __syncthreads();  // every lane has read its sub-block
+    if (t < 64) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) si[(o + c) * NB + o + i] = x[i];
+        for (int i = 0; i < 16; ++i) sl[(o1 + c1) * NB + o1 + i] = x[i];
     }
     __syncthreads();
-    // The products below run over full, unrolled k ranges: the inverse
-    // blocks in si are zero outside their lower triangles (si starts zeroed),
-    // so the extra terms add exact zeros, and fixed trip counts let every
-    // LDS read of a product be issued before its FMAs.
     // 2. two 32-blocks p: st_p = C_p * inv(D_2p) (16 x 16), then
-    //    si[32p+16.., 32p..] = -inv(D_2p+1) * st_p.  Thread t: block p = t>>7,
-    //    row i, columns j and j+8.
+    //    C_p <- -inv(D_2p+1) * st_p.  Thread t: block p = t>>7, row i,
+    //    columns j and j+8.
     {
         const int p = t >> 7, q = t & 127, i = q & 15, j = q >> 4, o = 32 * p;
         T a0 = T(0), a1 = T(0);
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const T c = sl[(o + k) * NB + o + 16 + i];
-            a0 += c * si[(o + j) * NB + o + k];
-            a1 += c * si[(o + j + 8) * NB + o + k];
+            a0 += c * sl[(o + j) * NB + o + k];
+            a1 += c * sl[(o + j + 8) * NB + o + k];
         }
         st[p * 256 + j * 16 + i] = a0;
         st[p * 256 + (j + 8) * 16 + i] = a1;
-        __syncthreads();
+        __syncthreads();  // C_p read by every lane; st_p complete
         a0 = T(0);
         a1 = T(0);
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const T c = si[(o + 16 + k) * NB + o + 16 + i];
+            const T c = sl[(o + 16 + k) * NB + o + 16 + i];
             a0 += c * st[p * 256 + j * 16 + k];
             a1 += c * st[p * 256 + (j + 8) * 16 + k];
         }
-        si[(o + j) * NB + o + 16 + i] = -a0;
-        si[(o + j + 8) * NB + o + 16 + i] = -a1;
+        sl[(o + j) * NB + o + 16 + i] = -a0;
+        sl[(o + j + 8) * NB + o + 16 + i] = -a1;
     }
     __syncthreads();
-    // 3. st = Q * inv(P) (32 x 32, inv(P) lower), si[32.., 0..32] = -inv(R) * st.
+    // 3. st = Q * inv(P) (32 x 32, inv(P) lower), Q <- -inv(R) * st.
     //    Thread t: row i = t & 31, columns j0 + 8u.
     {
         const int i = t & 31, j0 = t >> 5;
@@ -239,21 +252,22 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
         for (int k = 0; k < 32; ++k) {
             const T c = sl[k * NB + 32 + i];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) a[u] += c * si[(j0 + 8 * u) * NB + k];
+            for (int u = 0; u < 4; ++u) a[u] += c * sl[(j0 + 8 * u) * NB + k];
         }
+        // (step 2's reads of st ended at the barrier before this step)
 #pragma unroll
         for (int u = 0; u < 4; ++u) st[(j0 + 8 * u) * 32 + i] = a[u];
-        __syncthreads();
+        __syncthreads();  // st complete; Q read by every lane
 #pragma unroll
         for (int u = 0; u < 4; ++u) a[u] = T(0);
 #pragma unroll
         for (int k = 0; k < 32; ++k) {
-            const T c = si[(32 + k) * NB + 32 + i];
+            const T c = sl[(32 + k) * NB + 32 + i];
 #pragma unroll
             for (int u = 0; u < 4; ++u) a[u] += c * st[(j0 + 8 * u) * 32 + k];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) si[(j0 + 8 * u) * NB + 32 + i] = -a[u];
+        for (int u = 0; u < 4; ++u) sl[(j0 + 8 * u) * NB + 32 + i] = -a[u];
     }
     __syncthreads();
     // Condition of the block, kappa_1 = ||L||_1 ||inv(L)||_1 (both exact, the
@@ -261,22 +275,16 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
     // product has backward error growing with kappa * u, so a block above
     // LEAF_REFINE_KAPPA is flagged and the panels follow its product with
     // one step of refinement (X += inv(L) (B - L X)).
-    if (F != nullptr) {
-        T cs1 = T(0);
-        if (t < 2 * NB) {
-            const T *src = (t < NB) ? sl + t * NB : si + (t - NB) * NB;
+    if (F != nullptr && t < NB) {
+        T ni = T(0);
 #pragma unroll 8
-            for (int i = 0; i < NB; ++i) cs1 += src[i] < T(0) ? -src[i] : src[i];
-        }
-        // maxima over the 64 columns of each: waves 0 (L) and 1 (inverse)
+        for (int i = 0; i < NB; ++i) ni += sl[t * NB + i] < T(0) ? -sl[t * NB + i] : sl[t * NB + i];
         for (int o = 32; o > 0; o >>= 1) {
-            const T other = __shfl_xor(cs1, o);
-            cs1 = other > cs1 ? other : cs1;
+            const T other = __shfl_xor(ni, o);
+            ni = other > ni ? other : ni;
         }
-        if (t == NB) st[0] = cs1;
-        __syncthreads();
         if (t == 0) {
-            const T kappa = cs1 * st[0];
+            const T kappa = nl * ni;
             F[blk] = (kappa > T(LEAF_REFINE_KAPPA) || !(kappa == kappa)) ? 1 : 0;
         }
     }
@@ -285,12 +293,12 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
 #pragma unroll 4
     for (int e = t; e < NB * NB; e += NT) {
         int i = e % NB, j = e / NB;  // element (i, j) of the stored inverse
-        T v;
+        T val;
         if (lower || i >= nb || j >= nb)
-            v = (i < nb && j < nb) || lower ? si[j * NB + i] : (i == j ? T(1) : T(0));
+            val = (i < nb && j < nb) || lower ? sl[j * NB + i] : (i == j ? T(1) : T(0));
         else
-            v = si[(nb - 1 - j) * NB + (nb - 1 - i)];
-        w[e] = v;
+            val = sl[(nb - 1 - j) * NB + (nb - 1 - i)];
+        w[e] = val;
     }
 }
 
@@ -372,20 +380,18 @@ __global__ __launch_bounds__(LEAF_COLS) void trtri_kernel(int64_t n, const T *__
 
 // The 64-block inverse used by the recursion and the batched solver.
 template <typename T>
-__global__ __launch_bounds__(256) void trtri64_kernel(int64_t n, const T *__restrict__ A, int64_t rs, int64_t cs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trtri64_kernel(int64_t n, const T *__restrict__ A, int64_t rs, int64_t cs,
                                                       int lower, int unit, T *__restrict__ W, int *__restrict__ F) {
     __shared__ __attribute__((aligned(16))) T sl[64 * 64];
-    __shared__ __attribute__((aligned(16))) T si[64 * 64];
     __shared__ __attribute__((aligned(16))) T st[32 * 32];
-    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, unit != 0, W, sl, si, st, F);
+    trtri64_block<T>(n, blockIdx.x, A, rs, cs, lower != 0, unit != 0, W, sl, st, F);
 }
 
 // Batched form: one work-group per (system, block) pair.
 template <typename T>
-__global__ __launch_bounds__(256) void batch_trtri_kernel(const BatchSys<T> *__restrict__ sys,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void batch_trtri_kernel(const BatchSys<T> *__restrict__ sys,
                                                          const int2 *__restrict__ blk) {
     __shared__ __attribute__((aligned(16))) T sl[64 * 64];
-    __shared__ __attribute__((aligned(16))) T si[64 * 64];
     __shared__ __attribute__((aligned(16))) T st[32 * 32];
     const int2 b = blk[blockIdx.x];
     const BatchSys<T> &S = sys[b.x];
@@ -395,7 +401,7 @@ __global__ __launch_bounds__(256) void batch_trtri_kernel(const BatchSys<T> *__r
     const bool right = (S.wrs != 1);
     const int64_t rs = right ? S.mcs : S.mrs, cs = right ? S.mrs : S.mcs;
     const bool lower = right ? (S.lower == 0) : (S.lower != 0);
-    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, false, (T *)S.W, sl, si, st, (int *)S.F);
+    trtri64_block<T>(S.n, b.y, S.M, rs, cs, lower, false, (T *)S.W, sl, st, (int *)S.F);
 }
 
 template <typename T>
