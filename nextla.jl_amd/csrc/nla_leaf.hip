@@ -156,7 +156,7 @@ __global__ __launch_bounds__(LEAF_COLS) void leaf_kernel(int nb, int64_t m, cons
 // exact zeros.  Upper blocks are inverted as J U J.  40 KB of LDS in FP64
 // (four work-groups per CU; with a separate inverse it was 72 KB, two).
 template <typename T>
-__device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
+__device__ __forceinline__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
                               bool unit, T *__restrict__ W, T *sl, T *st, int *__restrict__ F) {
     constexpr int NB = 64, NT = 256;
     const int t = threadIdx.x;
@@ -188,10 +188,15 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
     }
     __syncthreads();
     // ||L||_1 (largest column sum) while L is still there: wave 0, lane = column
+    // (lane t walks its column from row t on, so that the 64 lanes of a read
+    // fall in different banks rather than all in one)
     T nl = T(0);
     if (F != nullptr && t < NB) {
 #pragma unroll 8
-        for (int i = 0; i < NB; ++i) nl += sl[t * NB + i] < T(0) ? -sl[t * NB + i] : sl[t * NB + i];
+        for (int i = 0; i < NB; ++i) {
+            const T a = sl[t * NB + ((i + t) & (NB - 1))];
+            nl += a < T(0) ? -a : a;
+        }
         for (int o = 32; o > 0; o >>= 1) {
             const T other = __shfl_xor(nl, o);
             nl = other > nl ? other : nl;
@@ -278,7 +283,10 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
     if (F != nullptr && t < NB) {
         T ni = T(0);
 #pragma unroll 8
-        for (int i = 0; i < NB; ++i) ni += sl[t * NB + i] < T(0) ? -sl[t * NB + i] : sl[t * NB + i];
+        for (int i = 0; i < NB; ++i) {
+            const T a = sl[t * NB + ((i + t) & (NB - 1))];
+            ni += a < T(0) ? -a : a;
+        }
         for (int o = 32; o > 0; o >>= 1) {
             const T other = __shfl_xor(ni, o);
             ni = other > ni ? other : ni;
@@ -308,7 +316,7 @@ __device__ void trtri64_block(int64_t n, int64_t blk, const T *__restrict__ A, i
 // solve; the inverses turn the diagonal-block leaves of the recursion into
 // MFMA products.
 template <typename T, int NB>
-__device__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
+__device__ __forceinline__ void trtri_block(int64_t n, int64_t blk, const T *__restrict__ A, int64_t rs, int64_t cs, bool lower,
                             bool unit, T *__restrict__ W, T *sl, T *srinv) {
     const int t = threadIdx.x;
     const int64_t off = blk * NB;

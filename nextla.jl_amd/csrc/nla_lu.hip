@@ -66,7 +66,7 @@ __device__ inline void wave_argmax(T &bv, T &bs, int64_t &bi) {
 }
 
 template <typename T>
-__device__ void lu_leaf_body(int64_t m, int n, T *__restrict__ A, int64_t lda, int64_t *__restrict__ ipiv,
+__device__ __forceinline__ void lu_leaf_body(int64_t m, int n, T *__restrict__ A, int64_t lda, int64_t *__restrict__ ipiv,
                              int64_t *__restrict__ info, int64_t r0) {
     __shared__ T s_val[2][LEAF_WAVES], s_sgn[2][LEAF_WAVES];
     __shared__ int64_t s_row[2][LEAF_WAVES];
