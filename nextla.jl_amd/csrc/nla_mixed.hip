@@ -428,15 +428,21 @@ hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s) {
 // ---- elementwise and reduction kernels -----------------------------------
 
 // Column-major rows x cols walks: threads along the rows (the unit-stride
-// dimension), work-groups of the grid's y dimension along the columns.
+// dimension), work-groups of the grid's y dimension along the columns.  A
+// thread takes four rows of a column (the loop unrolled, so their loads are
+// in flight together) and the grid holds at most EW_BLOCKS work-groups: with
+// one element per thread and a work-group per column the launch spent its
+// time dispatching work-groups (about a third of the HBM rate).
 #define NLA_EW_BEGIN                                                                \
     for (int64_t c = blockIdx.y; c < cols; c += gridDim.y)                          \
-        for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; \
-             r += (int64_t)gridDim.x * blockDim.x) {
+        _Pragma("unroll 4") for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; \
+                                 r += (int64_t)gridDim.x * blockDim.x) {
 #define NLA_EW_END }
 
-static inline dim3 ew_grid(int64_t rows, int64_t cols, int64_t max_blocks = 1 << 20) {
-    int64_t bx = (rows + 255) / 256;
+constexpr int64_t EW_BLOCKS = 8192;
+
+static inline dim3 ew_grid(int64_t rows, int64_t cols, int64_t max_blocks = EW_BLOCKS) {
+    int64_t bx = (rows + 1023) / 1024;
     if (bx > 64) bx = 64;
     int64_t by = cols < 65535 ? cols : 65535;
     if (bx * by > max_blocks) by = max_blocks / bx > 0 ? max_blocks / bx : 1;
@@ -656,13 +662,29 @@ __global__ __launch_bounds__(256) void sys_sumsq_col_kernel(int64_t rows, int64_
     int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = r0 + chunk < rows ? r0 + chunk : rows;
     if (tri == 1 && r0 < c) r0 = c;
     if (tri == 2 && r1 > c + 1) r1 = c + 1;
-    double ss = 0.0, mx = 0.0;
+    // four partial sums (rows r0 + lane + 64 (4 i + u) in sum u), so that
+    // four loads are in flight and the adds do not wait on each other
+    double su[4] = {0.0, 0.0, 0.0, 0.0}, mx = 0.0;
     const double *col = P + c * ld;
-    for (int64_t r = r0 + lane; r < r1; r += 64) {
-        const double v = col[r];
-        ss += v * v;
-        mx = fmax(mx, fabs(v));
+    int64_t r = r0 + lane;
+    for (; r + 192 < r1; r += 256) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = col[r + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            su[u] += v[u] * v[u];
+            mx = fmax(mx, fabs(v[u]));
+        }
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u, r += 64)
+        if (r < r1) {
+            const double v = col[r];
+            su[u] += v * v;
+            mx = fmax(mx, fabs(v));
+        }
+    double ss = (su[0] + su[1]) + (su[2] + su[3]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         ss += __shfl_xor(ss, o);

@@ -440,13 +440,21 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
     if constexpr (sizeof(T) == 4 && !TRMM) {
         if (xmax != nullptr) {
             // a NaN compares false above and never raises the maximum; the
-            // refinement's own checks see it in the residual
+            // refinement's own checks see it in the residual.  One global
+            // atomic per work-group: every work-group of the launch targets
+            // the same slot, and one per wave serialized there (+10 us per
+            // launch).
+            __shared__ unsigned s_max;
             unsigned b = __float_as_uint((float)vmax);
             for (int o = 32; o > 0; o >>= 1) {
                 const unsigned u = __shfl_xor(b, o);
                 b = u > b ? u : b;
             }
-            if ((t & 63) == 0 && b != 0) atomicMax(xmax, b);
+            if (t == 0) s_max = 0u;
+            __syncthreads();
+            if ((t & 63) == 0) atomicMax(&s_max, b);
+            __syncthreads();
+            if (t == 0 && s_max != 0u) atomicMax(xmax, s_max);
         }
     }
 }
