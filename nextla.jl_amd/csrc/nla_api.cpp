@@ -663,6 +663,33 @@ static hipError_t oz_timed(const OzPlan &P, const OzLayout &L, char *ws, bool lo
                  [&] { return oz_reconstruct(P, L, ws, out_t, n, m, S, lds, R, ldr, s); });
 }
 
+// A second stream per (host thread, device) for the mixed solve's work that
+// does not feed its first correction solve (A's norms and residue planes),
+// so that it runs beside that solve; fork and join through two events.
+// Default priority.  Kept until the thread ends (not destroyed).
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+static hipError_t side_stream(SideStream **out) {
+    constexpr int MAX_DEV = 64;
+    thread_local SideStream cache[MAX_DEV];
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEV) return hipErrorInvalidDevice;
+    SideStream &x = cache[dev];
+    if (x.s == nullptr) {
+        e = hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&x.fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&x.join, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    *out = &x;
+    return hipSuccess;
+}
+
 static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precision, double alpha, int64_t n,
                                   int64_t m, const double *A, int64_t lda, double *B, int64_t ldb, double tol,
                                   int max_iter, char *ws, const MixedLayout &L, hipStream_t s,
@@ -699,25 +726,33 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
     const bool by_col = (side == 'L');
     double *spart = (double *)(ws + L.spart), *xpart = (double *)(ws + L.xpart), *s2 = (double *)(ws + L.s2);
     const int ncha = sys_chunks(n, n), nchb = sys_chunks(L.nsys, n);
-    NLA_CK(launch_sys_sumsq(n, n, A, lda, true, uplo == 'L' ? 1 : 2, ncha, spart, s));
-    NLA_CK(launch_sys_fold(ncha, n, spart, nullptr, nrm + 0, s));
     NLA_CK(launch_sys_sumsq(L.rows, L.cols, S, L.ldr, by_col, 0, nchb, spart, s));
     NLA_CK(launch_sys_fold(nchb, L.nsys, spart, s2, nrm + 3, s));
-    // int8 residual: A's row scales and residue planes (Ahat = op(A) for side
-    // L, op(A)^T for side R, so that the residual is Ahat * Xhat or its
-    // transpose), kept for every step
+    // On the side stream, beside X0's solve: A's norms (partials in xpart,
+    // which the stopping tests use only after the join) and, for the int8
+    // residual, A's row scales and residue planes (Ahat = op(A) for side L,
+    // op(A)^T for side R, so that the residual is Ahat * Xhat or its
+    // transpose), kept for every step.  A non-finite A raises the residual's
+    // range flag, which the first stopping test reads (FP64 residuals then).
     bool use_oz = L.oz;
     const bool ahat_is_a = (side == 'L') != tr;
     const bool ahat_lower = ahat_is_a ? (uplo == 'L') : (uplo == 'U');
     int hflag = 0;
+    SideStream *side_s = nullptr;
+    NLA_CK(side_stream(&side_s));
+    NLA_CK(hipEventRecord(side_s->fork, s));
+    NLA_CK(hipStreamWaitEvent(side_s->s, side_s->fork, 0));
+    NLA_CK(launch_sys_sumsq(n, n, A, lda, true, uplo == 'L' ? 1 : 2, ncha, xpart, side_s->s));
+    NLA_CK(launch_sys_fold(ncha, n, xpart, nullptr, nrm + 0, side_s->s));
     if (use_oz)
-        NLA_CK(oz_prepare(L.op, L.ol, ws, A, ahat_is_a ? 1 : lda, ahat_is_a ? lda : 1, ahat_lower, n, s));
+        NLA_CK(oz_prepare(L.op, L.ol, ws, A, ahat_is_a ? 1 : lda, ahat_is_a ? lda : 1, ahat_lower, n, side_s->s));
+    NLA_CK(hipEventRecord(side_s->join, side_s->s));
+    // ||S||_F and max |S| (X0's scaling) from the main stream alone
     double h[6];
-    NLA_CK(hipMemcpyAsync(h, nrm, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
-    if (use_oz) NLA_CK(hipMemcpyAsync(&hflag, ws + L.ol.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    NLA_CK(hipMemcpyAsync(h + 3, nrm + 3, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
     NLA_CK(hipStreamSynchronize(s));
-    if (hflag) use_oz = false;  // A not finite: FP64 residuals
-    const double nA = std::sqrt(h[0]), nS = std::sqrt(h[3]);
+    double nA = 0.0, hA = 0.0;  // ||A||_F, read at the first stopping test
+    const double nS = std::sqrt(h[3]);
     const double maxS = h[4];
     (void)part;
 
@@ -859,6 +894,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         if (e2 == hipSuccess) e2 = launch_sys_sumsq(L.rows, L.cols, B, ldb, by_col, 0, nchb, xpart, s);
         if (e2 == hipSuccess) e2 = launch_sys_eta(nchb, L.nsys, spart, xpart, s2, nrm + 2, nrm + 6, s);
         if (e2 == hipSuccess) e2 = hipMemcpyAsync(h, nrm + 6, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(&hA, nrm, sizeof(double), hipMemcpyDeviceToHost, s);
         hflag = 0;
         if (e2 == hipSuccess && with_flag)
             e2 = hipMemcpyAsync(&hflag, ws + L.ol.flag, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -866,6 +902,8 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         return e2;
     };
 
+    // the side stream's work is complete before the first residual
+    NLA_CK(hipStreamWaitEvent(s, side_s->join, 0));
     double prev = INFINITY, bwd = INFINITY, bwd_f = INFINITY;
     int it = 0;
     for (;; ++it) {
@@ -881,6 +919,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
             NLA_CK(timed(NLA_KIND_RESIDUAL, res_fl, res_by, s, fp64_residual));
             NLA_CK(stop_norms(false));
         }
+        nA = std::sqrt(hA);
         const double nR = std::sqrt(h[0]), nX = std::sqrt(h[2]);
         const double den = nA * nX + nS;
         bwd_f = den > 0.0 ? nR / den : (nR > 0.0 ? INFINITY : 0.0);
