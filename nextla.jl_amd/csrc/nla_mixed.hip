@@ -76,15 +76,16 @@ __device__ inline float block_scale(unsigned bits) {
     return ldexpf(1.0f, k);
 }
 
-// Stages a 128 x 32 block of an fp32 operand into LDS as 128 rows of 32 k
-// (64 bytes) in the low-precision type.  Row r sits in row slot
+// Stages a BO x 32 block (BO = 128 with 256 threads, 256 with 512) of an
+// fp32 operand into LDS as BO rows of 32 k (64 bytes) in the low-precision
+// type.  Row r sits in row slot
 // r ^ ((r >> 2) & 3) (the low two bits flipped by the next two), and its
 // 16-byte chunk c at chunk c ^ (2 * (bit 2 ^ bit 4 of r)) (lp_lds_off):
 //   fragment reads (16 rows x one chunk per 16 lanes, 8 rows x 2 chunks per
 //     b128 phase) meet 16 distinct 16-byte bank groups;
 //   each 32-lane half of a tile store meets 32 distinct 8-byte bank pairs.
 // Each thread moves 4 rows x 4 k (one 8-byte quad per row):
-//   k-contiguous operand: quad t % 8 of rows t/8 + 32 v (v = 0..3), four
+//   k-contiguous operand: quad t % 8 of rows t/8 + (NT/8) v (v = 0..3), four
 //     16-byte loads along k (8 lanes cover 128 contiguous bytes of a row);
 //   outer-contiguous operand: rows 4 g .. 4 g + 3 (g = 8 (t / 64) + t % 8),
 //     quad (t / 8) % 8, four 16-byte loads along the rows (8 lanes cover
@@ -99,7 +100,8 @@ template <int LP, int BO, int BK, int NT, bool KCONTIG>
 struct LpLoader {
     typedef typename LpType<LP>::elem_t E;
     typedef typename LpType<LP>::quad_t Q;
-    static_assert(BO == 128 && BK == 32 && NT == 256, "the LDS layout is laid out for 128 x 32 tiles, 256 threads");
+    static_assert(BK == 32 && ((BO == 128 && NT == 256) || (BO == 256 && NT == 512)),
+                  "the LDS layout is laid out for BO x 32 tiles, 2 BO threads");
     static constexpr bool SPLIT = LpSplit<LP>::value;
     static constexpr int LDK = BK;
     static constexpr int TILE_ELEMS = BO * LDK;                     // one tile (hi parts for a split type)
@@ -111,7 +113,7 @@ struct LpLoader {
     __device__ static inline int g_of(int t) { return KCONTIG ? t / 8 : 8 * (t / 64) + (t % 8); }
     __device__ static inline int q_of(int t) { return KCONTIG ? t % 8 : (t / 8) % 8; }
     // tile row of the thread's v-th row (r[v])
-    __device__ static inline int row_of(int t, int v) { return KCONTIG ? g_of(t) + 32 * v : 4 * g_of(t) + v; }
+    __device__ static inline int row_of(int t, int v) { return KCONTIG ? g_of(t) + (NT / 8) * v : 4 * g_of(t) + v; }
 
     // Element (outer, k) at P[outer*so + k*sk]; (KCONTIG ? sk : so) == 1.
     template <bool CHECK>
@@ -419,8 +421,12 @@ hipError_t launch_lp_gemm(const LpGemmArgs &a, hipStream_t s) {
     // transposed operand) must start 16-byte aligned.
     const bool vec = ((uintptr_t)a.X & 15) == 0 && ((uintptr_t)a.Y & 15) == 0 && a.ldx % 4 == 0 && a.ldy % 4 == 0;
     if (!vec) return hipErrorInvalidValue;
-    if (a.lp == 0) return lp_launch_trans<0, 128, 128, 32, 2, 2>(a, s);
-    if (a.lp == 1) return lp_launch_trans<1, 128, 128, 32, 2, 2>(a, s);
+    // 256 x 256 tiles (eight waves of 64 x 128) halve the operand traffic of
+    // the large updates; fp16 pairs keep 128 x 128 (two tiles per operand
+    // would not fit the registers of the larger shape)
+    const bool big = a.M >= 1024 && a.N >= 1024;
+    if (a.lp == 0) return big ? lp_launch_trans<0, 256, 256, 32, 4, 2>(a, s) : lp_launch_trans<0, 128, 128, 32, 2, 2>(a, s);
+    if (a.lp == 1) return big ? lp_launch_trans<1, 256, 256, 32, 4, 2>(a, s) : lp_launch_trans<1, 128, 128, 32, 2, 2>(a, s);
     if (a.lp == 2) return lp_launch_trans<2, 128, 128, 32, 2, 2>(a, s);
     return hipErrorInvalidValue;
 }
