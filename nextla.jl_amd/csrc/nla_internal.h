@@ -322,8 +322,12 @@ struct OzConst {
     int y[OZ_TMAX];  // (M / p_t)^-1 mod p_t
     float invpf[OZ_TMAX];
     double invpd[OZ_TMAX];
-    double c1[OZ_TMAX], c2[OZ_TMAX], c3[OZ_TMAX];  // 1/p_t on grids 2^-45, 2^-86, 2^-127
-    double M;     // product of the T moduli
+    // w_t = y_t (M / p_t) (the CRT basis: w_t = 1 mod p_t, 0 mod the
+    // others) and M, as four base-2^40 limbs each (exact in double)
+    double w[OZ_TMAX][4];
+    double ml[4];
+    double M;     // product of the T moduli (rounded)
+    double invm;  // 1 / M (rounded)
     double smax;  // largest |S_int| the moduli cover (M / 8)
 };
 

@@ -460,13 +460,23 @@ __global__ __launch_bounds__(512) void oz_gemm_kernel(OzGemmArgs a, OzConst C) {
 
 // ---- reconstruction ---------------------------------------------------------
 
-// R(r, c) = S(r, c) - (A X) at (r, c) from the product residues: R_int =
-// S_int - c_t modulo p_t for every t, then x = M * centred-frac(sum_t u_t /
-// p_t) with u_t = (R_int mod p_t) (M/p_t)^-1 mod p_t.  1/p_t = c1 + c2 + c3
-// on fixed grids (2^-45, 2^-86, 2^-127) so that every partial sum u_t c is
-// exact in double; x has relative error ~2^-52 plus 2^-115 M.  er / ec: the
-// scale exponents of R's rows / columns.  |S_int| above C.smax, or a
-// non-finite S, sets *flag (the modulus product would not cover R_int).
+// R(r, c) = S(r, c) - (A X) at (r, c) from the product residues c_t, in
+// exact integer arithmetic on doubles.  With w_t = y_t (M / p_t) (1 modulo
+// p_t, 0 modulo the other moduli), V = sum_t c_t w_t = P modulo M, where
+// P = (A X)_int and |P|, |S_int| <= M/8.  Numbers are kept as four limbs of
+// base 2^40 (weights 1, 2^40, 2^80, 2^120); every limb operation is an exact
+// integer below 2^53:
+//   V's limbs: sum_t c_t w_t,i with c_t < 256, w_t,i < 2^40, T <= 20;
+//   D = V - S_int, S_int's limbs from |S_int| = m 2^sh (53-bit m) by exact
+//       floor / fma steps;
+//   k = round(D / M) from D's limbs in rounded double (|D - kM| = |R_int|
+//       <= M/4 leaves a margin of 1/4 against an error near 2^-39);
+//   E = D - kM = -R_int limb by limb, carries propagated (E0..E2 in
+//       [0, 2^40)), and E's value formed by three fmas: exact below 2^93,
+//       within 3 ulp above (no cancellation there).
+// R_int is scaled back by 2^-(er + ec), the scale exponents of R's rows and
+// columns.  |S_int| above C.smax, or a non-finite S, sets *flag (the modulus
+// product would not cover R_int).
 struct OzCrtArgs {
     int64_t rows, cols;
     const double *S;
@@ -476,67 +486,112 @@ struct OzCrtArgs {
     const uint8_t *Cr;
     int64_t cplane, ldc;
     const int *er, *ec;
-    const uint8_t *pw;
     int *flag;
 };
 
 // Four consecutive rows per thread: one 32-bit load gives their residues
 // of one plane, and every plane's load is issued before the first is used.
 // The planes are padded to multiples of 256 rows, so the 4-row word is
-// always inside the plane; rows past `rows` are skipped.
+// always inside the plane; rows past `rows` are skipped.  TT: the number of
+// moduli (C.T), or OZ_TMAX for any count (the basis padded with zeros); the
+// basis limbs are read from LDS (80 doubles do not fit the scalar registers).
+template <int TT>
 __global__ __launch_bounds__(256) void oz_crt_kernel(OzCrtArgs a, OzConst C) {
-    __shared__ uint8_t pw[OZ_TMAX * OZ_PW];
-    for (int i = threadIdx.x; i < C.T * OZ_PW / 4; i += 256)
-        reinterpret_cast<uint32_t *>(pw)[i] = reinterpret_cast<const uint32_t *>(a.pw)[i];
+    __shared__ double wl[TT][4];
+    for (int i = threadIdx.x; i < TT * 4; i += 256) wl[i / 4][i % 4] = (i / 4 < C.T) ? C.w[i / 4][i % 4] : 0.0;
     __syncthreads();
     const int64_t r0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4;
     if (r0 >= a.rows) return;
+    constexpr double B40 = 0x1p40, B80 = 0x1p80, B120 = 0x1p120;
     int er[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) er[j] = (r0 + j < a.rows) ? a.er[r0 + j] : 0;
     for (int64_t c = blockIdx.y; c < a.cols; c += gridDim.y) {
         const uint8_t *cp = a.Cr + c * a.ldc + r0;
-        uint32_t cw[OZ_TMAX];
+        uint32_t cw[TT];
 #pragma unroll
-        for (int t = 0; t < OZ_TMAX; ++t)
-            cw[t] = (t < C.T) ? *reinterpret_cast<const uint32_t *>(cp + t * a.cplane) : 0u;
+        for (int t = 0; t < TT; ++t)
+            cw[t] = (TT != OZ_TMAX || t < C.T) ? *reinterpret_cast<const uint32_t *>(cp + t * a.cplane) : 0u;
         const int ecc = a.ec[c];
         double sv[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) sv[j] = (r0 + j < a.rows) ? a.S[r0 + j + c * a.lds] : 0.0;
+        // V = sum_t c_t w_t, limb by limb, for the four rows
+        double v[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[j][i] = 0.0;
+        // the limbs of w_{t+1} are read while w_t's products run (the
+        // scheduler is held to that order, and the table's address is made
+        // opaque so the reads are not hoisted out of the column loop:
+        // holding every limb takes 136 registers)
+        const NLA_LDS double(*wt)[4] = (const NLA_LDS double(*)[4])wl;
+        asm volatile("" : "+v"(wt));
+        double wn[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wn[i] = wt[0][i];
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            double w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) w[i] = wn[i];
+            if (t + 1 < TT) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) wn[i] = wt[t + 1][i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double ct = (double)((cw[t] >> (8 * j)) & 255u);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[j][i] = fma(ct, w[i], v[j][i]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the sums are formed here, not inside each row's store branch
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) asm volatile("" : "+v"(v[j][i]));
+        // every row is computed (no branch), rows past `rows` not stored
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (r0 + j >= a.rows) break;
             const int sc = er[j] + ecc;
-            const double v = rint(ldexp(sv[j], sc));
-            double m = 0.0;
-            int sh = 0;
-            bool bad = !(fabs(v) <= C.smax);
-            if (!bad) {
-                oz_split(v, m, sh);
-                bad = sh >= OZ_PW;
+            const double sint = rint(ldexp(sv[j], sc));
+            const bool bad = !(fabs(sint) <= C.smax);
+            // |S_int| as limbs (each remainder has at most 53 significant bits)
+            const double sa = bad ? 0.0 : fabs(sint);
+            const double s3 = floor(sa * 0x1p-120);
+            double rem = fma(-s3, B120, sa);
+            const double s2 = floor(rem * 0x1p-80);
+            rem = fma(-s2, B80, rem);
+            const double s1 = floor(rem * 0x1p-40);
+            const double s0 = fma(-s1, B40, rem);
+            const double sg = sint < 0.0 ? -1.0 : 1.0;
+            // D = V - S_int
+            double d0 = fma(-sg, s0, v[j][0]), d1 = fma(-sg, s1, v[j][1]);
+            double d2 = fma(-sg, s2, v[j][2]), d3 = fma(-sg, s3, v[j][3]);
+            const double k = rint((fma(d3, B120, d2 * B80) + fma(d1, B40, d0)) * C.invm);
+            // E = D - k M = -R_int, then carries
+            d0 = fma(-k, C.ml[0], d0);
+            d1 = fma(-k, C.ml[1], d1);
+            d2 = fma(-k, C.ml[2], d2);
+            d3 = fma(-k, C.ml[3], d3);
+            double cy = floor(d0 * 0x1p-40);
+            d0 = fma(-cy, B40, d0);
+            d1 += cy;
+            cy = floor(d1 * 0x1p-40);
+            d1 = fma(-cy, B40, d1);
+            d2 += cy;
+            cy = floor(d2 * 0x1p-40);
+            d2 = fma(-cy, B40, d2);
+            d3 += cy;
+            const double e = fma(fma(fma(d3, B40, d2), B40, d1), B40, d0);
+            if (r0 + j < a.rows) {
+                a.R[r0 + j + c * a.ldr] = bad ? 0.0 : ldexp(0.0 - e, -sc);
+                if (bad) *a.flag = 1;
             }
-            if (bad) {
-                *a.flag = 1;
-                a.R[r0 + j + c * a.ldr] = 0.0;
-                continue;
-            }
-            double f1 = 0.0, f2 = 0.0, f3 = 0.0;
-#pragma unroll
-            for (int t = 0; t < OZ_TMAX; ++t) {
-                if (t >= C.T) break;
-                const int p = C.p[t];
-                const int st = oz_residue(m, sh, C, t, pw + t * OZ_PW);
-                const int ct = (int)((cw[t] >> (8 * j)) & 255u);
-                const int d = st - ct + p;  // in (0, 2p)
-                const double u = (double)oz_mod24((int)__umul24((unsigned)d, (unsigned)C.y[t]), p, C.invpf[t]);
-                f1 = fma(u, C.c1[t], f1);
-                f2 = fma(u, C.c2[t], f2);
-                f3 = fma(u, C.c3[t], f3);
-            }
-            const double q = rint(f1 + f2);
-            const double frac = ((f1 - q) + f2) + f3;
-            a.R[r0 + j + c * a.ldr] = ldexp(frac * C.M, -sc);
         }
     }
 }
@@ -603,24 +658,38 @@ OzPlan oz_plan(int64_t n, int64_t m) {
         for (int s = 0; s < T; ++s)
             if (s != t) mt = (mt * (OZ_MODULI[s] % p)) % p;
         C.y[t] = (int)modinv(mt, p);
-        // 1/p = q1 2^-45 + q2 2^-86 + q3 2^-127 + O(2^-128), each q by
-        // rounding the exact remainder (integers, no floating rounding)
-        const int64_t q1 = ((int64_t)1 << 45) / p + ((((int64_t)1 << 45) % p) * 2 >= p ? 1 : 0);
-        const int64_t n1 = ((int64_t)1 << 45) - q1 * p;  // |n1| <= p/2
-        const int64_t a2 = n1 * ((int64_t)1 << 41);
-        int64_t q2 = a2 / p;
-        int64_t r2 = a2 - q2 * p;
-        if (2 * r2 >= p) { ++q2; r2 -= p; }
-        if (2 * r2 < -p) { --q2; r2 += p; }
-        const int64_t a3 = r2 * ((int64_t)1 << 41);
-        int64_t q3 = a3 / p;
-        int64_t r3 = a3 - q3 * p;
-        if (2 * r3 >= p) ++q3;
-        if (2 * r3 < -p) --q3;
-        C.c1[t] = std::ldexp((double)q1, -45);
-        C.c2[t] = std::ldexp((double)q2, -86);
-        C.c3[t] = std::ldexp((double)q3, -127);
     }
+    // w_t = y_t prod_{s != t} p_s and M as base-2^40 limbs (32-bit words
+    // on the host: M < 256^20 = 2^160)
+    constexpr int NWD = 6;
+    auto limbs40 = [](const uint32_t *x, double *out) {
+        for (int i = 0; i < 4; ++i) {
+            uint64_t v = 0;
+            for (int b = 0; b < 40; ++b) {
+                const int bit = 40 * i + b;
+                v |= (uint64_t)((x[bit / 32] >> (bit % 32)) & 1u) << b;
+            }
+            out[i] = (double)v;
+        }
+    };
+    auto mul_small = [](uint32_t *x, uint32_t f) {
+        uint64_t carry = 0;
+        for (int i = 0; i < NWD; ++i) {
+            const uint64_t v = (uint64_t)x[i] * f + carry;
+            x[i] = (uint32_t)v;
+            carry = v >> 32;
+        }
+    };
+    uint32_t mw[NWD] = {1, 0, 0, 0, 0, 0};
+    for (int t = 0; t < T; ++t) mul_small(mw, (uint32_t)OZ_MODULI[t]);
+    limbs40(mw, C.ml);
+    for (int t = 0; t < T; ++t) {
+        uint32_t w[NWD] = {(uint32_t)C.y[t], 0, 0, 0, 0, 0};
+        for (int s = 0; s < T; ++s)
+            if (s != t) mul_small(w, (uint32_t)OZ_MODULI[s]);
+        limbs40(w, C.w[t]);
+    }
+    C.invm = 1.0 / M;
     return P;
 }
 
@@ -763,10 +832,15 @@ hipError_t oz_reconstruct(const OzPlan &P, const OzLayout &L, char *ws, bool out
     c.ldc = out_t ? P.mpad : P.npad;
     c.er = out_t ? fx : ea;
     c.ec = out_t ? ea : fx;
-    c.pw = (const uint8_t *)(ws + L.pw);
     c.flag = (int *)(ws + L.flag);
     const unsigned gy = (unsigned)(c.cols < 65535 ? c.cols : 65535);
-    hipLaunchKernelGGL(oz_crt_kernel, dim3((unsigned)((c.rows + 1023) / 1024), gy), dim3(256), 0, s, c, P.C);
+    const dim3 grid((unsigned)((c.rows + 1023) / 1024), gy);
+    switch (P.T) {
+        case 15: hipLaunchKernelGGL(oz_crt_kernel<15>, grid, dim3(256), 0, s, c, P.C); break;
+        case 16: hipLaunchKernelGGL(oz_crt_kernel<16>, grid, dim3(256), 0, s, c, P.C); break;
+        case 17: hipLaunchKernelGGL(oz_crt_kernel<17>, grid, dim3(256), 0, s, c, P.C); break;
+        default: hipLaunchKernelGGL(oz_crt_kernel<OZ_TMAX>, grid, dim3(256), 0, s, c, P.C); break;
+    }
     return hipGetLastError();
 }
 
