@@ -95,6 +95,29 @@ def test_low_precision_solve_accuracy(nla):
     assert bwd["fp16x2"] / bwd["fp32"] < 16.0, bwd         # about 2 bits apart
 
 
+@pytest.mark.parametrize("side,uplo,trans", COMBOS)
+def test_large_update_tiles_every_orientation(nla, side, uplo, trans):
+    """bf16 / fp16 updates of at least 1024 x 1024 run in 256 x 256 tiles,
+    here with ragged edges (n = 2300, 1100 right-hand sides) and every operand
+    orientation.  With no refinement
+    step each solve keeps its precision's accuracy relative to fp32, so every
+    tile computed the right product.  Here the fp32 panels set the floor
+    (fp16 lands 60-100 times above fp32), and the fp16 pairs sit on it."""
+    from nextla_amd.mixed import rectrxm_mixed_
+    n, m = 2300, 1100
+    A, B = case(n, m, side, uplo, 100.0, seed=19)
+    bwd = {}
+    for prec in ("bf16", "fp16", "fp16x2", "fp32"):
+        Bd = dev(B)
+        _, info = rectrxm_mixed_(side, uplo, trans, 1.0, dev(A), Bd, precision=prec, tol=1e-14, max_iter=0,
+                                 raise_on_stall=False)
+        bwd[prec] = O.backward_error(side, uplo, trans, 1.0, A, B, host(Bd))
+    assert bwd["bf16"] < 1e-3, bwd
+    assert 3.0 < bwd["bf16"] / bwd["fp16"] < 40.0, bwd
+    assert bwd["fp16"] / bwd["fp16x2"] > 20.0, bwd
+    assert bwd["fp16x2"] / bwd["fp32"] < 4.0, bwd
+
+
 def test_not_converged_reports_last_iterate(nla):
     from nextla_amd.mixed import rectrxm_mixed_
     A, B = case(700, 5, "L", "L", 100.0, seed=3)
