@@ -306,8 +306,6 @@ hipError_t launch_sys_eta(int nch, int64_t nsys, const double *pr, const double 
 constexpr int OZ_TMAX = 20;
 constexpr int OZ_MODULI[OZ_TMAX] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
                                     223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
-// Entries of the 2^s mod p tables (s < OZ_PW).
-constexpr int OZ_PW = 128;
 // Largest order: int32 sums of K products of residues (|r| <= 128) stay
 // exact for K <= 2^17; 2^16 also keeps the grids of every launch in range.
 constexpr int64_t OZ_N_MAX = 65536;
@@ -320,6 +318,8 @@ struct OzConst {
     int T;
     int p[OZ_TMAX];
     int y[OZ_TMAX];  // (M / p_t)^-1 mod p_t
+    int rc[OZ_TMAX][3];  // centred residues of 2^16, 2^32, 2^48 modulo p_t
+    int rhi[OZ_TMAX];    // centred residues lie in [rhi - p + 1, rhi], rhi = floor((p - 1) / 2)
     float invpf[OZ_TMAX];
     double invpd[OZ_TMAX];
     // w_t = y_t (M / p_t) (the CRT basis: w_t = 1 mod p_t, 0 mod the
@@ -345,7 +345,7 @@ OzPlan oz_plan(int64_t n, int64_t m);
 
 // Workspace offsets (bytes from the workspace start) of an OzPlan.
 struct OzLayout {
-    int64_t a, x, c, ea, fx, mxa, mxx, pw, flag, queue, total;
+    int64_t a, x, c, ea, fx, mxa, mxx, flag, queue, total;
 };
 OzLayout oz_layout(const OzPlan &P, int64_t base);
 

@@ -48,43 +48,37 @@ __device__ __forceinline__ int oz_mod53(double v, double p, double invp) {
     return r;
 }
 
-// x mod p in [0, p) for 0 <= x < 2^24 (exact in float); invp = 1/p.  The
-// quotient is below 2^17 and p below 2^9: a 24-bit multiply (full rate).
-__device__ __forceinline__ int oz_mod24(int x, int p, float invp) {
-    const int q = (int)((float)x * invp);
-    int r = x - (int)__umul24((unsigned)q, (unsigned)p);
-    r += (r < 0) ? p : 0;
-    r -= (r >= p) ? p : 0;
-    return r;
+// An integer-valued double |v| <= 2^62 as v = l3 2^48 + l2 2^32 + l1 2^16 + l0
+// with l0..l2 in [0, 2^16) and l3 in [-2^14, 2^14] (each remainder has at
+// most 53 significant bits, so every fma is exact).
+__device__ __forceinline__ void oz_limbs16(double v, int &l0, int &l1, int &l2, int &l3) {
+    const double h3 = floor(v * 0x1p-48);
+    double r = fma(-h3, 0x1p48, v);
+    const double h2 = floor(r * 0x1p-32);
+    r = fma(-h2, 0x1p32, r);
+    const double h1 = floor(r * 0x1p-16);
+    l0 = (int)fma(-h1, 0x1p16, r);
+    l1 = (int)h1;
+    l2 = (int)h2;
+    l3 = (int)h3;
 }
 
-// An integer-valued double v as m * 2^sh with |m| < 2^53 integer, sh >= 0.
-__device__ __forceinline__ void oz_split(double v, double &m, int &sh) {
-    sh = 0;
-    m = v;
-    if (fabs(v) >= 0x1p53) {
-        sh = ilogb(v) - 52;
-        m = ldexp(v, -sh);
-    }
-}
-
-// Residue of m * 2^sh modulo p_t: pw = the 2^s mod p_t table of modulus t.
-__device__ __forceinline__ int oz_residue(double m, int sh, const OzConst &C, int t, const uint8_t *pw) {
-    const int r = oz_mod53(m, (double)C.p[t], C.invpd[t]);
-    if (sh == 0) return r;
-    return oz_mod24((int)__umul24((unsigned)r, (unsigned)pw[sh]), C.p[t], C.invpf[t]);
-}
-
-// ---- tables -----------------------------------------------------------------
-
-// pw[t*OZ_PW + s] = 2^s mod p_t, s < OZ_PW.
-__global__ void oz_pow2_kernel(OzConst C, uint8_t *pw) {
-    const int t = blockIdx.x, s = threadIdx.x;
-    if (t >= C.T || s >= OZ_PW) return;
+// The centred residue of v (given by its limbs) modulo p_t, in [rlo_t, rhi_t]
+// (rhi_t = floor((p_t - 1) / 2), rlo_t = rhi_t - p_t + 1).  With the limb
+// weights' centred residues c_i (|c_i| <= 128), x = l0 + sum l_i c_i = v
+// (mod p_t) and |x| < 2^25; q = round(x / p_t) in float is off by at most
+// one and one conditional step each way brings x - q p_t into range.  All
+// 24-bit multiply-adds (full rate).
+__device__ __forceinline__ int oz_residue16(int l0, int l1, int l2, int l3, const OzConst &C, int t) {
+    int x = __mul24(l1, C.rc[t][0]) + l0;
+    x = __mul24(l2, C.rc[t][1]) + x;
+    x = __mul24(l3, C.rc[t][2]) + x;
     const int p = C.p[t];
-    int v = 1 % p;
-    for (int i = 0; i < s; ++i) v = (2 * v) % p;
-    pw[t * OZ_PW + s] = (uint8_t)v;
+    const int q = __float2int_rn((float)x * C.invpf[t]);
+    int r = __mul24(q, -p) + x;
+    r -= (r > C.rhi[t]) ? p : 0;
+    r += (r < C.rhi[t] - p + 1) ? p : 0;
+    return r;
 }
 
 // ---- scales -----------------------------------------------------------------
@@ -209,13 +203,10 @@ __device__ __forceinline__ bool oz_tile_stored(int panel, int64_t o0, int64_t k0
 // rounded value round(src 2^e) 2^-e replaces src (the refinement's iterate).
 // Grid: (npad_k/64, npad_o/64) over the whole padded destination, so the
 // padding holds zeros.
-__global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, OzDst d, int writeback, OzConst C,
-                                                         const uint8_t *pw_g) {
+__global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, OzDst d, int writeback, OzConst C) {
     __shared__ double tile[64][65];
-    __shared__ uint8_t pw[OZ_TMAX * 16];
     const int64_t o0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
     if (!oz_tile_stored(d.panel, o0, k0)) return;
-    for (int i = threadIdx.x; i < C.T * 16; i += 256) pw[i] = pw_g[(i >> 4) * OZ_PW + (i & 15)];
     oz_load_tile(s, o0, k0, tile);
     __syncthreads();
     const int oo = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
@@ -223,26 +214,24 @@ __global__ __launch_bounds__(256) void oz_residue_kernel(OzSrc s, const int *e, 
     const int eraw = (o < s.no) ? e[o] : 0;
     const bool bad = (eraw == OZ_BAD_EXP);  // a non-finite row: residues 0, values kept
     const int eo = bad ? 0 : eraw;
-    double m[16];
-    int sh[16];
+    // |v| <= 2^62: the scales keep every rounded entry within 2^bA / 2^bX
+    int l0[16], l1[16], l2[16], l3[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         const double v = bad ? 0.0 : rint(ldexp(tile[oo][kq + u], eo));
-        oz_split(v, m[u], sh[u]);
+        oz_limbs16(v, l0[u], l1[u], l2[u], l3[u]);
         if (writeback && !bad) tile[oo][kq + u] = ldexp(v, -eo);
     }
     const int64_t off = oz_tiled_off(d.panel, d.npad, o, k0 + kq);
     for (int t = 0; t < C.T; ++t) {
-        const int p = C.p[t];
-        const int half = p >> 1;  // residues above half map to r - p
         v4i packed;
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
             unsigned word = 0;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                int r = oz_residue(m[4 * w + b], sh[4 * w + b], C, t, pw + t * 16);
-                r = (r > half || (r == half && (p & 1) == 0)) ? r - p : r;
+                const int u = 4 * w + b;
+                const int r = oz_residue16(l0[u], l1[u], l2[u], l3[u], C, t);
                 word |= (unsigned)(r & 255) << (8 * b);
             }
             packed[w] = (int)word;
@@ -658,6 +647,13 @@ OzPlan oz_plan(int64_t n, int64_t m) {
         for (int s = 0; s < T; ++s)
             if (s != t) mt = (mt * (OZ_MODULI[s] % p)) % p;
         C.y[t] = (int)modinv(mt, p);
+        // centred residues of 2^16, 2^32, 2^48 and the centred range's top
+        C.rhi[t] = (int)((p - 1) / 2);
+        int64_t w = 1;
+        for (int i = 0; i < 3; ++i) {
+            w = (w * 65536) % p;
+            C.rc[t][i] = (int)(w > C.rhi[t] ? w - p : w);
+        }
     }
     // w_t = y_t prod_{s != t} p_s and M as base-2^40 limbs (32-bit words
     // on the host: M < 256^20 = 2^160)
@@ -704,7 +700,6 @@ OzLayout oz_layout(const OzPlan &P, int64_t base) {
     L.fx = o;    o = al(o + P.mpad * 4);
     L.mxa = o;   o = al(o + P.npad * 8);
     L.mxx = o;   o = al(o + P.mpad * 8);
-    L.pw = o;    o = al(o + OZ_TMAX * OZ_PW);
     L.flag = o;  o = al(o + 4);
     L.queue = o; o = al(o + 8 * 4);
     L.total = o;
@@ -731,17 +726,13 @@ hipError_t oz_prepare(const OzPlan &P, const OzLayout &L, char *ws, const double
                       bool lower, int64_t n, hipStream_t s) {
     hipError_t err = oz_fill(ws + L.flag, 1, s);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(oz_pow2_kernel, dim3(P.T), dim3(OZ_PW), 0, s, P.C, (uint8_t *)(ws + L.pw));
-    err = hipGetLastError();
-    if (err != hipSuccess) return err;
     OzSrc src{const_cast<double *>(A), ars, acs, n, n, lower ? 1 : 2};
     int *ea = (int *)(ws + L.ea);
     err = oz_scales(src, (unsigned long long *)(ws + L.mxa), P.bA, ea, (int *)(ws + L.flag), s);
     if (err != hipSuccess) return err;
     OzDst d{(int8_t *)(ws + L.a), P.aplane, P.npad, lower ? 1 : 2};
     dim3 g((unsigned)(P.npad / 64), (unsigned)(P.npad / 64));
-    hipLaunchKernelGGL(oz_residue_kernel, g, dim3(256), 0, s, src, (const int *)ea, d, 0, P.C,
-                       (const uint8_t *)(ws + L.pw));
+    hipLaunchKernelGGL(oz_residue_kernel, g, dim3(256), 0, s, src, (const int *)ea, d, 0, P.C);
     return hipGetLastError();
 }
 
@@ -789,7 +780,7 @@ hipError_t oz_convert_x(const OzPlan &P, const OzLayout &L, char *ws, double *X,
     if (err != hipSuccess) return err;
     OzDst d{(int8_t *)(ws + L.x), P.xplane, P.npad, 0};
     hipLaunchKernelGGL(oz_residue_kernel, dim3((unsigned)(P.npad / 64), (unsigned)(P.mpad / 64)), dim3(256), 0, s,
-                       xs, (const int *)fx, d, 1, P.C, (const uint8_t *)(ws + L.pw));
+                       xs, (const int *)fx, d, 1, P.C);
     return hipGetLastError();
 }
 
