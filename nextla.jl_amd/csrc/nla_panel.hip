@@ -23,6 +23,10 @@ namespace nla {
 
 #define NLA_LDS __attribute__((address_space(3)))
 
+// The source of the M-operand elements outside a block (or its triangle):
+// reading zeros there leaves nothing to mask when the fragment is used.
+static __device__ __align__(16) const double panel_zero[2] = {0.0, 0.0};
+
 // 16x16x4 f32 MFMA (2048 flops, 32 cycles): D[i][j] in lane j+16*(i/4),
 // register i%4 (verified on gfx950 by a lane-layout probe).
 struct Mfma16f {
@@ -55,26 +59,22 @@ struct PanelShape {
     __device__ static inline int cb0(int) { return 0; }
 };
 
-// Phase p of the blocked algorithm for nsb 64-blocks.  kind: 0 = diagonal
+// A phase of the blocked algorithm for nsb 64-blocks.  kind: 0 = diagonal
 // (TRSM: inv(M_kk) block; TRMM: masked M_ii), 1 = off-diagonal M_ik.
 struct Phase {
     int i, k, kind;
     bool first, last;  // first / last phase of an output block group
 };
 
-__device__ inline Phase phase_of(int p, int nsb, bool lower, bool trmm) {
+// Phase (step, q) of the blocked algorithm, walked in order with phase_next
+// (step s holds s + 1 phases for TRSM, nsb - s for TRMM).
+__device__ inline Phase phase_at(int step, int q, int nsb, bool lower, bool trmm) {
     Phase ph;
     if (!trmm) {
         // Left-looking: output block i (top-down for lower, bottom-up for
         // upper) gathers its sources k (already solved) in registers, then
         // its diagonal phase subtracts them from B_i and applies inv(M_ii).
         // Step s has s source phases and one diagonal phase.
-        int step = 0, base = 0;
-        while (p >= base + step + 1) {
-            base += step + 1;
-            ++step;
-        }
-        const int q = p - base;
         const int i = lower ? step : nsb - 1 - step;
         ph.i = i;
         if (q < step) {
@@ -85,19 +85,12 @@ __device__ inline Phase phase_of(int p, int nsb, bool lower, bool trmm) {
         } else {
             ph.k = i;
             ph.kind = 0;
-            ph.first = (step == 0);  // diagonal phase of a block with no sources
+            ph.first = (step == 0);
             ph.last = true;
         }
     } else {
         // output block i (bottom-up for lower): diagonal, then its sources
-        int step = 0, base = 0;
-        auto cnt = [&](int s) { return nsb - s; };  // diagonal + sources (k < i lower, k > i upper)
-        while (p >= base + cnt(step)) {
-            base += cnt(step);
-            ++step;
-        }
-        int q = p - base;
-        int i = lower ? nsb - 1 - step : step;
+        const int i = lower ? nsb - 1 - step : step;
         ph.i = i;
         if (q == 0) {
             ph.k = i;
@@ -107,9 +100,17 @@ __device__ inline Phase phase_of(int p, int nsb, bool lower, bool trmm) {
             ph.kind = 1;
         }
         ph.first = (q == 0);
-        ph.last = (q == cnt(step) - 1);
+        ph.last = (q == nsb - step - 1);
     }
     return ph;
+}
+
+__device__ inline void phase_next(int &step, int &q, int nsb, bool trmm) {
+    ++q;
+    if (trmm ? (q >= nsb - step) : (q > step)) {
+        ++step;
+        q = 0;
+    }
 }
 
 __device__ inline int phase_count(int nsb, bool trmm) {
@@ -197,41 +198,56 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
 
     // M-operand fragment of phase ph: op(i, k) for this lane's row.  Elements
     // outside the block (or, for the TRMM diagonal, outside the triangle) are
-    // read from a valid address and flagged in *okb; they are zeroed when the
-    // fragment is used, one phase later, so no instruction waits for the
-    // loads before then.
-    auto fetch = [&](const Phase &ph, T *af, uint32_t *okb) {
+    // read from panel_zero, so no instruction waits for the loads before the
+    // fragment is used, one phase later.
+    //
+    // The lane's elements of a phase are op(i, kb + KD sw), sw < NKW, each
+    // KD * acs past the last: one 64-bit add per load (a multiply per load
+    // before, at a quarter of the rate).  Which of them lie inside the block
+    // (and its triangle) is a prefix / suffix of sw, computed once per phase
+    // as a bit mask.
+    static_assert(MF::KD == 4, "four k per MFMA step");
+    const int fi = rb * MF::BD + lo;            // the lane's row of a block
+    const int kb = kh * NKW * MF::KD + lk;      // the lane's first k
+    auto span = [](int c) { return c <= 0 ? 0u : (c >= NKW ? (NKW >= 32 ? ~0u : (1u << NKW) - 1u) : (1u << c) - 1u); };
+    // global address space spelled out: a select between the block and
+    // panel_zero otherwise degrades the loads to flat ones; its address is
+    // taken once (the compiler would rebuild it before every select)
+    typedef const __attribute__((address_space(1))) T GT;
+    GT *zp = (GT *)panel_zero;
+    asm volatile("" : "+v"(zp));
+    // the lane's offset and the step between its k in M and W blocks
+    const int64_t offM = (int64_t)fi * mrs + (int64_t)kb * mcs, stM = (int64_t)MF::KD * mcs;
+    const int64_t offW = (int64_t)fi * wrs + (int64_t)kb * wcs, stW = (int64_t)MF::KD * wcs;
+    auto fetch = [&](const Phase &ph, T *af) {
         const T *A;
-        int64_t ars, acs;
+        int64_t off, st;
         int tri = 0;
         if (ph.kind == 0 && !TRMM) {
             A = Wp + (int64_t)ph.k * 64 * 64;
-            ars = wrs;
-            acs = wcs;
+            off = offW;
+            st = stW;
         } else {
             A = Mp + (int64_t)(64 * ph.i) * mrs + (int64_t)(64 * ph.k) * mcs;
-            ars = mrs;
-            acs = mcs;
+            off = offM;
+            st = stM;
             if (ph.kind == 0) tri = lower ? 1 : 2;
         }
         const int ni = bsize(ph.i), nk = bsize(ph.k);
-        const int i = rb * MF::BD + lo;
-        uint32_t bits = 0;
+        // k < nk: the first ceil((nk - kb) / 4) steps; a unit diagonal is not
+        // read (the strict triangle here, the block itself added when the
+        // result is written)
+        uint32_t bits = fi < ni ? span((nk - kb + 3) >> 2) : 0u;
+        if (tri == 1) bits &= span((((unit ? fi - 1 : fi) - kb) >> 2) + 1);             // k <= i (k < i)
+        if (tri == 2) bits &= ~span(((unit ? fi + 1 : fi) - kb + 3) >> 2);            // k >= i (k > i)
+        GT *p = (GT *)(A + off);
 #pragma unroll
         for (int sw = 0; sw < NKW; ++sw) {
-            const int s = kh * NKW + sw;
-            const int k = s * MF::KD + lk;
-            bool ok = i < ni && k < nk;
-            // a unit diagonal is not read: the strict triangle here, and the
-            // block itself added when the result is written
-            if (tri == 1) ok = ok && (unit ? k < i : k <= i);
-            if (tri == 2) ok = ok && (unit ? k > i : k >= i);
-            af[sw] = *(ok ? A + i * ars + k * acs : A);
-            bits |= (ok ? 1u : 0u) << sw;
+            af[sw] = *(((bits >> sw) & 1u) ? p : zp);
+            p += st;
         }
-        *okb = bits;
     };
-    auto compute = [&](const Phase &ph, const T *af, uint32_t okb) {
+    auto compute = [&](const Phase &ph, const T *af) {
         const NLA_LDS T *src = bs + ph.k * 64;
         // all LDS operands of the phase are requested before the first MFMA
         T bv[NKW][PS::CBW];
@@ -245,7 +261,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         __builtin_amdgcn_s_setprio(1);  // issue priority while in the MFMA cluster
 #pragma unroll
         for (int sw = 0; sw < NKW; ++sw) {
-            const T a = ((okb >> sw) & 1u) ? af[sw] : T(0);
+            const T a = af[sw];
 #pragma unroll
             for (int cb = 0; cb < PS::CBW; ++cb) {
                 if (sw & 1) acc2[cb] = MF::mma(a, bv[sw][cb], acc2[cb]);
@@ -354,11 +370,11 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         for (int j = 0; j < NC; ++j) blk[(cg + NW * j) * LDB + r] = x[j];
         __syncthreads();  // X_i complete for the blocks that follow
     };
-    auto run = [&](const Phase &ph, const T *af, uint32_t okb) {
+    auto run = [&](const Phase &ph, const T *af) {
         if (!TRMM) {
             if (ph.kind == 1) {  // acc += M_ik X_k, kept in registers
                 if (ph.first) zero_acc();
-                compute(ph, af, okb);
+                compute(ph, af);
             } else {  // B_i -= acc; X_i = inv(M_ii) B_i, in place
                 if (!ph.first) write_block(ph.i, T(-1), T(1));
                 __syncthreads();
@@ -367,7 +383,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
                     return;
                 }
                 zero_acc();
-                compute(ph, af, okb);
+                compute(ph, af);
                 __syncthreads();  // every wave has read B_i
                 write_block(ph.i, T(1), T(0));
                 __syncthreads();  // X_i complete for the blocks that follow
@@ -375,7 +391,7 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
             return;
         }
         if (ph.first) zero_acc();
-        compute(ph, af, okb);
+        compute(ph, af);
         if (ph.last) {
             __syncthreads();  // every wave has read B_i
             write_block(ph.i, T(1), unit ? T(1) : T(0));
@@ -392,27 +408,30 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
     constexpr int PD = 2;
     const int np = phase_count(nsb, TRMM);
     T fr[PD][NKW];
-    uint32_t okr[PD];
     Phase phr[PD];
+    // the next phase to fetch, as (step, q) and its index
+    int fstep = 0, fq = 0, fp = 0;
+    auto fetch_next = [&](int slot) {
+        phr[slot] = phase_at(fstep, fq, nsb, lower, TRMM);
+        fetch(phr[slot], fr[slot]);
+        if (fp + 1 < np) {  // past the last phase, repeat it
+            phase_next(fstep, fq, nsb, TRMM);
+            ++fp;
+        }
+    };
 #pragma unroll
-    for (int j = 0; j < PD - 1; ++j) {
-        phr[j] = phase_of(j < np ? j : np - 1, nsb, lower, TRMM);
-        fetch(phr[j], fr[j], &okr[j]);
-    }
+    for (int j = 0; j < PD - 1; ++j) fetch_next(j);
     const int np_full = np - np % PD;
     for (int p = 0; p < np_full; p += PD) {
 #pragma unroll
         for (int u = 0; u < PD; ++u) {
-            const int nxt = p + u + PD - 1;
-            const int slot = (u + PD - 1) % PD;
-            phr[slot] = phase_of(nxt < np ? nxt : np - 1, nsb, lower, TRMM);
-            fetch(phr[slot], fr[slot], &okr[slot]);
-            run(phr[u], fr[u], okr[u]);
+            fetch_next((u + PD - 1) % PD);
+            run(phr[u], fr[u]);
         }
     }
 #pragma unroll
     for (int u = 0; u < PD - 1; ++u)
-        if (np_full + u < np) run(phr[u], fr[u], okr[u]);
+        if (np_full + u < np) run(phr[u], fr[u]);
 
     // ---- write the strip back ----
     T vmax = T(0);  // largest |X| written (fp32 TRSM with xmax)
