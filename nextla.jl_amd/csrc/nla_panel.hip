@@ -241,10 +241,18 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         if (tri == 1) bits &= span((((unit ? fi - 1 : fi) - kb) >> 2) + 1);             // k <= i (k < i)
         if (tri == 2) bits &= ~span(((unit ? fi + 1 : fi) - kb + 3) >> 2);            // k >= i (k > i)
         GT *p = (GT *)(A + off);
+        if (tri == 0 && ni == 64 && nk == 64) {  // a whole block (uniform): nothing to select
 #pragma unroll
-        for (int sw = 0; sw < NKW; ++sw) {
-            af[sw] = *(((bits >> sw) & 1u) ? p : zp);
-            p += st;
+            for (int sw = 0; sw < NKW; ++sw) {
+                af[sw] = *p;
+                p += st;
+            }
+        } else {
+#pragma unroll
+            for (int sw = 0; sw < NKW; ++sw) {
+                af[sw] = *(((bits >> sw) & 1u) ? p : zp);
+                p += st;
+            }
         }
     };
     auto compute = [&](const Phase &ph, const T *af) {
