@@ -378,31 +378,35 @@ __device__ __forceinline__ void panel_strip(int nbig, int64_t m, int64_t c0, con
         for (int j = 0; j < NC; ++j) blk[(cg + NW * j) * LDB + r] = x[j];
         __syncthreads();  // X_i complete for the blocks that follow
     };
+    // The accumulators are zero when an output block's first phase starts:
+    // cleared once here and after every write that consumes them (a
+    // conditional clear in the source phases cost a select per register).
+    zero_acc();
     auto run = [&](const Phase &ph, const T *af) {
         if (!TRMM) {
             if (ph.kind == 1) {  // acc += M_ik X_k, kept in registers
-                if (ph.first) zero_acc();
                 compute(ph, af);
             } else {  // B_i -= acc; X_i = inv(M_ii) B_i, in place
                 if (!ph.first) write_block(ph.i, T(-1), T(1));
+                zero_acc();
                 __syncthreads();
                 if (Fp != nullptr && Fp[ph.i] != 0) {
                     solve_refined(ph.i);
                     return;
                 }
-                zero_acc();
                 compute(ph, af);
                 __syncthreads();  // every wave has read B_i
                 write_block(ph.i, T(1), T(0));
+                zero_acc();
                 __syncthreads();  // X_i complete for the blocks that follow
             }
             return;
         }
-        if (ph.first) zero_acc();
         compute(ph, af);
         if (ph.last) {
             __syncthreads();  // every wave has read B_i
             write_block(ph.i, T(1), unit ? T(1) : T(0));
+            zero_acc();
             __syncthreads();
         }
     };
