@@ -564,14 +564,15 @@ int32_t nla_serve_destroy(nla_server server);
  *                      one fused panel launch (default, needs leaf mode 1),
  *                      0 = recurse to 64-leaves;
  *   NLA_CFG_PANEL_ROWS_F32 / _F64: panel size for that dtype, a multiple of
- *                      64 in [64, 1024] (defaults 256 for FP32, 512 for FP64;
- *                      above 512 rows FP64 panels use 16 columns per group);
+ *                      64 in [64, 1024], or 0 (default): by layout, 512
+ *                      where the panel's rows are contiguous (left side
+ *                      with trans 'N', right side with 'T'), else 256;
+ *                      above 512 rows FP64 panels use 16 columns per group;
  *   NLA_CFG_PANEL_STRIP_F32 / _F64: right-hand-side columns per panel
  *                      work-group, 16 (default) or 32;
  *   NLA_CFG_PANEL_WAVES: waves per panel work-group (panels up to 512 rows):
  *                      4, or 8, where two waves share each 16-row output
- *                      tile and split its k range; 0 (default) = 4 for FP64
- *                      TRSM, 8 otherwise;
+ *                      tile and split its k range; 0 (default) = 4;
  *   NLA_CFG_BATCH_CLASSES: size classes new batch plans may use, bit c for
  *                      systems up to 128 << c rows (c = 0..3; the 1024 class
  *                      is always on; default 0x1F: all, with bit 4).  A system goes to the

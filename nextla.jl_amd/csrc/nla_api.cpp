@@ -111,16 +111,23 @@ struct View {
 static std::atomic<int> g_leaf_mode{-1};
 static std::atomic<int> g_panel{-1};
 // Panel shape per dtype (index NLA_F32 / NLA_F64): the largest diagonal
-// block handed to one fused panel launch (a multiple of 64, <= PANEL_MAX)
-// and its RHS columns per work-group.  Defaults from a sweep on MI355X at
-// n = 8192 (rows 256/384/512 x strips 16/32): FP64 512 rows, FP32 256 rows
-// (3-5 % faster than 512 there), 16 columns for both.
-static std::atomic<int> g_panel_rows[2] = {{256}, {512}};
+// block handed to one fused panel launch (a multiple of 64, <= PANEL_MAX,
+// or 0: by layout) and its RHS columns per work-group.  By layout: where the
+// panel's rows are contiguous in memory (left side with op(A) = A, right
+// side with op(A) = A^T) its fragment loads are coalesced and 512 rows win;
+// otherwise every fragment element is its own cache line and 256 rows
+// win.  Measured on MI355X at n = 8192, 4 waves (rows 256 / 512): FP32 TRSM
+// L-L-N 5.06 / 4.83 ms, R-L-T 5.48 / 5.17, L-L-T 5.49 / 6.06, R-L-N
+// 5.84 / 6.33; FP64 TRSM L-L-N 9.39 / 9.10, L-L-T 9.89 / 10.10, R-L-N
+// 10.08 / 10.19; FP64 TRMM L-L-T 9.96 / 10.22.  16 columns for both dtypes.
+static std::atomic<int> g_panel_rows[2] = {{0}, {0}};
+static int panel_rows_default(bool contiguous) { return contiguous ? 512 : 256; }
 static std::atomic<int> g_panel_strip[2] = {{PANEL_STRIP}, {PANEL_STRIP}};
-// Waves per panel work-group: 0 = by operation (measured on MI355X at
-// n = 8192: 8 waves, splitting each phase's k range, are 1-3 % faster for
-// FP32 TRSM and for TRMM; FP64 TRSM is 1.4 % faster with 4, its diagonal
-// steps paying for the extra barriers), else 4 or 8 for every panel.
+// Waves per panel work-group: 0 = the default, 4 (measured on MI355X at
+// n = 8192 with 512-row panels: 4 waves against 8, which split each phase's
+// k range, FP32 TRSM 4.83 / 4.95 ms, FP32 TRMM 4.80 / 4.87, FP64 TRMM
+// 9.03 / 9.10, FP64 TRSM 9.36 / 9.62, the refined solve's fp32 panels
+// 33.1 / 33.5 ms), else 4 or 8 for every panel.
 static std::atomic<int> g_panel_waves{0};
 // Batch size classes: all four, systems above 512 rows staged (measured on
 // MI355X over 10,000 systems of 16..1024 rows: 5.5 ms vs 6.9 ms with
@@ -538,10 +545,13 @@ static int32_t rectrxm_typed(char side, char uplo, char trans, char func, double
     }
     {
         const int di = std::is_same<T, double>::value ? NLA_F64 : NLA_F32;
-        r.panel_rows = g_panel_rows[di].load();
+        const int pr = g_panel_rows[di].load();
+        // the panel's row stride: A's for the left side, its transpose's for the right
+        const bool contiguous = (side == 'L' ? A.rs : A.cs) == 1;
+        r.panel_rows = pr ? pr : panel_rows_default(contiguous);
         r.panel_strip = g_panel_strip[di].load();
         const int pw = g_panel_waves.load();
-        r.panel_waves = pw ? pw : ((std::is_same<T, double>::value && func == 'S') ? 4 : 8);
+        r.panel_waves = pw ? pw : 4;
     }
     T *W = nullptr;
     std::unique_lock<std::mutex> hold;
@@ -769,7 +779,7 @@ static int32_t rectrxm_mixed_impl(char side, char uplo, char trans, int precisio
         r.use_panel = true;
         r.W = W;
         r.F = WF;
-        r.panel_waves = g_panel_waves.load() ? g_panel_waves.load() : 8;
+        r.panel_waves = g_panel_waves.load() ? g_panel_waves.load() : 4;
         const bool scaled = (prec == NLA_LP_FP16 || prec == NLA_LP_FP16X2);
         if (prec != NLA_LP_FP32) r.lp = (prec == NLA_LP_FP16X2) ? 2 : prec;
         if (scaled) {
@@ -1239,7 +1249,7 @@ int32_t nla_set_config(int32_t key, int32_t value) {
         return prev;
     }
     if (key == NLA_CFG_PANEL_ROWS_F32 || key == NLA_CFG_PANEL_ROWS_F64) {
-        if (value < 64 || value > PANEL_MAX || value % 64 != 0) return -1;
+        if (value != 0 && (value < 64 || value > PANEL_MAX || value % 64 != 0)) return -1;
         return g_panel_rows[key == NLA_CFG_PANEL_ROWS_F64 ? NLA_F64 : NLA_F32].exchange(value);
     }
     if (key == NLA_CFG_BATCH_CLASSES) {

@@ -114,6 +114,22 @@ def test_shape_errors(mirror):
                          torch.zeros(3, 2, dtype=torch.float64))
 
 
+def test_panel_rows_by_layout_setting(nla):
+    """NLA_CFG_PANEL_ROWS_*: 0 selects the panel size by layout (the
+    default), a multiple of 64 in [64, 1024] fixes it, anything else is
+    rejected and leaves the setting unchanged."""
+    from nextla_amd import _lib
+    for key in (_lib.NLA_CFG_PANEL_ROWS_F32, _lib.NLA_CFG_PANEL_ROWS_F64):
+        before = _lib.set_config(key, 0)
+        assert before == 0 or (before % 64 == 0 and 64 <= before <= 1024)
+        try:
+            assert _lib.set_config(key, 32) == -1
+            assert _lib.set_config(key, 256) == 0
+            assert _lib.set_config(key, 0) == 256
+        finally:
+            _lib.set_config(key, before)
+
+
 def test_tuner_picks_fastest_and_restores(nla):
     """The tuner's selection and configuration handling, with stand-in timings
     (no GPU): it returns the fastest candidate and leaves the process-wide
