@@ -8,7 +8,11 @@ device at hand and kept per device:
                       solves (a multiple of 64 up to 1024; 256-512 are
                       measured) and the right-hand
                       sides per work-group (16 or 32), per dtype
-                      (NLA_CFG_PANEL_ROWS_*, NLA_CFG_PANEL_STRIP_*);
+                      (NLA_CFG_PANEL_ROWS_*, NLA_CFG_PANEL_STRIP_*); a
+                      tuned size replaces the library's default (0: 512
+                      rows where the panel's rows are contiguous in
+                      memory, 256 where they are strided) for every
+                      layout, so it is tuned on the layout that matters;
   batch classes       which LDS-sized variants of the persistent batched
                       solver a plan may use, and whether systems above 512
                       rows are one 1024-row panel or two 512-row stages
