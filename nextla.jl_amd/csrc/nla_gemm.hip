@@ -83,12 +83,14 @@ struct TileLoader {
                 }
             } else {
                 if (c == 0) okbits = 0;
+                // the chunk's elements are consecutive in memory (the stride
+                // along the chunk is 1): one address per chunk, not a 64-bit
+                // multiply per element
+                const T *pc = P + go * so + gk * sk;
 #pragma unroll
                 for (int e = 0; e < VW; ++e) {
-                    int64_t eo = KCONTIG ? go : go + e;
-                    int64_t ek = KCONTIG ? gk + e : gk;
-                    bool ok = (eo < no) && (ek < nk);
-                    r[c][e] = *(ok ? P + eo * so + ek * sk : P);
+                    const bool ok = (KCONTIG ? go : go + e) < no && (KCONTIG ? gk + e : gk) < nk;
+                    r[c][e] = *(ok ? pc + e : P);
                     okbits |= (ok ? 1u : 0u) << (c * VW + e);
                 }
             }
