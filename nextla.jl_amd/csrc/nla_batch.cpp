@@ -17,8 +17,9 @@
 // independent of the number of systems.  Panel tasks are one per (panel,
 // 16-column strip), grouped by size class (the LDS strip of the kernel that
 // solves a class fits its largest panel, so small panels run at higher
-// occupancy) and longest first within a class; GEMM tasks are one per 64 x 64
-// output tile, longest k-range first.  Descriptors are uploaded in stream
+// occupancy) and longest first within a class; GEMM tasks are one per output
+// tile (64 rows by 32 right-hand sides in FP64, where most systems have few;
+// 64 x 64 in FP32), longest k-range first.  Descriptors are uploaded in stream
 // order into the caller's workspace; executing a plan launches kernels on
 // the caller's stream and nothing else, so it can be captured in a hipGraph
 // and replayed.
@@ -71,7 +72,8 @@ static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, i
         if (func == 'S') L.nblk += (n + 63) / 64;
         if (st > 1) {
             // every stage but one has a GEMM; its output is at most n rows x m
-            const int64_t tn = (m + GROUP_TILE - 1) / GROUP_TILE, tr = (n + GROUP_TILE - 1) / GROUP_TILE;
+            const int64_t tc = group_tile_c((int)es);
+            const int64_t tn = (m + tc - 1) / tc, tr = (n + GROUP_TILE_R - 1) / GROUP_TILE_R;
             L.ngem += st - 1;
             L.ngtasks += (st - 1) * tr * tn;
         }
@@ -102,6 +104,7 @@ struct nla_batch_plan_s {
     int32_t dtype;
     bool trmm;
     bool tx, ty;  // grouped GEMM operand orientation
+    bool rows_m;  // the grouped GEMMs' M runs along the systems' rows (side 'L')
     char *ws;
     int nblk;
     std::vector<nla::Launch> launches;
@@ -279,7 +282,8 @@ static int32_t plan_host(char side, char uplo, char trans, char func, const nla_
             }
             g.alpha = ga;
             g.beta = gb;
-            g.tiles_m = (int)((g.M + GROUP_TILE - 1) / GROUP_TILE);
+            const int tm_edge = group_tile_m(side == 'L', (int)sizeof(T));
+            g.tiles_m = (int)((g.M + tm_edge - 1) / tm_edge);
             gem.push_back(g);
             gem_stage.push_back(s);
             H.gem_item.push_back(i);
@@ -316,7 +320,8 @@ static int32_t plan_host(char side, char uplo, char trans, char func, const nla_
     gtasks.reserve((size_t)L.ngtasks);
     for (int gi : gorder) {
         const GroupGemm<T> &g = gem[(size_t)gi];
-        const int tiles = g.tiles_m * (int)((g.N + GROUP_TILE - 1) / GROUP_TILE);
+        const int tn_edge = group_tile_n(side == 'L', (int)sizeof(T));
+        const int tiles = g.tiles_m * (int)((g.N + tn_edge - 1) / tn_edge);
         for (int t = 0; t < tiles; ++t) gtasks.push_back(make_int2(gi, t));
     }
     if ((int64_t)pan.size() > L.npan || (int64_t)ptasks.size() > L.nptasks || (int64_t)gem.size() > L.ngem ||
@@ -366,6 +371,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
     int32_t st = plan_host<T>(side, uplo, trans, func, items, count, L, (uintptr_t)P->ws, batch_task_order(), H);
     if (st != NLA_OK) return st;
     P->tx = H.tx;
+    P->rows_m = (side == 'L');
     P->ty = H.ty;
     const std::vector<BatchSys<T>> &sys = H.sys, &pan = H.pan;
     const std::vector<BatchTask> &ptasks = H.ptasks;
@@ -384,7 +390,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
     P->launches = H.launches;
     for (Launch &l : P->launches)
         l.groups = l.cls < 4 ? grid(batch_kernel_ptr<T>(trmm, BATCH_CLASSES[l.cls]), l.count)
-                             : grid(grouped_gemm_ptr<T>(P->tx, P->ty, l.vec), l.count);
+                             : grid(grouped_gemm_ptr<T>(P->tx, P->ty, l.vec, P->rows_m), l.count);
     // one staged image of the descriptor region, copied in stream order
     P->host.assign((size_t)(L.inv - L.sys), 0);
     char *h = P->host.data() - L.sys;
@@ -418,7 +424,8 @@ static hipError_t enqueue(nla_batch_plan_s *P, hipStream_t s) {
         if (l.cls < 4)
             e = launch_batch<T>(pan, ptasks + l.start, l.count, ctr, P->trmm, BATCH_CLASSES[l.cls], l.groups, s);
         else
-            e = launch_grouped_gemm<T>(gem, gtasks + l.start, l.count, ctr, P->tx, P->ty, l.vec, l.groups, s);
+            e = launch_grouped_gemm<T>(gem, gtasks + l.start, l.count, ctr, P->tx, P->ty, l.vec, P->rows_m, l.groups,
+                                       s);
     }
     return e;
 }
@@ -524,8 +531,10 @@ int64_t nla_batch_plan_describe(char side, char uplo, char trans, char func, int
                     const int2 &gt = H.gtasks[(size_t)t];
                     const GroupGemm<T> &g = H.gem[(size_t)gt.x];
                     const int64_t tm = gt.y % g.tiles_m, tn = gt.y / g.tiles_m;
-                    const int64_t a0 = tm * GROUP_TILE, b0 = tn * GROUP_TILE;
-                    const int64_t an = std::min<int64_t>(GROUP_TILE, g.M - a0), bn = std::min<int64_t>(GROUP_TILE, g.N - b0);
+                    const int64_t em = group_tile_m(side == 'L', (int)sizeof(T));
+                    const int64_t en = group_tile_n(side == 'L', (int)sizeof(T));
+                    const int64_t a0 = tm * em, b0 = tn * en;
+                    const int64_t an = std::min<int64_t>(em, g.M - a0), bn = std::min<int64_t>(en, g.N - b0);
                     d.kind = 2;
                     d.system = H.gem_item[(size_t)gt.x];
                     if (side == 'L') {  // tile rows along R, columns along the RHS

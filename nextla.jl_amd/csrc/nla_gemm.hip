@@ -573,48 +573,52 @@ __global__ void zero_counter_kernel(int *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
-// 64 x 64 tiles: FP64 four waves of 32 x 32 (2 x 2 MFMA blocks), FP32 four
-// waves of one 32 x 32 block each.
+// FP64: 64 x 32 tiles for side 'L', 32 x 64 for side 'R', four waves of
+// 32 x 16 (16 x 32) in 16 k per step; FP32: 64 x 64, four waves of one
+// 32 x 32 block in 32 k per step.
 template <typename T, bool TX, bool TY, int VW>
-static const void *gg_ptr() {
-    if constexpr (sizeof(T) == 8)
-        return (const void *)grouped_gemm_kernel<T, GROUP_TILE, GROUP_TILE, 16, 2, 2, TX, TY, VW>;
-    else
-        return (const void *)grouped_gemm_kernel<T, GROUP_TILE, GROUP_TILE, 32, 2, 2, TX, TY, VW>;
+static const void *gg_ptr(bool rows_m) {
+    if constexpr (sizeof(T) == 8) {
+        if (rows_m) return (const void *)grouped_gemm_kernel<T, 64, 32, 16, 2, 2, TX, TY, VW>;
+        return (const void *)grouped_gemm_kernel<T, 32, 64, 16, 2, 2, TX, TY, VW>;
+    } else {
+        (void)rows_m;
+        return (const void *)grouped_gemm_kernel<T, 64, 64, 32, 2, 2, TX, TY, VW>;
+    }
 }
 
 template <typename T>
-const void *grouped_gemm_ptr(bool tx, bool ty, bool vec) {
+const void *grouped_gemm_ptr(bool tx, bool ty, bool vec, bool rows_m) {
     constexpr int V = 16 / sizeof(T);
     if (vec) {
-        if (!tx && !ty) return gg_ptr<T, false, false, V>();
-        if (tx && !ty) return gg_ptr<T, true, false, V>();
-        if (!tx && ty) return gg_ptr<T, false, true, V>();
-        return gg_ptr<T, true, true, V>();
+        if (!tx && !ty) return gg_ptr<T, false, false, V>(rows_m);
+        if (tx && !ty) return gg_ptr<T, true, false, V>(rows_m);
+        if (!tx && ty) return gg_ptr<T, false, true, V>(rows_m);
+        return gg_ptr<T, true, true, V>(rows_m);
     }
-    if (!tx && !ty) return gg_ptr<T, false, false, 1>();
-    if (tx && !ty) return gg_ptr<T, true, false, 1>();
-    if (!tx && ty) return gg_ptr<T, false, true, 1>();
-    return gg_ptr<T, true, true, 1>();
+    if (!tx && !ty) return gg_ptr<T, false, false, 1>(rows_m);
+    if (tx && !ty) return gg_ptr<T, true, false, 1>(rows_m);
+    if (!tx && ty) return gg_ptr<T, false, true, 1>(rows_m);
+    return gg_ptr<T, true, true, 1>(rows_m);
 }
 
 template <typename T>
 hipError_t launch_grouped_gemm(const GroupGemm<T> *g, const int2 *tasks, int ntasks, int *counter, bool tx, bool ty,
-                               bool vec, int groups, hipStream_t s) {
+                               bool vec, bool rows_m, int groups, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, s, counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     void *args[] = {(void *)&g, (void *)&tasks, (void *)&ntasks, (void *)&counter};
-    return hipLaunchKernel(grouped_gemm_ptr<T>(tx, ty, vec), dim3(groups), dim3(256), args, 0, s);
+    return hipLaunchKernel(grouped_gemm_ptr<T>(tx, ty, vec, rows_m), dim3(groups), dim3(256), args, 0, s);
 }
 
-template const void *grouped_gemm_ptr<float>(bool, bool, bool);
-template const void *grouped_gemm_ptr<double>(bool, bool, bool);
+template const void *grouped_gemm_ptr<float>(bool, bool, bool, bool);
+template const void *grouped_gemm_ptr<double>(bool, bool, bool, bool);
 template hipError_t launch_grouped_gemm<float>(const GroupGemm<float> *, const int2 *, int, int *, bool, bool, bool,
-                                               int, hipStream_t);
+                                               bool, int, hipStream_t);
 template hipError_t launch_grouped_gemm<double>(const GroupGemm<double> *, const int2 *, int, int *, bool, bool,
-                                                bool, int, hipStream_t);
+                                                bool, bool, int, hipStream_t);
 
 template hipError_t launch_gemm<float>(const GemmArgs<float> &, hipStream_t);
 template hipError_t launch_gemm<double>(const GemmArgs<double> &, hipStream_t);

@@ -112,18 +112,25 @@ struct GroupGemm {
     int tiles_m;
 };
 
-// Tile edge of the grouped GEMM (square, so a GEMM and its transpose give the
-// same tile count).
-constexpr int GROUP_TILE = 64;
+// Tile of the grouped GEMM: GROUP_TILE_R along a system's rows and
+// group_tile_c(element size) along its right-hand sides (FP64 32: most staged
+// systems have few; FP32 64, its 32 x 32 MFMA blocks filling four waves), so
+// a GEMM and its transpose (side 'R') give the same tile count.  rows_m: the
+// GEMM's M runs along the system's rows (side 'L'; tiles R x C), else along
+// the right-hand sides (side 'R'; tiles C x R).
+constexpr int GROUP_TILE_R = 64;
+inline int group_tile_c(int elem_bytes) { return elem_bytes == 8 ? 32 : 64; }
+inline int group_tile_m(bool rows_m, int elem_bytes) { return rows_m ? GROUP_TILE_R : group_tile_c(elem_bytes); }
+inline int group_tile_n(bool rows_m, int elem_bytes) { return rows_m ? group_tile_c(elem_bytes) : GROUP_TILE_R; }
 
 // Persistent grouped GEMM over tasks (gemm index, tile index) taken from a
 // counter (cleared by a kernel first); vec: every X and Y of the launch has
 // 16-byte aligned columns and leading dimensions.
 template <typename T>
 hipError_t launch_grouped_gemm(const GroupGemm<T> *g, const int2 *tasks, int ntasks, int *counter, bool tx, bool ty,
-                               bool vec, int groups, hipStream_t s);
+                               bool vec, bool rows_m, int groups, hipStream_t s);
 template <typename T>
-const void *grouped_gemm_ptr(bool tx, bool ty, bool vec);
+const void *grouped_gemm_ptr(bool tx, bool ty, bool vec, bool rows_m);
 
 // One system of a batch in the panel's canonical form (see PanelArgs).
 template <typename T>
