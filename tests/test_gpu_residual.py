@@ -71,6 +71,26 @@ def test_int8_residual_matches_exact_model(nla, side, uplo, trans):
     assert np.all(err <= 4 * 2.0 ** -53 * np.abs(Rm) + scale_bound(side, uplo, trans, A, X, bA, bX)), err.max()
 
 
+@pytest.mark.parametrize("side,uplo,trans,n", [("L", "L", "N", 12), ("R", "U", "T", 12), ("L", "U", "N", 300),
+                                                ("R", "L", "T", 300)])
+def test_int8_reconstruction_rounds_once(nla, side, uplo, trans, n):
+    """The reconstruction forms the integer residual exactly (base-2^40 limbs
+    in exact double arithmetic) and rounds it once: every entry lies within
+    3 ulp of the exact fixed-point residual, with no absolute floor.  n = 12
+    and 300 take 15 and 16 moduli (the other residual tests reach 17)."""
+    rng = np.random.default_rng(n)
+    m = 7
+    A = tri(n, uplo, rng)
+    shp = (n, m) if side == "L" else (m, n)
+    X, B = rng.standard_normal(shp), rng.standard_normal(shp)
+    R, Xr = run(side, uplo, trans, -0.75, A, X, B, "int8")
+    T, bA, bX = RO.plan_bits(n)
+    assert T == (15 if n == 12 else 16)
+    Rm, Xm = RO.fixed_point_model(side, uplo, trans, -0.75, A, X, B, bA, bX)
+    assert np.array_equal(Xr, Xm)
+    assert np.all(np.abs(R - Rm) <= 3 * 2.0 ** -53 * np.abs(Rm)), np.abs(R - Rm).max()
+
+
 @pytest.mark.parametrize("side,uplo,trans", [("L", "L", "N"), ("L", "U", "T"), ("R", "L", "N"), ("R", "U", "T")])
 def test_int8_residual_near_solution(nla, side, uplo, trans):
     """X = the plain FP64 solve: R is ~1e-16 of |A||X|, so it is made of the
