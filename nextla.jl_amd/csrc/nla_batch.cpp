@@ -13,13 +13,14 @@
 // the block itself from the rows not yet multiplied).  Stage s of every
 // system runs in the same launches, so a plan of any mix of sizes takes
 //   1 inversion launch (TRSM) + per stage: a panel launch per size class
-//   and one grouped GEMM launch,
+//   and one grouped GEMM launch (two in FP64: systems of at most 16
+//   right-hand sides take narrow tiles in the second),
 // independent of the number of systems.  Panel tasks are one per (panel,
 // 16-column strip), grouped by size class (the LDS strip of the kernel that
 // solves a class fits its largest panel, so small panels run at higher
 // occupancy) and longest first within a class; GEMM tasks are one per output
-// tile (64 rows by 32 right-hand sides in FP64, where most systems have few;
-// 64 x 64 in FP32), longest k-range first.  Descriptors are uploaded in stream
+// tile (FP64: 64 rows by 32 right-hand sides, by 16 for systems of at most
+// 16, as most have few; FP32: 64 x 64), longest k-range first.  Descriptors are uploaded in stream
 // order into the caller's workspace; executing a plan launches kernels on
 // the caller's stream and nothing else, so it can be captured in a hipGraph
 // and replayed.
@@ -72,14 +73,14 @@ static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, i
         if (func == 'S') L.nblk += (n + 63) / 64;
         if (st > 1) {
             // every stage but one has a GEMM; its output is at most n rows x m
-            const int64_t tc = group_tile_c((int)es);
+            const int64_t tc = group_tile_c((int)es, group_narrow(m, (int)es));
             const int64_t tn = (m + tc - 1) / tc, tr = (n + GROUP_TILE_R - 1) / GROUP_TILE_R;
             L.ngem += st - 1;
             L.ngtasks += (st - 1) * tr * tn;
         }
     }
     L.counter = 0;
-    L.sys = align256(std::max<int64_t>(256, L.nstages * 5 * 4));
+    L.sys = align256(std::max<int64_t>(256, L.nstages * 6 * 4));
     L.pan = align256(L.sys + count * sys_bytes);
     L.ptasks = align256(L.pan + L.npan * sys_bytes);
     L.gem = align256(L.ptasks + L.nptasks * (int64_t)sizeof(BatchTask));
@@ -91,7 +92,8 @@ static Layout layout_of(char func, int32_t dtype, const nla_batch_item *items, i
     return L;
 }
 
-// One launch of the plan: a panel launch (cls 0..3) or a grouped GEMM (cls 4).
+// One launch of the plan: a panel launch (cls 0..3) or a grouped GEMM (cls 4;
+// cls 5: the FP64 GEMMs of few right-hand sides, in narrow tiles).
 struct Launch {
     int stage, cls, start, count, groups;
     bool vec;
@@ -131,6 +133,7 @@ struct HostPlan {
     std::vector<GroupGemm<T>> gem;
     std::vector<int> gem_stage;
     std::vector<int64_t> gem_item, gem_r0, gem_rn, gem_k0, gem_kn;
+    std::vector<char> gem_narrow;
     std::vector<int2> gtasks, blk;
     std::vector<Launch> launches;
     bool tx = false, ty = false;
@@ -282,9 +285,11 @@ static int32_t plan_host(char side, char uplo, char trans, char func, const nla_
             }
             g.alpha = ga;
             g.beta = gb;
-            const int tm_edge = group_tile_m(side == 'L', (int)sizeof(T));
+            const bool narrow = group_narrow(it.m, (int)sizeof(T));
+            const int tm_edge = group_tile_m(side == 'L', (int)sizeof(T), narrow);
             g.tiles_m = (int)((g.M + tm_edge - 1) / tm_edge);
             gem.push_back(g);
+            H.gem_narrow.push_back(narrow ? 1 : 0);
             gem_stage.push_back(s);
             H.gem_item.push_back(i);
             H.gem_r0.push_back(R0);
@@ -309,18 +314,19 @@ static int32_t plan_host(char side, char uplo, char trans, char func, const nla_
         const int pa = (na + 63) / 64, pb = (nb2 + 63) / 64;
         return pa * (pa + 1) > pb * (pb + 1);
     });
-    // GEMM tiles: by stage, longest k-range first.
+    // GEMM tiles: by stage, wide tiles before narrow ones, longest k-range first.
     std::vector<int> gorder(gem.size());
     for (size_t k = 0; k < gorder.size(); ++k) gorder[k] = (int)k;
     std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) {
         if (gem_stage[(size_t)a] != gem_stage[(size_t)b]) return gem_stage[(size_t)a] < gem_stage[(size_t)b];
+        if (H.gem_narrow[(size_t)a] != H.gem_narrow[(size_t)b]) return H.gem_narrow[(size_t)a] < H.gem_narrow[(size_t)b];
         return gem[(size_t)a].K > gem[(size_t)b].K;
     });
     std::vector<int2> &gtasks = H.gtasks;
     gtasks.reserve((size_t)L.ngtasks);
     for (int gi : gorder) {
         const GroupGemm<T> &g = gem[(size_t)gi];
-        const int tn_edge = group_tile_n(side == 'L', (int)sizeof(T));
+        const int tn_edge = group_tile_n(side == 'L', (int)sizeof(T), H.gem_narrow[(size_t)gi] != 0);
         const int tiles = g.tiles_m * (int)((g.N + tn_edge - 1) / tn_edge);
         for (int t = 0; t < tiles; ++t) gtasks.push_back(make_int2(gi, t));
     }
@@ -339,15 +345,18 @@ static int32_t plan_host(char side, char uplo, char trans, char func, const nla_
                 ++pt;
             if (pt > start) H.launches.push_back(Launch{s, c, (int)start, (int)(pt - start), 0, false});
         }
-        const size_t start = gt;
-        bool vec = true;
-        while (gt < gtasks.size() && gem_stage[(size_t)gtasks[gt].x] == s) {
-            const GroupGemm<T> &g = gem[(size_t)gtasks[gt].x];
-            constexpr int V = 16 / sizeof(T);
-            vec = vec && al16(g.X) && al16(g.Y) && g.ldx % V == 0 && g.ldy % V == 0;
-            ++gt;
+        for (int nw = 0; nw < 2; ++nw) {  // wide tiles (cls 4), then narrow ones (cls 5)
+            const size_t start = gt;
+            bool vec = true;
+            while (gt < gtasks.size() && gem_stage[(size_t)gtasks[gt].x] == s &&
+                   H.gem_narrow[(size_t)gtasks[gt].x] == nw) {
+                const GroupGemm<T> &g = gem[(size_t)gtasks[gt].x];
+                constexpr int V = 16 / sizeof(T);
+                vec = vec && al16(g.X) && al16(g.Y) && g.ldx % V == 0 && g.ldy % V == 0;
+                ++gt;
+            }
+            if (gt > start) H.launches.push_back(Launch{s, 4 + nw, (int)start, (int)(gt - start), 0, vec});
         }
-        if (gt > start) H.launches.push_back(Launch{s, 4, (int)start, (int)(gt - start), 0, vec});
     }
     // a perturbed task order within every launch (testing: a task's result
     // must not depend on which work-group runs it or when)
@@ -390,7 +399,7 @@ static int32_t build(char side, char uplo, char trans, char func, const nla_batc
     P->launches = H.launches;
     for (Launch &l : P->launches)
         l.groups = l.cls < 4 ? grid(batch_kernel_ptr<T>(trmm, BATCH_CLASSES[l.cls]), l.count)
-                             : grid(grouped_gemm_ptr<T>(P->tx, P->ty, l.vec, P->rows_m), l.count);
+                             : grid(grouped_gemm_ptr<T>(P->tx, P->ty, l.vec, P->rows_m, l.cls == 5), l.count);
     // one staged image of the descriptor region, copied in stream order
     P->host.assign((size_t)(L.inv - L.sys), 0);
     char *h = P->host.data() - L.sys;
@@ -420,12 +429,12 @@ static hipError_t enqueue(nla_batch_plan_s *P, hipStream_t s) {
     int *counters = (int *)(P->ws + L.counter);
     for (const Launch &l : P->launches) {
         if (e != hipSuccess) break;
-        int *ctr = counters + l.stage * 5 + l.cls;
+        int *ctr = counters + l.stage * 6 + l.cls;
         if (l.cls < 4)
             e = launch_batch<T>(pan, ptasks + l.start, l.count, ctr, P->trmm, BATCH_CLASSES[l.cls], l.groups, s);
         else
-            e = launch_grouped_gemm<T>(gem, gtasks + l.start, l.count, ctr, P->tx, P->ty, l.vec, P->rows_m, l.groups,
-                                       s);
+            e = launch_grouped_gemm<T>(gem, gtasks + l.start, l.count, ctr, P->tx, P->ty, l.vec, P->rows_m,
+                                       l.cls == 5, l.groups, s);
     }
     return e;
 }
@@ -531,8 +540,9 @@ int64_t nla_batch_plan_describe(char side, char uplo, char trans, char func, int
                     const int2 &gt = H.gtasks[(size_t)t];
                     const GroupGemm<T> &g = H.gem[(size_t)gt.x];
                     const int64_t tm = gt.y % g.tiles_m, tn = gt.y / g.tiles_m;
-                    const int64_t em = group_tile_m(side == 'L', (int)sizeof(T));
-                    const int64_t en = group_tile_n(side == 'L', (int)sizeof(T));
+                    const bool nar = H.gem_narrow[(size_t)gt.x] != 0;
+                    const int64_t em = group_tile_m(side == 'L', (int)sizeof(T), nar);
+                    const int64_t en = group_tile_n(side == 'L', (int)sizeof(T), nar);
                     const int64_t a0 = tm * em, b0 = tn * en;
                     const int64_t an = std::min<int64_t>(em, g.M - a0), bn = std::min<int64_t>(en, g.N - b0);
                     d.kind = 2;

@@ -574,51 +574,57 @@ __global__ void zero_counter_kernel(int *p) {
 }
 
 // FP64: 64 x 32 tiles for side 'L', 32 x 64 for side 'R', four waves of
-// 32 x 16 (16 x 32) in 16 k per step; FP32: 64 x 64, four waves of one
-// 32 x 32 block in 32 k per step.
+// 32 x 16 (16 x 32) in 16 k per step; narrow GEMMs 64 x 16 (16 x 64), four
+// waves of one 16 x 16 block in 32 k per step.  FP32: 64 x 64, four waves
+// of one 32 x 32 block in 32 k per step.
 template <typename T, bool TX, bool TY, int VW>
-static const void *gg_ptr(bool rows_m) {
+static const void *gg_ptr(bool rows_m, bool narrow) {
     if constexpr (sizeof(T) == 8) {
+        if (narrow) {
+            if (rows_m) return (const void *)grouped_gemm_kernel<T, 64, 16, 32, 4, 1, TX, TY, VW>;
+            return (const void *)grouped_gemm_kernel<T, 16, 64, 32, 1, 4, TX, TY, VW>;
+        }
         if (rows_m) return (const void *)grouped_gemm_kernel<T, 64, 32, 16, 2, 2, TX, TY, VW>;
         return (const void *)grouped_gemm_kernel<T, 32, 64, 16, 2, 2, TX, TY, VW>;
     } else {
         (void)rows_m;
+        (void)narrow;
         return (const void *)grouped_gemm_kernel<T, 64, 64, 32, 2, 2, TX, TY, VW>;
     }
 }
 
 template <typename T>
-const void *grouped_gemm_ptr(bool tx, bool ty, bool vec, bool rows_m) {
+const void *grouped_gemm_ptr(bool tx, bool ty, bool vec, bool rows_m, bool narrow) {
     constexpr int V = 16 / sizeof(T);
     if (vec) {
-        if (!tx && !ty) return gg_ptr<T, false, false, V>(rows_m);
-        if (tx && !ty) return gg_ptr<T, true, false, V>(rows_m);
-        if (!tx && ty) return gg_ptr<T, false, true, V>(rows_m);
-        return gg_ptr<T, true, true, V>(rows_m);
+        if (!tx && !ty) return gg_ptr<T, false, false, V>(rows_m, narrow);
+        if (tx && !ty) return gg_ptr<T, true, false, V>(rows_m, narrow);
+        if (!tx && ty) return gg_ptr<T, false, true, V>(rows_m, narrow);
+        return gg_ptr<T, true, true, V>(rows_m, narrow);
     }
-    if (!tx && !ty) return gg_ptr<T, false, false, 1>(rows_m);
-    if (tx && !ty) return gg_ptr<T, true, false, 1>(rows_m);
-    if (!tx && ty) return gg_ptr<T, false, true, 1>(rows_m);
-    return gg_ptr<T, true, true, 1>(rows_m);
+    if (!tx && !ty) return gg_ptr<T, false, false, 1>(rows_m, narrow);
+    if (tx && !ty) return gg_ptr<T, true, false, 1>(rows_m, narrow);
+    if (!tx && ty) return gg_ptr<T, false, true, 1>(rows_m, narrow);
+    return gg_ptr<T, true, true, 1>(rows_m, narrow);
 }
 
 template <typename T>
 hipError_t launch_grouped_gemm(const GroupGemm<T> *g, const int2 *tasks, int ntasks, int *counter, bool tx, bool ty,
-                               bool vec, bool rows_m, int groups, hipStream_t s) {
+                               bool vec, bool rows_m, bool narrow, int groups, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, s, counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     void *args[] = {(void *)&g, (void *)&tasks, (void *)&ntasks, (void *)&counter};
-    return hipLaunchKernel(grouped_gemm_ptr<T>(tx, ty, vec, rows_m), dim3(groups), dim3(256), args, 0, s);
+    return hipLaunchKernel(grouped_gemm_ptr<T>(tx, ty, vec, rows_m, narrow), dim3(groups), dim3(256), args, 0, s);
 }
 
-template const void *grouped_gemm_ptr<float>(bool, bool, bool, bool);
-template const void *grouped_gemm_ptr<double>(bool, bool, bool, bool);
+template const void *grouped_gemm_ptr<float>(bool, bool, bool, bool, bool);
+template const void *grouped_gemm_ptr<double>(bool, bool, bool, bool, bool);
 template hipError_t launch_grouped_gemm<float>(const GroupGemm<float> *, const int2 *, int, int *, bool, bool, bool,
-                                               bool, int, hipStream_t);
+                                               bool, bool, int, hipStream_t);
 template hipError_t launch_grouped_gemm<double>(const GroupGemm<double> *, const int2 *, int, int *, bool, bool,
-                                                bool, bool, int, hipStream_t);
+                                                bool, bool, bool, int, hipStream_t);
 
 template hipError_t launch_gemm<float>(const GemmArgs<float> &, hipStream_t);
 template hipError_t launch_gemm<double>(const GemmArgs<double> &, hipStream_t);

@@ -118,19 +118,26 @@ struct GroupGemm {
 // a GEMM and its transpose (side 'R') give the same tile count.  rows_m: the
 // GEMM's M runs along the system's rows (side 'L'; tiles R x C), else along
 // the right-hand sides (side 'R'; tiles C x R).
-constexpr int GROUP_TILE_R = 64;
-inline int group_tile_c(int elem_bytes) { return elem_bytes == 8 ? 32 : 64; }
-inline int group_tile_m(bool rows_m, int elem_bytes) { return rows_m ? GROUP_TILE_R : group_tile_c(elem_bytes); }
-inline int group_tile_n(bool rows_m, int elem_bytes) { return rows_m ? group_tile_c(elem_bytes) : GROUP_TILE_R; }
+// FP64 GEMMs of at most GROUP_NARROW right-hand sides (half the staged
+// systems) take tiles GROUP_NARROW wide, in a launch of their own.
+constexpr int GROUP_TILE_R = 64, GROUP_NARROW = 16;
+inline bool group_narrow(int64_t rhs, int elem_bytes) { return elem_bytes == 8 && rhs <= GROUP_NARROW; }
+inline int group_tile_c(int elem_bytes, bool narrow) { return elem_bytes == 8 ? (narrow ? GROUP_NARROW : 32) : 64; }
+inline int group_tile_m(bool rows_m, int elem_bytes, bool narrow) {
+    return rows_m ? GROUP_TILE_R : group_tile_c(elem_bytes, narrow);
+}
+inline int group_tile_n(bool rows_m, int elem_bytes, bool narrow) {
+    return rows_m ? group_tile_c(elem_bytes, narrow) : GROUP_TILE_R;
+}
 
 // Persistent grouped GEMM over tasks (gemm index, tile index) taken from a
 // counter (cleared by a kernel first); vec: every X and Y of the launch has
 // 16-byte aligned columns and leading dimensions.
 template <typename T>
 hipError_t launch_grouped_gemm(const GroupGemm<T> *g, const int2 *tasks, int ntasks, int *counter, bool tx, bool ty,
-                               bool vec, bool rows_m, int groups, hipStream_t s);
+                               bool vec, bool rows_m, bool narrow, int groups, hipStream_t s);
 template <typename T>
-const void *grouped_gemm_ptr(bool tx, bool ty, bool vec, bool rows_m);
+const void *grouped_gemm_ptr(bool tx, bool ty, bool vec, bool rows_m, bool narrow);
 
 // One system of a batch in the panel's canonical form (see PanelArgs).
 template <typename T>
