@@ -101,13 +101,21 @@ __device__ __forceinline__ bool oz_keep(const OzSrc &s, int64_t o, int64_t k) {
 // Stages the 64 x 64 tile (o0, k0) of the source into LDS (zero outside the
 // kept region), loading along whichever stride is 1.
 __device__ __forceinline__ void oz_load_tile(const OzSrc &s, int64_t o0, int64_t k0, double (*tile)[65]) {
+    // all 16 loads of a thread are issued before the first LDS store, so
+    // they are in flight together
     const bool kfast = (s.sk == 1);
-    for (int idx = threadIdx.x; idx < 64 * 64; idx += 256) {
-        const int a = idx & 63, b = idx >> 6;
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u, a = idx & 63, b = idx >> 6;
         const int oo = kfast ? b : a, ko = kfast ? a : b;
         const int64_t o = o0 + oo, k = k0 + ko;
-        const bool ok = oz_keep(s, o, k);
-        tile[oo][ko] = ok ? s.P[o * s.so + k * s.sk] : 0.0;
+        v[u] = oz_keep(s, o, k) ? s.P[o * s.so + k * s.sk] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int idx = threadIdx.x + 256 * u, a = idx & 63, b = idx >> 6;
+        tile[kfast ? b : a][kfast ? a : b] = v[u];
     }
 }
 
@@ -119,17 +127,63 @@ __global__ __launch_bounds__(256) void oz_absmax_kernel(OzSrc s, unsigned long l
     const int64_t o0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
     if (s.tri == 1 && k0 > o0 + 63) return;
     if (s.tri == 2 && k0 + 63 < o0) return;
+    __shared__ unsigned long long part[3][64];
     oz_load_tile(s, o0, k0, tile);
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int oo = threadIdx.x;
-        unsigned long long best = 0;
-        for (int k = 0; k < 64; ++k) {
-            const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(tile[oo][k]));
-            best = b > best ? b : best;
-        }
+    // wave w reduces columns 16w .. 16w + 15 of every row; wave 0 combines
+    const int oo = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long best = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(tile[oo][16 * w + k]));
+        best = b > best ? b : best;
+    }
+    if (w > 0) part[w - 1][oo] = best;
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) best = part[q][oo] > best ? part[q][oo] : best;
         if (o0 + oo < s.no && best != 0) atomicMax(mx + o0 + oo, best);
     }
+}
+
+// The same maxima for a source whose rows are contiguous (sk = 1) and whole
+// (no triangle), as X's systems are: one wave per row and chunk of k, four
+// coalesced loads in flight per lane, one atomicMax per wave (no staging of
+// tiles through LDS).
+__global__ __launch_bounds__(256) void oz_absmax_rows_kernel(OzSrc s, int64_t chunk, unsigned long long *mx) {
+    const int lane = threadIdx.x & 63;
+    const int64_t o = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (o >= s.no) return;
+    const int64_t k0 = (int64_t)blockIdx.y * chunk, k1 = k0 + chunk < s.nk ? k0 + chunk : s.nk;
+    const double *row = s.P + o * s.so;
+    unsigned long long b[4] = {0, 0, 0, 0};
+    int64_t k = k0 + lane;
+    for (; k + 192 < k1; k += 256) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = row[k + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const unsigned long long x = (unsigned long long)__double_as_longlong(fabs(v[u]));
+            b[u] = x > b[u] ? x : b[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u, k += 64)
+        if (k < k1) {
+            const unsigned long long x = (unsigned long long)__double_as_longlong(fabs(row[k]));
+            b[u] = x > b[u] ? x : b[u];
+        }
+    unsigned long long best = b[0] > b[1] ? b[0] : b[1];
+    best = b[2] > best ? b[2] : best;
+    best = b[3] > best ? b[3] : best;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long x = __shfl_xor(best, off);
+        best = x > best ? x : best;
+    }
+    if (lane == 0 && best != 0) atomicMax(mx + o, best);
 }
 
 // e[o] = bits - 1 - floor(log2 mx[o]) (so that max |src(o,.)| 2^e[o] lies in
@@ -713,8 +767,19 @@ static hipError_t oz_fill(void *p, int64_t words, hipStream_t s) {
 static hipError_t oz_scales(const OzSrc &src, unsigned long long *mx, int bits, int *e, int *flag, hipStream_t s) {
     hipError_t err = oz_fill(mx, 2 * src.no, s);
     if (err != hipSuccess) return err;
-    dim3 g((unsigned)((src.nk + 63) / 64), (unsigned)((src.no + 63) / 64));
-    hipLaunchKernelGGL(oz_absmax_kernel, g, dim3(256), 0, s, src, mx);
+    if (src.sk == 1 && src.tri == 0) {
+        // chunks of at least 1024 k, about 16k work-groups in all
+        const int64_t wg = (src.no + 3) / 4;
+        int64_t nch = (16384 + wg - 1) / wg;
+        const int64_t most = (src.nk + 1023) / 1024;
+        nch = nch < 1 ? 1 : (nch > most ? most : nch);
+        const int64_t chunk = (src.nk + nch - 1) / nch;
+        hipLaunchKernelGGL(oz_absmax_rows_kernel, dim3((unsigned)wg, (unsigned)nch), dim3(256), 0, s, src, chunk,
+                           mx);
+    } else {
+        dim3 g((unsigned)((src.nk + 63) / 64), (unsigned)((src.no + 63) / 64));
+        hipLaunchKernelGGL(oz_absmax_kernel, g, dim3(256), 0, s, src, mx);
+    }
     err = hipGetLastError();
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((src.no + 255) / 256)), dim3(256), 0, s, src.no, mx, bits,
