@@ -468,7 +468,18 @@ __global__ __launch_bounds__(256) void tile_absmax_kernel(int64_t n, const float
     if (lower ? ti >= tj : ti <= tj) {
         const int64_t r = ti * 64 + (t & 63);
         const int64_t c0 = tj * 64 + (t >> 6) * 16;
-        if (r < n) {
+        if (ti != tj && (ti + 1) * 64 <= n && (tj + 1) * 64 <= n) {
+            // a whole off-diagonal tile: 16 loads in flight, no checks
+            const float *p = A32 + r + c0 * ld32;
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = p[u * ld32];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const unsigned b = __float_as_uint(fabsf(v[u]));
+                m = b > m ? b : m;
+            }
+        } else if (r < n) {
 #pragma unroll 4
             for (int u = 0; u < 16; ++u) {
                 const int64_t c = c0 + u;
