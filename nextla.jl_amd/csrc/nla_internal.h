@@ -334,7 +334,6 @@ struct OzConst {
     int y[OZ_TMAX];  // (M / p_t)^-1 mod p_t
     int rc[OZ_TMAX][3];  // centred residues of 2^16, 2^32, 2^48 modulo p_t
     int rhi[OZ_TMAX];    // centred residues lie in [rhi - p + 1, rhi], rhi = floor((p - 1) / 2)
-    float invpf[OZ_TMAX];
     double invpd[OZ_TMAX];
     // w_t = y_t (M / p_t) (the CRT basis: w_t = 1 mod p_t, 0 mod the
     // others) and M, as four base-2^40 limbs each (exact in double)

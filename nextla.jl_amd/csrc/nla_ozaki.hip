@@ -63,22 +63,20 @@ __device__ __forceinline__ void oz_limbs16(double v, int &l0, int &l1, int &l2, 
     l3 = (int)h3;
 }
 
-// The centred residue of v (given by its limbs) modulo p_t, in [rlo_t, rhi_t]
-// (rhi_t = floor((p_t - 1) / 2), rlo_t = rhi_t - p_t + 1).  With the limb
-// weights' centred residues c_i (|c_i| <= 128), x = l0 + sum l_i c_i = v
-// (mod p_t) and |x| < 2^25; q = round(x / p_t) in float is off by at most
-// one and one conditional step each way brings x - q p_t into range.  All
-// 24-bit multiply-adds (full rate).
+// The low byte of the centred residue of v (given by its limbs) modulo p_t
+// (the residue lies in [rhi_t - p_t + 1, rhi_t], rhi_t = floor((p_t - 1) / 2),
+// so its byte is its int8 value).  With the limb weights' centred residues
+// c_i (|c_i| <= 128), x = l0 + sum l_i c_i = v (mod p_t) and |x| < 2^25.  For
+// odd p_t, q = round(x / p_t) in double is exact (the product's error is below
+// 2^-34 and x / p_t lies at least 1 / (2 p_t) from a half-integer), so
+// x - q p_t is the centred residue itself; for p_t = 256 the byte of x - q p_t
+// is x's low byte whatever q is.  No correction steps.
 __device__ __forceinline__ int oz_residue16(int l0, int l1, int l2, int l3, const OzConst &C, int t) {
     int x = __mul24(l1, C.rc[t][0]) + l0;
     x = __mul24(l2, C.rc[t][1]) + x;
     x = __mul24(l3, C.rc[t][2]) + x;
-    const int p = C.p[t];
-    const int q = __float2int_rn((float)x * C.invpf[t]);
-    int r = __mul24(q, -p) + x;
-    r -= (r > C.rhi[t]) ? p : 0;
-    r += (r < C.rhi[t] - p + 1) ? p : 0;
-    return r;
+    const int q = __double2int_rn((double)x * C.invpd[t]);
+    return __mul24(q, -C.p[t]) + x;
 }
 
 // ---- scales -----------------------------------------------------------------
@@ -695,7 +693,6 @@ OzPlan oz_plan(int64_t n, int64_t m) {
     for (int t = 0; t < T; ++t) {
         const int64_t p = OZ_MODULI[t];
         C.p[t] = (int)p;
-        C.invpf[t] = 1.0f / (float)p;
         C.invpd[t] = 1.0 / (double)p;
         int64_t mt = 1;  // (M / p_t) mod p_t
         for (int s = 0; s < T; ++s)
